@@ -1,0 +1,167 @@
+// Forward Kennett reflectivity over a (slowness, frequency) grid.
+//
+// One work-item walks one (p, omega) lane through all interfaces, bottom to
+// top (Kennett_Reflectivity/kennett_reflectivity_gpu.py:216-272):
+//
+//   RRd <- 0 below the deepest interface
+//   for each interface i (deepest first), with b = i + 1 the layer below:
+//     MT  = E_b RRd E_b,          E_b = diag(exp(i w eta_b h_b), exp(i w nu_b h_b))
+//     U   = (I - Ru_i MT)^-1      (closed-form 2x2 inverse)
+//     RRd = Rd_i + Tu_i MT U Td_i
+//   R = exp(2 i w eta_0 h_0) RRd_00            (+ free surface: R / (1 + R))
+//
+// A workgroup owns one slowness and a tile of frequencies.  Its prologue
+// stages the layer slownesses and the frequency-independent interface
+// coefficients of that slowness in LDS (one work-item per layer, then per
+// interface); every lane then reads them as wave-uniform LDS broadcasts.
+// Phase factors and the 2x2 algebra stay in registers; each store writes a
+// coalesced run of complex128 along omega.
+#include "kh_physics.hpp"
+#include "kh_runtime.hpp"
+#include "kh_sweep.hpp"
+
+namespace kh {
+
+__global__ void vertical_slowness_kernel(const double* __restrict__ layers, int nl,
+                                         const double* __restrict__ p, int np,
+                                         cd* __restrict__ eta, cd* __restrict__ neta) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)nl * np) return;
+  const int l = (int)(idx / np);
+  const int j = (int)(idx % np);
+  const LayerPt L = layer_at(layers + (size_t)l * NCOL, l, p[j]);
+  eta[idx] = L.eta;
+  neta[idx] = L.neta;
+}
+
+__global__ void interface_kernel(const double* __restrict__ layers, int nl,
+                                 const double* __restrict__ p, int np, cd* __restrict__ Rd,
+                                 cd* __restrict__ Ru, cd* __restrict__ Tu, cd* __restrict__ Td) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)(nl - 1) * np) return;
+  const int i = (int)(idx / np);
+  const int j = (int)(idx % np);
+  const double pv = p[j];
+  const LayerPt up = layer_at(layers + (size_t)i * NCOL, i, pv);
+  const LayerPt dn = layer_at(layers + (size_t)(i + 1) * NCOL, i + 1, pv);
+  const Iface f = interface_at(i, pv, up, dn);
+  cd* rd = Rd + idx * 4;
+  cd* ru = Ru + idx * 4;
+  cd* tu = Tu + idx * 4;
+  cd* td = Td + idx * 4;
+  rd[0] = f.rd00; rd[1] = f.rd01; rd[2] = f.rd01; rd[3] = f.rd11;
+  ru[0] = f.ru00; ru[1] = f.ru01; ru[2] = f.ru01; ru[3] = f.ru11;
+  td[0] = f.td00; td[1] = f.td01; td[2] = f.td10; td[3] = f.td11;
+  tu[0] = f.td00; tu[1] = f.td10; tu[2] = f.td01; tu[3] = f.td11;
+}
+
+// grid.x = np * ntiles; block b covers slowness b / ntiles and frequencies
+// [tile * blockDim * wpt, (tile + 1) * blockDim * wpt).
+__global__ void __launch_bounds__(1024) forward_kernel(
+    const double* __restrict__ layers, int nl, const double* __restrict__ p,
+    const cd* __restrict__ omega, int nw, int free_surface, cd* __restrict__ R, int ntiles,
+    int wpt) {
+  extern __shared__ double smem_d[];
+  LayerPt* s_layer = reinterpret_cast<LayerPt*>(smem_d);
+  Iface* s_if = reinterpret_cast<Iface*>(s_layer + nl);
+
+  const int ip = blockIdx.x / ntiles;
+  const int tile = blockIdx.x - ip * ntiles;
+  const double pv = p[ip];
+  stage_slowness(layers, nl, pv, s_layer, s_if);
+
+  const int base = tile * blockDim.x * wpt + threadIdx.x;
+  cd* Rrow = R + (size_t)ip * nw;
+  for (int k = 0; k < wpt; ++k) {
+    const int w = base + k * blockDim.x;
+    if (w >= nw) break;
+    Rrow[w] = sweep_lane(s_layer, s_if, nl, omega[w], free_surface);
+  }
+}
+
+size_t forward_lds_bytes(int nl) {
+  return (size_t)nl * sizeof(LayerPt) + (size_t)(nl - 1) * sizeof(Iface);
+}
+
+// Default launch shape: as many frequencies per work-item as still leaves
+// >= 4 workgroups per CU on a 256-CU part.
+static void default_shape(int np, int nw, int* block, int* wpt) {
+  *block = 256;
+  *wpt = 1;
+  for (int cand = 4; cand > 1; cand >>= 1) {
+    const long tiles = (nw + 256L * cand - 1) / (256L * cand);
+    if (tiles * (long)np >= 1024) {
+      *wpt = cand;
+      break;
+    }
+  }
+  if ((long)np * ((nw + 255) / 256) < 1024 && nw <= 64) *block = 64;
+}
+
+}  // namespace kh
+
+using namespace kh;
+
+extern "C" int kh_vertical_slowness(const double* layers, int nl, const double* p, int np,
+                                    double* eta, double* neta, void* stream) {
+  clear_error();
+  KH_TRY(check_model("kh_vertical_slowness", layers, nl));
+  KH_REQUIRE(np >= 0, KH_ERR_INVALID_ARGUMENT, "kh_vertical_slowness: np < 0");
+  if (np == 0) return KH_OK;
+  KH_REQUIRE(p && eta && neta, KH_ERR_INVALID_ARGUMENT, "kh_vertical_slowness: NULL pointer");
+  const long n = (long)nl * np;
+  const int blk = 256;
+  hipLaunchKernelGGL(vertical_slowness_kernel, dim3((unsigned)((n + blk - 1) / blk)), dim3(blk),
+                     0, as_stream(stream), layers, nl, p, np, reinterpret_cast<cd*>(eta),
+                     reinterpret_cast<cd*>(neta));
+  return check_launch("kh_vertical_slowness");
+}
+
+extern "C" int kh_interface_coefficients(const double* layers, int nl, const double* p, int np,
+                                         double* Rd, double* Ru, double* Tu, double* Td,
+                                         void* stream) {
+  clear_error();
+  KH_TRY(check_model("kh_interface_coefficients", layers, nl));
+  KH_REQUIRE(np >= 0, KH_ERR_INVALID_ARGUMENT, "kh_interface_coefficients: np < 0");
+  if (np == 0) return KH_OK;
+  KH_REQUIRE(p && Rd && Ru && Tu && Td, KH_ERR_INVALID_ARGUMENT,
+             "kh_interface_coefficients: NULL pointer");
+  const long n = (long)(nl - 1) * np;
+  const int blk = 256;
+  hipLaunchKernelGGL(interface_kernel, dim3((unsigned)((n + blk - 1) / blk)), dim3(blk), 0,
+                     as_stream(stream), layers, nl, p, np, reinterpret_cast<cd*>(Rd),
+                     reinterpret_cast<cd*>(Ru), reinterpret_cast<cd*>(Tu),
+                     reinterpret_cast<cd*>(Td));
+  return check_launch("kh_interface_coefficients");
+}
+
+extern "C" int kh_forward_ex(const double* layers, int nl, const double* p, int np,
+                             const double* omega, int nw, int free_surface, double* R, int block,
+                             int omega_per_item, void* stream) {
+  clear_error();
+  KH_TRY(check_model("kh_forward", layers, nl));
+  KH_REQUIRE(np >= 0 && nw >= 0, KH_ERR_INVALID_ARGUMENT, "kh_forward: negative size");
+  if (np == 0 || nw == 0) return KH_OK;
+  KH_REQUIRE(p && omega && R, KH_ERR_INVALID_ARGUMENT, "kh_forward: NULL pointer");
+  KH_REQUIRE(block == 64 || block == 128 || block == 256 || block == 512 || block == 1024,
+             KH_ERR_INVALID_ARGUMENT, "kh_forward: block %d not in {64,...,1024}", block);
+  KH_REQUIRE(omega_per_item >= 1 && omega_per_item <= 16, KH_ERR_INVALID_ARGUMENT,
+             "kh_forward: omega_per_item %d not in 1..16", omega_per_item);
+  const long span = (long)block * omega_per_item;
+  const long ntiles = (nw + span - 1) / span;
+  const long nblocks = ntiles * (long)np;
+  KH_REQUIRE(nblocks <= 0x7fffffffL, KH_ERR_UNSUPPORTED, "kh_forward: grid too large");
+  const size_t lds = forward_lds_bytes(nl);
+  hipLaunchKernelGGL(forward_kernel, dim3((unsigned)nblocks), dim3(block), lds,
+                     as_stream(stream), layers, nl, p, reinterpret_cast<const cd*>(omega), nw,
+                     free_surface, reinterpret_cast<cd*>(R), (int)ntiles, omega_per_item);
+  return check_launch("kh_forward");
+}
+
+extern "C" int kh_forward(const double* layers, int nl, const double* p, int np,
+                          const double* omega, int nw, int free_surface, double* R,
+                          void* stream) {
+  int block, wpt;
+  default_shape(np, nw, &block, &wpt);
+  return kh_forward_ex(layers, nl, p, np, omega, nw, free_surface, R, block, wpt, stream);
+}

@@ -1,0 +1,96 @@
+"""Device-level entry points: torch tensors in HBM in, torch tensors out.
+
+These are the calls the drop-in modules, the gather and the benchmark build
+on.  Every function launches on torch's current HIP stream of the tensors'
+device and returns without synchronising.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _abi
+from .layer_model import LayerModel, pack_layers
+
+__all__ = [
+    "layers_tensor",
+    "forward",
+    "vertical_slowness",
+    "interface_coefficients",
+]
+
+
+def layers_tensor(model: LayerModel | np.ndarray | torch.Tensor, dev=None) -> torch.Tensor:
+    """The (nl, 6) float64 layer table of ``model`` on the device."""
+    dev = _abi.device(dev)
+    if isinstance(model, LayerModel):
+        tab = torch.from_numpy(pack_layers(model))
+    elif isinstance(model, np.ndarray):
+        tab = torch.from_numpy(np.ascontiguousarray(model, dtype=np.float64))
+    else:
+        tab = model.detach().to(torch.float64)
+    if tab.ndim != 2 or tab.shape[1] != 6:
+        raise ValueError(f"layer table must be (n_layers, 6), got {tuple(tab.shape)}")
+    return tab.to(dev).contiguous()
+
+
+def _as_dev(x, dtype, dev) -> torch.Tensor:
+    if isinstance(x, torch.Tensor):
+        return x.detach().to(device=dev, dtype=dtype).contiguous()
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(x).astype(
+        np.complex128 if dtype == torch.complex128 else np.float64))).to(dev)
+
+
+def forward(layers: torch.Tensor, p: torch.Tensor, omega: torch.Tensor,
+            free_surface: bool = False, out: torch.Tensor | None = None,
+            block: int | None = None, omega_per_item: int | None = None) -> torch.Tensor:
+    """R (np, nw) complex128 on the device for a layer table, slownesses and
+    (possibly damped) frequencies already on the device."""
+    dev = layers.device
+    p = _as_dev(p, torch.float64, dev)
+    omega = _as_dev(omega, torch.complex128, dev)
+    n_p, n_w = p.numel(), omega.numel()
+    if out is None:
+        out = torch.empty((n_p, n_w), dtype=torch.complex128, device=dev)
+    elif out.shape != (n_p, n_w) or out.dtype != torch.complex128 or not out.is_contiguous():
+        raise ValueError("out must be a contiguous complex128 tensor of shape (np, nw)")
+    lib = _abi.lib()
+    s = _abi.stream_handle(dev)
+    nl = layers.shape[0]
+    if block is None and omega_per_item is None:
+        rc = lib.kh_forward(_abi.ptr(layers), nl, _abi.ptr(p), n_p, _abi.ptr(omega), n_w,
+                            int(bool(free_surface)), _abi.ptr(out), s)
+    else:
+        rc = lib.kh_forward_ex(_abi.ptr(layers), nl, _abi.ptr(p), n_p, _abi.ptr(omega), n_w,
+                               int(bool(free_surface)), _abi.ptr(out), int(block or 256),
+                               int(omega_per_item or 1), s)
+    _abi.check(rc, "kh_forward")
+    return out
+
+
+def vertical_slowness(layers: torch.Tensor, p) -> tuple[torch.Tensor, torch.Tensor]:
+    """(eta, neta), each (nl, np) complex128 on the device."""
+    dev = layers.device
+    p = _as_dev(p, torch.float64, dev)
+    nl = layers.shape[0]
+    eta = torch.empty((nl, p.numel()), dtype=torch.complex128, device=dev)
+    neta = torch.empty_like(eta)
+    rc = _abi.lib().kh_vertical_slowness(_abi.ptr(layers), nl, _abi.ptr(p), p.numel(),
+                                         _abi.ptr(eta), _abi.ptr(neta), _abi.stream_handle(dev))
+    _abi.check(rc, "kh_vertical_slowness")
+    return eta, neta
+
+
+def interface_coefficients(layers: torch.Tensor, p):
+    """(Rd, Ru, Tu, Td), each (nl-1, np, 2, 2) complex128 on the device."""
+    dev = layers.device
+    p = _as_dev(p, torch.float64, dev)
+    nl = layers.shape[0]
+    outs = [torch.empty((nl - 1, p.numel(), 2, 2), dtype=torch.complex128, device=dev)
+            for _ in range(4)]
+    rc = _abi.lib().kh_interface_coefficients(_abi.ptr(layers), nl, _abi.ptr(p), p.numel(),
+                                              *[_abi.ptr(o) for o in outs],
+                                              _abi.stream_handle(dev))
+    _abi.check(rc, "kh_interface_coefficients")
+    return tuple(outs)

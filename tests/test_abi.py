@@ -1,0 +1,62 @@
+"""The C-ABI library: it loads, and it exports every entry point that
+include/kennett_hip.h declares (symbol lookup only; no GPU work)."""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from seismicinversion_amd import _abi, _build
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                      "include", "kennett_hip.h")
+
+
+def declared_entry_points():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(kh_\w+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(_build.LIB_PATH):
+        _build.build()
+    return ctypes.CDLL(_build.LIB_PATH)
+
+
+def test_header_declares_entry_points():
+    names = declared_entry_points()
+    assert "kh_forward" in names and "kh_last_error" in names
+
+
+def test_library_exports_every_declared_symbol(lib):
+    missing = [n for n in declared_entry_points() if not hasattr(lib, n)]
+    assert not missing, f"not exported: {missing}"
+
+
+def test_binding_table_matches_header():
+    assert sorted(_abi.EXPORTS) == declared_entry_points()
+
+
+def test_version_and_error_strings(lib):
+    lib.kh_version.restype = ctypes.c_char_p
+    lib.kh_last_error.restype = ctypes.c_char_p
+    assert b"gfx950" in lib.kh_version()
+    assert lib.kh_last_error() == b""
+
+
+def test_invalid_arguments_are_rejected_before_any_launch(lib):
+    """Argument checks run on the host and return a status; no GPU needed."""
+    lib.kh_last_error.restype = ctypes.c_char_p
+    vp = ctypes.c_void_p
+    rc = lib.kh_forward(vp(0), 1, vp(0), 4, vp(0), 4, 0, vp(0), vp(0))
+    assert rc == 1 and b"at least 2 layers" in lib.kh_last_error()
+    rc = lib.kh_forward(vp(0), 300, vp(0), 4, vp(0), 4, 0, vp(0), vp(0))
+    assert rc == 4 and b"KH_MAX_LAYERS" in lib.kh_last_error()
+    rc = lib.kh_forward_ex(vp(16), 3, vp(16), 4, vp(16), 4, 0, vp(16), 96, 1, vp(0))
+    assert rc == 1 and b"block" in lib.kh_last_error()
+    assert lib.kh_forward(vp(16), 3, vp(16), 0, vp(16), 4, 0, vp(16), vp(0)) == 0
