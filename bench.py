@@ -88,12 +88,14 @@ def main():
     model, p, omega = c2_forward_inputs()
     layers = engine.layers_tensor(model, dev)
     p_t = torch.from_numpy(p).to(dev)
-    w_t = torch.from_numpy(omega).to(dev)
     out = torch.empty((len(p), len(omega)), dtype=torch.complex128, device=dev)
     updates = len(p) * len(omega) * (model.n_layers - 1)
 
+    dw = 2.0 * np.pi / 64.0
+
     def step():
-        engine.forward(layers, p_t, w_t, free_surface=True, out=out)
+        engine.forward_grid(layers, p_t, 1.0, dw, np.pi / 64.0, len(omega), free_surface=True,
+                            out=out)
 
     for _ in range(args.warmup):
         step()
@@ -121,7 +123,7 @@ def main():
             traffic = json.load(f).get("bytes_per_launch")
     roofline = {"bound": "mfma", "pipe": "fp64-vector", "achieved": achieved,
                 "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
-                "traffic": traffic, "kernel": "kh::forward_kernel",
+                "traffic": traffic, "kernel": "kh::forward_kernel<W, true>",
                 "kernel_ms": kern_ms, "flop_per_update": FLOP_PER_UPDATE,
                 "algorithmic_bytes": 16 * len(p) * len(omega),
                 "hbm_gbs": 16 * len(p) * len(omega) / (kern_ms * 1e-3) / 1e9}

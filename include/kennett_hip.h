@@ -32,7 +32,7 @@ extern "C" {
 
 /* Largest layer count one launch accepts (the per-slowness layer and
  * interface tables are staged in LDS). */
-#define KH_MAX_LAYERS 256
+#define KH_MAX_LAYERS 240
 
 /* Library version string. */
 const char* kh_version(void);
@@ -71,11 +71,26 @@ int kh_forward(const double* layers, int nl, const double* p, int np,
 
 /* kh_forward with an explicit launch shape: `block` work-items per
  * workgroup (64, 128, 256, 512 or 1024) and `omega_per_item` frequencies
- * walked by each work-item (1..16).  Results are identical to kh_forward's
- * for every shape. */
+ * walked by each work-item (1, 2, 3, 4 or 8).  Results are identical to
+ * kh_forward's for every shape. */
 int kh_forward_ex(const double* layers, int nl, const double* p, int np,
                   const double* omega, int nw, int free_surface, double* R,
                   int block, int omega_per_item, void* stream);
+
+/* kh_forward on the uniform frequency grid omega_j = (k0 + j)*dw + i*gamma,
+ * j = 0..nw-1 — the grids the reference builds:
+ * omega = arange(1, nw)*dw + 1j*gamma (Kennett_Reflectivity/kennett_gather_gpu.py:226-231)
+ * and arange(1, nfreq+1)*dw (Kennett_Reflectivity/taup_inversion.py:298-301).
+ * No frequency array is read; phase factors of the frequencies a work-item
+ * carries beyond its first come from exp(i(w + s)eta h) = exp(i w eta h) exp(i s eta h). */
+int kh_forward_grid(const double* layers, int nl, const double* p, int np,
+                    double k0, double dw, double gamma, int nw, int free_surface,
+                    double* R, void* stream);
+
+/* kh_forward_grid with an explicit launch shape (as kh_forward_ex). */
+int kh_forward_grid_ex(const double* layers, int nl, const double* p, int np,
+                       double k0, double dw, double gamma, int nw, int free_surface,
+                       double* R, int block, int omega_per_item, void* stream);
 
 #ifdef __cplusplus
 }

@@ -18,6 +18,7 @@ from ._build import LIB_PATH
 __all__ = ["lib", "check", "device", "stream_handle", "ptr", "EXPORTS"]
 
 _c_int = ctypes.c_int
+_c_double = ctypes.c_double
 _vp = ctypes.c_void_p
 
 # name -> argtypes (every entry point returns int unless listed in _RESTYPES)
@@ -28,6 +29,10 @@ EXPORTS: dict[str, list] = {
     "kh_interface_coefficients": [_vp, _c_int, _vp, _c_int, _vp, _vp, _vp, _vp, _vp],
     "kh_forward": [_vp, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _vp, _vp],
     "kh_forward_ex": [_vp, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _vp, _c_int, _c_int, _vp],
+    "kh_forward_grid": [_vp, _c_int, _vp, _c_int, _c_double, _c_double, _c_double, _c_int, _c_int,
+                        _vp, _vp],
+    "kh_forward_grid_ex": [_vp, _c_int, _vp, _c_int, _c_double, _c_double, _c_double, _c_int,
+                           _c_int, _vp, _c_int, _c_int, _vp],
 }
 _RESTYPES = {"kh_version": ctypes.c_char_p, "kh_last_error": ctypes.c_char_p}
 

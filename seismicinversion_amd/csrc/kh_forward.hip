@@ -55,47 +55,108 @@ __global__ void interface_kernel(const double* __restrict__ layers, int nl,
   tu[0] = f.td00; tu[1] = f.td10; tu[2] = f.td01; tu[3] = f.td11;
 }
 
-// grid.x = np * ntiles; block b covers slowness b / ntiles and frequencies
-// [tile * blockDim * wpt, (tile + 1) * blockDim * wpt).
+// grid.x = np * ntiles; block b covers slowness b / ntiles and the
+// frequencies tile * blockDim * W + threadIdx.x + k * blockDim, k < W.
+// GRID: omega_j = (k0 + j) * dw + i * gamma (omega unused); otherwise omega[j].
+template <int W, bool GRID>
 __global__ void __launch_bounds__(1024) forward_kernel(
     const double* __restrict__ layers, int nl, const double* __restrict__ p,
-    const cd* __restrict__ omega, int nw, int free_surface, cd* __restrict__ R, int ntiles,
-    int wpt) {
+    const cd* __restrict__ omega, double k0, double dw, double gamma, int nw, int free_surface,
+    cd* __restrict__ R, int ntiles) {
   extern __shared__ double smem_d[];
   LayerPt* s_layer = reinterpret_cast<LayerPt*>(smem_d);
   Iface* s_if = reinterpret_cast<Iface*>(s_layer + nl);
+  LayerStep* s_step = reinterpret_cast<LayerStep*>(s_if + (nl - 1));
 
   const int ip = blockIdx.x / ntiles;
   const int tile = blockIdx.x - ip * ntiles;
-  const double pv = p[ip];
-  stage_slowness(layers, nl, pv, s_layer, s_if);
+  stage_slowness(layers, nl, p[ip], s_layer, s_if);
+  if (GRID && W > 1) {
+    const double shift = (double)blockDim.x * dw;
+    for (int l = threadIdx.x; l < nl; l += blockDim.x) s_step[l] = layer_step(s_layer[l], shift);
+    __syncthreads();
+  }
 
-  const int base = tile * blockDim.x * wpt + threadIdx.x;
+  const int base = tile * blockDim.x * W + threadIdx.x;
+  cd om[W];
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    const int w = min(base + k * (int)blockDim.x, nw - 1);
+    om[k] = GRID ? cd{(k0 + (double)w) * dw, gamma} : omega[w];
+  }
+  cd out[W];
+  sweep_lanes<W, GRID>(s_layer, s_if, s_step, nl, om, free_surface, out);
   cd* Rrow = R + (size_t)ip * nw;
-  for (int k = 0; k < wpt; ++k) {
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
     const int w = base + k * blockDim.x;
-    if (w >= nw) break;
-    Rrow[w] = sweep_lane(s_layer, s_if, nl, omega[w], free_surface);
+    if (w < nw) Rrow[w] = out[k];
   }
 }
 
 size_t forward_lds_bytes(int nl) {
-  return (size_t)nl * sizeof(LayerPt) + (size_t)(nl - 1) * sizeof(Iface);
+  return (size_t)nl * (sizeof(LayerPt) + sizeof(LayerStep)) + (size_t)(nl - 1) * sizeof(Iface);
 }
 
-// Default launch shape: as many frequencies per work-item as still leaves
-// >= 4 workgroups per CU on a 256-CU part.
-static void default_shape(int np, int nw, int* block, int* wpt) {
+// Default launch shape: the most frequencies per work-item that still leaves
+// >= 1024 workgroups (4 per CU on a 256-CU part).
+static void default_shape(int np, int nw, bool grid, int* block, int* wpt) {
   *block = 256;
   *wpt = 1;
-  for (int cand = 4; cand > 1; cand >>= 1) {
-    const long tiles = (nw + 256L * cand - 1) / (256L * cand);
-    if (tiles * (long)np >= 1024) {
-      *wpt = cand;
+  const int cands[3] = {4, 2, 1};
+  for (int c : cands) {
+    if (!grid && c > 2) continue;
+    const long tiles = (nw + 256L * c - 1) / (256L * c);
+    if (tiles * (long)np >= 1024 || c == 1) {
+      *wpt = c;
       break;
     }
   }
-  if ((long)np * ((nw + 255) / 256) < 1024 && nw <= 64) *block = 64;
+  if (nw <= 64) *block = 64;
+}
+
+static bool valid_wpt(int w) { return w == 1 || w == 2 || w == 3 || w == 4 || w == 8; }
+
+template <bool GRID>
+static int launch_forward(const double* layers, int nl, const double* p, int np, const cd* omega,
+                          double k0, double dw, double gamma, int nw, int free_surface, cd* R,
+                          int block, int wpt, hipStream_t stream) {
+  const long span = (long)block * wpt;
+  const long ntiles = (nw + span - 1) / span;
+  const long nblocks = ntiles * (long)np;
+  KH_REQUIRE(nblocks <= 0x7fffffffL, KH_ERR_UNSUPPORTED, "kh_forward: grid too large");
+  const size_t lds = forward_lds_bytes(nl);
+  const dim3 g((unsigned)nblocks), b(block);
+  const int nt = (int)ntiles;
+#define KH_FWD_CASE(WV)                                                                       \
+  case WV:                                                                                   \
+    hipLaunchKernelGGL((forward_kernel<WV, GRID>), g, b, lds, stream, layers, nl, p, omega, k0, \
+                       dw, gamma, nw, free_surface, R, nt);                                  \
+    break;
+  switch (wpt) {
+    KH_FWD_CASE(1)
+    KH_FWD_CASE(2)
+    KH_FWD_CASE(3)
+    KH_FWD_CASE(4)
+    KH_FWD_CASE(8)
+    default:
+      break;
+  }
+#undef KH_FWD_CASE
+  return check_launch("kh_forward");
+}
+
+static int check_forward_args(const char* what, const double* layers, int nl, const double* p,
+                              int np, int nw, const void* R, int block, int wpt) {
+  KH_TRY(check_model(what, layers, nl));
+  KH_REQUIRE(np >= 0 && nw >= 0, KH_ERR_INVALID_ARGUMENT, "%s: negative size", what);
+  if (np == 0 || nw == 0) return KH_OK;
+  KH_REQUIRE(p && R, KH_ERR_INVALID_ARGUMENT, "%s: NULL pointer", what);
+  KH_REQUIRE(block == 64 || block == 128 || block == 256 || block == 512 || block == 1024,
+             KH_ERR_INVALID_ARGUMENT, "%s: block %d not in {64,...,1024}", what, block);
+  KH_REQUIRE(valid_wpt(wpt), KH_ERR_INVALID_ARGUMENT,
+             "%s: omega_per_item %d not in {1,2,3,4,8}", what, wpt);
+  return KH_OK;
 }
 
 }  // namespace kh
@@ -139,29 +200,37 @@ extern "C" int kh_forward_ex(const double* layers, int nl, const double* p, int 
                              const double* omega, int nw, int free_surface, double* R, int block,
                              int omega_per_item, void* stream) {
   clear_error();
-  KH_TRY(check_model("kh_forward", layers, nl));
-  KH_REQUIRE(np >= 0 && nw >= 0, KH_ERR_INVALID_ARGUMENT, "kh_forward: negative size");
+  KH_TRY(check_forward_args("kh_forward", layers, nl, p, np, nw, R, block, omega_per_item));
   if (np == 0 || nw == 0) return KH_OK;
-  KH_REQUIRE(p && omega && R, KH_ERR_INVALID_ARGUMENT, "kh_forward: NULL pointer");
-  KH_REQUIRE(block == 64 || block == 128 || block == 256 || block == 512 || block == 1024,
-             KH_ERR_INVALID_ARGUMENT, "kh_forward: block %d not in {64,...,1024}", block);
-  KH_REQUIRE(omega_per_item >= 1 && omega_per_item <= 16, KH_ERR_INVALID_ARGUMENT,
-             "kh_forward: omega_per_item %d not in 1..16", omega_per_item);
-  const long span = (long)block * omega_per_item;
-  const long ntiles = (nw + span - 1) / span;
-  const long nblocks = ntiles * (long)np;
-  KH_REQUIRE(nblocks <= 0x7fffffffL, KH_ERR_UNSUPPORTED, "kh_forward: grid too large");
-  const size_t lds = forward_lds_bytes(nl);
-  hipLaunchKernelGGL(forward_kernel, dim3((unsigned)nblocks), dim3(block), lds,
-                     as_stream(stream), layers, nl, p, reinterpret_cast<const cd*>(omega), nw,
-                     free_surface, reinterpret_cast<cd*>(R), (int)ntiles, omega_per_item);
-  return check_launch("kh_forward");
+  KH_REQUIRE(omega, KH_ERR_INVALID_ARGUMENT, "kh_forward: omega is NULL");
+  return launch_forward<false>(layers, nl, p, np, reinterpret_cast<const cd*>(omega), 0.0, 0.0,
+                               0.0, nw, free_surface, reinterpret_cast<cd*>(R), block,
+                               omega_per_item, as_stream(stream));
 }
 
 extern "C" int kh_forward(const double* layers, int nl, const double* p, int np,
                           const double* omega, int nw, int free_surface, double* R,
                           void* stream) {
   int block, wpt;
-  default_shape(np, nw, &block, &wpt);
+  default_shape(np, nw, false, &block, &wpt);
   return kh_forward_ex(layers, nl, p, np, omega, nw, free_surface, R, block, wpt, stream);
+}
+
+extern "C" int kh_forward_grid_ex(const double* layers, int nl, const double* p, int np, double k0,
+                                  double dw, double gamma, int nw, int free_surface, double* R,
+                                  int block, int omega_per_item, void* stream) {
+  clear_error();
+  KH_TRY(check_forward_args("kh_forward_grid", layers, nl, p, np, nw, R, block, omega_per_item));
+  if (np == 0 || nw == 0) return KH_OK;
+  return launch_forward<true>(layers, nl, p, np, nullptr, k0, dw, gamma, nw, free_surface,
+                              reinterpret_cast<cd*>(R), block, omega_per_item, as_stream(stream));
+}
+
+extern "C" int kh_forward_grid(const double* layers, int nl, const double* p, int np, double k0,
+                               double dw, double gamma, int nw, int free_surface, double* R,
+                               void* stream) {
+  int block, wpt;
+  default_shape(np, nw, true, &block, &wpt);
+  return kh_forward_grid_ex(layers, nl, p, np, k0, dw, gamma, nw, free_surface, R, block, wpt,
+                            stream);
 }

@@ -1,12 +1,31 @@
 // The per-lane Kennett upward sweep and the workgroup prologue that stages
 // one slowness's layer and interface tables in LDS.  Shared by the forward,
-// multi-model and gather kernels so that every caller runs the identical
-// arithmetic for a lane.
+// multi-model and gather kernels so every caller runs identical arithmetic
+// for a lane.
+//
+// A lane carries the full 2x2 RRd and applies the reference's update
+// (kennett_reflectivity_gpu.py:235-257) in the same association:
+//     MT = E RRd E,  U = (I - Ru MT)^-1,  RRd = Rd + (Tu (MT U)) Td,
+// with Tu read as Td^T and U formed as adj(I - Ru MT) times one reciprocal
+// of the determinant.  (Dropping the lower triangle of the symmetric RRd
+// saves ~10% of the FLOPs but changes the rounding enough to leave the
+// reference's result at large slowness, so the full form is kept.)
+//
+// A lane may carry W frequencies at once (the interface coefficients are
+// then read once for all W).  On a uniform grid omega_j = (k0 + j) dw + i gamma
+// the W frequencies of a lane are B apart, and the phase factors of the
+// 2nd..W-th come from the first by the exact identity
+//     exp(i (w + B dw) eta h) = exp(i w eta h) * exp(i B dw eta h),
+// with exp(i B dw eta h) computed once per layer in the prologue.
 #pragma once
 
 #include "kh_physics.hpp"
 
 namespace kh {
+
+struct LayerStep {  // per-layer phase step for a frequency shift of B*dw
+  cd a, b;
+};
 
 // Workgroup-cooperative: layer slownesses of every layer at slowness pv,
 // then the coefficients of every interface.  Ends with a barrier.
@@ -20,62 +39,135 @@ __device__ __forceinline__ void stage_slowness(const double* __restrict__ layers
   __syncthreads();
 }
 
-struct Mat2 {
-  cd a00, a01, a10, a11;
-};
+__host__ __device__ __forceinline__ LayerStep layer_step(const LayerPt& L, double shift) {
+  LayerStep s{{1.0, 0.0}, {1.0, 0.0}};
+  if (!L.halfspace) {
+    const cd w{shift, 0.0};
+    s.a = phase(L.eta, w, L.h);
+    s.b = phase(L.neta, w, L.h);
+  }
+  return s;
+}
 
-// RRd at the top of the stack (just below the water layer) for one omega.
-__host__ __device__ __forceinline__ Mat2 sweep_rrd(const LayerPt* __restrict__ L,
-                                         const Iface* __restrict__ F, int nl, cd om) {
-  // Deepest interface: RRd below it is 0, so RRd = Rd.
-  const Iface& fb = F[nl - 2];
-  Mat2 r{fb.rd00, fb.rd01, fb.rd01, fb.rd11};
+// One interface step for one frequency: RRd <- Rd + Tu MT U Td.
+__host__ __device__ __forceinline__ void kennett_step(cd& r00, cd& r01, cd& r10, cd& r11, cd ea,
+                                                      cd eb, const Iface& f) {
+  const cd eaea = ea * ea, ebeb = eb * eb, eaeb = ea * eb;
+  const cd m00 = eaea * r00, m01 = eaeb * r01, m10 = eaeb * r10, m11 = ebeb * r11;
+  const cd one{1.0, 0.0};
+  // A = I - Ru MT
+  const cd a00 = one - (f.ru00 * m00 + f.ru01 * m10);
+  const cd a01 = -(f.ru00 * m01 + f.ru01 * m11);
+  const cd a10 = -(f.ru01 * m00 + f.ru11 * m10);
+  const cd a11 = one - (f.ru01 * m01 + f.ru11 * m11);
+  const cd rdet = recip(a00 * a11 - a01 * a10);
+  const cd u00 = a11 * rdet, u01 = -a01 * rdet, u10 = -a10 * rdet, u11 = a00 * rdet;
+  // X = MT U
+  const cd x00 = m00 * u00 + m01 * u10, x01 = m00 * u01 + m01 * u11;
+  const cd x10 = m10 * u00 + m11 * u10, x11 = m10 * u01 + m11 * u11;
+  // Y = Tu X with Tu = Td^T
+  const cd y00 = f.td00 * x00 + f.td10 * x10, y01 = f.td00 * x01 + f.td10 * x11;
+  const cd y10 = f.td01 * x00 + f.td11 * x10, y11 = f.td01 * x01 + f.td11 * x11;
+  // RRd = Rd + Y Td
+  r00 = f.rd00 + (y00 * f.td00 + y01 * f.td10);
+  r01 = f.rd01 + (y00 * f.td01 + y01 * f.td11);
+  r10 = f.rd01 + (y10 * f.td00 + y11 * f.td10);
+  r11 = f.rd11 + (y10 * f.td01 + y11 * f.td11);
+}
+
+// RRd00 just below the water layer for W frequencies of one lane.  With
+// GRID, om[k] = om[0] + k * shift (real shift) and `steps` holds the
+// per-layer factors for that shift; otherwise every phase is evaluated.
+template <int W, bool GRID>
+__host__ __device__ __forceinline__ void sweep_rrd00(const LayerPt* __restrict__ L,
+                                                     const Iface* __restrict__ F,
+                                                     const LayerStep* __restrict__ steps, int nl,
+                                                     const cd (&om)[W], cd (&out)[W]) {
+  cd q00[W], q01[W], q10[W], q11[W];
+  {
+    const Iface& fb = F[nl - 2];  // below it RRd = 0, so RRd = Rd
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      q00[k] = fb.rd00;
+      q01[k] = fb.rd01;
+      q10[k] = fb.rd01;
+      q11[k] = fb.rd11;
+    }
+  }
   for (int i = nl - 3; i >= 0; --i) {
     const LayerPt& lb = L[i + 1];
-    cd ea{1.0, 0.0}, eb{1.0, 0.0};
-    if (!lb.halfspace) {
-      ea = phase(lb.eta, om, lb.h);
-      eb = phase(lb.neta, om, lb.h);
+    const Iface f = F[i];
+    cd ea[W], eb[W];
+    if (lb.halfspace) {
+#pragma unroll
+      for (int k = 0; k < W; ++k) ea[k] = eb[k] = cd{1.0, 0.0};
+    } else if (GRID) {
+      ea[0] = phase(lb.eta, om[0], lb.h);
+      eb[0] = phase(lb.neta, om[0], lb.h);
+      const LayerStep s = steps[i + 1];
+#pragma unroll
+      for (int k = 1; k < W; ++k) {
+        ea[k] = ea[k - 1] * s.a;
+        eb[k] = eb[k - 1] * s.b;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < W; ++k) {
+        ea[k] = phase(lb.eta, om[k], lb.h);
+        eb[k] = phase(lb.neta, om[k], lb.h);
+      }
     }
-    const cd eaea = ea * ea, ebeb = eb * eb, eaeb = ea * eb;
-    const cd m00 = eaea * r.a00, m01 = eaeb * r.a01, m10 = eaeb * r.a10, m11 = ebeb * r.a11;
-    const Iface& f = F[i];
-    // A = I - Ru MT
-    const cd one{1.0, 0.0};
-    const cd a00 = one - (f.ru00 * m00 + f.ru01 * m10);
-    const cd a01 = -(f.ru00 * m01 + f.ru01 * m11);
-    const cd a10 = -(f.ru01 * m00 + f.ru11 * m10);
-    const cd a11 = one - (f.ru01 * m01 + f.ru11 * m11);
-    const cd rdet = recip(a00 * a11 - a01 * a10);
-    const cd u00 = a11 * rdet, u01 = -a01 * rdet, u10 = -a10 * rdet, u11 = a00 * rdet;
-    // X = MT U
-    const cd x00 = m00 * u00 + m01 * u10, x01 = m00 * u01 + m01 * u11;
-    const cd x10 = m10 * u00 + m11 * u10, x11 = m10 * u01 + m11 * u11;
-    // Y = Tu X, with Tu = Td^T
-    const cd y00 = f.td00 * x00 + f.td10 * x10, y01 = f.td00 * x01 + f.td10 * x11;
-    const cd y10 = f.td01 * x00 + f.td11 * x10, y11 = f.td01 * x01 + f.td11 * x11;
-    // RRd = Rd + Y Td
-    r.a00 = f.rd00 + (y00 * f.td00 + y01 * f.td10);
-    r.a01 = f.rd01 + (y00 * f.td01 + y01 * f.td11);
-    r.a10 = f.rd01 + (y10 * f.td00 + y11 * f.td10);
-    r.a11 = f.rd11 + (y10 * f.td01 + y11 * f.td11);
+#pragma unroll
+    for (int k = 0; k < W; ++k) kennett_step(q00[k], q01[k], q10[k], q11[k], ea[k], eb[k], f);
   }
-  return r;
+#pragma unroll
+  for (int k = 0; k < W; ++k) out[k] = q00[k];
 }
 
-// PP reflectivity seen from the sea surface (kennett_reflectivity_gpu.py:259-272).
-__host__ __device__ __forceinline__ cd surface_pp(const LayerPt& water, cd rrd00, cd om, int free_surface) {
-  cd ea0{1.0, 0.0};
-  if (!water.halfspace) ea0 = phase(water.eta, om, water.h);
-  const cd top = (ea0 * ea0) * rrd00;
-  if (free_surface) return top / (cd{1.0, 0.0} + top);
-  return top;
+// Water-layer phase and optional free surface (kennett_reflectivity_gpu.py:259-272).
+template <int W, bool GRID>
+__host__ __device__ __forceinline__ void surface_pp(const LayerPt& water, const LayerStep& step,
+                                                    const cd (&om)[W], cd (&r)[W],
+                                                    int free_surface) {
+  cd ea0[W];
+  if (water.halfspace) {
+#pragma unroll
+    for (int k = 0; k < W; ++k) ea0[k] = cd{1.0, 0.0};
+  } else if (GRID) {
+    ea0[0] = phase(water.eta, om[0], water.h);
+#pragma unroll
+    for (int k = 1; k < W; ++k) ea0[k] = ea0[k - 1] * step.a;
+  } else {
+#pragma unroll
+    for (int k = 0; k < W; ++k) ea0[k] = phase(water.eta, om[k], water.h);
+  }
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    const cd top = (ea0[k] * ea0[k]) * r[k];
+    r[k] = free_surface ? top / (cd{1.0, 0.0} + top) : top;
+  }
 }
 
-__host__ __device__ __forceinline__ cd sweep_lane(const LayerPt* __restrict__ L, const Iface* __restrict__ F,
-                                         int nl, cd om, int free_surface) {
-  const Mat2 r = sweep_rrd(L, F, nl, om);
-  return surface_pp(L[0], r.a00, om, free_surface);
+// Full lane: PP reflectivity at W frequencies.
+template <int W, bool GRID>
+__host__ __device__ __forceinline__ void sweep_lanes(const LayerPt* __restrict__ L,
+                                                     const Iface* __restrict__ F,
+                                                     const LayerStep* __restrict__ steps, int nl,
+                                                     const cd (&om)[W], int free_surface,
+                                                     cd (&out)[W]) {
+  sweep_rrd00<W, GRID>(L, F, steps, nl, om, out);
+  const LayerStep s0 = GRID ? steps[0] : LayerStep{{1.0, 0.0}, {1.0, 0.0}};
+  surface_pp<W, GRID>(L[0], s0, om, out, free_surface);
+}
+
+// Single frequency, generic omega (host tests and simple callers).
+__host__ __device__ __forceinline__ cd sweep_lane(const LayerPt* __restrict__ L,
+                                                  const Iface* __restrict__ F, int nl, cd om,
+                                                  int free_surface) {
+  const cd o[1] = {om};
+  cd r[1];
+  sweep_lanes<1, false>(L, F, nullptr, nl, o, free_surface, r);
+  return r[0];
 }
 
 }  // namespace kh

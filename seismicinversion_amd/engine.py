@@ -15,7 +15,9 @@ from .layer_model import LayerModel, pack_layers
 
 __all__ = [
     "layers_tensor",
+    "uniform_grid",
     "forward",
+    "forward_grid",
     "vertical_slowness",
     "interface_coefficients",
 ]
@@ -66,6 +68,55 @@ def forward(layers: torch.Tensor, p: torch.Tensor, omega: torch.Tensor,
                                int(bool(free_surface)), _abi.ptr(out), int(block or 256),
                                int(omega_per_item or 1), s)
     _abi.check(rc, "kh_forward")
+    return out
+
+
+def uniform_grid(omega) -> tuple[float, float, float] | None:
+    """(k0, dw, gamma) if omega[j] == (k0 + j)*dw + i*gamma holds exactly for
+    every j (the grids the reference builds with arange(...)*dw), else None."""
+    w = np.asarray(omega).astype(np.complex128).reshape(-1)
+    n = w.shape[0]
+    if n < 2 or not np.all(w.imag == w.imag[0]) or not np.all(np.isfinite(w)):
+        return None
+    re = w.real
+    d1 = re[1] - re[0]
+    if not d1 > 0:
+        return None
+    k0 = float(np.round(re[0] / d1))
+    cands = [d1, (re[-1] - re[0]) / (n - 1)]
+    if k0 != 0:
+        cands.append(re[0] / k0)
+    j = np.arange(n, dtype=np.float64)
+    for dw in cands:
+        if np.array_equal((k0 + j) * dw, re):
+            return k0, float(dw), float(w.imag[0])
+    return None
+
+
+def forward_grid(layers: torch.Tensor, p: torch.Tensor, k0: float, dw: float, gamma: float,
+                 n_omega: int, free_surface: bool = False, out: torch.Tensor | None = None,
+                 block: int | None = None, omega_per_item: int | None = None) -> torch.Tensor:
+    """R (np, n_omega) on the uniform grid omega_j = (k0 + j)*dw + i*gamma."""
+    dev = layers.device
+    p = _as_dev(p, torch.float64, dev)
+    n_p = p.numel()
+    if out is None:
+        out = torch.empty((n_p, n_omega), dtype=torch.complex128, device=dev)
+    elif out.shape != (n_p, n_omega) or out.dtype != torch.complex128 or not out.is_contiguous():
+        raise ValueError("out must be a contiguous complex128 tensor of shape (np, n_omega)")
+    lib = _abi.lib()
+    s = _abi.stream_handle(dev)
+    nl = layers.shape[0]
+    if block is None and omega_per_item is None:
+        rc = lib.kh_forward_grid(_abi.ptr(layers), nl, _abi.ptr(p), n_p, float(k0), float(dw),
+                                 float(gamma), int(n_omega), int(bool(free_surface)),
+                                 _abi.ptr(out), s)
+    else:
+        rc = lib.kh_forward_grid_ex(_abi.ptr(layers), nl, _abi.ptr(p), n_p, float(k0),
+                                    float(dw), float(gamma), int(n_omega),
+                                    int(bool(free_surface)), _abi.ptr(out), int(block or 256),
+                                    int(omega_per_item or 1), s)
+    _abi.check(rc, "kh_forward_grid")
     return out
 
 

@@ -4,7 +4,8 @@
 convention (kennett_reflectivity_gpu.py:88-124): NumPy in, a new (np, nw)
 complex128 NumPy array out, inputs untouched.  The recursion runs in the HIP
 kernel ``kh_forward`` (one work-item per (p, omega) lane) instead of the
-reference's full-grid NumPy temporaries.
+reference's full-grid NumPy temporaries; frequency grids of the form
+(k0 + j)*dw + i*gamma (every grid the reference builds) take ``kh_forward_grid``.
 """
 
 from __future__ import annotations
@@ -36,6 +37,11 @@ def kennett_reflectivity_batch(
     omega_c = np.ascontiguousarray(np.asarray(omega).astype(np.complex128).reshape(-1))
     dev = get_device()
     layers = engine.layers_tensor(model, dev)
-    R = engine.forward(layers, torch.from_numpy(p_samples).to(dev),
-                       torch.from_numpy(omega_c).to(dev), free_surface=free_surface)
+    p_t = torch.from_numpy(p_samples).to(dev)
+    grid = engine.uniform_grid(omega_c)
+    if grid is not None:
+        R = engine.forward_grid(layers, p_t, *grid, omega_c.shape[0], free_surface=free_surface)
+    else:
+        R = engine.forward(layers, p_t, torch.from_numpy(omega_c).to(dev),
+                           free_surface=free_surface)
     return R.cpu().numpy()

@@ -7,18 +7,60 @@
 
 using namespace kh;
 
+static void stage(const double* layers, int nl, double p, std::vector<LayerPt>& L,
+                  std::vector<Iface>& F) {
+  for (int l = 0; l < nl; ++l) L[l] = layer_at(layers + (size_t)l * NCOL, l, p);
+  for (int i = 0; i < nl - 1; ++i) F[i] = interface_at(i, p, L[i], L[i + 1]);
+}
+
 extern "C" int host_forward(const double* layers, int nl, const double* p, int np,
                             const double* omega, int nw, int free_surface, double* R) {
   std::vector<LayerPt> L(nl);
   std::vector<Iface> F(nl - 1);
   for (int j = 0; j < np; ++j) {
-    for (int l = 0; l < nl; ++l) L[l] = layer_at(layers + (size_t)l * NCOL, l, p[j]);
-    for (int i = 0; i < nl - 1; ++i) F[i] = interface_at(i, p[j], L[i], L[i + 1]);
+    stage(layers, nl, p[j], L, F);
     for (int w = 0; w < nw; ++w) {
       const cd om{omega[2 * w], omega[2 * w + 1]};
       const cd r = sweep_lane(L.data(), F.data(), nl, om, free_surface);
       R[2 * ((size_t)j * nw + w)] = r.r;
       R[2 * ((size_t)j * nw + w) + 1] = r.i;
+    }
+  }
+  return 0;
+}
+
+// The uniform-grid path of the kernel: frequencies omega_j = (k0 + j) dw + i gamma,
+// four per lane, `block` apart (as forward_kernel<4, true> with blockDim = block).
+extern "C" int host_forward_grid(const double* layers, int nl, const double* p, int np, double k0,
+                                 double dw, double gamma, int nw, int free_surface, int block,
+                                 double* R) {
+  constexpr int W = 4;
+  std::vector<LayerPt> L(nl);
+  std::vector<Iface> F(nl - 1);
+  std::vector<LayerStep> S(nl);
+  for (int j = 0; j < np; ++j) {
+    stage(layers, nl, p[j], L, F);
+    for (int l = 0; l < nl; ++l) S[l] = layer_step(L[l], (double)block * dw);
+    const int span = block * W;
+    for (int tile = 0; tile * span < nw; ++tile) {
+      for (int t = 0; t < block; ++t) {
+        const int base = tile * span + t;
+        cd om[W];
+        for (int k = 0; k < W; ++k) {
+          int w = base + k * block;
+          if (w > nw - 1) w = nw - 1;
+          om[k] = cd{(k0 + (double)w) * dw, gamma};
+        }
+        cd out[W];
+        sweep_lanes<W, true>(L.data(), F.data(), S.data(), nl, om, free_surface, out);
+        for (int k = 0; k < W; ++k) {
+          const int w = base + k * block;
+          if (w < nw) {
+            R[2 * ((size_t)j * nw + w)] = out[k].r;
+            R[2 * ((size_t)j * nw + w) + 1] = out[k].i;
+          }
+        }
+      }
     }
   }
   return 0;

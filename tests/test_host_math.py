@@ -68,3 +68,23 @@ def test_host_math_deep_model_vs_oracle(host_lib):
     R = host_forward(host_lib, m, p, w, True)
     O = K.reflectivity_batch(m, p, w, True)
     assert_rows_close(R, O, row_tolerance(m, p, w, True), "60-layer")
+
+
+@pytest.mark.parametrize("block", [64, 256])
+def test_host_math_uniform_grid_path(host_lib, block):
+    """The phase recurrence of the uniform-grid kernel path against the oracle
+    evaluated on the same grid (gather frequencies: k*2pi/64 + i*pi/64)."""
+    m = seeded_model(20, 42)
+    p = uniform_slownesses(16, 1.2)
+    nw, dw, gamma = 1023, 2 * np.pi / 64.0, np.pi / 64.0
+    R = np.zeros((len(p), nw), np.complex128)
+    tab = pack_layers(m)
+    vp = ctypes.c_void_p
+    host_lib.host_forward_grid.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_int, vp]
+    host_lib.host_forward_grid(tab.ctypes.data, len(tab), p.ctypes.data, len(p), 1.0, dw, gamma,
+                               nw, 1, block, R.ctypes.data)
+    w = np.arange(1, nw + 1) * dw + 1j * gamma
+    O = K.reflectivity_batch(m, p, w, True)
+    assert_rows_close(R, O, row_tolerance(m, p, w, True), "grid path")
