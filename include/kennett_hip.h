@@ -20,6 +20,8 @@
 #ifndef KENNETT_HIP_H
 #define KENNETT_HIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -91,6 +93,33 @@ int kh_forward_grid(const double* layers, int nl, const double* p, int np,
 int kh_forward_grid_ex(const double* layers, int nl, const double* p, int np,
                        double k0, double dw, double gamma, int nw, int free_surface,
                        double* R, int block, int omega_per_item, void* stream);
+
+/* ---- gather: discrete-wavenumber sum and inverse FFT to traces ---- */
+
+/* Workspace bytes kh_bessel_sum needs for np slownesses, nw - 1 frequencies
+ * and nr offsets (0 if any is empty). */
+size_t kh_bessel_workspace_bytes(int np, int nw, int nr);
+
+/* U[r, k] = S_k * sum_j R[j, k] * taper_j * p_j * w_k * dp * J0(w_k p_j r) in
+ * float64, for k = 0..nw-2 with real w_k = (k0 + k)*dw, taper_j =
+ * (1 + cos(pi p_j / p_max))/2.  R (np, nw-1) complex128, p (np,), offsets
+ * (nr,), S (nw-1,) complex128 or NULL (S = 1), U (nr, nw-1) complex128.
+ * The slowness sum is reduced in a fixed order (bitwise reproducible).
+ * Replaces _gpu_bessel_summation (Kennett_Reflectivity/kennett_gather_gpu.py:56-166,
+ * float32 there) and the per-p accumulation of compute_gather
+ * (Kennett_Reflectivity/kennett_gather.py:100-124, 271-293). */
+int kh_bessel_sum(const double* R, int np, int nw, const double* p, double dp,
+                  double p_max, double k0, double dw, const double* offsets, int nr,
+                  const double* S, double* U, void* workspace, size_t workspace_bytes,
+                  void* stream);
+
+/* Real traces (nr, nt = 2*nw) float64 from U (nr, nw-1) complex128:
+ * trace_r[n] = Re(FFT_nt([0, U_r, 0, conj(reversed U_r)])[n]) * exp(-gamma n T/nt),
+ * nw a power of two with nt*16 bytes <= the device's LDS per workgroup.
+ * Replaces the per-trace np.fft.fft loop of
+ * Kennett_Reflectivity/kennett_gather_gpu.py:273-282 (and kennett_gather.py:295-303). */
+int kh_hermitian_traces(const double* U, int nr, int nw, double gamma, double T,
+                        double* traces, void* stream);
 
 #ifdef __cplusplus
 }

@@ -33,8 +33,13 @@ EXPORTS: dict[str, list] = {
                         _vp, _vp],
     "kh_forward_grid_ex": [_vp, _c_int, _vp, _c_int, _c_double, _c_double, _c_double, _c_int,
                            _c_int, _vp, _c_int, _c_int, _vp],
+    "kh_bessel_workspace_bytes": [_c_int, _c_int, _c_int],
+    "kh_bessel_sum": [_vp, _c_int, _c_int, _vp, _c_double, _c_double, _c_double, _c_double, _vp,
+                      _c_int, _vp, _vp, _vp, ctypes.c_size_t, _vp],
+    "kh_hermitian_traces": [_vp, _c_int, _c_int, _c_double, _c_double, _vp, _vp],
 }
-_RESTYPES = {"kh_version": ctypes.c_char_p, "kh_last_error": ctypes.c_char_p}
+_RESTYPES = {"kh_version": ctypes.c_char_p, "kh_last_error": ctypes.c_char_p,
+             "kh_bessel_workspace_bytes": ctypes.c_size_t}
 
 KH_ERR_INVALID_ARGUMENT = 1
 

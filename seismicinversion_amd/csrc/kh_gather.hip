@@ -1,0 +1,219 @@
+// Discrete-wavenumber gather: Bessel-weighted slowness sum and the Hermitian
+// inverse FFT to real traces (Kennett_Reflectivity/kennett_gather_gpu.py:56-284,
+// float64 twin kennett_gather.py:195-303).
+//
+//   U[r, k]  = S_k * sum_j R[j, k] * taper_j * p_j * w_k * dp * J0(w_k p_j r)
+//   taper_j  = (1 + cos(pi p_j / p_max)) / 2
+//   trace_r[n] = Re( sum_k X_k exp(-2 pi i k n / nt) ) * exp(-gamma t_n),  nt = 2 nw,
+//   X = [0, U_1..U_{nw-1}, 0, conj(U_{nw-1})..conj(U_1)]     (real frequencies w_k)
+//
+// The sum runs in float64 (the reference's device path sums in float32; its
+// float64 twin is the parity target).  Stage 1: a one-wave workgroup owns 64
+// consecutive frequencies x a group of offsets x a chunk of slownesses and
+// writes one partial sum per (chunk, r, k); stage 2 adds the chunks in a fixed
+// order and applies S_k, so U does not depend on scheduling.  J0 is the ROCm
+// device library's double-precision j0.
+#include "kh_complex.hpp"
+#include "kh_runtime.hpp"
+
+namespace kh {
+
+constexpr int BS_WAVE = 64;   // frequencies per stage-1 workgroup (one wave)
+constexpr int BS_RG = 10;     // offsets per stage-1 workgroup
+constexpr int BS_PCHUNK = 128;  // slownesses per stage-1 workgroup
+
+__global__ void __launch_bounds__(BS_WAVE) bessel_partial_kernel(
+    const cd* __restrict__ R, int np, int nwm, const double* __restrict__ p, double dp,
+    double p_max, double k0, double dw, const double* __restrict__ offsets, int nr,
+    cd* __restrict__ partial, int n_rgroups, int n_wtiles) {
+  __shared__ double s_tp[BS_PCHUNK];  // taper_j * p_j
+  __shared__ double s_p[BS_PCHUNK];
+  int b = blockIdx.x;
+  const int wtile = b % n_wtiles;
+  b /= n_wtiles;
+  const int rgroup = b % n_rgroups;
+  const int chunk = b / n_rgroups;
+  const int j0_ = chunk * BS_PCHUNK;
+  const int nj = min(BS_PCHUNK, np - j0_);
+  for (int t = threadIdx.x; t < nj; t += blockDim.x) {
+    const double pj = p[j0_ + t];
+    const double taper = 0.5 * (1.0 + cos(M_PI * pj / p_max));
+    s_p[t] = pj;
+    s_tp[t] = taper * pj;
+  }
+  __syncthreads();
+
+  const int k = wtile * BS_WAVE + threadIdx.x;
+  const int kc = min(k, nwm - 1);
+  const double w = (k0 + (double)kc) * dw;
+  const int r0 = rgroup * BS_RG;
+  const int nrg = min(BS_RG, nr - r0);
+  double rr[BS_RG];
+#pragma unroll
+  for (int q = 0; q < BS_RG; ++q) rr[q] = offsets[min(r0 + q, nr - 1)];
+  cd acc[BS_RG];
+#pragma unroll
+  for (int q = 0; q < BS_RG; ++q) acc[q] = cd{0.0, 0.0};
+
+  const cd* Rcol = R + (size_t)j0_ * nwm + kc;
+  for (int t = 0; t < nj; ++t) {
+    const double pj = s_p[t];
+    const double wgt = (s_tp[t] * w) * dp;
+    const cd Rv = Rcol[(size_t)t * nwm];
+    const cd Wv = Rv * wgt;
+    const double x0 = w * pj;
+#pragma unroll
+    for (int q = 0; q < BS_RG; ++q) {
+      const double J = j0(x0 * rr[q]);
+      acc[q].r += J * Wv.r;
+      acc[q].i += J * Wv.i;
+    }
+  }
+  if (k < nwm) {
+    cd* out = partial + ((size_t)chunk * nr + r0) * nwm + k;
+    for (int q = 0; q < nrg; ++q) out[(size_t)q * nwm] = acc[q];
+  }
+}
+
+__global__ void zero_kernel(cd* __restrict__ x, long n) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < n) x[idx] = cd{0.0, 0.0};
+}
+
+// U[r, k] = S_k * sum_c partial[c, r, k] (chunks in ascending order).
+__global__ void bessel_reduce_kernel(const cd* __restrict__ partial, int nchunks, int nr, int nwm,
+                                     const cd* __restrict__ S, cd* __restrict__ U) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n = (long)nr * nwm;
+  if (idx >= n) return;
+  cd s = partial[idx];
+  for (int c = 1; c < nchunks; ++c) s = s + partial[(size_t)c * n + idx];
+  if (S) s = s * S[idx % nwm];
+  U[idx] = s;
+}
+
+// One workgroup per trace: in-place radix-2 decimation-in-time FFT of the
+// length-nt Hermitian vector in LDS (loaded in bit-reversed order), then
+// trace[n] = Re(X^[n]) * exp(-gamma n dt).
+__global__ void __launch_bounds__(1024) hermitian_trace_kernel(const cd* __restrict__ U, int nw,
+                                                               int log2nt, double gamma, double dt,
+                                                               double* __restrict__ traces) {
+  extern __shared__ double smem_f[];
+  cd* X = reinterpret_cast<cd*>(smem_f);
+  const int nt = 1 << log2nt;
+  const int nwm = nw - 1;
+  const int r = blockIdx.x;
+  const cd* Ur = U + (size_t)r * nwm;
+  for (int k = threadIdx.x; k < nt; k += blockDim.x) {
+    cd v{0.0, 0.0};
+    if (k >= 1 && k < nw) v = Ur[k - 1];
+    else if (k > nw) v = conj(Ur[nt - k - 1]);
+    const int rev = (int)(__brev((unsigned)k) >> (32 - log2nt));
+    X[rev] = v;
+  }
+  __syncthreads();
+  for (int s = 1; s <= log2nt; ++s) {
+    const int half = 1 << (s - 1);
+    for (int t = threadIdx.x; t < nt / 2; t += blockDim.x) {
+      const int grp = t >> (s - 1);
+      const int pos = t & (half - 1);
+      const int i0 = (grp << s) + pos;
+      const int i1 = i0 + half;
+      double sn, cs;
+      // twiddle exp(-2 pi i pos / 2^s)
+      sincospi(-(double)pos / (double)half, &sn, &cs);
+      const cd tw{cs, sn};
+      const cd a = X[i0];
+      const cd bb = X[i1] * tw;
+      X[i0] = a + bb;
+      X[i1] = a - bb;
+    }
+    __syncthreads();
+  }
+  double* tr = traces + (size_t)r * nt;
+  for (int n = threadIdx.x; n < nt; n += blockDim.x) {
+    const double t = (double)n * dt;
+    tr[n] = X[n].r * exp(-gamma * t);
+  }
+}
+
+static int max_lds_bytes() {
+  int dev = 0, v = 65536;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  return v;
+}
+
+static int ilog2_exact(long n) {
+  int l = 0;
+  while ((1L << l) < n) ++l;
+  return ((1L << l) == n) ? l : -1;
+}
+
+}  // namespace kh
+
+using namespace kh;
+
+extern "C" size_t kh_bessel_workspace_bytes(int np, int nw, int nr) {
+  if (np <= 0 || nw <= 1 || nr <= 0) return 0;
+  const long nchunks = (np + BS_PCHUNK - 1) / BS_PCHUNK;
+  return (size_t)nchunks * nr * (nw - 1) * sizeof(cd);
+}
+
+extern "C" int kh_bessel_sum(const double* R, int np, int nw, const double* p, double dp,
+                             double p_max, double k0, double dw, const double* offsets, int nr,
+                             const double* S, double* U, void* workspace, size_t workspace_bytes,
+                             void* stream) {
+  clear_error();
+  KH_REQUIRE(np >= 0 && nw >= 1 && nr >= 0, KH_ERR_INVALID_ARGUMENT, "kh_bessel_sum: bad sizes");
+  const int nwm = nw - 1;
+  if (nr == 0 || nwm == 0) return KH_OK;
+  KH_REQUIRE(U, KH_ERR_INVALID_ARGUMENT, "kh_bessel_sum: U is NULL");
+  KH_REQUIRE(p_max > 0.0, KH_ERR_INVALID_ARGUMENT, "kh_bessel_sum: p_max must be > 0");
+  hipStream_t st = as_stream(stream);
+  if (np == 0) {  // empty sum; a fill kernel (not a memset) keeps the call graph-capturable
+    const long n = (long)nr * nwm;
+    hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                       reinterpret_cast<cd*>(U), n);
+    return check_launch("kh_bessel_sum (zero)");
+  }
+  KH_REQUIRE(R && p && offsets, KH_ERR_INVALID_ARGUMENT, "kh_bessel_sum: NULL pointer");
+  const size_t need = kh_bessel_workspace_bytes(np, nw, nr);
+  KH_REQUIRE(workspace && workspace_bytes >= need, KH_ERR_INVALID_ARGUMENT,
+             "kh_bessel_sum: workspace of %zu bytes needed, %zu given", need, workspace_bytes);
+  const int nchunks = (np + BS_PCHUNK - 1) / BS_PCHUNK;
+  const int n_rgroups = (nr + BS_RG - 1) / BS_RG;
+  const int n_wtiles = (nwm + BS_WAVE - 1) / BS_WAVE;
+  const long nblocks = (long)nchunks * n_rgroups * n_wtiles;
+  KH_REQUIRE(nblocks <= 0x7fffffffL, KH_ERR_UNSUPPORTED, "kh_bessel_sum: grid too large");
+  cd* partial = reinterpret_cast<cd*>(workspace);
+  hipLaunchKernelGGL(bessel_partial_kernel, dim3((unsigned)nblocks), dim3(BS_WAVE), 0, st,
+                     reinterpret_cast<const cd*>(R), np, nwm, p, dp, p_max, k0, dw, offsets, nr,
+                     partial, n_rgroups, n_wtiles);
+  KH_TRY(check_launch("kh_bessel_sum (partial)"));
+  const long n = (long)nr * nwm;
+  hipLaunchKernelGGL(bessel_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                     partial, nchunks, nr, nwm, reinterpret_cast<const cd*>(S),
+                     reinterpret_cast<cd*>(U));
+  return check_launch("kh_bessel_sum (reduce)");
+}
+
+extern "C" int kh_hermitian_traces(const double* U, int nr, int nw, double gamma, double T,
+                                   double* traces, void* stream) {
+  clear_error();
+  KH_REQUIRE(nr >= 0 && nw >= 1, KH_ERR_INVALID_ARGUMENT, "kh_hermitian_traces: bad sizes");
+  if (nr == 0) return KH_OK;
+  KH_REQUIRE(U && traces, KH_ERR_INVALID_ARGUMENT, "kh_hermitian_traces: NULL pointer");
+  const long nt = 2L * nw;
+  const int log2nt = ilog2_exact(nt);
+  KH_REQUIRE(log2nt >= 1, KH_ERR_INVALID_ARGUMENT, "kh_hermitian_traces: nw=%d is not a power of 2",
+             nw);
+  const size_t lds = (size_t)nt * sizeof(cd);
+  KH_REQUIRE(lds <= (size_t)max_lds_bytes(), KH_ERR_UNSUPPORTED,
+             "kh_hermitian_traces: nt=%ld needs %zu bytes of LDS per workgroup", nt, lds);
+  const int block = (int)(nt / 2 < 1024 ? (nt / 2 < 64 ? 64 : nt / 2) : 1024);
+  const double dt = T / (double)nt;
+  hipLaunchKernelGGL(hermitian_trace_kernel, dim3(nr), dim3(block), lds, as_stream(stream),
+                     reinterpret_cast<const cd*>(U), nw, log2nt, gamma, dt, traces);
+  return check_launch("kh_hermitian_traces");
+}

@@ -145,3 +145,41 @@ def interface_coefficients(layers: torch.Tensor, p):
                                               _abi.stream_handle(dev))
     _abi.check(rc, "kh_interface_coefficients")
     return tuple(outs)
+
+
+def bessel_sum(R: torch.Tensor, p: torch.Tensor, dp: float, p_max: float, k0: float, dw: float,
+               offsets: torch.Tensor, S: torch.Tensor | None = None,
+               out: torch.Tensor | None = None) -> torch.Tensor:
+    """U (nr, nw-1) complex128: Bessel-weighted slowness sum of R (np, nw-1)
+    over real frequencies (k0 + k)*dw, times S if given."""
+    dev = R.device
+    n_p, nwm = R.shape
+    p = _as_dev(p, torch.float64, dev)
+    offsets = _as_dev(offsets, torch.float64, dev)
+    nr = offsets.numel()
+    if out is None:
+        out = torch.empty((nr, nwm), dtype=torch.complex128, device=dev)
+    lib = _abi.lib()
+    nbytes = lib.kh_bessel_workspace_bytes(n_p, nwm + 1, nr)
+    ws = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=dev)
+    S_ptr = None if S is None else _abi.ptr(_as_dev(S, torch.complex128, dev))
+    rc = lib.kh_bessel_sum(_abi.ptr(R.contiguous()), n_p, nwm + 1, _abi.ptr(p), float(dp),
+                           float(p_max), float(k0), float(dw), _abi.ptr(offsets), nr, S_ptr,
+                           _abi.ptr(out), _abi.ptr(ws), ws.numel(), _abi.stream_handle(dev))
+    _abi.check(rc, "kh_bessel_sum")
+    return out
+
+
+def hermitian_traces(U: torch.Tensor, nw: int, gamma: float, T: float,
+                     out: torch.Tensor | None = None) -> torch.Tensor:
+    """Real traces (nr, 2*nw) float64 from U (nr, nw-1)."""
+    dev = U.device
+    nr = U.shape[0]
+    if U.shape[1] != nw - 1:
+        raise ValueError(f"U has {U.shape[1]} frequencies, expected nw-1 = {nw - 1}")
+    if out is None:
+        out = torch.empty((nr, 2 * nw), dtype=torch.float64, device=dev)
+    rc = _abi.lib().kh_hermitian_traces(_abi.ptr(U.contiguous()), nr, int(nw), float(gamma),
+                                        float(T), _abi.ptr(out), _abi.stream_handle(dev))
+    _abi.check(rc, "kh_hermitian_traces")
+    return out
