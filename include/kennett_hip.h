@@ -36,6 +36,9 @@ extern "C" {
  * interface tables are staged in LDS). */
 #define KH_MAX_LAYERS 240
 
+/* Largest layer count the derivative entry points accept. */
+#define KH_MAX_LAYERS_DERIV 120
+
 /* Library version string. */
 const char* kh_version(void);
 
@@ -72,7 +75,7 @@ int kh_forward(const double* layers, int nl, const double* p, int np,
                void* stream);
 
 /* kh_forward with an explicit launch shape: `block` work-items per
- * workgroup (64, 128, 256, 512 or 1024) and `omega_per_item` frequencies
+ * workgroup (64, 128 or 256) and `omega_per_item` frequencies
  * walked by each work-item (1, 2, 3, 4 or 8).  Results are identical to
  * kh_forward's for every shape. */
 int kh_forward_ex(const double* layers, int nl, const double* p, int np,
@@ -120,6 +123,68 @@ int kh_bessel_sum(const double* R, int np, int nw, const double* p, double dp,
  * Kennett_Reflectivity/kennett_gather_gpu.py:273-282 (and kennett_gather.py:295-303). */
 int kh_hermitian_traces(const double* U, int nr, int nw, double gamma, double T,
                         double* traces, void* stream);
+
+/* ---- derivatives ----
+ * Parameters are the six columns of every layer row; derivative tables are
+ * row-major (nl, 6) per slowness: d/d[alpha, beta, rho, h, Q_alpha, Q_beta].
+ * Directions `dlayers` are (ndir, nl, 6) float64.  Cotangents follow torch's
+ * convention for complex outputs of real inputs:
+ *     grad_x = Re( sum_w conj(G_w) dR_w/dx ).
+ * Thickness derivatives of half-space layers (h = inf) are 0.  nl must not
+ * exceed KH_MAX_LAYERS_DERIV.  The adjoint entry points keep the forward
+ * states every ceil(sqrt(nl-1)) interfaces in the workspace and recompute the
+ * rest; per-frequency cotangents are reduced in a fixed order. */
+
+/* Workspace bytes for kh_vjp / kh_misfit_grad (dual = 0) or kh_misfit_hvp /
+ * kh_vjp_hvp (dual = 1) with ndir directions. */
+size_t kh_deriv_workspace_bytes(int nl, int np, int nw, int ndir, int dual);
+
+/* Tangent-linear (forward-mode) products dR[d] = J dlayers[d], (ndir, np, nw)
+ * complex128; R (np, nw) is written too when non-NULL.  One launch gives the
+ * analytical Frechet derivatives along ndir directions
+ * (Kennett_Reflectivity/frechet_analytical.py:583-911, one column per direction). */
+int kh_jvp(const double* layers, const double* dlayers, int ndir, int nl,
+           const double* p, int np, const double* omega, int nw, int free_surface,
+           double* R, double* dR, void* stream);
+
+/* Vector-Jacobian product grad (np, nl, 6) for the cotangent G (np, nw)
+ * complex128 (the backward of kennett_reflectivity_torch,
+ * Kennett_Reflectivity/kennett_torch.py:312-480, under torch.autograd). */
+int kh_vjp(const double* layers, int nl, const double* p, int np,
+           const double* omega, int nw, int free_surface, const double* G,
+           double* grad, double* R, void* workspace, size_t workspace_bytes,
+           void* stream);
+
+/* Misfit chi_p = sum_w |R - R_obs|^2 per slowness (chi (np, 2): value, 0) and
+ * its gradient (np, nl, 6) (the per-p terms of taup_misfit_grad_hessian,
+ * Kennett_Reflectivity/taup_inversion.py:154-174). */
+int kh_misfit_grad(const double* layers, int nl, const double* p, int np,
+                   const double* omega, int nw, int free_surface, const double* R_obs,
+                   double* chi, double* grad, double* R, void* workspace,
+                   size_t workspace_bytes, void* stream);
+
+/* Tangent-over-adjoint Hessian-vector products of the misfit: for every
+ * direction d, hvp[d] (np, nl, 6) = H dlayers[d] per slowness (exact Hessian,
+ * as torch.func.hessian in taup_inversion.py:177), chi (ndir, np, 2) =
+ * [chi, dchi], grad (np, nl, 6) (optional), dR (ndir, np, nw) = J dlayers[d]
+ * (optional).  second_only != 0 drops the Gauss-Newton part 2 Re(J^H J dm),
+ * leaving 2 Re(sum conj(R - R_obs) d^2R[dm]). */
+int kh_misfit_hvp(const double* layers, const double* dlayers, int ndir, int nl,
+                  const double* p, int np, const double* omega, int nw,
+                  int free_surface, const double* R_obs, int second_only, double* chi,
+                  double* grad, double* hvp, double* dR, void* workspace,
+                  size_t workspace_bytes, void* stream);
+
+/* Directional derivative of the vector-Jacobian product: for cotangent G
+ * (np, nw) with tangents dG (ndir, np, nw) (or NULL = 0) and parameter
+ * directions dlayers, dgrad[d] = d/dt grad(m + t dm_d, G + t dG_d)
+ * (ndir, np, nl, 6); grad (np, nl, 6) and dR (ndir, np, nw) optional.
+ * This is the forward-mode derivative of kennett_reflectivity_torch's
+ * backward, needed by forward-over-reverse Hessians. */
+int kh_vjp_hvp(const double* layers, const double* dlayers, int ndir, int nl,
+               const double* p, int np, const double* omega, int nw, int free_surface,
+               const double* G, const double* dG, double* grad, double* dgrad,
+               double* dR, void* workspace, size_t workspace_bytes, void* stream);
 
 #ifdef __cplusplus
 }

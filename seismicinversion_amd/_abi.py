@@ -37,8 +37,18 @@ EXPORTS: dict[str, list] = {
     "kh_bessel_sum": [_vp, _c_int, _c_int, _vp, _c_double, _c_double, _c_double, _c_double, _vp,
                       _c_int, _vp, _vp, _vp, ctypes.c_size_t, _vp],
     "kh_hermitian_traces": [_vp, _c_int, _c_int, _c_double, _c_double, _vp, _vp],
+    "kh_deriv_workspace_bytes": [_c_int, _c_int, _c_int, _c_int, _c_int],
+    "kh_jvp": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _vp, _vp, _vp],
+    "kh_vjp": [_vp, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _vp, _vp, _vp, _vp,
+               ctypes.c_size_t, _vp],
+    "kh_misfit_grad": [_vp, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp,
+                       ctypes.c_size_t, _vp],
+    "kh_misfit_hvp": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _vp, _c_int,
+                      _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp],
+    "kh_vjp_hvp": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _vp, _vp, _vp,
+                   _vp, _vp, _vp, ctypes.c_size_t, _vp],
 }
-_RESTYPES = {"kh_version": ctypes.c_char_p, "kh_last_error": ctypes.c_char_p,
+_RESTYPES = {"kh_deriv_workspace_bytes": ctypes.c_size_t, "kh_version": ctypes.c_char_p, "kh_last_error": ctypes.c_char_p,
              "kh_bessel_workspace_bytes": ctypes.c_size_t}
 
 KH_ERR_INVALID_ARGUMENT = 1

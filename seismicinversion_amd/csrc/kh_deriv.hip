@@ -1,0 +1,383 @@
+// Derivative kernels of the Kennett reflectivity (the differentiable path of
+// Kennett_Reflectivity/kennett_torch.py:312-480, its Jacobian/Hessian API
+// :611-684, the analytical Frechet derivatives of frechet_analytical.py:583-911
+// and the misfit/gradient/Hessian of taup_inversion.py:112-187).
+//
+//   jvp_kernel            tangent-linear sweep (Z = Dual<cd>): dR = J dm for
+//                         one parameter direction per grid slice
+//   adjoint_kernel<Z, M>  one wave per (direction, p, 64-frequency tile):
+//                         forward with sqrt(N) checkpoints in the workspace,
+//                         reverse sweep, per-level cotangents reduced over the
+//                         wave (fixed butterfly) and written per tile.
+//                         Z = cd: vector-Jacobian product (M = 0, given
+//                         cotangent) or misfit gradient (M = 1);
+//                         Z = Dual<cd>, M = 1: tangent-over-adjoint, i.e. the
+//                         misfit gradient and the Hessian-vector product H dm
+//   chain_kernel<Z>       per (direction, p, layer): sums the tiles in order
+//                         and chains coefficient/slowness cotangents to the six
+//                         layer parameters
+#include <type_traits>
+
+#include "kh_deriv.hpp"
+#include "kh_runtime.hpp"
+
+namespace kh {
+
+constexpr int AD_WAVE = 64;
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int o = AD_WAVE / 2; o >= 1; o >>= 1) x += __shfl_xor(x, o, AD_WAVE);
+  return x;
+}
+__device__ __forceinline__ cd wave_sum(const cd& z) { return cd{wave_sum(z.r), wave_sum(z.i)}; }
+template <class T>
+__device__ __forceinline__ Dual<T> wave_sum(const Dual<T>& z) {
+  return {wave_sum(z.v), wave_sum(z.d)};
+}
+template <class Z>
+__device__ __forceinline__ IfaceZ<Z> wave_sum(const IfaceZ<Z>& f) {
+  IfaceZ<Z> o;
+  o.rd00 = wave_sum(f.rd00); o.rd01 = wave_sum(f.rd01); o.rd11 = wave_sum(f.rd11);
+  o.ru00 = wave_sum(f.ru00); o.ru01 = wave_sum(f.ru01); o.ru11 = wave_sum(f.ru11);
+  o.td00 = wave_sum(f.td00); o.td01 = wave_sum(f.td01); o.td10 = wave_sum(f.td10);
+  o.td11 = wave_sum(f.td11);
+  return o;
+}
+
+template <class Z>
+struct WaveSink {
+  IfaceZ<Z>* cif;
+  LayerCot<Z>* cly;
+  __device__ void iface(int i, const IfaceZ<Z>& f) {
+    const IfaceZ<Z> r = wave_sum(f);
+    if (threadIdx.x == 0) cif[i] = r;
+  }
+  __device__ void layer(int l, const LayerCot<Z>& c) {
+    const LayerCot<Z> r{wave_sum(c.eta), wave_sum(c.neta), wave_sum(c.h)};
+    if (threadIdx.x == 0) cly[l] = r;
+  }
+};
+
+template <class Z>
+struct GlobalStore {
+  Z* base;
+  long stride;
+  __device__ Z& operator()(int e) { return base[(long)e * stride]; }
+};
+
+template <class Z>
+__device__ __forceinline__ void stage_z(const double* __restrict__ layers,
+                                        const double* __restrict__ drow_base, int nl, double pv,
+                                        LayerZ<Z>* s_layer, IfaceZ<Z>* s_if) {
+  for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+    Z x[NCOL];
+    row_z(layers + (size_t)l * NCOL, drow_base ? drow_base + (size_t)l * NCOL : nullptr, x);
+    s_layer[l] = layer_t<Z>(x, l, pv);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nl - 1; i += blockDim.x)
+    s_if[i] = interface_t<Z>(i, pv, s_layer[i], s_layer[i + 1]);
+  __syncthreads();
+}
+
+// grid.x = ndir * np * ntiles, blockDim = 256: dR[dir, p, w] = J dm_dir.
+__global__ void __launch_bounds__(256) jvp_kernel(const double* __restrict__ layers,
+                                                  const double* __restrict__ dlayers, int nl,
+                                                  const double* __restrict__ p, int np,
+                                                  const cd* __restrict__ omega, int nw,
+                                                  int free_surface, cd* __restrict__ R,
+                                                  cd* __restrict__ dR, int ntiles) {
+  extern __shared__ double smem_j[];
+  LayerZ<dcd>* s_layer = reinterpret_cast<LayerZ<dcd>*>(smem_j);
+  IfaceZ<dcd>* s_if = reinterpret_cast<IfaceZ<dcd>*>(s_layer + nl);
+  int b = blockIdx.x;
+  const int tile = b % ntiles;
+  b /= ntiles;
+  const int ip = b % np;
+  const int dir = b / np;
+  stage_z<dcd>(layers, dlayers + (size_t)dir * nl * NCOL, nl, p[ip], s_layer, s_if);
+  const int w = tile * blockDim.x + threadIdx.x;
+  if (w >= nw) return;
+  const cd om = omega[w];
+  const IfaceZ<dcd>& fb = s_if[nl - 2];
+  M2<dcd> S{fb.rd00, fb.rd01, fb.rd01, fb.rd11};
+  for (int i = nl - 3; i >= 0; --i) {
+    dcd ea, eb;
+    phases_of(s_layer[i + 1], om, ea, eb);
+    S = step_fwd(S, ea, eb, s_if[i]);
+  }
+  const LayerZ<dcd>& water = s_layer[0];
+  dcd ea0 = lift<dcd>(1.0);
+  if (!water.halfspace) ea0 = phase_t(water.eta, om, water.h);
+  const dcd top = (ea0 * ea0) * S.a00;
+  const dcd r = free_surface ? top / (lift<dcd>(1.0) + top) : top;
+  const size_t o = (size_t)ip * nw + w;
+  dR[(size_t)dir * np * nw + o] = r.d;
+  if (R && dir == 0) R[o] = r.v;
+}
+
+// grid.x = ndir * np * ntiles, blockDim = 64 (one wave per block).
+template <class Z, bool MISFIT>
+__global__ void __launch_bounds__(AD_WAVE) adjoint_kernel(
+    const double* __restrict__ layers, const double* __restrict__ dlayers, int nl,
+    const double* __restrict__ p, int np, const cd* __restrict__ omega, int nw, int free_surface,
+    const cd* __restrict__ Gin, const cd* __restrict__ dGin, int second_only, Z* __restrict__ ws,
+    long nlanes,
+    IfaceZ<Z>* __restrict__ cot_if, LayerCot<Z>* __restrict__ cot_ly, Z* __restrict__ chi_part,
+    cd* __restrict__ R, cd* __restrict__ dR, int ntiles) {
+  extern __shared__ double smem_a[];
+  LayerZ<Z>* s_layer = reinterpret_cast<LayerZ<Z>*>(smem_a);
+  IfaceZ<Z>* s_if = reinterpret_cast<IfaceZ<Z>*>(s_layer + nl);
+  int b = blockIdx.x;
+  const int tile = b % ntiles;
+  b /= ntiles;
+  const int ip = b % np;
+  const int dir = b / np;
+  stage_z<Z>(layers, dlayers ? dlayers + (size_t)dir * nl * NCOL : nullptr, nl, p[ip], s_layer,
+             s_if);
+  const int w = tile * AD_WAVE + threadIdx.x;
+  const bool valid = w < nw;
+  const int wc = valid ? w : nw - 1;
+  const cd om = omega[wc];
+  const size_t slot = (size_t)blockIdx.x;  // (dir, p, tile)
+  GlobalStore<Z> st{ws + (long)blockIdx.x * AD_WAVE + threadIdx.x, nlanes};
+  WaveSink<Z> sink{cot_if + slot * (nl - 1), cot_ly + slot * nl};
+  const cd gin = Gin[(size_t)ip * nw + wc];
+  const cd dgin = dGin ? dGin[((size_t)dir * np + ip) * nw + wc] : cd{0.0, 0.0};
+  Z chi = lift<Z>(0.0);
+  const Z r = lane_adjoint<Z>(s_layer, s_if, nl, om, free_surface, st, sink, [&](const Z& Rv) {
+    if (!valid) return lift<Z>(0.0);
+    if (MISFIT) {
+      const Z res = Rv - lift_c<Z>(gin);
+      chi = re_part(conj(res) * res);
+      Z g = res * 2.0;
+      if (second_only) g = lift_c<Z>(prim(g));
+      return g;
+    }
+    Z g = lift_c<Z>(gin);
+    if constexpr (!std::is_same<Z, cd>::value) g.d = dgin;
+    return g;
+  });
+  if (threadIdx.x == 0) {
+    const Z zero = lift<Z>(0.0);
+    sink.cly[nl - 1] = LayerCot<Z>{zero, zero, zero};
+  }
+  if (MISFIT) {
+    const Z c = wave_sum(chi);
+    if (threadIdx.x == 0) chi_part[slot] = c;
+  }
+  if (valid) {
+    const size_t o = (size_t)ip * nw + w;
+    if (R && dir == 0) R[o] = prim(r);
+    if constexpr (!std::is_same<Z, cd>::value) {
+      if (dR) dR[(size_t)dir * np * nw + o] = r.d;
+    }
+  }
+}
+
+template <class Z>
+__device__ __forceinline__ IfaceZ<Z> iface_add(const IfaceZ<Z>& a, const IfaceZ<Z>& b) {
+  IfaceZ<Z> o;
+  o.rd00 = a.rd00 + b.rd00; o.rd01 = a.rd01 + b.rd01; o.rd11 = a.rd11 + b.rd11;
+  o.ru00 = a.ru00 + b.ru00; o.ru01 = a.ru01 + b.ru01; o.ru11 = a.ru11 + b.ru11;
+  o.td00 = a.td00 + b.td00; o.td01 = a.td01 + b.td01; o.td10 = a.td10 + b.td10;
+  o.td11 = a.td11 + b.td11;
+  return o;
+}
+
+__device__ __forceinline__ double tangent_re(const cd&) { return 0.0; }
+__device__ __forceinline__ double tangent_re(const dcd& z) { return z.d.r; }
+
+// grid-stride over (dir, p, l).  grad (np, nl, 6) from dir 0; hvp (ndir, np, nl, 6)
+// for dual Z; chi (ndir, np, 2) [value, tangent] when chi_part is given.
+template <class Z>
+__global__ void chain_kernel(const double* __restrict__ layers, const double* __restrict__ dlayers,
+                             int nl, const double* __restrict__ p, int np, int ndir,
+                             const IfaceZ<Z>* __restrict__ cot_if,
+                             const LayerCot<Z>* __restrict__ cot_ly,
+                             const Z* __restrict__ chi_part, int ntiles, double* __restrict__ grad,
+                             double* __restrict__ hvp, double* __restrict__ chi) {
+  const long n = (long)ndir * np * nl;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const int l = (int)(idx % nl);
+  const int ip = (int)((idx / nl) % np);
+  const int dir = (int)(idx / ((long)nl * np));
+  const size_t base = ((size_t)dir * np + ip) * ntiles;
+  const Z zero = lift<Z>(0.0);
+  IfaceZ<Z> up{zero, zero, zero, zero, zero, zero, zero, zero, zero, zero}, dn = up;
+  LayerCot<Z> lc{zero, zero, zero};
+  for (int t = 0; t < ntiles; ++t) {
+    const size_t s = base + t;
+    if (l >= 1) up = iface_add(up, cot_if[s * (nl - 1) + (l - 1)]);
+    if (l <= nl - 2) dn = iface_add(dn, cot_if[s * (nl - 1) + l]);
+    const LayerCot<Z>& c = cot_ly[s * nl + l];
+    lc.eta = lc.eta + c.eta;
+    lc.neta = lc.neta + c.neta;
+    lc.h = lc.h + c.h;
+  }
+  Z out[NCOL];
+  chain_layer<Z>(layers, dlayers ? dlayers + (size_t)dir * nl * NCOL : nullptr, nl, l, p[ip], up,
+                 dn, lc, out);
+  const size_t g = ((size_t)ip * nl + l) * NCOL;
+  if (grad && dir == 0)
+    for (int c = 0; c < NCOL; ++c) grad[g + c] = prim(out[c]).r;
+  if (hvp)
+    for (int c = 0; c < NCOL; ++c) hvp[(size_t)dir * np * nl * NCOL + g + c] = tangent_re(out[c]);
+  if (chi && chi_part && l == 0) {
+    Z s = zero;
+    for (int t = 0; t < ntiles; ++t) s = s + chi_part[base + t];
+    chi[((size_t)dir * np + ip) * 2] = prim(s).r;
+    chi[((size_t)dir * np + ip) * 2 + 1] = tangent_re(s);
+  }
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct DerivLayout {
+  size_t ws, cif, cly, chi, total;
+  long nlanes;
+  int ntiles;
+};
+
+template <class Z>
+static DerivLayout deriv_layout(int nl, int np, int nw, int ndir) {
+  DerivLayout d;
+  d.ntiles = (nw + AD_WAVE - 1) / AD_WAVE;
+  const long nblocks = (long)ndir * np * d.ntiles;
+  d.nlanes = nblocks * AD_WAVE;
+  d.ws = 0;
+  d.cif = align256(d.ws + (size_t)d.nlanes * lane_slots(nl) * sizeof(Z));
+  d.cly = align256(d.cif + (size_t)nblocks * (nl - 1) * sizeof(IfaceZ<Z>));
+  d.chi = align256(d.cly + (size_t)nblocks * nl * sizeof(LayerCot<Z>));
+  d.total = align256(d.chi + (size_t)nblocks * sizeof(Z));
+  return d;
+}
+
+static int check_deriv_args(const char* what, const double* layers, int nl, int np, int nw,
+                            int ndir) {
+  KH_TRY(check_model(what, layers, nl));
+  KH_REQUIRE(nl <= KH_MAX_LAYERS_DERIV, KH_ERR_UNSUPPORTED,
+             "%s: %d layers exceeds KH_MAX_LAYERS_DERIV=%d", what, nl, KH_MAX_LAYERS_DERIV);
+  KH_REQUIRE(np >= 0 && nw >= 0 && ndir >= 1, KH_ERR_INVALID_ARGUMENT, "%s: bad sizes", what);
+  return KH_OK;
+}
+
+template <class Z, bool MISFIT>
+static int run_adjoint(const char* what, const double* layers, const double* dlayers, int ndir,
+                       int nl, const double* p, int np, const double* omega, int nw, int fs,
+                       const double* Gin, const double* dGin, int second_only, double* chi,
+                       double* grad,
+                       double* hvp, double* R, double* dR, void* workspace, size_t ws_bytes,
+                       hipStream_t st) {
+  const DerivLayout d = deriv_layout<Z>(nl, np, nw, ndir);
+  KH_REQUIRE(workspace && ws_bytes >= d.total, KH_ERR_INVALID_ARGUMENT,
+             "%s: workspace of %zu bytes needed, %zu given", what, d.total, ws_bytes);
+  char* wsb = reinterpret_cast<char*>(workspace);
+  const long nblocks = (long)ndir * np * d.ntiles;
+  KH_REQUIRE(nblocks <= 0x7fffffffL, KH_ERR_UNSUPPORTED, "%s: grid too large", what);
+  const size_t lds = (size_t)nl * sizeof(LayerZ<Z>) + (size_t)(nl - 1) * sizeof(IfaceZ<Z>);
+  auto* cif = reinterpret_cast<IfaceZ<Z>*>(wsb + d.cif);
+  auto* cly = reinterpret_cast<LayerCot<Z>*>(wsb + d.cly);
+  auto* chip = reinterpret_cast<Z*>(wsb + d.chi);
+  hipLaunchKernelGGL((adjoint_kernel<Z, MISFIT>), dim3((unsigned)nblocks), dim3(AD_WAVE), lds, st,
+                     layers, dlayers, nl, p, np, reinterpret_cast<const cd*>(omega), nw, fs,
+                     reinterpret_cast<const cd*>(Gin), reinterpret_cast<const cd*>(dGin), second_only,
+                     reinterpret_cast<Z*>(wsb + d.ws), d.nlanes, cif, cly, chip,
+                     reinterpret_cast<cd*>(R), reinterpret_cast<cd*>(dR), d.ntiles);
+  KH_TRY(check_launch(what));
+  const long n = (long)ndir * np * nl;
+  hipLaunchKernelGGL((chain_kernel<Z>), dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st,
+                     layers, dlayers, nl, p, np, ndir, cif, cly, MISFIT ? chip : nullptr,
+                     d.ntiles, grad, hvp, chi);
+  return check_launch(what);
+}
+
+}  // namespace kh
+
+using namespace kh;
+
+extern "C" size_t kh_deriv_workspace_bytes(int nl, int np, int nw, int ndir, int dual) {
+  if (nl < 2 || np <= 0 || nw <= 0 || ndir <= 0) return 0;
+  return dual ? deriv_layout<dcd>(nl, np, nw, ndir).total : deriv_layout<cd>(nl, np, nw, ndir).total;
+}
+
+extern "C" int kh_jvp(const double* layers, const double* dlayers, int ndir, int nl,
+                      const double* p, int np, const double* omega, int nw, int free_surface,
+                      double* R, double* dR, void* stream) {
+  clear_error();
+  KH_TRY(check_deriv_args("kh_jvp", layers, nl, np, nw, ndir));
+  if (np == 0 || nw == 0) return KH_OK;
+  KH_REQUIRE(dlayers && p && omega && dR, KH_ERR_INVALID_ARGUMENT, "kh_jvp: NULL pointer");
+  const int blk = 256;
+  const int ntiles = (nw + blk - 1) / blk;
+  const long nblocks = (long)ndir * np * ntiles;
+  KH_REQUIRE(nblocks <= 0x7fffffffL, KH_ERR_UNSUPPORTED, "kh_jvp: grid too large");
+  const size_t lds = (size_t)nl * sizeof(LayerZ<dcd>) + (size_t)(nl - 1) * sizeof(IfaceZ<dcd>);
+  hipLaunchKernelGGL(jvp_kernel, dim3((unsigned)nblocks), dim3(blk), lds, as_stream(stream),
+                     layers, dlayers, nl, p, np, reinterpret_cast<const cd*>(omega), nw,
+                     free_surface, reinterpret_cast<cd*>(R), reinterpret_cast<cd*>(dR), ntiles);
+  return check_launch("kh_jvp");
+}
+
+extern "C" int kh_vjp(const double* layers, int nl, const double* p, int np, const double* omega,
+                      int nw, int free_surface, const double* G, double* grad, double* R,
+                      void* workspace, size_t workspace_bytes, void* stream) {
+  clear_error();
+  KH_TRY(check_deriv_args("kh_vjp", layers, nl, np, nw, 1));
+  if (np == 0) return KH_OK;
+  KH_REQUIRE(p && omega && G && grad, KH_ERR_INVALID_ARGUMENT, "kh_vjp: NULL pointer");
+  KH_REQUIRE(nw > 0, KH_ERR_INVALID_ARGUMENT, "kh_vjp: nw must be > 0");
+  return run_adjoint<cd, false>("kh_vjp", layers, nullptr, 1, nl, p, np, omega, nw, free_surface,
+                                G, nullptr, 0, nullptr, grad, nullptr, R, nullptr, workspace,
+                                workspace_bytes, as_stream(stream));
+}
+
+extern "C" int kh_misfit_grad(const double* layers, int nl, const double* p, int np,
+                              const double* omega, int nw, int free_surface, const double* R_obs,
+                              double* chi, double* grad, double* R, void* workspace,
+                              size_t workspace_bytes, void* stream) {
+  clear_error();
+  KH_TRY(check_deriv_args("kh_misfit_grad", layers, nl, np, nw, 1));
+  if (np == 0) return KH_OK;
+  KH_REQUIRE(p && omega && R_obs && grad && chi, KH_ERR_INVALID_ARGUMENT,
+             "kh_misfit_grad: NULL pointer");
+  KH_REQUIRE(nw > 0, KH_ERR_INVALID_ARGUMENT, "kh_misfit_grad: nw must be > 0");
+  return run_adjoint<cd, true>("kh_misfit_grad", layers, nullptr, 1, nl, p, np, omega, nw,
+                               free_surface, R_obs, nullptr, 0, chi, grad, nullptr, R, nullptr,
+                               workspace,
+                               workspace_bytes, as_stream(stream));
+}
+
+extern "C" int kh_misfit_hvp(const double* layers, const double* dlayers, int ndir, int nl,
+                             const double* p, int np, const double* omega, int nw,
+                             int free_surface, const double* R_obs, int second_only, double* chi,
+                             double* grad, double* hvp, double* dR, void* workspace,
+                             size_t workspace_bytes, void* stream) {
+  clear_error();
+  KH_TRY(check_deriv_args("kh_misfit_hvp", layers, nl, np, nw, ndir));
+  if (np == 0) return KH_OK;
+  KH_REQUIRE(dlayers && p && omega && R_obs && hvp, KH_ERR_INVALID_ARGUMENT,
+             "kh_misfit_hvp: NULL pointer");
+  KH_REQUIRE(nw > 0, KH_ERR_INVALID_ARGUMENT, "kh_misfit_hvp: nw must be > 0");
+  return run_adjoint<dcd, true>("kh_misfit_hvp", layers, dlayers, ndir, nl, p, np, omega, nw,
+                                free_surface, R_obs, nullptr, second_only, chi, grad, hvp, nullptr,
+                                dR,
+                                workspace, workspace_bytes, as_stream(stream));
+}
+
+extern "C" int kh_vjp_hvp(const double* layers, const double* dlayers, int ndir, int nl,
+                          const double* p, int np, const double* omega, int nw, int free_surface,
+                          const double* G, const double* dG, double* grad, double* dgrad,
+                          double* dR, void* workspace, size_t workspace_bytes, void* stream) {
+  clear_error();
+  KH_TRY(check_deriv_args("kh_vjp_hvp", layers, nl, np, nw, ndir));
+  if (np == 0) return KH_OK;
+  KH_REQUIRE(dlayers && p && omega && G && dgrad, KH_ERR_INVALID_ARGUMENT,
+             "kh_vjp_hvp: NULL pointer");
+  KH_REQUIRE(nw > 0, KH_ERR_INVALID_ARGUMENT, "kh_vjp_hvp: nw must be > 0");
+  return run_adjoint<dcd, false>("kh_vjp_hvp", layers, dlayers, ndir, nl, p, np, omega, nw,
+                                 free_surface, G, dG, 0, nullptr, grad, dgrad, nullptr, dR,
+                                 workspace, workspace_bytes, as_stream(stream));
+}

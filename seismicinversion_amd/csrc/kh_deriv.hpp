@@ -1,0 +1,259 @@
+// Per-lane derivative algorithms of the Kennett recursion, shared by the
+// device kernels (kh_deriv.hip) and the host build used by the CPU tests.
+//
+// lane_adjoint<Z>: one (p, omega) lane.
+//   1. forward sweep from the deepest interface, saving the state S_j every
+//      K = ceil(sqrt(nl-1)) levels (j % K == 0) in per-lane storage;
+//   2. R at the surface and its cotangent G (given, or 2 (R - R_obs) for the
+//      misfit |R - R_obs|^2);
+//   3. reverse sweep top-down in segments of K interfaces: each segment's
+//      states are recomputed from the checkpoint below it (with the phase
+//      factors) into a K-entry buffer, then walked back with step_rev.
+//   The cotangents of every interface's coefficients and of every layer's
+//   (eta, nu, h) are handed to a sink (the kernel reduces them over the
+//   wavefront; the host harness sums them).
+// With Z = Dual<cd> seeded along a parameter direction dm every quantity
+// also carries its derivative along dm: the surface tangent is J dm and the
+// cotangents' tangents, chained by chain_layer, give the Hessian-vector
+// product (tangent-over-adjoint).
+//
+// chain_layer<Z>: cotangents of the six parameters [alpha, beta, rho, h,
+// Q_alpha, Q_beta] of layer l from the (omega-summed) cotangents of the two
+// interfaces around it and of its own (eta, nu, h), using the derivatives of
+// the coefficient formulas along each parameter (one dual level above Z).
+#pragma once
+
+#include "kh_adjoint.hpp"
+#include "kennett_hip.h"
+
+namespace kh {
+
+template <class Z>
+struct LayerCot {
+  Z eta, neta, h;
+};
+
+__host__ __device__ __forceinline__ int ckpt_stride(int nl) {
+  int k = 1;
+  while (k * k < nl - 1) ++k;
+  return k;
+}
+__host__ __device__ __forceinline__ int ckpt_count(int nl, int K) { return (nl - 2) / K + 1; }
+// Z-slots of per-lane storage: nck checkpoints of 4 + K buffer entries of 6.
+__host__ __device__ __forceinline__ int lane_slots(int nl) {
+  const int K = ckpt_stride(nl);
+  return ckpt_count(nl, K) * 4 + K * 6;
+}
+
+template <class Z>
+__host__ __device__ __forceinline__ Z zero_z() {
+  return lift<Z>(0.0);
+}
+
+template <class Z>
+__host__ __device__ __forceinline__ void phases_of(const LayerZ<Z>& lb, cd om, Z& ea, Z& eb) {
+  if (lb.halfspace) {
+    ea = lift<Z>(1.0);
+    eb = lift<Z>(1.0);
+  } else {
+    ea = phase_t(lb.eta, om, lb.h);
+    eb = phase_t(lb.neta, om, lb.h);
+  }
+}
+
+// Storage: st(e) returns a reference to Z-slot e of this lane.
+// Sink: sink.iface(i, IfaceZ<Z>), sink.layer(l, LayerCot<Z>).
+// G_of(R) -> cotangent of R.  Returns R.
+template <class Z, class Store, class Sink, class GFn>
+__host__ __device__ inline Z lane_adjoint(const LayerZ<Z>* __restrict__ L,
+                                          const IfaceZ<Z>* __restrict__ F, int nl, cd om,
+                                          int free_surface, Store& st, Sink& sink, GFn&& G_of) {
+  const int K = ckpt_stride(nl);
+  const int nck = ckpt_count(nl, K);
+  auto save = [&](int slot, const M2<Z>& S) {
+    st(slot * 4 + 0) = S.a00;
+    st(slot * 4 + 1) = S.a01;
+    st(slot * 4 + 2) = S.a10;
+    st(slot * 4 + 3) = S.a11;
+  };
+  auto load = [&](int slot) {
+    return M2<Z>{st(slot * 4 + 0), st(slot * 4 + 1), st(slot * 4 + 2), st(slot * 4 + 3)};
+  };
+  auto buf = [&](int k, int q) -> Z& { return st(nck * 4 + k * 6 + q); };
+
+  // 1. forward with checkpoints
+  const IfaceZ<Z>& fb = F[nl - 2];
+  M2<Z> S{fb.rd00, fb.rd01, fb.rd01, fb.rd11};
+  if ((nl - 2) % K == 0) save((nl - 2) / K, S);
+  for (int i = nl - 3; i >= 0; --i) {
+    Z ea, eb;
+    phases_of(L[i + 1], om, ea, eb);
+    S = step_fwd(S, ea, eb, F[i]);
+    if (i % K == 0) save(i / K, S);
+  }
+
+  // 2. surface value and its cotangent
+  const LayerZ<Z>& water = L[0];
+  Z ea0 = lift<Z>(1.0);
+  if (!water.halfspace) ea0 = phase_t(water.eta, om, water.h);
+  const Z e2 = ea0 * ea0;
+  const Z top = e2 * S.a00;
+  const Z one = lift<Z>(1.0);
+  const Z R = free_surface ? top / (one + top) : top;
+  const Z G = G_of(R);
+  const Z opt = one + top;
+  const Z tb = free_surface ? conj(recip(opt * opt)) * G : G;
+  M2<Z> sb{conj(e2) * tb, zero_z<Z>(), zero_z<Z>(), zero_z<Z>()};
+  {
+    LayerCot<Z> c0{zero_z<Z>(), zero_z<Z>(), zero_z<Z>()};
+    if (!water.halfspace) {
+      const Z ea0b = conj(ea0 * 2.0 * S.a00) * tb;
+      phase_rev(water.eta, om, water.h, ea0, ea0b, c0.eta, c0.h);
+    }
+    sink.layer(0, c0);
+  }
+
+  // 3. reverse, segment by segment
+  for (int i0 = 0; i0 <= nl - 2; i0 += K) {
+    const int i1 = (i0 + K - 1 < nl - 2) ? i0 + K - 1 : nl - 2;
+    // recompute S_{i+1} (and the phases of layer i+1) for i = i1 .. i0
+    M2<Z> T;
+    if (i1 == nl - 2) {
+      T = M2<Z>{fb.rd00, fb.rd01, fb.rd01, fb.rd11};  // S_{nl-2}
+    } else {
+      T = load((i1 + 1) / K);  // (i1 + 1) is a multiple of K
+    }
+    for (int i = i1; i >= i0; --i) {
+      if (i == nl - 2) continue;  // bottom interface: no step (S_{nl-2} = Rd)
+      Z ea, eb;
+      phases_of(L[i + 1], om, ea, eb);
+      const int k = i - i0;
+      buf(k, 0) = T.a00;
+      buf(k, 1) = T.a01;
+      buf(k, 2) = T.a10;
+      buf(k, 3) = T.a11;
+      buf(k, 4) = ea;
+      buf(k, 5) = eb;
+      T = step_fwd(T, ea, eb, F[i]);
+    }
+    for (int i = i0; i <= i1; ++i) {
+      IfaceZ<Z> fbar;
+      fbar.rd00 = fbar.rd01 = fbar.rd11 = zero_z<Z>();
+      fbar.ru00 = fbar.ru01 = fbar.ru11 = zero_z<Z>();
+      fbar.td00 = fbar.td01 = fbar.td10 = fbar.td11 = zero_z<Z>();
+      if (i == nl - 2) {
+        fbar.rd00 = sb.a00;
+        fbar.rd01 = sb.a01 + sb.a10;
+        fbar.rd11 = sb.a11;
+        sink.iface(i, fbar);
+        continue;
+      }
+      const int k = i - i0;
+      const M2<Z> Sin{buf(k, 0), buf(k, 1), buf(k, 2), buf(k, 3)};
+      const Z ea = buf(k, 4), eb = buf(k, 5);
+      M2<Z> rb;
+      Z eab, ebb;
+      step_rev(Sin, ea, eb, F[i], sb, rb, fbar, eab, ebb);
+      sink.iface(i, fbar);
+      LayerCot<Z> c{zero_z<Z>(), zero_z<Z>(), zero_z<Z>()};
+      const LayerZ<Z>& lb = L[i + 1];
+      if (!lb.halfspace) {
+        phase_rev(lb.eta, om, lb.h, ea, eab, c.eta, c.h);
+        Z hb2 = zero_z<Z>();
+        phase_rev(lb.neta, om, lb.h, eb, ebb, c.neta, hb2);
+        c.h = c.h + hb2;
+      }
+      sink.layer(i + 1, c);
+      sb = rb;
+    }
+  }
+  return R;
+}
+
+// One dual level above a scalar type, with the seeding of its inputs.
+template <class Z>
+struct Up;
+template <>
+struct Up<cd> {
+  using W = dcd;
+  // value x, direction dm component (unused), outer seed e
+  __host__ __device__ static W seed(double x, double /*dm*/, double e) { return seed1(x, e); }
+  __host__ __device__ static cd outer(const W& w) { return w.d; }
+};
+template <>
+struct Up<dcd> {
+  using W = ddcd;
+  __host__ __device__ static W seed(double x, double dm, double e) { return seed2(x, dm, e); }
+  __host__ __device__ static dcd outer(const W& w) { return w.d; }
+};
+
+// Z-valued layer from a parameter row (+ direction row when Z is dual).
+__host__ __device__ __forceinline__ void row_z(const double* row, const double*, cd (&x)[NCOL]) {
+  for (int c = 0; c < NCOL; ++c) x[c] = cd{row[c], 0.0};
+}
+__host__ __device__ __forceinline__ void row_z(const double* row, const double* drow,
+                                               dcd (&x)[NCOL]) {
+  for (int c = 0; c < NCOL; ++c) x[c] = seed1(row[c], drow ? drow[c] : 0.0);
+}
+
+template <class Z>
+__host__ __device__ __forceinline__ Z re_dot(const Z& a, const Z& b) {
+  return re_part(conj(a) * b);
+}
+
+template <class Z>
+__host__ __device__ __forceinline__ Z iface_dot(const IfaceZ<Z>& dc, const IfaceZ<Z>& cb) {
+  return re_dot(dc.rd00, cb.rd00) + re_dot(dc.rd01, cb.rd01) + re_dot(dc.rd11, cb.rd11) +
+         re_dot(dc.ru00, cb.ru00) + re_dot(dc.ru01, cb.ru01) + re_dot(dc.ru11, cb.ru11) +
+         re_dot(dc.td00, cb.td00) + re_dot(dc.td01, cb.td01) + re_dot(dc.td10, cb.td10) +
+         re_dot(dc.td11, cb.td11);
+}
+
+template <class W, class Z>
+__host__ __device__ __forceinline__ IfaceZ<Z> outer_iface(const IfaceZ<W>& f) {
+  IfaceZ<Z> o;
+  o.rd00 = f.rd00.d; o.rd01 = f.rd01.d; o.rd11 = f.rd11.d;
+  o.ru00 = f.ru00.d; o.ru01 = f.ru01.d; o.ru11 = f.ru11.d;
+  o.td00 = f.td00.d; o.td01 = f.td01.d; o.td10 = f.td10.d; o.td11 = f.td11.d;
+  return o;
+}
+
+// Parameter cotangents of layer l (out[c] for c in COL_*), given the summed
+// cotangents: cb_up = interface l-1 (ignored for l = 0), cb_dn = interface l
+// (ignored for l = nl-1), lc = layer l.  `layers`/`dlayers` are the row-major
+// (nl, 6) parameter and direction tables (dlayers only for dual Z).
+template <class Z>
+__host__ __device__ inline void chain_layer(const double* layers, const double* dlayers, int nl,
+                                            int l, double p, const IfaceZ<Z>& cb_up,
+                                            const IfaceZ<Z>& cb_dn, const LayerCot<Z>& lc,
+                                            Z (&out)[NCOL]) {
+  using W = typename Up<Z>::W;
+  const int cols[5] = {COL_ALPHA, COL_BETA, COL_RHO, COL_QA, COL_QB};
+  out[COL_H] = lc.h;
+  for (int t = 0; t < 5; ++t) {
+    const int col = cols[t];
+    LayerZ<W> Ls[3];
+    for (int q = 0; q < 3; ++q) {
+      const int ll = l - 1 + q;
+      if (ll < 0 || ll >= nl) continue;
+      W x[NCOL];
+      for (int c = 0; c < NCOL; ++c) {
+        const double dm = dlayers ? dlayers[(size_t)ll * NCOL + c] : 0.0;
+        x[c] = Up<Z>::seed(layers[(size_t)ll * NCOL + c], dm, (ll == l && c == col) ? 1.0 : 0.0);
+      }
+      Ls[q] = layer_t<W>(x, ll, p);
+    }
+    Z acc = re_dot(Up<Z>::outer(Ls[1].eta), lc.eta) + re_dot(Up<Z>::outer(Ls[1].neta), lc.neta);
+    if (l >= 1) {
+      const IfaceZ<W> f = interface_t<W>(l - 1, p, Ls[0], Ls[1]);
+      acc = acc + iface_dot(outer_iface<W, Z>(f), cb_up);
+    }
+    if (l <= nl - 2) {
+      const IfaceZ<W> f = interface_t<W>(l, p, Ls[1], Ls[2]);
+      acc = acc + iface_dot(outer_iface<W, Z>(f), cb_dn);
+    }
+    out[col] = acc;
+  }
+}
+
+}  // namespace kh

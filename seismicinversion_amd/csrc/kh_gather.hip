@@ -71,7 +71,9 @@ __global__ void __launch_bounds__(BS_WAVE) bessel_partial_kernel(
   }
   if (k < nwm) {
     cd* out = partial + ((size_t)chunk * nr + r0) * nwm + k;
-    for (int q = 0; q < nrg; ++q) out[(size_t)q * nwm] = acc[q];
+#pragma unroll
+    for (int q = 0; q < BS_RG; ++q)
+      if (q < nrg) out[(size_t)q * nwm] = acc[q];
   }
 }
 

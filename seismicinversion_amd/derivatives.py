@@ -1,0 +1,152 @@
+"""Device derivative entry points (kh_jvp, kh_vjp, kh_misfit_grad,
+kh_misfit_hvp, kh_vjp_hvp) on torch tensors in HBM.
+
+Parameter layout everywhere: per layer the six columns
+[alpha, beta, rho, h, Q_alpha, Q_beta] of the (nl, 6) layer table; gradients
+and Hessian-vector products come back as (np, nl, 6) per slowness (callers
+sum over slowness themselves).  Directions are (ndir, nl, 6).
+"""
+
+from __future__ import annotations
+
+import torch
+
+from . import _abi
+from .engine import check_precision
+
+__all__ = ["jvp", "vjp", "misfit_grad", "misfit_hvp", "vjp_hvp", "WORKSPACE_LIMIT"]
+
+# Directions per launch are capped so that one launch's workspace stays below this.
+WORKSPACE_LIMIT = 4 << 30
+
+
+def _f64(x, dev):
+    check_precision(x)
+    return x.detach().to(device=dev, dtype=torch.float64).contiguous()
+
+
+def _c128(x, dev):
+    check_precision(x)
+    return x.detach().to(device=dev, dtype=torch.complex128).contiguous()
+
+
+def _workspace(nl, n_p, nw, ndir, dual, dev):
+    nbytes = int(_abi.lib().kh_deriv_workspace_bytes(nl, n_p, nw, ndir, int(dual)))
+    return torch.empty(max(nbytes, 1), dtype=torch.uint8, device=dev)
+
+
+def _dir_chunks(nl, n_p, nw, ndir):
+    one = int(_abi.lib().kh_deriv_workspace_bytes(nl, n_p, nw, 1, 1)) or 1
+    step = max(1, min(ndir, WORKSPACE_LIMIT // one))
+    return [(s, min(ndir, s + step)) for s in range(0, ndir, step)]
+
+
+def jvp(layers, dlayers, p, omega, free_surface=False):
+    """(R (np, nw), dR (ndir, np, nw)) with dR[d] = J dlayers[d]."""
+    dev = layers.device
+    p = _f64(p, dev)
+    omega = _c128(omega, dev)
+    dlayers = _f64(dlayers, dev).reshape(-1, layers.shape[0], 6)
+    ndir, nl = dlayers.shape[0], layers.shape[0]
+    n_p, nw = p.numel(), omega.numel()
+    R = torch.empty((n_p, nw), dtype=torch.complex128, device=dev)
+    dR = torch.empty((ndir, n_p, nw), dtype=torch.complex128, device=dev)
+    rc = _abi.lib().kh_jvp(_abi.ptr(layers), _abi.ptr(dlayers), ndir, nl, _abi.ptr(p), n_p,
+                           _abi.ptr(omega), nw, int(bool(free_surface)), _abi.ptr(R),
+                           _abi.ptr(dR), _abi.stream_handle(dev))
+    _abi.check(rc, "kh_jvp")
+    return R, dR
+
+
+def vjp(layers, p, omega, G, free_surface=False):
+    """(grad (np, nl, 6), R (np, nw)) for the cotangent G (np, nw)."""
+    dev = layers.device
+    p = _f64(p, dev)
+    omega = _c128(omega, dev)
+    nl, n_p, nw = layers.shape[0], p.numel(), omega.numel()
+    G = _c128(G, dev).reshape(n_p, nw)
+    grad = torch.empty((n_p, nl, 6), dtype=torch.float64, device=dev)
+    R = torch.empty((n_p, nw), dtype=torch.complex128, device=dev)
+    ws = _workspace(nl, n_p, nw, 1, False, dev)
+    rc = _abi.lib().kh_vjp(_abi.ptr(layers), nl, _abi.ptr(p), n_p, _abi.ptr(omega), nw,
+                           int(bool(free_surface)), _abi.ptr(G), _abi.ptr(grad), _abi.ptr(R),
+                           _abi.ptr(ws), ws.numel(), _abi.stream_handle(dev))
+    _abi.check(rc, "kh_vjp")
+    return grad, R
+
+
+def misfit_grad(layers, p, omega, R_obs, free_surface=False):
+    """(chi (np,), grad (np, nl, 6), R (np, nw)) of chi_p = sum_w |R - R_obs|^2."""
+    dev = layers.device
+    p = _f64(p, dev)
+    omega = _c128(omega, dev)
+    nl, n_p, nw = layers.shape[0], p.numel(), omega.numel()
+    R_obs = _c128(R_obs, dev).reshape(n_p, nw)
+    chi = torch.empty((n_p, 2), dtype=torch.float64, device=dev)
+    grad = torch.empty((n_p, nl, 6), dtype=torch.float64, device=dev)
+    R = torch.empty((n_p, nw), dtype=torch.complex128, device=dev)
+    ws = _workspace(nl, n_p, nw, 1, False, dev)
+    rc = _abi.lib().kh_misfit_grad(_abi.ptr(layers), nl, _abi.ptr(p), n_p, _abi.ptr(omega), nw,
+                                   int(bool(free_surface)), _abi.ptr(R_obs), _abi.ptr(chi),
+                                   _abi.ptr(grad), _abi.ptr(R), _abi.ptr(ws), ws.numel(),
+                                   _abi.stream_handle(dev))
+    _abi.check(rc, "kh_misfit_grad")
+    return chi[:, 0], grad, R
+
+
+def misfit_hvp(layers, dlayers, p, omega, R_obs, free_surface=False, second_only=False,
+               want_dR=False):
+    """Exact misfit Hessian-vector products along each direction.
+
+    Returns (chi (np,), grad (np, nl, 6), hvp (ndir, np, nl, 6), dR (ndir, np, nw) or None)."""
+    dev = layers.device
+    p = _f64(p, dev)
+    omega = _c128(omega, dev)
+    nl, n_p, nw = layers.shape[0], p.numel(), omega.numel()
+    R_obs = _c128(R_obs, dev).reshape(n_p, nw)
+    dlayers = _f64(dlayers, dev).reshape(-1, nl, 6)
+    ndir = dlayers.shape[0]
+    grad = torch.empty((n_p, nl, 6), dtype=torch.float64, device=dev)
+    hvp = torch.empty((ndir, n_p, nl, 6), dtype=torch.float64, device=dev)
+    dR = torch.empty((ndir, n_p, nw), dtype=torch.complex128, device=dev) if want_dR else None
+    chi = None
+    for s, e in _dir_chunks(nl, n_p, nw, ndir):
+        nd = e - s
+        ws = _workspace(nl, n_p, nw, nd, True, dev)
+        chi_c = torch.empty((nd, n_p, 2), dtype=torch.float64, device=dev)
+        rc = _abi.lib().kh_misfit_hvp(
+            _abi.ptr(layers), _abi.ptr(dlayers[s:e]), nd, nl, _abi.ptr(p), n_p, _abi.ptr(omega),
+            nw, int(bool(free_surface)), _abi.ptr(R_obs), int(bool(second_only)),
+            _abi.ptr(chi_c), _abi.ptr(grad) if s == 0 else None, _abi.ptr(hvp[s:e]),
+            _abi.ptr(dR[s:e]) if want_dR else None, _abi.ptr(ws), ws.numel(),
+            _abi.stream_handle(dev))
+        _abi.check(rc, "kh_misfit_hvp")
+        if chi is None:
+            chi = chi_c[0, :, 0]
+    return chi, grad, hvp, dR
+
+
+def vjp_hvp(layers, dlayers, p, omega, G, dG=None, free_surface=False):
+    """(grad (np, nl, 6), dgrad (ndir, np, nl, 6)): directional derivatives of
+    the vector-Jacobian product along (dlayers[d], dG[d])."""
+    dev = layers.device
+    p = _f64(p, dev)
+    omega = _c128(omega, dev)
+    nl, n_p, nw = layers.shape[0], p.numel(), omega.numel()
+    G = _c128(G, dev).reshape(n_p, nw)
+    dlayers = _f64(dlayers, dev).reshape(-1, nl, 6)
+    ndir = dlayers.shape[0]
+    if dG is not None:
+        dG = _c128(dG, dev).reshape(ndir, n_p, nw)
+    grad = torch.empty((n_p, nl, 6), dtype=torch.float64, device=dev)
+    dgrad = torch.empty((ndir, n_p, nl, 6), dtype=torch.float64, device=dev)
+    for s, e in _dir_chunks(nl, n_p, nw, ndir):
+        nd = e - s
+        ws = _workspace(nl, n_p, nw, nd, True, dev)
+        rc = _abi.lib().kh_vjp_hvp(
+            _abi.ptr(layers), _abi.ptr(dlayers[s:e]), nd, nl, _abi.ptr(p), n_p, _abi.ptr(omega),
+            nw, int(bool(free_surface)), _abi.ptr(G), _abi.ptr(dG[s:e]) if dG is not None else None,
+            _abi.ptr(grad) if s == 0 else None, _abi.ptr(dgrad[s:e]), None, _abi.ptr(ws),
+            ws.numel(), _abi.stream_handle(dev))
+        _abi.check(rc, "kh_vjp_hvp")
+    return grad, dgrad

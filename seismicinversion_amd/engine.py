@@ -31,14 +31,26 @@ def layers_tensor(model: LayerModel | np.ndarray | torch.Tensor, dev=None) -> to
     elif isinstance(model, np.ndarray):
         tab = torch.from_numpy(np.ascontiguousarray(model, dtype=np.float64))
     else:
+        check_precision(model, "layer table")
         tab = model.detach().to(torch.float64)
     if tab.ndim != 2 or tab.shape[1] != 6:
         raise ValueError(f"layer table must be (n_layers, 6), got {tuple(tab.shape)}")
     return tab.to(dev).contiguous()
 
 
+_LOW_PRECISION = (torch.float16, torch.bfloat16, torch.float32, torch.complex32, torch.complex64)
+
+
+def check_precision(x, what="input"):
+    """Reject reduced-precision tensors: the path computes in float64/complex128
+    and a float32 slowness or frequency would silently change the result."""
+    if isinstance(x, torch.Tensor) and x.dtype in _LOW_PRECISION:
+        raise TypeError(f"{what} must be float64/complex128, got {x.dtype}")
+
+
 def _as_dev(x, dtype, dev) -> torch.Tensor:
     if isinstance(x, torch.Tensor):
+        check_precision(x)
         return x.detach().to(device=dev, dtype=dtype).contiguous()
     return torch.from_numpy(np.ascontiguousarray(np.asarray(x).astype(
         np.complex128 if dtype == torch.complex128 else np.float64))).to(dev)

@@ -1,0 +1,122 @@
+// Host build of the derivative algorithms (csrc/kh_deriv.hpp): the per-lane
+// forward/checkpoint/reverse sweep and the parameter chain rule, run over the
+// frequencies sequentially (the kernels reduce the same per-lane cotangents
+// over a wavefront).  Compared on the CPU with torch autograd of the oracle.
+#include "kh_deriv.hpp"
+
+#include <vector>
+
+using namespace kh;
+
+template <class Z>
+static IfaceZ<Z> add(const IfaceZ<Z>& a, const IfaceZ<Z>& b) {
+  IfaceZ<Z> o;
+  o.rd00 = a.rd00 + b.rd00; o.rd01 = a.rd01 + b.rd01; o.rd11 = a.rd11 + b.rd11;
+  o.ru00 = a.ru00 + b.ru00; o.ru01 = a.ru01 + b.ru01; o.ru11 = a.ru11 + b.ru11;
+  o.td00 = a.td00 + b.td00; o.td01 = a.td01 + b.td01; o.td10 = a.td10 + b.td10;
+  o.td11 = a.td11 + b.td11;
+  return o;
+}
+
+template <class Z>
+struct Sink {
+  std::vector<IfaceZ<Z>> cif;
+  std::vector<LayerCot<Z>> cly;
+  explicit Sink(int nl) : cif(nl), cly(nl) {
+    const Z z = lift<Z>(0.0);
+    for (auto& f : cif) f = IfaceZ<Z>{z, z, z, z, z, z, z, z, z, z};
+    for (auto& c : cly) c = LayerCot<Z>{z, z, z};
+  }
+  void iface(int i, const IfaceZ<Z>& f) { cif[i] = add(cif[i], f); }
+  void layer(int l, const LayerCot<Z>& c) {
+    cly[l].eta = cly[l].eta + c.eta;
+    cly[l].neta = cly[l].neta + c.neta;
+    cly[l].h = cly[l].h + c.h;
+  }
+};
+
+template <class Z>
+struct Store {
+  std::vector<Z> v;
+  explicit Store(int n) : v(n) {}
+  Z& operator()(int e) { return v[e]; }
+};
+
+template <class Z>
+static void stage(const double* layers, const double* dlayers, int nl, double p,
+                  std::vector<LayerZ<Z>>& L, std::vector<IfaceZ<Z>>& F) {
+  for (int l = 0; l < nl; ++l) {
+    Z x[NCOL];
+    row_z(layers + (size_t)l * NCOL, dlayers ? dlayers + (size_t)l * NCOL : nullptr, x);
+    L[l] = layer_t<Z>(x, l, p);
+  }
+  for (int i = 0; i < nl - 1; ++i) F[i] = interface_t<Z>(i, p, L[i], L[i + 1]);
+}
+
+// grad (nl, 6) of Re(sum_w conj(G_w) R_w); R (nw,) complex.
+extern "C" int host_vjp(const double* layers, int nl, double p, const double* omega, int nw,
+                        int fs, const double* G, double* grad, double* R) {
+  std::vector<LayerZ<cd>> L(nl);
+  std::vector<IfaceZ<cd>> F(nl - 1);
+  stage<cd>(layers, nullptr, nl, p, L, F);
+  Sink<cd> sink(nl);
+  Store<cd> st(lane_slots(nl));
+  for (int w = 0; w < nw; ++w) {
+    const cd om{omega[2 * w], omega[2 * w + 1]};
+    const cd g{G[2 * w], G[2 * w + 1]};
+    const cd r = lane_adjoint<cd>(L.data(), F.data(), nl, om, fs, st, sink,
+                                  [&](const cd&) { return g; });
+    R[2 * w] = r.r;
+    R[2 * w + 1] = r.i;
+  }
+  const cd z{0.0, 0.0};
+  const IfaceZ<cd> none{z, z, z, z, z, z, z, z, z, z};
+  for (int l = 0; l < nl; ++l) {
+    cd out[NCOL];
+    chain_layer<cd>(layers, nullptr, nl, l, p, l >= 1 ? sink.cif[l - 1] : none,
+                    l <= nl - 2 ? sink.cif[l] : none, sink.cly[l], out);
+    for (int c = 0; c < NCOL; ++c) grad[l * NCOL + c] = out[c].r;
+  }
+  return 0;
+}
+
+// chi = sum |R - Robs|^2, its gradient, and (H dm) along the direction
+// dlayers (nl, 6); dR = J dm.  second_only drops the Gauss-Newton part.
+extern "C" int host_misfit_hvp(const double* layers, const double* dlayers, int nl, double p,
+                               const double* omega, int nw, int fs, const double* Robs,
+                               int second_only, double* chi, double* grad, double* hvp,
+                               double* dR) {
+  std::vector<LayerZ<dcd>> L(nl);
+  std::vector<IfaceZ<dcd>> F(nl - 1);
+  stage<dcd>(layers, dlayers, nl, p, L, F);
+  Sink<dcd> sink(nl);
+  Store<dcd> st(lane_slots(nl));
+  dcd chis = lift<dcd>(0.0);
+  for (int w = 0; w < nw; ++w) {
+    const cd om{omega[2 * w], omega[2 * w + 1]};
+    const dcd ro = lift_c<dcd>(cd{Robs[2 * w], Robs[2 * w + 1]});
+    const dcd r = lane_adjoint<dcd>(L.data(), F.data(), nl, om, fs, st, sink, [&](const dcd& R) {
+      const dcd res = R - ro;
+      chis = chis + re_part(conj(res) * res);
+      dcd g = res * 2.0;
+      if (second_only) g.d = cd{0.0, 0.0};
+      return g;
+    });
+    dR[2 * w] = r.d.r;
+    dR[2 * w + 1] = r.d.i;
+  }
+  chi[0] = chis.v.r;
+  chi[1] = chis.d.r;
+  const dcd z = lift<dcd>(0.0);
+  const IfaceZ<dcd> none{z, z, z, z, z, z, z, z, z, z};
+  for (int l = 0; l < nl; ++l) {
+    dcd out[NCOL];
+    chain_layer<dcd>(layers, dlayers, nl, l, p, l >= 1 ? sink.cif[l - 1] : none,
+                     l <= nl - 2 ? sink.cif[l] : none, sink.cly[l], out);
+    for (int c = 0; c < NCOL; ++c) {
+      grad[l * NCOL + c] = out[c].v.r;
+      hvp[l * NCOL + c] = out[c].d.r;
+    }
+  }
+  return 0;
+}

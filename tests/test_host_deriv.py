@@ -1,0 +1,104 @@
+"""The derivative algorithms (csrc/kh_deriv.hpp: checkpointed adjoint sweep,
+tangent-over-adjoint, parameter chain rule) compiled for the host and
+compared with torch autograd through the oracle's torch restatement."""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import kennett_torch_ref as T
+from seismicinversion_amd.layer_model import pack_layers
+from seismicinversion_amd.workloads import default_ocean_crust_model, seeded_model
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+vp = ctypes.c_void_p
+
+
+@pytest.fixture(scope="module")
+def lib(tmp_path_factory):
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    out = tmp_path_factory.mktemp("hostderiv") / "libkh_host_deriv.so"
+    subprocess.run([hipcc, "--offload-host-only", "-O2", "-std=c++17", "-fPIC", "-shared",
+                    f"-I{REPO}/include", f"-I{REPO}/seismicinversion_amd/csrc",
+                    os.path.join(HERE, "host", "kh_host_deriv.cpp"), "-o", str(out)],
+                   check=True, capture_output=True)
+    L = ctypes.CDLL(str(out))
+    L.host_vjp.argtypes = [vp, ctypes.c_int, ctypes.c_double, vp, ctypes.c_int, ctypes.c_int, vp,
+                           vp, vp]
+    L.host_misfit_hvp.argtypes = [vp, vp, ctypes.c_int, ctypes.c_double, vp, ctypes.c_int,
+                                  ctypes.c_int, vp, ctypes.c_int, vp, vp, vp, vp]
+    return L
+
+
+def ref(tab, p, w, fs):
+    t = torch.tensor(tab.copy())
+    t[:, 3][torch.isinf(t[:, 3])] = 1e30
+    return t, lambda tt: T.reflectivity(tt[:, 0], tt[:, 1], tt[:, 2], tt[:, 3], tt[:, 4],
+                                        tt[:, 5], p, torch.tensor(w), fs)
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / np.max(np.abs(b))
+
+
+CASES = [(default_ocean_crust_model(), 0.2, False), (default_ocean_crust_model(), 0.45, True),
+         (seeded_model(12, 3), 0.3, True), (seeded_model(27, 11), 0.12, False)]
+
+
+@pytest.mark.parametrize("model,p,fs", CASES)
+def test_host_vjp_and_hvp_vs_torch(lib, model, p, fs):
+    tab = pack_layers(model)
+    nl = len(tab)
+    fin = np.isfinite(tab[:, 3])
+    w = np.arange(1, 40) * 2 * np.pi / 64
+    wc = w.astype(np.complex128)
+    rng = np.random.default_rng(0)
+    G = rng.standard_normal(len(w)) + 1j * rng.standard_normal(len(w))
+    grad = np.zeros((nl, 6))
+    R = np.zeros(len(w), np.complex128)
+    lib.host_vjp(tab.ctypes.data, nl, p, wc.ctypes.data, len(w), int(fs), G.ctypes.data,
+                 grad.ctypes.data, R.ctypes.data)
+    t0, f = ref(tab, p, w, fs)
+    t = t0.clone().requires_grad_(True)
+    Rt = f(t)
+    Gt = torch.tensor(G)
+    (g,) = torch.autograd.grad((Rt.real * Gt.real + Rt.imag * Gt.imag).sum(), t)
+    g = g.numpy()
+    g[~fin, 3] = 0
+    assert rel(R, Rt.detach().numpy()) < 1e-12
+    assert rel(grad, g) < 1e-9
+
+    R_obs = Rt.detach().numpy() * (1 + 0.05j)
+    dm = rng.standard_normal((nl, 6)) * np.array([1, 1, 1, 1, 10, 10])
+    dm[~fin, 3] = 0
+    chi, gr, hv = np.zeros(2), np.zeros((nl, 6)), np.zeros((nl, 6))
+    dR = np.zeros(len(w), np.complex128)
+    lib.host_misfit_hvp(tab.ctypes.data, dm.ctypes.data, nl, p, wc.ctypes.data, len(w), int(fs),
+                        R_obs.ctypes.data, 0, chi.ctypes.data, gr.ctypes.data, hv.ctypes.data,
+                        dR.ctypes.data)
+    Ro = torch.tensor(R_obs)
+
+    def chi_f(tt):
+        r = f(tt) - Ro
+        return (r.real ** 2 + r.imag ** 2).sum()
+
+    gt = torch.func.grad(chi_f)(t0).numpy()
+    _, hvt = torch.func.jvp(torch.func.grad(chi_f), (t0,), (torch.tensor(dm),))
+    hvt = hvt.numpy()
+    gt[~fin, 3] = 0
+    hvt[~fin, 3] = 0
+    assert abs(chi[0] - chi_f(t0).item()) / chi[0] < 1e-11
+    assert rel(gr, gt) < 1e-9
+    assert rel(hv, hvt) < 1e-9
+    _, dRt = torch.func.jvp(f, (t0,), (torch.tensor(dm),))
+    assert rel(dR, dRt.numpy()) < 1e-9
