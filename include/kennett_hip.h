@@ -63,6 +63,18 @@ int kh_interface_coefficients(const double* layers, int nl, const double* p, int
                               double* Rd, double* Ru, double* Tu, double* Td,
                               void* stream);
 
+/* Coefficients of n independent interfaces from given vertical slownesses:
+ * p, rho1, rho2 (n,) float64; eta1, neta1, beta1, eta2, neta2, beta2 (n,)
+ * complex128 (beta = complex S velocity 1/s_s); ocean != 0 selects the
+ * acoustic-over-elastic form (neta1, beta1 ignored, may be NULL).  Outputs
+ * Rd, Ru, Tu, Td (n, 2, 2) complex128.  Replaces solid_solid_interface and
+ * ocean_bottom_interface (Kennett_Reflectivity/scattering_matrices.py:42-298). */
+int kh_interfaces_direct(int n, int ocean, const double* p, const double* eta1,
+                         const double* neta1, const double* rho1, const double* beta1,
+                         const double* eta2, const double* neta2, const double* rho2,
+                         const double* beta2, double* Rd, double* Ru, double* Tu, double* Td,
+                         void* stream);
+
 /* PP reflectivity R (np, nw) complex128 (row-major, slowness-major) for
  * slownesses p (np,) float64 and frequencies omega (nw,) complex128 (may be
  * damped, omega + i*gamma; must not be 0).  free_surface != 0 applies the
@@ -185,6 +197,19 @@ int kh_vjp_hvp(const double* layers, const double* dlayers, int ndir, int nl,
                const double* p, int np, const double* omega, int nw, int free_surface,
                const double* G, const double* dG, double* grad, double* dgrad,
                double* dR, void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---- Levenberg-Marquardt support ---- */
+
+/* Workspace bytes of kh_gram for a (P, K) input. */
+size_t kh_gram_workspace_bytes(int P, long K);
+
+/* G (P, P) = scale * X X^T for X (P, K) float64 row-major, on the FP64
+ * matrix cores; chunks of K reduced in a fixed order.  With X the real view
+ * of Jacobian columns this is the Gauss-Newton block Re(J^H J) of the
+ * Hessian that Kennett_Reflectivity/taup_inversion.py:177 forms through
+ * torch.func.hessian. */
+int kh_gram(const double* X, int P, long K, double scale, double* G, void* workspace,
+            size_t workspace_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -47,8 +47,13 @@ EXPORTS: dict[str, list] = {
                       _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp],
     "kh_vjp_hvp": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _vp, _vp, _vp,
                    _vp, _vp, _vp, ctypes.c_size_t, _vp],
+    "kh_interfaces_direct": [_c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                             _vp, _vp, _vp, _vp],
+    "kh_gram_workspace_bytes": [_c_int, ctypes.c_long],
+    "kh_gram": [_vp, _c_int, ctypes.c_long, _c_double, _vp, _vp, ctypes.c_size_t, _vp],
 }
-_RESTYPES = {"kh_deriv_workspace_bytes": ctypes.c_size_t, "kh_version": ctypes.c_char_p, "kh_last_error": ctypes.c_char_p,
+_RESTYPES = {"kh_deriv_workspace_bytes": ctypes.c_size_t, "kh_gram_workspace_bytes": ctypes.c_size_t,
+             "kh_version": ctypes.c_char_p, "kh_last_error": ctypes.c_char_p,
              "kh_bessel_workspace_bytes": ctypes.c_size_t}
 
 KH_ERR_INVALID_ARGUMENT = 1

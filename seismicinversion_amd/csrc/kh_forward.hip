@@ -55,6 +55,42 @@ __global__ void interface_kernel(const double* __restrict__ layers, int nl,
   tu[0] = f.td00; tu[1] = f.td10; tu[2] = f.td01; tu[3] = f.td11;
 }
 
+// Coefficients of n independent interfaces given their vertical slownesses
+// (the reference's scalar API, scattering_matrices.py:42-298).
+__global__ void interface_direct_kernel(int n, int ocean, const double* __restrict__ p,
+                                        const cd* __restrict__ eta1, const cd* __restrict__ neta1,
+                                        const double* __restrict__ rho1,
+                                        const cd* __restrict__ beta1, const cd* __restrict__ eta2,
+                                        const cd* __restrict__ neta2,
+                                        const double* __restrict__ rho2,
+                                        const cd* __restrict__ beta2, cd* __restrict__ Rd,
+                                        cd* __restrict__ Ru, cd* __restrict__ Tu,
+                                        cd* __restrict__ Td) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  LayerPt up, dn;
+  up.eta = eta1[k];
+  up.neta = ocean ? cd{0.0, 0.0} : neta1[k];
+  up.beta_c = ocean ? cd{0.0, 0.0} : beta1[k];
+  up.rho = rho1[k];
+  dn.eta = eta2[k];
+  dn.neta = neta2[k];
+  dn.beta_c = beta2[k];
+  dn.rho = rho2[k];
+  up.h = dn.h = 0.0;
+  up.halfspace = dn.halfspace = 0;
+  up.pad_ = dn.pad_ = 0;
+  const Iface f = ocean ? ocean_bottom(p[k], up, dn) : solid_solid(p[k], up, dn);
+  cd* rd = Rd + k * 4;
+  cd* ru = Ru + k * 4;
+  cd* tu = Tu + k * 4;
+  cd* td = Td + k * 4;
+  rd[0] = f.rd00; rd[1] = f.rd01; rd[2] = f.rd01; rd[3] = f.rd11;
+  ru[0] = f.ru00; ru[1] = f.ru01; ru[2] = f.ru01; ru[3] = f.ru11;
+  td[0] = f.td00; td[1] = f.td01; td[2] = f.td10; td[3] = f.td11;
+  tu[0] = f.td00; tu[1] = f.td10; tu[2] = f.td01; tu[3] = f.td11;
+}
+
 // grid.x = np * ntiles; block b covers slowness b / ntiles and the
 // frequencies tile * blockDim * W + threadIdx.x + k * blockDim, k < W.
 // GRID: omega_j = (k0 + j) * dw + i * gamma (omega unused); otherwise omega[j].
@@ -194,6 +230,28 @@ extern "C" int kh_interface_coefficients(const double* layers, int nl, const dou
                      reinterpret_cast<cd*>(Ru), reinterpret_cast<cd*>(Tu),
                      reinterpret_cast<cd*>(Td));
   return check_launch("kh_interface_coefficients");
+}
+
+extern "C" int kh_interfaces_direct(int n, int ocean, const double* p, const double* eta1,
+                                    const double* neta1, const double* rho1, const double* beta1,
+                                    const double* eta2, const double* neta2, const double* rho2,
+                                    const double* beta2, double* Rd, double* Ru, double* Tu,
+                                    double* Td, void* stream) {
+  clear_error();
+  KH_REQUIRE(n >= 0, KH_ERR_INVALID_ARGUMENT, "kh_interfaces_direct: n < 0");
+  if (n == 0) return KH_OK;
+  KH_REQUIRE(p && eta1 && rho1 && eta2 && neta2 && rho2 && beta2 && Rd && Ru && Tu && Td &&
+                 (ocean || (neta1 && beta1)),
+             KH_ERR_INVALID_ARGUMENT, "kh_interfaces_direct: NULL pointer");
+  const int blk = 256;
+  hipLaunchKernelGGL(interface_direct_kernel, dim3((unsigned)((n + blk - 1) / blk)), dim3(blk), 0,
+                     as_stream(stream), n, ocean, p, reinterpret_cast<const cd*>(eta1),
+                     reinterpret_cast<const cd*>(neta1), rho1, reinterpret_cast<const cd*>(beta1),
+                     reinterpret_cast<const cd*>(eta2), reinterpret_cast<const cd*>(neta2), rho2,
+                     reinterpret_cast<const cd*>(beta2), reinterpret_cast<cd*>(Rd),
+                     reinterpret_cast<cd*>(Ru), reinterpret_cast<cd*>(Tu),
+                     reinterpret_cast<cd*>(Td));
+  return check_launch("kh_interfaces_direct");
 }
 
 extern "C" int kh_forward_ex(const double* layers, int nl, const double* p, int np,

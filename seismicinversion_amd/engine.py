@@ -195,3 +195,38 @@ def hermitian_traces(U: torch.Tensor, nw: int, gamma: float, T: float,
                                         float(T), _abi.ptr(out), _abi.stream_handle(dev))
     _abi.check(rc, "kh_hermitian_traces")
     return out
+
+
+def interfaces_direct(p, eta1, neta1, rho1, beta1, eta2, neta2, rho2, beta2, ocean: bool,
+                      dev=None):
+    """(Rd, Ru, Tu, Td) each (n, 2, 2) complex128 on the device for n interfaces
+    given their vertical slownesses (kh_interfaces_direct)."""
+    dev = _abi.device(dev)
+    f = [_as_dev(np.atleast_1d(np.asarray(x)), torch.float64, dev) for x in (p, rho1, rho2)]
+    c = [None if x is None else _as_dev(np.atleast_1d(np.asarray(x)), torch.complex128, dev)
+         for x in (eta1, neta1, beta1, eta2, neta2, beta2)]
+    n = f[0].numel()
+    outs = [torch.empty((n, 2, 2), dtype=torch.complex128, device=dev) for _ in range(4)]
+    ptr = lambda t: None if t is None else _abi.ptr(t)  # noqa: E731
+    rc = _abi.lib().kh_interfaces_direct(
+        n, int(bool(ocean)), ptr(f[0]), ptr(c[0]), ptr(c[1]), ptr(f[1]), ptr(c[2]), ptr(c[3]),
+        ptr(c[4]), ptr(f[2]), ptr(c[5]), *[_abi.ptr(o) for o in outs], _abi.stream_handle(dev))
+    _abi.check(rc, "kh_interfaces_direct")
+    return tuple(outs)
+
+
+def gram(X: torch.Tensor, scale: float = 1.0, out: torch.Tensor | None = None) -> torch.Tensor:
+    """scale * X X^T (P, P) float64 on the FP64 matrix cores (kh_gram); X (P, K)."""
+    check_precision(X, "X")
+    X = X.detach().to(torch.float64).contiguous()
+    P, K = X.shape
+    dev = X.device
+    if out is None:
+        out = torch.empty((P, P), dtype=torch.float64, device=dev)
+    lib = _abi.lib()
+    nbytes = int(lib.kh_gram_workspace_bytes(P, K))
+    ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=dev)
+    rc = lib.kh_gram(_abi.ptr(X), P, K, float(scale), _abi.ptr(out), _abi.ptr(ws), ws.numel(),
+                     _abi.stream_handle(dev))
+    _abi.check(rc, "kh_gram")
+    return out

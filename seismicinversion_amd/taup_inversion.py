@@ -1,0 +1,240 @@
+"""Drop-in for the solver part of Kennett_Reflectivity/taup_inversion.py
+(taup_inversion.py:56-470, 527-571): Newton-Levenberg-Marquardt in
+log-parameter space with the exact Hessian.
+
+Per iteration the device evaluates, for all slownesses in one launch, the
+misfit chi = sum_p sum_w |R - R_obs|^2, its gradient (adjoint sweep) and the
+exact Hessian in the packed parameters:
+
+* hessian_mode "gram" (default): the P Jacobian columns and the second-order
+  term 2 Re(sum conj(R - R_obs) d^2R[e_k]) come from one tangent-over-adjoint
+  launch (kh_misfit_hvp, second_only); the Gauss-Newton block 2 Re(J^H J) is
+  a Gram matrix on the FP64 matrix cores (kh_gram).
+* hessian_mode "hvp": the P exact Hessian-vector products H e_k directly.
+
+The host keeps what the reference keeps there: the P x P damped solve
+(np.linalg.solve), the damping schedule (lambda_0 = 0.01 max|diag H|,
+x5 on rejection, /2 on acceptance, at most 10 trials) and the convergence
+tests, so the iteration sequence follows taup_inversion.py:389-439.
+"""
+
+from __future__ import annotations
+
+import logging
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _abi, derivatives, engine
+from .kennett_torch import _pack_params, _unpack_params, model_to_tensors, packed_index
+from .layer_model import LayerModel
+
+__all__ = [
+    "InversionResult",
+    "taup_misfit_grad_hessian",
+    "newton_lm_step",
+    "invert_taup",
+    "compute_taup_traces",
+]
+
+logger = logging.getLogger(__name__)
+_F = torch.float64
+_C = torch.complex128
+
+
+@dataclass
+class InversionResult:
+    """Convergence history and models of a tau-p inversion (taup_inversion.py:56-67)."""
+
+    true_model: LayerModel
+    initial_model: LayerModel
+    recovered_model: LayerModel
+    misfit_history: list[float] = field(default_factory=list)
+    grad_norm_history: list[float] = field(default_factory=list)
+    param_error_history: list[float] = field(default_factory=list)
+    n_iterations: int = 0
+    converged: bool = False
+
+
+class _Problem:
+    """Device-resident inversion data: reference layer table, slownesses,
+    frequencies, observed reflectivity and the packed-parameter map."""
+
+    def __init__(self, alpha_ref, beta_ref, rho_ref, thickness_ref, Q_alpha, Q_beta, p_values,
+                 omega, R_obs, free_surface=False):
+        self.dev = _abi.device()
+        cols = [torch.as_tensor(x, dtype=_F).detach().cpu() for x in
+                (alpha_ref, beta_ref, rho_ref, thickness_ref, Q_alpha, Q_beta)]
+        self.refs = cols
+        self.nl = cols[0].shape[0]
+        self.p_values = [float(p) for p in p_values]
+        self.p = torch.tensor(self.p_values, dtype=_F, device=self.dev)
+        self.omega = torch.as_tensor(omega).detach().to(_C).to(self.dev)
+        if isinstance(R_obs, dict):
+            R_obs = torch.stack([torch.as_tensor(R_obs[p]) for p in p_values])
+        self.R_obs = torch.as_tensor(R_obs).to(_C).reshape(len(self.p_values), -1).to(self.dev)
+        self.free_surface = free_surface
+        idx = packed_index(self.nl)
+        self.P = len(idx)
+        self.li = torch.tensor([l for l, _ in idx], device=self.dev)
+        self.ci = torch.tensor([c for _, c in idx], device=self.dev)
+        basis = torch.zeros((self.P, self.nl, 6), dtype=_F)
+        for k, (l, c) in enumerate(idx):
+            basis[k, l, c] = 1.0
+        self.basis = basis.to(self.dev)
+
+    def table(self, phys: np.ndarray) -> torch.Tensor:
+        a, b, r, h = _unpack_params(torch.as_tensor(phys, dtype=_F), *self.refs[:4])
+        h = torch.where(h < 1e30, h, torch.full_like(h, float("inf")))
+        tab = torch.stack([a, b, r, h, self.refs[4], self.refs[5]], dim=1)
+        return engine.layers_tensor(tab, self.dev)
+
+    def misfit(self, phys: np.ndarray) -> float:
+        R = engine.forward(self.table(phys), self.p, self.omega, free_surface=self.free_surface)
+        res = R - self.R_obs
+        return float((res.real ** 2 + res.imag ** 2).sum().item())
+
+    def grad_hessian(self, phys: np.ndarray, mode: str = "gram"):
+        layers = self.table(phys)
+        if mode == "hvp":
+            chi, grad, hvp, _ = derivatives.misfit_hvp(layers, self.basis, self.p, self.omega,
+                                                       self.R_obs, self.free_surface)
+            H = hvp.sum(1)[:, self.li, self.ci].T
+        elif mode == "gram":
+            chi, grad, S, dR = derivatives.misfit_hvp(layers, self.basis, self.p, self.omega,
+                                                      self.R_obs, self.free_surface,
+                                                      second_only=True, want_dR=True)
+            X = torch.view_as_real(dR).reshape(self.P, -1)
+            H = engine.gram(X, 2.0) + S.sum(1)[:, self.li, self.ci].T
+        else:
+            raise ValueError(f"unknown hessian mode {mode!r}")
+        g = grad.sum(0)[self.li, self.ci]
+        return float(chi.sum().item()), g.cpu().numpy(), H.cpu().numpy()
+
+
+def _log_transform(m, g, H):
+    """g_log = m g, H_log = m m^T H + diag(g m) (taup_inversion.py:180-185)."""
+    return m * g, np.outer(m, m) * H + np.diag(g * m)
+
+
+def taup_misfit_grad_hessian(log_params, alpha_ref, beta_ref, rho_ref, thickness_ref, Q_alpha,
+                             Q_beta, p_values, omega, R_obs, hessian_mode: str = "gram"):
+    """(chi, g_log, H_log) summed over the slownesses (taup_inversion.py:112-187)."""
+    prob = _Problem(alpha_ref, beta_ref, rho_ref, thickness_ref, Q_alpha, Q_beta, p_values, omega,
+                    R_obs)
+    m = np.exp(np.asarray(log_params, dtype=np.float64))
+    chi, g, H = prob.grad_hessian(m, hessian_mode)
+    g_log, H_log = _log_transform(m, g, H)
+    return chi, g_log, H_log
+
+
+def _eval_misfit_only(log_params, alpha_ref, beta_ref, rho_ref, thickness_ref, Q_alpha, Q_beta,
+                      p_values, omega, R_obs) -> float:
+    """Misfit without derivatives (taup_inversion.py:190-230)."""
+    prob = _Problem(alpha_ref, beta_ref, rho_ref, thickness_ref, Q_alpha, Q_beta, p_values, omega,
+                    R_obs)
+    return prob.misfit(np.exp(np.asarray(log_params, dtype=np.float64)))
+
+
+def newton_lm_step(g: np.ndarray, H: np.ndarray, lam: float) -> np.ndarray:
+    """Solve (H + lam I) dx = -g (taup_inversion.py:238-254)."""
+    return np.linalg.solve(H + lam * np.eye(len(g)), -g)
+
+
+def _to_model(tab_cols, ref_model: LayerModel) -> LayerModel:
+    a, b, r, h = (t.detach().cpu().numpy() for t in tab_cols)
+    h = h.copy()
+    h[h >= 1e29] = np.inf
+    return LayerModel.from_arrays(alpha=a, beta=b, rho=r, thickness=h,
+                                  Q_alpha=ref_model.Q_alpha, Q_beta=ref_model.Q_beta)
+
+
+def invert_taup(true_model: LayerModel, p_values: list[float], nfreq: int = 64,
+                perturbation: float = 0.15, max_iter: int = 100, seed: int = 42,
+                tol: float = 1e-8, hessian_mode: str = "gram",
+                max_trials: int = 10) -> InversionResult:
+    """Newton-LM inversion from a perturbed start model (taup_inversion.py:262-470)."""
+    t = model_to_tensors(true_model)
+    refs = (t["alpha"], t["beta"], t["rho"], t["thickness"], t["Q_alpha"], t["Q_beta"])
+    dw = 2.0 * np.pi / 64.0
+    omega = torch.arange(1, nfreq + 1, dtype=_F) * dw
+    dev = _abi.device()
+    obs = engine.forward(engine.layers_tensor(true_model, dev),
+                         torch.tensor([float(p) for p in p_values], dtype=_F), omega.to(_C))
+    prob = _Problem(*refs, p_values, omega, obs)
+
+    true_params = _pack_params(*refs[:4]).numpy()
+    n_params = len(true_params)
+    n_sub = true_model.n_layers - 1
+    rng = np.random.default_rng(seed)
+    perturb = np.ones(n_params)
+    n_vel = 2 * n_sub
+    perturb[:n_vel] = rng.uniform(1.0 - perturbation, 1.0 + perturbation, n_vel)
+    perturb[n_vel:n_vel + n_sub] = rng.uniform(1.0 - perturbation * 0.67,
+                                               1.0 + perturbation * 0.67, n_sub)
+    perturb[n_vel + n_sub:] = rng.uniform(1.0 - perturbation * 1.33,
+                                          1.0 + perturbation * 1.33, n_sub - 1)
+    init_params = true_params * perturb
+    initial_model = _to_model(_unpack_params(torch.tensor(init_params), *refs[:4]), true_model)
+
+    log_x = np.log(init_params)
+    misfits: list[float] = []
+    gnorms: list[float] = []
+    perrs: list[float] = []
+    lam = None
+    for iteration in range(max_iter):
+        m = np.exp(log_x)
+        chi, g_phys, H_phys = prob.grad_hessian(m, hessian_mode)
+        g, H = _log_transform(m, g_phys, H_phys)
+        grad_norm = float(np.linalg.norm(g))
+        param_err = float(np.linalg.norm(m - true_params) / np.linalg.norm(true_params))
+        misfits.append(chi)
+        gnorms.append(grad_norm)
+        perrs.append(param_err)
+        logger.info("iter %3d  chi=%.4e  ||g||=%.4e  err=%.4e", iteration, chi, grad_norm,
+                    param_err)
+        if lam is None:
+            lam = 0.01 * max(float(np.abs(np.diag(H)).max()), 1e-10)
+        if grad_norm < tol:
+            break
+        if len(misfits) > 1 and abs(misfits[-2] - chi) / max(abs(chi), 1e-30) < tol:
+            break
+        accepted = False
+        for _ in range(max_trials):
+            trial = log_x + newton_lm_step(g, H, lam)
+            if prob.misfit(np.exp(trial)) < chi:
+                log_x = trial
+                lam = max(lam / 2.0, 1e-15)
+                accepted = True
+                break
+            lam *= 5.0
+        if not accepted:
+            logger.warning("Line search failed at iteration %d", iteration)
+            break
+
+    recovered = _to_model(_unpack_params(torch.tensor(np.exp(log_x)), *refs[:4]), true_model)
+    return InversionResult(true_model=true_model, initial_model=initial_model,
+                           recovered_model=recovered, misfit_history=misfits,
+                           grad_norm_history=gnorms, param_error_history=perrs,
+                           n_iterations=len(misfits),
+                           converged=bool(perrs and perrs[-1] < 0.01))
+
+
+def compute_taup_traces(model: LayerModel, p_values: list[float], T: float = 64.0,
+                        nw: int = 1024):
+    """Ricker-convolved tau-p traces (taup_inversion.py:527-571): R * S through
+    the Hermitian inverse FFT, all slownesses in one forward launch and one
+    trace launch."""
+    from .source import ricker_spectrum
+
+    dev = _abi.device()
+    dw = 2.0 * np.pi / T
+    omega = np.arange(1, nw, dtype=np.float64) * dw
+    S = torch.from_numpy(ricker_spectrum(omega, nw * dw)).to(dev)
+    R = engine.forward_grid(engine.layers_tensor(model, dev),
+                            torch.tensor([float(p) for p in p_values], dtype=_F), 1.0, dw, 0.0,
+                            nw - 1)
+    tr = engine.hermitian_traces(R * S[None, :], nw, 0.0, T).cpu().numpy()
+    time = np.arange(2 * nw, dtype=np.float64) * (T / float(2 * nw))
+    return time, {p: tr[i] for i, p in enumerate(p_values)}
