@@ -109,6 +109,38 @@ int kh_forward_grid_ex(const double* layers, int nl, const double* p, int np,
                        double k0, double dw, double gamma, int nw, int free_surface,
                        double* R, int block, int omega_per_item, void* stream);
 
+/* ---- many models in one persistent launch ---- */
+
+/* One model of a kh_forward_multi batch (80 bytes, all offsets in elements). */
+typedef struct {
+  long long layer_off; /* first row of this model in layers_all ((rows, 6) float64) */
+  long long p_off;     /* first slowness of this model in p_all */
+  long long r_off;     /* first complex element of this model's (np, nw) block in R_all */
+  long long item_off;  /* first work item of this model (prefix sum of np * ntiles) */
+  double k0, dw, gamma; /* frequencies omega_j = (k0 + j)*dw + i*gamma, j < nw */
+  int nl, np, nw;
+  int ntiles;          /* ceil(nw / (block * omega_per_item)) for the launch shape */
+  int free_surface;
+  int pad;
+} kh_model_desc;
+
+/* Work items (slowness x frequency tiles) of one model for a launch shape. */
+long long kh_multi_items(int np, int nw, int block, int omega_per_item);
+
+/* Reflectivity of nmodels models in one persistent launch: workgroups take
+ * (model, slowness, frequency tile) items from a device counter until all
+ * total_items are done.  desc (nmodels,) is a DEVICE array sorted by item_off
+ * (list the costliest models first for the best drain order); max_nl = the
+ * largest nl; counter = 8 bytes of device memory (reset by the call);
+ * workgroups <= 0 picks CUs x resident workgroups.  block in {64, 128, 256},
+ * omega_per_item in {1, 2, 4}.  Each model's output equals kh_forward_grid_ex
+ * of that model alone with the same block and omega_per_item, bit for bit.
+ * Replaces a loop of kennett_reflectivity_batch calls, one per model
+ * (Kennett_Reflectivity/kennett_reflectivity_gpu.py:88). */
+int kh_forward_multi(const kh_model_desc* desc, int nmodels, long long total_items, int max_nl,
+                     const double* layers_all, const double* p_all, double* R_all, int block,
+                     int omega_per_item, int workgroups, void* counter, void* stream);
+
 /* ---- gather: discrete-wavenumber sum and inverse FFT to traces ---- */
 
 /* Workspace bytes kh_bessel_sum needs for np slownesses, nw - 1 frequencies

@@ -1,0 +1,130 @@
+"""Many earth models in one persistent launch (kh_forward_multi).
+
+``reflectivity_multi`` takes models of different depths with their own
+slownesses and uniform frequency grids and returns each model's (np, nw)
+complex128 reflectivity.  Models are queued costliest first (interfaces x
+frequencies per work item) so the persistent workgroups drain the long items
+early; results are bitwise identical to one ``kh_forward_grid_ex`` launch
+per model with the same launch shape.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _abi, autotune, engine
+from .layer_model import LayerModel, pack_layers
+
+__all__ = ["ModelSpec", "MultiPlan", "plan_multi", "reflectivity_multi", "DESC_DTYPE"]
+
+# numpy mirror of kh_model_desc (include/kennett_hip.h)
+DESC_DTYPE = np.dtype([("layer_off", "<i8"), ("p_off", "<i8"), ("r_off", "<i8"),
+                       ("item_off", "<i8"), ("k0", "<f8"), ("dw", "<f8"), ("gamma", "<f8"),
+                       ("nl", "<i4"), ("np", "<i4"), ("nw", "<i4"), ("ntiles", "<i4"),
+                       ("free_surface", "<i4"), ("pad", "<i4")])
+assert DESC_DTYPE.itemsize == 80
+
+
+@dataclass
+class ModelSpec:
+    """One forward request: a model, its slownesses and its frequency grid
+    omega_j = (k0 + j)*dw + i*gamma, j < nw."""
+
+    model: LayerModel | np.ndarray
+    p: np.ndarray
+    nw: int
+    dw: float = 2.0 * np.pi / 64.0
+    k0: float = 1.0
+    gamma: float = 0.0
+    free_surface: bool = False
+
+    def table(self) -> np.ndarray:
+        if isinstance(self.model, LayerModel):
+            return pack_layers(self.model)
+        return np.ascontiguousarray(self.model, dtype=np.float64)
+
+    @property
+    def n_layers(self) -> int:
+        return self.table().shape[0]
+
+    def updates(self) -> int:
+        return len(self.p) * self.nw * (self.n_layers - 1)
+
+
+@dataclass
+class MultiPlan:
+    desc: torch.Tensor          # device uint8 view of nmodels kh_model_desc
+    layers: torch.Tensor        # (rows, 6) float64 on the device
+    p: torch.Tensor             # all slownesses
+    R: torch.Tensor             # all outputs, complex128, flat
+    r_off: np.ndarray           # per model (input order)
+    shapes: list[tuple[int, int]]
+    total_items: int
+    max_nl: int
+    block: int
+    wpt: int
+    counter: torch.Tensor
+
+
+def plan_multi(specs: list[ModelSpec], block: int, wpt: int, dev=None) -> MultiPlan:
+    """Pack specs into device buffers and descriptors for one launch shape."""
+    dev = _abi.device(dev)
+    tabs = [s.table() for s in specs]
+    for t in tabs:
+        if t.shape[0] < 2:
+            raise ValueError("every model needs at least 2 layers")
+    span = block * wpt
+    cost = [(len(s.p) * -(-s.nw // span)) * (t.shape[0] - 1) * span for s, t in zip(specs, tabs)]
+    order = sorted(range(len(specs)), key=lambda i: -cost[i])
+    desc = np.zeros(len(specs), DESC_DTYPE)
+    layer_off = np.cumsum([0] + [t.shape[0] for t in tabs])
+    p_off = np.cumsum([0] + [len(s.p) for s in specs])
+    r_off = np.cumsum([0] + [len(s.p) * s.nw for s in specs])
+    item = 0
+    for slot, i in enumerate(order):
+        s = specs[i]
+        ntiles = -(-s.nw // span)
+        desc[slot] = (layer_off[i], p_off[i], r_off[i], item, float(s.k0), float(s.dw),
+                      float(s.gamma), tabs[i].shape[0], len(s.p), s.nw, ntiles,
+                      int(bool(s.free_surface)), 0)
+        item += len(s.p) * ntiles
+    layers = torch.from_numpy(np.concatenate(tabs, axis=0)).to(dev)
+    p = torch.from_numpy(np.concatenate([np.asarray(s.p, np.float64) for s in specs])
+                         if specs else np.zeros(0)).to(dev)
+    R = torch.empty(int(r_off[-1]), dtype=torch.complex128, device=dev)
+    return MultiPlan(desc=torch.from_numpy(desc.view(np.uint8)).to(dev), layers=layers, p=p, R=R,
+                     r_off=r_off, shapes=[(len(s.p), s.nw) for s in specs], total_items=item,
+                     max_nl=max(t.shape[0] for t in tabs) if tabs else 2, block=block, wpt=wpt,
+                     counter=torch.zeros(1, dtype=torch.int64, device=dev))
+
+
+def launch(plan: MultiPlan, workgroups: int = 0) -> torch.Tensor:
+    """Enqueue the persistent launch of a plan on the current stream."""
+    dev = plan.R.device
+    rc = _abi.lib().kh_forward_multi(_abi.ptr(plan.desc), len(plan.shapes), plan.total_items,
+                                     plan.max_nl, _abi.ptr(plan.layers), _abi.ptr(plan.p),
+                                     _abi.ptr(plan.R), plan.block, plan.wpt, int(workgroups),
+                                     _abi.ptr(plan.counter), _abi.stream_handle(dev))
+    _abi.check(rc, "kh_forward_multi")
+    return plan.R
+
+
+def split(plan: MultiPlan) -> list[torch.Tensor]:
+    return [plan.R[int(plan.r_off[i]):int(plan.r_off[i + 1])].view(n_p, nw)
+            for i, (n_p, nw) in enumerate(plan.shapes)]
+
+
+def reflectivity_multi(specs: list[ModelSpec], block: int | None = None,
+                       omega_per_item: int | None = None, dev=None) -> list[torch.Tensor]:
+    """Each spec's R (np, nw) on the device, all from one persistent launch.
+    The launch shape comes from the autotuner unless given."""
+    if not specs:
+        return []
+    if block is None or omega_per_item is None:
+        block, omega_per_item = autotune.multi_shape(specs, dev=dev)
+    plan = plan_multi(specs, block, omega_per_item, dev)
+    launch(plan)
+    return split(plan)

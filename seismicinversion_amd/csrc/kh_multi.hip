@@ -1,0 +1,149 @@
+// Many earth models of different depths, slowness counts and frequency
+// grids in one persistent launch.
+//
+// The work is a list of items (model, slowness, frequency tile).  Items are
+// numbered model by model (callers list the costliest models first, so the
+// queue drains longest-first); each workgroup repeatedly takes the next item
+// number from a device-wide counter (one atomic per item), stages that
+// model's layers and interfaces at that slowness in LDS and sweeps its
+// frequency tile, until the counter passes the item count.  A lane's
+// arithmetic is that of forward_kernel with the same (block, frequencies per
+// item), so every model's output is bitwise identical to a kh_forward_grid_ex
+// launch of that model alone with the same shape.
+#include "kh_physics.hpp"
+#include "kh_runtime.hpp"
+#include "kh_sweep.hpp"
+
+namespace kh {
+
+static_assert(sizeof(kh_model_desc) == 80, "kh_model_desc layout");
+
+__global__ void zero_counter_kernel(unsigned long long* c) { *c = 0ULL; }
+
+__device__ __forceinline__ int find_model(const kh_model_desc* __restrict__ d, int n, long long item) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {  // last model with item_off <= item
+    const int mid = (lo + hi + 1) >> 1;
+    if (d[mid].item_off <= item) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int W>
+__global__ void __launch_bounds__(256) multi_kernel(const kh_model_desc* __restrict__ desc,
+                                                    int nmodels, long long total,
+                                                    const double* __restrict__ layers_all,
+                                                    const double* __restrict__ p_all,
+                                                    cd* __restrict__ R_all, int max_nl,
+                                                    unsigned long long* __restrict__ counter) {
+  extern __shared__ double smem_m[];
+  LayerPt* s_layer = reinterpret_cast<LayerPt*>(smem_m);
+  Iface* s_if = reinterpret_cast<Iface*>(s_layer + max_nl);
+  LayerStep* s_step = reinterpret_cast<LayerStep*>(s_if + (max_nl - 1));
+  __shared__ long long s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = (long long)atomicAdd(counter, 1ULL);
+    __syncthreads();
+    const long long item = s_item;
+    __syncthreads();
+    if (item >= total) break;  // every wave of every workgroup reaches this
+    const kh_model_desc& md = desc[find_model(desc, nmodels, item)];
+    const long long local = item - md.item_off;
+    const int ip = (int)(local / md.ntiles);
+    const int tile = (int)(local - (long long)ip * md.ntiles);
+    const int nl = md.nl, nw = md.nw;
+    stage_slowness(layers_all + md.layer_off * NCOL, nl, p_all[md.p_off + ip], s_layer, s_if);
+    if (W > 1) {
+      const double shift = (double)blockDim.x * md.dw;
+      for (int l = threadIdx.x; l < nl; l += blockDim.x) s_step[l] = layer_step(s_layer[l], shift);
+      __syncthreads();
+    }
+    const int base = tile * blockDim.x * W + threadIdx.x;
+    cd om[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      const int w = min(base + k * (int)blockDim.x, nw - 1);
+      om[k] = cd{(md.k0 + (double)w) * md.dw, md.gamma};
+    }
+    cd out[W];
+    sweep_lanes<W, true>(s_layer, s_if, s_step, nl, om, md.free_surface, out);
+    cd* Rrow = R_all + md.r_off + (long long)ip * nw;
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      const int w = base + k * blockDim.x;
+      if (w < nw) Rrow[w] = out[k];
+    }
+    __syncthreads();  // LDS is restaged by the next item
+  }
+}
+
+static size_t multi_lds_bytes(int max_nl) {
+  return (size_t)max_nl * (sizeof(LayerPt) + sizeof(LayerStep)) +
+         (size_t)(max_nl - 1) * sizeof(Iface);
+}
+
+template <int W>
+static int launch_multi(const kh_model_desc* desc, int nmodels, long long total,
+                        const double* layers_all, const double* p_all, cd* R_all, int block,
+                        int max_nl, unsigned long long* counter, int wgs, hipStream_t st) {
+  hipLaunchKernelGGL((multi_kernel<W>), dim3(wgs), dim3(block), multi_lds_bytes(max_nl), st, desc,
+                     nmodels, total, layers_all, p_all, R_all, max_nl, counter);
+  return check_launch("kh_forward_multi");
+}
+
+}  // namespace kh
+
+using namespace kh;
+
+extern "C" long long kh_multi_items(int np, int nw, int block, int omega_per_item) {
+  if (np <= 0 || nw <= 0 || block <= 0 || omega_per_item <= 0) return 0;
+  const long long span = (long long)block * omega_per_item;
+  return (long long)np * ((nw + span - 1) / span);
+}
+
+extern "C" int kh_forward_multi(const kh_model_desc* desc, int nmodels, long long total_items,
+                                int max_nl, const double* layers_all, const double* p_all,
+                                double* R_all, int block, int omega_per_item, int workgroups,
+                                void* counter, void* stream) {
+  clear_error();
+  KH_REQUIRE(nmodels >= 0 && total_items >= 0, KH_ERR_INVALID_ARGUMENT,
+             "kh_forward_multi: negative size");
+  if (nmodels == 0 || total_items == 0) return KH_OK;
+  KH_REQUIRE(desc && layers_all && p_all && R_all && counter, KH_ERR_INVALID_ARGUMENT,
+             "kh_forward_multi: NULL pointer");
+  KH_REQUIRE(max_nl >= 2 && max_nl <= KH_MAX_LAYERS, KH_ERR_UNSUPPORTED,
+             "kh_forward_multi: max_nl %d outside [2, %d]", max_nl, KH_MAX_LAYERS);
+  KH_REQUIRE(block == 64 || block == 128 || block == 256, KH_ERR_INVALID_ARGUMENT,
+             "kh_forward_multi: block %d not in {64, 128, 256}", block);
+  hipStream_t st = as_stream(stream);
+  int wgs = workgroups;
+  if (wgs <= 0) {  // fill the device: CUs x resident workgroups per CU
+    int dev = 0, cus = 256, per_cu = 1;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const size_t lds = multi_lds_bytes(max_nl);
+    hipError_t e = hipSuccess;
+    switch (omega_per_item) {
+      case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, multi_kernel<1>, block, lds); break;
+      case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, multi_kernel<2>, block, lds); break;
+      case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, multi_kernel<4>, block, lds); break;
+      default: break;
+    }
+    if (e != hipSuccess || per_cu < 1) per_cu = 1;
+    wgs = cus * per_cu;
+  }
+  if ((long long)wgs > total_items) wgs = (int)total_items;
+  auto* c = reinterpret_cast<unsigned long long*>(counter);
+  hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, st, c);
+  KH_TRY(check_launch("kh_forward_multi (counter)"));
+  cd* R = reinterpret_cast<cd*>(R_all);
+  switch (omega_per_item) {
+    case 1: return launch_multi<1>(desc, nmodels, total_items, layers_all, p_all, R, block, max_nl, c, wgs, st);
+    case 2: return launch_multi<2>(desc, nmodels, total_items, layers_all, p_all, R, block, max_nl, c, wgs, st);
+    case 4: return launch_multi<4>(desc, nmodels, total_items, layers_all, p_all, R, block, max_nl, c, wgs, st);
+    default:
+      set_error("kh_forward_multi: omega_per_item %d not in {1, 2, 4}", omega_per_item);
+      return KH_ERR_INVALID_ARGUMENT;
+  }
+}
