@@ -132,6 +132,8 @@ long long kh_multi_items(int np, int nw, int block, int omega_per_item);
  * total_items are done.  desc (nmodels,) is a DEVICE array sorted by item_off
  * (list the costliest models first for the best drain order); max_nl = the
  * largest nl; counter = 8 bytes of device memory (reset by the call);
+ * nonfinite = NULL or a device int array (nmodels,) that receives, for each
+ * desc entry k, the number of NaN/Inf outputs of that model (reset by the call);
  * workgroups <= 0 picks CUs x resident workgroups.  block in {64, 128, 256},
  * omega_per_item in {1, 2, 4}.  Each model's output equals kh_forward_grid_ex
  * of that model alone with the same block and omega_per_item, bit for bit.
@@ -139,7 +141,8 @@ long long kh_multi_items(int np, int nw, int block, int omega_per_item);
  * (Kennett_Reflectivity/kennett_reflectivity_gpu.py:88). */
 int kh_forward_multi(const kh_model_desc* desc, int nmodels, long long total_items, int max_nl,
                      const double* layers_all, const double* p_all, double* R_all, int block,
-                     int omega_per_item, int workgroups, void* counter, void* stream);
+                     int omega_per_item, int workgroups, void* counter, int* nonfinite,
+                     void* stream);
 
 /* ---- gather: discrete-wavenumber sum and inverse FFT to traces ---- */
 

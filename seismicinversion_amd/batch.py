@@ -42,9 +42,15 @@ class ModelSpec:
     free_surface: bool = False
 
     def table(self) -> np.ndarray:
-        if isinstance(self.model, LayerModel):
-            return pack_layers(self.model)
-        return np.ascontiguousarray(self.model, dtype=np.float64)
+        """The validated (nl, 6) layer table (computed once)."""
+        tab = self.__dict__.get("_table")
+        if tab is None:
+            if isinstance(self.model, LayerModel):
+                tab = pack_layers(self.model)
+            else:
+                tab = np.ascontiguousarray(self.model, dtype=np.float64)
+            self.__dict__["_table"] = tab
+        return tab
 
     @property
     def n_layers(self) -> int:
@@ -67,6 +73,8 @@ class MultiPlan:
     block: int
     wpt: int
     counter: torch.Tensor
+    nonfinite: torch.Tensor     # int32 (nmodels,) NaN/Inf output counts, queue order
+    order: list[int]            # queue slot -> input index
 
 
 def plan_multi(specs: list[ModelSpec], block: int, wpt: int, dev=None) -> MultiPlan:
@@ -98,7 +106,9 @@ def plan_multi(specs: list[ModelSpec], block: int, wpt: int, dev=None) -> MultiP
     return MultiPlan(desc=torch.from_numpy(desc.view(np.uint8)).to(dev), layers=layers, p=p, R=R,
                      r_off=r_off, shapes=[(len(s.p), s.nw) for s in specs], total_items=item,
                      max_nl=max(t.shape[0] for t in tabs) if tabs else 2, block=block, wpt=wpt,
-                     counter=torch.zeros(1, dtype=torch.int64, device=dev))
+                     counter=torch.zeros(1, dtype=torch.int64, device=dev),
+                     nonfinite=torch.zeros(max(len(specs), 1), dtype=torch.int32, device=dev),
+                     order=order)
 
 
 def launch(plan: MultiPlan, workgroups: int = 0) -> torch.Tensor:
@@ -107,9 +117,19 @@ def launch(plan: MultiPlan, workgroups: int = 0) -> torch.Tensor:
     rc = _abi.lib().kh_forward_multi(_abi.ptr(plan.desc), len(plan.shapes), plan.total_items,
                                      plan.max_nl, _abi.ptr(plan.layers), _abi.ptr(plan.p),
                                      _abi.ptr(plan.R), plan.block, plan.wpt, int(workgroups),
-                                     _abi.ptr(plan.counter), _abi.stream_handle(dev))
+                                     _abi.ptr(plan.counter), _abi.ptr(plan.nonfinite),
+                                     _abi.stream_handle(dev))
     _abi.check(rc, "kh_forward_multi")
     return plan.R
+
+
+def nonfinite_counts(plan: MultiPlan, counts_by_slot) -> list[int]:
+    """Per input model, the NaN/Inf output count from the plan's queue-order
+    counts (``plan.nonfinite`` after the launch, as a host sequence)."""
+    out = [0] * len(plan.shapes)
+    for slot, i in enumerate(plan.order):
+        out[i] = int(counts_by_slot[slot])
+    return out
 
 
 def split(plan: MultiPlan) -> list[torch.Tensor]:

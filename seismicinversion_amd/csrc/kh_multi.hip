@@ -18,7 +18,17 @@ namespace kh {
 
 static_assert(sizeof(kh_model_desc) == 80, "kh_model_desc layout");
 
-__global__ void zero_counter_kernel(unsigned long long* c) { *c = 0ULL; }
+// Resets the work counter and, when requested, the per-model counts of
+// non-finite outputs.
+__global__ void zero_counter_kernel(unsigned long long* c, int* nonfinite, int nmodels) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *c = 0ULL;
+  if (nonfinite != nullptr && i < nmodels) nonfinite[i] = 0;
+}
+
+__device__ __forceinline__ bool finite_cd(const cd& z) {
+  return z.r == z.r && z.i == z.i && fabs(z.r) != INFINITY && fabs(z.i) != INFINITY;
+}
 
 __device__ __forceinline__ int find_model(const kh_model_desc* __restrict__ d, int n, long long item) {
   int lo = 0, hi = n - 1;
@@ -36,7 +46,8 @@ __global__ void __launch_bounds__(256) multi_kernel(const kh_model_desc* __restr
                                                     const double* __restrict__ layers_all,
                                                     const double* __restrict__ p_all,
                                                     cd* __restrict__ R_all, int max_nl,
-                                                    unsigned long long* __restrict__ counter) {
+                                                    unsigned long long* __restrict__ counter,
+                                                    int* __restrict__ nonfinite) {
   extern __shared__ double smem_m[];
   LayerPt* s_layer = reinterpret_cast<LayerPt*>(smem_m);
   Iface* s_if = reinterpret_cast<Iface*>(s_layer + max_nl);
@@ -48,7 +59,8 @@ __global__ void __launch_bounds__(256) multi_kernel(const kh_model_desc* __restr
     const long long item = s_item;
     __syncthreads();
     if (item >= total) break;  // every wave of every workgroup reaches this
-    const kh_model_desc& md = desc[find_model(desc, nmodels, item)];
+    const int mi = find_model(desc, nmodels, item);
+    const kh_model_desc& md = desc[mi];
     const long long local = item - md.item_off;
     const int ip = (int)(local / md.ntiles);
     const int tile = (int)(local - (long long)ip * md.ntiles);
@@ -69,11 +81,16 @@ __global__ void __launch_bounds__(256) multi_kernel(const kh_model_desc* __restr
     cd out[W];
     sweep_lanes<W, true>(s_layer, s_if, s_step, nl, om, md.free_surface, out);
     cd* Rrow = R_all + md.r_off + (long long)ip * nw;
+    int bad = 0;
 #pragma unroll
     for (int k = 0; k < W; ++k) {
       const int w = base + k * blockDim.x;
-      if (w < nw) Rrow[w] = out[k];
+      if (w < nw) {
+        Rrow[w] = out[k];
+        bad += finite_cd(out[k]) ? 0 : 1;
+      }
     }
+    if (nonfinite != nullptr && bad != 0) atomicAdd(nonfinite + mi, bad);
     __syncthreads();  // LDS is restaged by the next item
   }
 }
@@ -86,9 +103,10 @@ static size_t multi_lds_bytes(int max_nl) {
 template <int W>
 static int launch_multi(const kh_model_desc* desc, int nmodels, long long total,
                         const double* layers_all, const double* p_all, cd* R_all, int block,
-                        int max_nl, unsigned long long* counter, int wgs, hipStream_t st) {
+                        int max_nl, unsigned long long* counter, int* nonfinite, int wgs,
+                        hipStream_t st) {
   hipLaunchKernelGGL((multi_kernel<W>), dim3(wgs), dim3(block), multi_lds_bytes(max_nl), st, desc,
-                     nmodels, total, layers_all, p_all, R_all, max_nl, counter);
+                     nmodels, total, layers_all, p_all, R_all, max_nl, counter, nonfinite);
   return check_launch("kh_forward_multi");
 }
 
@@ -105,7 +123,7 @@ extern "C" long long kh_multi_items(int np, int nw, int block, int omega_per_ite
 extern "C" int kh_forward_multi(const kh_model_desc* desc, int nmodels, long long total_items,
                                 int max_nl, const double* layers_all, const double* p_all,
                                 double* R_all, int block, int omega_per_item, int workgroups,
-                                void* counter, void* stream) {
+                                void* counter, int* nonfinite, void* stream) {
   clear_error();
   KH_REQUIRE(nmodels >= 0 && total_items >= 0, KH_ERR_INVALID_ARGUMENT,
              "kh_forward_multi: negative size");
@@ -135,13 +153,14 @@ extern "C" int kh_forward_multi(const kh_model_desc* desc, int nmodels, long lon
   }
   if ((long long)wgs > total_items) wgs = (int)total_items;
   auto* c = reinterpret_cast<unsigned long long*>(counter);
-  hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, st, c);
+  const int zero_blocks = nonfinite != nullptr ? (nmodels + 255) / 256 : 1;
+  hipLaunchKernelGGL(zero_counter_kernel, dim3(zero_blocks), dim3(256), 0, st, c, nonfinite, nmodels);
   KH_TRY(check_launch("kh_forward_multi (counter)"));
   cd* R = reinterpret_cast<cd*>(R_all);
   switch (omega_per_item) {
-    case 1: return launch_multi<1>(desc, nmodels, total_items, layers_all, p_all, R, block, max_nl, c, wgs, st);
-    case 2: return launch_multi<2>(desc, nmodels, total_items, layers_all, p_all, R, block, max_nl, c, wgs, st);
-    case 4: return launch_multi<4>(desc, nmodels, total_items, layers_all, p_all, R, block, max_nl, c, wgs, st);
+    case 1: return launch_multi<1>(desc, nmodels, total_items, layers_all, p_all, R, block, max_nl, c, nonfinite, wgs, st);
+    case 2: return launch_multi<2>(desc, nmodels, total_items, layers_all, p_all, R, block, max_nl, c, nonfinite, wgs, st);
+    case 4: return launch_multi<4>(desc, nmodels, total_items, layers_all, p_all, R, block, max_nl, c, nonfinite, wgs, st);
     default:
       set_error("kh_forward_multi: omega_per_item %d not in {1, 2, 4}", omega_per_item);
       return KH_ERR_INVALID_ARGUMENT;

@@ -1,0 +1,474 @@
+"""Continuous-batching reflectivity service.
+
+Requests (forward models and small Levenberg-Marquardt inversions, each
+with its own layer count, slownesses and frequency grid) arrive from any
+thread through ``submit_forward`` / ``submit_invert`` and get a
+``concurrent.futures.Future``.  One scheduler thread runs the loop:
+
+1. admission by device memory: a request is admitted when its HBM footprint
+   (layer table, slownesses, output, and for inversions the derivative
+   workspace) fits in what the budget has free; requests larger than the
+   whole budget fail with MemoryError, the rest wait in arrival order;
+2. batching: every tick, all admitted forward requests not yet launched go
+   into ONE persistent kh_forward_multi launch (requests that arrived while
+   earlier batches run join the next launch), enqueued on the next of
+   several default-priority HIP streams together with its host->device and
+   device->host copies, so consecutive batches overlap;
+3. completion: a second thread waits for each batch (HIP event), copies its
+   outputs from pinned staging into per-request host arrays on a pool of
+   copy threads, resolves the futures and returns the memory and the stream
+   slot, while the scheduler keeps forming and launching batches;
+4. inversions run their LM iterations on their own stream, concurrently.
+
+Each request keeps a record (arrival, admission, launch, completion times,
+updates, status) from which ``stats()`` reports p50/p99 latency and
+updates/s.  Results do not depend on the batch a request lands in: the
+persistent kernel's per-lane arithmetic is fixed by the launch shape, which
+the server holds constant.
+
+The device work goes through a backend object so that the scheduling logic
+can be exercised without a GPU (the tests substitute a host backend).
+"""
+
+from __future__ import annotations
+
+import queue
+import threading
+import time
+from concurrent.futures import Future, ThreadPoolExecutor
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .batch import ModelSpec
+
+__all__ = ["InvertSpec", "Request", "KennettServer", "DeviceBackend", "percentile_stats"]
+
+
+@dataclass
+class InvertSpec:
+    """A small LM inversion: recover `true_model` (sub-ocean Vp, Vs, rho, h)
+    from its own reflectivity at slownesses p and nfreq frequencies, starting
+    from a perturbed model, for `iterations` Newton-LM steps."""
+
+    true_model: object
+    p: np.ndarray
+    nfreq: int = 64
+    perturbation: float = 0.05
+    iterations: int = 5
+    seed: int = 42
+    hessian_mode: str = "hvp"
+    max_trials: int = 10
+
+    @property
+    def n_layers(self) -> int:
+        return len(self.true_model.alpha)
+
+    def updates(self) -> int:
+        return len(self.p) * self.nfreq * (self.n_layers - 1)
+
+
+@dataclass
+class Request:
+    kind: str
+    spec: object
+    future: Future
+    arrival: float
+    deadline: float | None = None
+    mem: int = 0
+    record: dict = field(default_factory=dict)
+
+
+def percentile_stats(records, kind="forward", n_layers=None) -> dict:
+    lat = [r["end"] - r["arrival"] for r in records
+           if r.get("status") == "ok" and r["kind"] == kind and
+           (n_layers is None or r["n_layers"] == n_layers)]
+    if not lat:
+        return {"count": 0}
+    lat = np.asarray(lat) * 1e3
+    return {"count": int(lat.size), "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean())}
+
+
+class DeviceBackend:
+    """HIP backend: persistent multi-model launches and device LM inversions.
+
+    Each of the ``n_streams`` stream slots owns a pinned host staging buffer
+    of ``staging_bytes``.  A batch's outputs reach it with one device->host
+    copy on the batch's stream (pinned memory copies at the link rate;
+    pageable memory would be staged through the runtime at a fraction of
+    it), and ``results`` then copies them into one host array per request on
+    ``copy_threads`` threads while later batches run on the other slots.
+    The server caps batches at ``max_batch_bytes`` (the staging size); a
+    single request larger than that gets a one-off pinned buffer."""
+
+    def __init__(self, n_streams: int = 3, shape=(256, 4), device=None,
+                 staging_bytes: int = 512 << 20, copy_threads: int = 8):
+        import torch
+
+        from . import _abi
+
+        self.torch = torch
+        self.dev = _abi.device(device)
+        self.streams = [torch.cuda.Stream(device=self.dev) for _ in range(n_streams)]
+        self.lm_stream = torch.cuda.Stream(device=self.dev)
+        self.slots = n_streams
+        self.shape = shape
+        self.max_batch_bytes = int(staging_bytes)
+        self.staging = [torch.empty(self.max_batch_bytes, dtype=torch.uint8, pin_memory=True)
+                        for _ in range(n_streams)]
+        self._free = list(range(n_streams))
+        self._lock = threading.Lock()
+        self.pool = ThreadPoolExecutor(max_workers=copy_threads,
+                                       thread_name_prefix="kennett-copy")
+
+    def free_bytes(self) -> int:
+        free, _ = self.torch.cuda.mem_get_info(self.dev)
+        return int(free)
+
+    @staticmethod
+    def forward_bytes(spec: ModelSpec) -> int:
+        return 16 * len(spec.p) * spec.nw + 8 * len(spec.p) + 48 * spec.n_layers + 84
+
+    @staticmethod
+    def invert_bytes(spec: InvertSpec) -> int:
+        from . import derivatives
+
+        nl, n_p, nw = spec.n_layers, len(spec.p), spec.nfreq
+        return min(derivatives.WORKSPACE_LIMIT, 1 << 34) + 64 * n_p * nw * 4 * nl
+
+    def launch_forward(self, specs: list[ModelSpec]):
+        """Enqueue one persistent launch for the batch on a free stream slot,
+        then the device->host copies of its outputs and NaN/Inf counts."""
+        from . import batch
+
+        torch = self.torch
+        with self._lock:
+            if not self._free:
+                raise RuntimeError("no free stream slot: more batches in flight than streams")
+            slot = self._free.pop()
+        s = self.streams[slot]
+        try:
+            nbytes = 16 * sum(len(x.p) * x.nw for x in specs)
+            with torch.cuda.stream(s):
+                plan = batch.plan_multi(specs, *self.shape, dev=self.dev)
+                batch.launch(plan)
+                if nbytes <= self.max_batch_bytes:
+                    host = self.staging[slot][:nbytes].view(torch.complex128)
+                else:
+                    host = torch.empty(plan.R.numel(), dtype=torch.complex128, pin_memory=True)
+                host.copy_(plan.R, non_blocking=True)
+                counts = torch.empty(len(specs), dtype=torch.int32, pin_memory=True)
+                counts.copy_(plan.nonfinite[:len(specs)], non_blocking=True)
+                ev = torch.cuda.Event()
+                ev.record(s)
+        except BaseException:
+            self._release_slot(slot)
+            raise
+        return {"plan": plan, "host": host, "counts": counts, "event": ev, "slot": slot}
+
+    def _release_slot(self, slot: int) -> None:
+        with self._lock:
+            self._free.append(slot)
+
+    def done(self, handle) -> bool:
+        return handle["event"].query()
+
+    def results(self, handle):
+        """(per-request host arrays, per-request NaN/Inf counts); frees the slot."""
+        from . import batch
+
+        try:
+            handle["event"].synchronize()
+            plan = handle["plan"]
+            flat = handle["host"].numpy()
+            outs = [np.empty(shape, np.complex128) for shape in plan.shapes]
+            jobs = []
+            chunk = 1 << 19  # complex elements (8 MiB) per copy job
+            for i, (n_p, nw) in enumerate(plan.shapes):
+                off = int(plan.r_off[i])
+                rows = max(1, chunk // nw)
+                for r0 in range(0, n_p, rows):
+                    jobs.append((i, off, nw, r0, min(n_p, r0 + rows)))
+
+            def copy(job):
+                i, off, nw, r0, r1 = job
+                np.copyto(outs[i][r0:r1], flat[off + r0 * nw:off + r1 * nw].reshape(r1 - r0, nw))
+
+            for _ in self.pool.map(copy, jobs):
+                pass
+            counts = batch.nonfinite_counts(plan, handle["counts"].numpy())
+        finally:
+            self._release_slot(handle["slot"])
+        return outs, counts
+
+    def invert(self, spec: InvertSpec):
+        from .taup_inversion import invert_taup
+
+        with self.torch.cuda.stream(self.lm_stream):
+            res = invert_taup(spec.true_model, list(spec.p), nfreq=spec.nfreq,
+                              perturbation=spec.perturbation, max_iter=spec.iterations,
+                              seed=spec.seed, tol=0.0, hessian_mode=spec.hessian_mode,
+                              max_trials=spec.max_trials)
+            self.lm_stream.synchronize()
+        return res
+
+
+class KennettServer:
+    def __init__(self, backend=None, hbm_budget: int | None = None, max_batch_updates: float = 1e9,
+                 max_lm_concurrency: int = 2, idle_sleep: float = 2e-4,
+                 max_inflight: int | None = None, max_batch_bytes: int | None = None):
+        self.backend = backend if backend is not None else DeviceBackend()
+        # batches in flight at once (one per stream): further arrivals wait for
+        # a free slot and then launch together
+        self.max_inflight = max_inflight or getattr(self.backend, "slots", 2)
+        if hbm_budget is None:
+            hbm_budget = int(0.8 * self.backend.free_bytes())
+        self.budget = int(hbm_budget)
+        self.in_use = 0
+        self.max_batch_updates = max_batch_updates
+        self.max_batch_bytes = (max_batch_bytes if max_batch_bytes is not None
+                                else getattr(self.backend, "max_batch_bytes", None))
+        self.incoming: queue.Queue = queue.Queue()
+        self.waiting: list[Request] = []
+        self.completions: queue.Queue = queue.Queue()
+        self.n_inflight = 0
+        self.outstanding = 0  # submitted and not yet finished
+        self.records: list[dict] = []
+        self.lock = threading.Lock()
+        self.lm_threads: list[threading.Thread] = []
+        self.lm_slots = threading.Semaphore(max_lm_concurrency)
+        self.idle_sleep = idle_sleep
+        self._stop = threading.Event()
+        self._thread: threading.Thread | None = None
+        self._completer: threading.Thread | None = None
+        self.batches = 0
+
+    # ------------------------------------------------------------------ client API
+    def submit_forward(self, spec: ModelSpec, deadline_s: float | None = None) -> Future:
+        return self._submit("forward", spec, deadline_s)
+
+    def submit_invert(self, spec: InvertSpec, deadline_s: float | None = None) -> Future:
+        return self._submit("invert", spec, deadline_s)
+
+    def _submit(self, kind, spec, deadline_s) -> Future:
+        fut: Future = Future()
+        now = time.perf_counter()
+        req = Request(kind=kind, spec=spec, future=fut, arrival=now,
+                      deadline=None if deadline_s is None else now + deadline_s)
+        req.record = {"kind": kind, "arrival": now, "n_layers": 0, "updates": 0,
+                      "status": "queued"}
+        try:
+            req.record["n_layers"] = spec.n_layers
+            req.record["updates"] = spec.updates()
+        except ValueError as e:  # unusable model: this request fails, the service goes on
+            self._finish(req, error=e)
+            return fut
+        with self.lock:
+            self.outstanding += 1
+        req.record["counted"] = True
+        self.incoming.put(req)
+        return fut
+
+    def start(self) -> "KennettServer":
+        self._stop.clear()
+        self._thread = threading.Thread(target=self._loop, name="kennett-scheduler", daemon=True)
+        self._completer = threading.Thread(target=self._complete_loop, name="kennett-completion",
+                                           daemon=True)
+        self._thread.start()
+        self._completer.start()
+        return self
+
+    def stop(self, drain: bool = True) -> None:
+        if drain:
+            while True:
+                with self.lock:
+                    if self.outstanding == 0:
+                        break
+                time.sleep(1e-3)
+        self._stop.set()
+        if self._thread is not None:
+            self._thread.join()
+        if self._completer is not None:
+            self.completions.put(None)
+            self._completer.join()
+        for t in self.lm_threads:
+            t.join()
+
+    def __enter__(self):
+        return self.start()
+
+    def __exit__(self, *exc):
+        self.stop()
+
+    def stats(self) -> dict:
+        with self.lock:
+            recs = list(self.records)
+        done = [r for r in recs if r.get("status") == "ok"]
+        fwd = [r for r in done if r["kind"] == "forward"]
+        span = (max(r["end"] for r in done) - min(r["arrival"] for r in done)) if done else 0.0
+        return {
+            "completed": len(done),
+            "failed": sum(1 for r in recs if r.get("status") not in ("ok", "queued")),
+            "batches": self.batches,
+            "forward_updates": int(sum(r["updates"] for r in fwd)),
+            "forward_updates_per_s": (sum(r["updates"] for r in fwd) / span) if span else 0.0,
+            "latency_forward": percentile_stats(recs, "forward"),
+            "latency_forward_20_layers": percentile_stats(recs, "forward", 20),
+            "latency_invert": percentile_stats(recs, "invert"),
+        }
+
+    # ------------------------------------------------------------------ scheduler
+    def _finish(self, req: Request, result=None, error: BaseException | None = None,
+                nonfinite: int | None = None):
+        now = time.perf_counter()
+        req.record["end"] = now
+        if error is not None:
+            req.record["status"] = type(error).__name__
+            req.future.set_exception(error)
+        else:
+            req.record["status"] = "ok"
+            if nonfinite is not None:
+                req.record["nonfinite"] = int(nonfinite)
+            elif isinstance(result, np.ndarray):
+                req.record["nonfinite"] = int(np.count_nonzero(~np.isfinite(result)))
+            req.future.set_result(result)
+        with self.lock:
+            self.records.append(req.record)
+            if req.record.pop("counted", False):
+                self.outstanding -= 1
+
+    def _mem_of(self, req: Request) -> int:
+        if req.kind == "forward":
+            return self.backend.forward_bytes(req.spec)
+        return self.backend.invert_bytes(req.spec)
+
+    def _validate(self, req: Request):
+        spec = req.spec
+        model = spec.model if req.kind == "forward" else spec.true_model
+        if not isinstance(model, np.ndarray):
+            from .layer_model import pack_layers
+
+            pack_layers(model)  # raises ValueError for an unusable model
+        if req.kind == "forward" and (spec.nw <= 0 or len(spec.p) == 0):
+            raise ValueError("forward request needs at least one slowness and one frequency")
+
+    def _admit(self) -> list[Request]:
+        while True:
+            try:
+                req = self.incoming.get_nowait()
+            except queue.Empty:
+                break
+            try:
+                self._validate(req)
+                req.mem = self._mem_of(req)
+                if req.mem > self.budget:
+                    raise MemoryError(f"request needs {req.mem} bytes, budget is {self.budget}")
+            except (ValueError, MemoryError) as e:
+                self._finish(req, error=e)
+                continue
+            self.waiting.append(req)
+        admitted, keep = [], []
+        now = time.perf_counter()
+        for req in self.waiting:
+            if req.deadline is not None and now > req.deadline:
+                self._finish(req, error=TimeoutError("deadline passed before admission"))
+                continue
+            # arrival order is kept: once one request waits for memory, later ones wait too
+            if not keep and self._reserve(req.mem):
+                req.record["admit"] = now
+                admitted.append(req)
+            else:
+                keep.append(req)
+        self.waiting = keep
+        return admitted
+
+    def _reserve(self, nbytes: int) -> bool:
+        with self.lock:
+            if self.in_use + nbytes > self.budget:
+                return False
+            self.in_use += nbytes
+            return True
+
+    def _release(self, nbytes: int) -> None:
+        with self.lock:
+            self.in_use -= nbytes
+
+    def _loop(self):
+        pending_fwd: list[Request] = []
+        while not self._stop.is_set():
+            progressed = False
+            for req in self._admit():
+                progressed = True
+                if req.kind == "forward":
+                    pending_fwd.append(req)
+                else:
+                    t = threading.Thread(target=self._run_invert, args=(req,), daemon=True)
+                    self.lm_threads.append(t)
+                    t.start()
+            if pending_fwd and self.n_inflight < self.max_inflight:
+                batch, total, nbytes = [], 0.0, 0
+                cap = self.max_batch_bytes
+                while pending_fwd:
+                    nxt = pending_fwd[0].spec
+                    b = 16 * len(nxt.p) * nxt.nw
+                    if batch and (total + nxt.updates() > self.max_batch_updates or
+                                  (cap is not None and nbytes + b > cap)):
+                        break
+                    r = pending_fwd.pop(0)
+                    batch.append(r)
+                    total += nxt.updates()
+                    nbytes += b
+                launched = time.perf_counter()
+                try:
+                    handle = self.backend.launch_forward([r.spec for r in batch])
+                except Exception as e:  # launch refused: fail the batch, free its memory
+                    for r in batch:
+                        self._release(r.mem)
+                        self._finish(r, error=e)
+                else:
+                    for r in batch:
+                        r.record["launch"] = launched
+                        r.record["batch"] = self.batches
+                    self.batches += 1
+                    with self.lock:
+                        self.n_inflight += 1
+                    self.completions.put((handle, batch))
+                progressed = True
+            self.lm_threads = [t for t in self.lm_threads if t.is_alive()]
+            if not progressed:
+                time.sleep(self.idle_sleep)
+
+    def _complete_loop(self):
+        """Completion thread: waits for each launched batch in launch order,
+        hands every request its host result and returns the batch's memory and
+        stream slot, so the scheduler keeps launching meanwhile."""
+        while True:
+            item = self.completions.get()
+            if item is None:
+                return
+            handle, batch = item
+            try:
+                outs, counts = self.backend.results(handle)
+            except Exception as e:
+                for r in batch:
+                    self._release(r.mem)
+                    self._finish(r, error=e)
+            else:
+                for i, (r, out) in enumerate(zip(batch, outs)):
+                    self._release(r.mem)
+                    self._finish(r, result=out, nonfinite=None if counts is None else counts[i])
+            with self.lock:
+                self.n_inflight -= 1
+
+    def _run_invert(self, req: Request):
+        with self.lm_slots:
+            req.record["launch"] = time.perf_counter()
+            try:
+                res = self.backend.invert(req.spec)
+            except Exception as e:
+                self._release(req.mem)
+                self._finish(req, error=e)
+                return
+            self._release(req.mem)
+            self._finish(req, result=res)

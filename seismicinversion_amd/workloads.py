@@ -137,3 +137,30 @@ def uneven_batch_models(n_models: int = 256, seed: int = 42,
         (seeded_model(int(nl), seed + i), uniform_slownesses(int(npp), p_max))
         for i, (nl, npp) in enumerate(zip(nls, nps))
     ]
+
+
+@dataclass
+class StreamItem:
+    """One request of the serving stream: arrival offset (s), kind and sizes."""
+
+    t: float
+    kind: str
+    n_layers: int
+    n_p: int
+    n_omega: int
+    seed: int
+
+
+def serving_stream(rate: float, n_requests: int, seed: int = 42, lm_fraction: float = 0.2,
+                   nl_range=(3, 100), np_range=(16, 2048), omegas=(256, 1024, 4096)):
+    """Poisson arrivals at `rate` per second: a fraction `lm_fraction` of
+    inversion requests, the rest forward requests; layer and slowness counts
+    log-uniform over their ranges, frequency counts uniform over `omegas`."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n_requests)) if rate > 0 else np.zeros(n_requests)
+    kinds = np.where(rng.random(n_requests) < lm_fraction, "invert", "forward")
+    nls = np.rint(np.exp(rng.uniform(np.log(nl_range[0]), np.log(nl_range[1]), n_requests)))
+    nps = np.rint(np.exp(rng.uniform(np.log(np_range[0]), np.log(np_range[1]), n_requests)))
+    nws = rng.choice(np.asarray(omegas), n_requests)
+    return [StreamItem(float(t[i]), str(kinds[i]), int(nls[i]), int(nps[i]), int(nws[i]),
+                       seed + 1000 + i) for i in range(n_requests)]

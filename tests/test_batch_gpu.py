@@ -61,3 +61,20 @@ def test_full_uneven_batch_and_autotuner():
                                  s.gamma, s.nw, False, block=shape[0], omega_per_item=shape[1])
         assert torch.equal(outs[i], Rp)
     assert all(torch.isfinite(torch.view_as_real(R)).all() for R in outs)
+
+
+def test_nonfinite_counts_per_model():
+    """A NaN slowness makes its whole row NaN; the launch counts the non-finite
+    outputs of each model and the others stay at zero."""
+    specs = specs_c3(6, nw=300)
+    specs[2].p = specs[2].p.copy()
+    specs[2].p[1] = np.nan
+    plan = batch.plan_multi(specs, 256, 2)
+    batch.launch(plan)
+    counts = batch.nonfinite_counts(plan, plan.nonfinite.cpu().numpy())
+    assert counts == [300 if i == 2 else 0 for i in range(6)]
+    R = batch.split(plan)[2]
+    assert not torch.isfinite(torch.view_as_real(R[1])).any()
+    # a second launch of the same plan resets the counts before counting again
+    batch.launch(plan)
+    assert batch.nonfinite_counts(plan, plan.nonfinite.cpu().numpy())[2] == 300

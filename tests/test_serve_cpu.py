@@ -1,0 +1,153 @@
+"""Scheduling logic of the continuous-batching server on the CPU: a host
+backend computes forward requests with the oracle and counts launches, so
+admission by memory, batching, deadlines and per-request failures are
+exercised without a GPU."""
+
+from __future__ import annotations
+
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from oracle import kennett_numpy as K
+from seismicinversion_amd.batch import ModelSpec
+from seismicinversion_amd.layer_model import LayerModel
+from seismicinversion_amd.serve import InvertSpec, KennettServer
+from seismicinversion_amd.workloads import seeded_model
+
+
+class HostBackend:
+    """Runs each batch on a worker thread with the NumPy oracle; a batch
+    becomes 'done' when its thread ends (like an event on a stream)."""
+
+    slots = 1
+
+    def __init__(self, delay=0.0, poison=None):
+        self.launches = []
+        self.delay = delay
+        self.poison = poison or set()
+
+    def free_bytes(self):
+        return 1 << 30
+
+    @staticmethod
+    def forward_bytes(spec):
+        return 16 * len(spec.p) * spec.nw
+
+    @staticmethod
+    def invert_bytes(spec):
+        return 1000
+
+    def launch_forward(self, specs):
+        out = {}
+
+        def run():
+            time.sleep(self.delay)
+            res = []
+            for s in specs:
+                w = (s.k0 + np.arange(s.nw)) * s.dw + 1j * s.gamma
+                R = K.reflectivity_batch(s.model, s.p, w, s.free_surface)
+                if id(s) in self.poison:
+                    R[0, 0] = np.nan
+                res.append(R)
+            out["res"] = res
+
+        t = threading.Thread(target=run)
+        t.start()
+        self.launches.append(len(specs))
+        return {"t": t, "out": out}
+
+    def done(self, h):
+        return not h["t"].is_alive()
+
+    def results(self, h):
+        h["t"].join()
+        return h["out"]["res"], None
+
+    def invert(self, spec):
+        return {"iterations": spec.iterations}
+
+
+def fwd(nl, n_p, nw, seed=1, fs=False):
+    return ModelSpec(model=seeded_model(nl, seed), p=np.linspace(0.05, 0.6, n_p), nw=nw,
+                     free_surface=fs)
+
+
+def test_requests_are_batched_and_results_match_alone():
+    be = HostBackend(delay=0.05)
+    specs = [fwd(3 + i % 7, 4 + i % 5, 16 + i, seed=i, fs=bool(i % 2)) for i in range(24)]
+    with KennettServer(backend=be, hbm_budget=1 << 30) as srv:
+        futs = []
+        for s in specs:
+            futs.append(srv.submit_forward(s))
+            time.sleep(0.002)
+        outs = [f.result(timeout=60) for f in futs]
+    assert len(be.launches) < len(specs), "requests were not batched"
+    assert sum(be.launches) == len(specs)
+    for s, R in zip(specs, outs):
+        w = (s.k0 + np.arange(s.nw)) * s.dw
+        np.testing.assert_array_equal(R, K.reflectivity_batch(s.model, s.p, w, s.free_surface))
+    st = srv.stats()
+    assert st["completed"] == len(specs) and st["latency_forward"]["count"] == len(specs)
+    assert st["latency_forward"]["p99_ms"] >= st["latency_forward"]["p50_ms"] > 0
+
+
+def test_memory_budget_queues_and_rejects():
+    be = HostBackend(delay=0.02)
+    small = [fwd(4, 8, 32, seed=i) for i in range(6)]
+    need = HostBackend.forward_bytes(small[0])
+    with KennettServer(backend=be, hbm_budget=2 * need) as srv:
+        futs = [srv.submit_forward(s) for s in small]
+        big = srv.submit_forward(fwd(4, 64, 512))
+        with pytest.raises(MemoryError):
+            big.result(timeout=30)
+        for f in futs:
+            f.result(timeout=60)
+        assert srv.in_use <= 2 * need
+    assert max(be.launches) <= 2, "admission exceeded the memory budget"
+
+
+def test_invalid_model_fails_alone_and_others_proceed():
+    be = HostBackend()
+    bad = ModelSpec(model=LayerModel.from_arrays(alpha=[1.5], beta=[0.0], rho=[1.0],
+                                                 thickness=[np.inf], Q_alpha=[1e4],
+                                                 Q_beta=[1e4]), p=np.array([0.1]), nw=8)
+    with KennettServer(backend=be, hbm_budget=1 << 30) as srv:
+        f_bad = srv.submit_forward(bad)
+        f_ok = srv.submit_forward(fwd(5, 3, 8))
+        with pytest.raises(ValueError):
+            f_bad.result(timeout=30)
+        assert f_ok.result(timeout=30).shape == (3, 8)
+    assert srv.stats()["failed"] == 1
+
+
+def test_nonfinite_flag_and_deadline():
+    spec = fwd(5, 3, 8)
+    be = HostBackend(poison={id(spec)})
+    with KennettServer(backend=be, hbm_budget=1 << 30) as srv:
+        srv.submit_forward(spec).result(timeout=30)
+    assert srv.records[-1]["nonfinite"] == 1
+    # a request that cannot be admitted before its deadline is cancelled
+    be = HostBackend(delay=0.3)
+    first = fwd(4, 8, 32)
+    need = HostBackend.forward_bytes(first)
+    with KennettServer(backend=be, hbm_budget=need) as srv:
+        f1 = srv.submit_forward(first)
+        time.sleep(0.05)
+        f2 = srv.submit_forward(fwd(4, 8, 32, seed=3), deadline_s=0.05)
+        with pytest.raises(TimeoutError):
+            f2.result(timeout=30)
+        f1.result(timeout=30)
+
+
+def test_inversions_run_beside_forward_batches():
+    be = HostBackend()
+    with KennettServer(backend=be, hbm_budget=1 << 30) as srv:
+        fi = srv.submit_invert(InvertSpec(true_model=seeded_model(4, 2), p=np.array([0.1, 0.2]),
+                                          nfreq=16, iterations=3))
+        ff = srv.submit_forward(fwd(4, 2, 8))
+        assert fi.result(timeout=30) == {"iterations": 3}
+        ff.result(timeout=30)
+    assert srv.stats()["latency_invert"]["count"] == 1
