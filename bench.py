@@ -1,19 +1,34 @@
-"""Benchmark: Kennett P-SV reflectivity throughput on one MI355X.
+"""Benchmark: Kennett P-SV reflectivity served vs plain on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-Workload (one "step"): the forward reflectivity of the gather workload —
-20-layer seeded model, free surface on, 8192 slownesses uniform in (0, 1.2]
-s/km, 1023 damped frequencies omega_k = k*2*pi/64 + i*pi/64 (nt = 2048,
-T = 64 s) — with the layer table, slownesses and frequencies resident in HBM.
+Headline workload (one "step"): N = 400 forward requests drawn from the
+mixed-size serving distribution (seed 42: layer counts round(exp(U[ln 3,
+ln 100])), slownesses round(exp(U[ln 16, ln 2048])) uniform in (0, 0.8] s/km,
+frequency counts uniform over {256, 1024, 4096}, omega_k = k*2*pi/64),
+submitted at once to the continuous-batching server and served with
+device-resident results.  ``value`` = p*omega*layer updates per second
+(one update = one interface step of one (p, omega) lane).
 
-Metric: p*omega*layer updates per second, one update = one interface step of
-one (p, omega) lane (8192 * 1023 * 19 = 1.59e8 per step).
+Beside it, on the same requests:
+* plain: the one-model-per-launch synchronous GPU path (one kh_forward_grid
+  launch per request, synchronised), with device results and, through the
+  kennett_reflectivity_batch drop-in, with host results;
+* served with host results (pinned staging + copy-out);
+* an open-loop run: Poisson arrivals at --rate req/s for --open-seconds,
+  20 % of them 5-iteration LM inversions (slownesses <= 32, frequencies <= 128),
+  results to the host; p50/p99 latency from arrival to result for all forward
+  requests, for 20-layer forward requests and for inversions;
+* kernels: the gather workload's forward kernel (20 layers, 8192 p x 1023
+  damped omega, free surface), the full gather (forward + J0 wavenumber sum +
+  inverse FFT, 100 offsets) and the 256-model uneven batch (persistent
+  launch vs one launch per model).
 
-Prints one JSON line with the throughput, a roofline entry for the dominant
-kernel (FP64 algorithmic FLOPs = 340 per update, timed with HIP events on the
-launch stream) and a CPU baseline (the NumPy oracle, one core, on a bounded
-sample of the same slowness grid).
+``roofline``: the persistent multi-model kernel of the served steps, FP64
+algorithmic FLOPs (340 per update) per launch over the average launch
+duration (HIP events on the server's compute stream, where the batch
+kernels run back to back).  ``cpu_baseline``: the NumPy
+oracle, one process, on the middle slowness row of every request.
 """
 
 from __future__ import annotations
@@ -30,126 +45,251 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 FLOP_PER_UPDATE = 340.0          # 41 cmul (6) + 25 cadd (2) + 4 cdiv (11)
-PEAK_FP64_TFLOPS = 78.6          # MI355X FP64 (vector = matrix)
+PEAK_FP64_TFLOPS = 78.6          # MI355X FP64 vector
 PEAK_HBM_GBS = 8000.0
+UNIT = "p*omega*layer updates/s"
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-sample-rows", type=int, default=512,
-                    help="slowness rows of the CPU baseline sample (0 = skip)")
-    ap.add_argument("--traffic-json", default=os.path.join(HERE, "profiles", "traffic_forward_c2.json"),
-                    help="HBM bytes per launch measured with rocprofv3 --pmc (optional)")
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--requests", type=int, default=400)
+    ap.add_argument("--rate", type=float, default=200.0, help="open-loop arrivals per second")
+    ap.add_argument("--open-seconds", type=float, default=4.0)
+    ap.add_argument("--traffic-json", default=os.path.join(HERE, "profiles", "traffic_multi_served.json"),
+                    help="HBM bytes per launch of the multi-model kernel from rocprofv3 --pmc")
     return ap.parse_args()
 
 
-def c2_forward_inputs():
-    from seismicinversion_amd.workloads import seeded_model, uniform_slownesses, omega_grid
+def timed(fn, reps, torch):
+    """Mean wall seconds of fn() over reps, synchronised on both sides."""
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
 
-    model = seeded_model(20, 42)
-    p = uniform_slownesses(8192, 1.2)
-    omega = omega_grid(1023) + 1j * (np.pi / 64.0)
-    return model, p, omega
+
+def event_ms(fn, reps, torch):
+    """Mean HIP-event milliseconds of fn() on the current stream."""
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(reps)]
+    for a, b in ev:
+        a.record()
+        fn()
+        b.record()
+    torch.cuda.synchronize()
+    return float(np.mean([a.elapsed_time(b) for a, b in ev]))
 
 
-def cpu_baseline(model, p, omega, rows):
-    """The oracle (NumPy, one process) on every (len(p)/rows)-th slowness row."""
+def cpu_baseline(specs):
+    """The oracle (NumPy, one process) on the middle slowness row of every request."""
     from oracle import kennett_numpy as K
 
-    stride = max(1, len(p) // rows)
-    ps = p[::stride][:rows]
-    K.reflectivity_batch(model, ps[:8], omega, True)  # warm-up
+    s0 = specs[0]
+    K.reflectivity_batch(s0.model, s0.p[:1], (s0.k0 + np.arange(8)) * s0.dw, False)  # warm-up
+    upd = 0
     t0 = time.perf_counter()
-    K.reflectivity_batch(model, ps, omega, True)
+    for s in specs:
+        mid = len(s.p) // 2
+        w = (s.k0 + np.arange(s.nw)) * s.dw + 1j * s.gamma
+        K.reflectivity_batch(s.model, s.p[mid:mid + 1], w, s.free_surface)
+        upd += s.nw * (s.n_layers - 1)
     dt = time.perf_counter() - t0
-    upd = len(ps) * len(omega) * (model.n_layers - 1)
-    return {"value": upd / dt, "unit": "p*omega*layer updates/s", "cores": 1, "kind": "port",
-            "sample": f"{len(ps)} of {len(p)} slowness rows (every {stride}th) x {len(omega)} "
-                      f"omega x {model.n_layers - 1} interfaces, NumPy oracle, 1 process, "
-                      f"{dt:.1f} s"}
+    return {"value": upd / dt, "unit": UNIT, "cores": 1, "kind": "port",
+            "sample": f"middle slowness row of each of the {len(specs)} requests "
+                      f"({upd:.3e} updates), NumPy oracle, 1 process, {dt:.1f} s"}
+
+
+def kernel_figures(torch, dev):
+    from seismicinversion_amd import autotune, batch, engine
+    from seismicinversion_amd.kennett_gather_gpu import gather_device
+    from seismicinversion_amd.source import ricker_spectrum
+    from seismicinversion_amd.workloads import gather_config_c2, uneven_batch_models
+
+    out = {}
+    # forward kernel on the gather workload
+    cfg = gather_config_c2()
+    L = engine.layers_tensor(cfg.model, dev)
+    p = torch.arange(1, cfg.np_slow + 1, dtype=torch.float64, device=dev) * (cfg.p_max / cfg.np_slow)
+    dw, gamma, nw = 2 * np.pi / cfg.T, np.pi / cfg.T, cfg.nw - 1
+    R = torch.empty((cfg.np_slow, nw), dtype=torch.complex128, device=dev)
+    fwd = lambda: engine.forward_grid(L, p, 1.0, dw, gamma, nw, True, R)  # noqa: E731
+    fwd()
+    ms = event_ms(fwd, 10, torch)
+    upd = cfg.np_slow * nw * (cfg.model.n_layers - 1)
+    out["forward_gather_workload"] = {
+        "kernel": "kh::forward_kernel<4, true>", "ms": ms, "updates": upd,
+        "updates_per_s": upd / (ms * 1e-3),
+        "tflops": FLOP_PER_UPDATE * upd / (ms * 1e-3) / 1e12,
+        "frac_fp64_peak": FLOP_PER_UPDATE * upd / (ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS}
+    # full gather, inputs resident
+    off = torch.from_numpy(cfg.offsets).to(dev)
+    w = np.arange(1, cfg.nw) * dw
+    S = torch.from_numpy(ricker_spectrum(w, cfg.ricker_omega_max)).to(dev)
+    gat = lambda: gather_device(L, off, cfg.T, cfg.nw, cfg.np_slow, cfg.p_max, gamma, S, True)  # noqa: E731
+    gat()
+    out["gather"] = {"workload": "20 layers, 8192 p x 1023 omega, 100 offsets, nt 2048",
+                     "ms": event_ms(gat, 5, torch)}
+    # 256 uneven models: persistent launch vs one launch per model
+    specs = [batch.ModelSpec(model=m, p=pp, nw=1024) for m, pp in uneven_batch_models(256, 42)]
+    upd = sum(s.updates() for s in specs)
+    Ls = [engine.layers_tensor(s.model, dev) for s in specs]
+    Ps = [torch.from_numpy(s.p).to(dev) for s in specs]
+    Os = [torch.empty((len(s.p), s.nw), dtype=torch.complex128, device=dev) for s in specs]
+
+    def plain():
+        for s, Lm, Pm, Om in zip(specs, Ls, Ps, Os):
+            engine.forward_grid(Lm, Pm, s.k0, s.dw, s.gamma, s.nw, False, Om)
+
+    plain()
+    t_plain = timed(plain, 3, torch)
+    shape = autotune.multi_shape(specs, dev=dev)
+    plan = batch.plan_multi(specs, *shape, dev=dev)
+    batch.launch(plan)
+    t_multi = timed(lambda: batch.launch(plan), 3, torch)
+    out["uneven_256_models"] = {"updates": upd, "shape": list(shape),
+                                "plain_ms": t_plain * 1e3, "multi_ms": t_multi * 1e3,
+                                "multi_updates_per_s": upd / t_multi,
+                                "speedup": t_plain / t_multi}
+    return out
 
 
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(every box has one MI355X; sharding is described in DESIGN.md)", file=sys.stderr)
+              "(every box has one MI355X; how the path shards is described in the design notes)",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
 
     from seismicinversion_amd import engine
+    from seismicinversion_amd.kennett_reflectivity_gpu import kennett_reflectivity_batch
+    from seismicinversion_amd.serve import DeviceBackend, KennettServer, stream_requests
+    from seismicinversion_amd.workloads import serving_stream
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
-    model, p, omega = c2_forward_inputs()
-    layers = engine.layers_tensor(model, dev)
-    p_t = torch.from_numpy(p).to(dev)
-    out = torch.empty((len(p), len(omega)), dtype=torch.complex128, device=dev)
-    updates = len(p) * len(omega) * (model.n_layers - 1)
+    items = serving_stream(0.0, args.requests, seed=42, lm_fraction=0.0)
+    specs = [s for _, s in stream_requests(items)]
+    updates = sum(s.updates() for s in specs)
 
-    dw = 2.0 * np.pi / 64.0
+    # ---- served, device-resident results (headline)
+    be = DeviceBackend(n_streams=3, kernel_timing=True)
+    srv = KennettServer(backend=be).start()
 
-    def step():
-        engine.forward_grid(layers, p_t, 1.0, dw, np.pi / 64.0, len(omega), free_surface=True,
-                            out=out)
+    def served(on_device=True):
+        futs = [srv.submit_forward(s, on_device=on_device) for s in specs]
+        for f in futs:
+            f.result()
 
     for _ in range(args.warmup):
-        step()
+        served()
     torch.cuda.synchronize()
+    be.kernel_log.clear()
+    b0 = srv.batches
+    t_served = timed(served, args.steps, torch)
+    batches_per_step = (srv.batches - b0) / args.steps
+    log = list(be.kernel_log)
+    served()  # warm the host-result path's staging and pages
+    served(False)
+    t_served_host = timed(lambda: served(False), 1, torch)
+    srv.stop()
 
-    stream = torch.cuda.current_stream(dev)
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for a, b in ev:
-        a.record(stream)
-        step()
-        b.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
-    ms_per_step = wall * 1e3 / args.steps
-    value = updates / (wall / args.steps)
+    # ---- plain: one synchronous launch per request
+    def plain():
+        for s in specs:
+            engine.forward_grid(engine.layers_tensor(s.model, dev), torch.from_numpy(s.p).to(dev),
+                                s.k0, s.dw, s.gamma, s.nw, s.free_surface)
+            torch.cuda.synchronize()
 
-    achieved = FLOP_PER_UPDATE * updates / (kern_ms * 1e-3) / 1e12
+    def plain_host():
+        for s in specs:
+            kennett_reflectivity_batch(s.model, s.p, (s.k0 + np.arange(s.nw)) * s.dw,
+                                       s.free_surface)
+
+    plain()
+    t_plain = timed(plain, 1, torch)
+    t_plain_host = timed(plain_host, 1, torch)
+
+    # ---- open loop with inversions, results to the host
+    open_loop = None
+    if args.open_seconds > 0:
+        n_open = max(1, int(args.rate * args.open_seconds))
+        stream = serving_stream(args.rate, n_open, seed=42, lm_fraction=0.2)
+        reqs = stream_requests(stream)
+        with KennettServer(backend=DeviceBackend(n_streams=3)) as osrv:
+            t0 = time.perf_counter()
+            futs = []
+            for it, (kind, spec) in zip(stream, reqs):
+                d = it.t - (time.perf_counter() - t0)
+                if d > 0:
+                    time.sleep(d)
+                futs.append(osrv.submit_forward(spec) if kind == "forward"
+                            else osrv.submit_invert(spec))
+            for f in futs:
+                f.result()
+        st = osrv.stats()
+        open_loop = {"rate_req_per_s": args.rate, "requests": n_open,
+                     "inversions": sum(1 for k, _ in reqs if k == "invert"),
+                     "forward_updates_per_s": st["forward_updates_per_s"],
+                     "batches": st["batches"], "failed": st["failed"],
+                     "latency_forward_ms": st["latency_forward"],
+                     "latency_forward_20_layers_ms": st["latency_forward_20_layers"],
+                     "latency_invert_ms": st["latency_invert"]}
+
+    kernels = kernel_figures(torch, dev)
+
+    # ---- roofline of the persistent multi-model kernel over the timed served steps
+    kern_ms = float(np.mean([m for m, _, _ in log]))
+    upd_per_launch = float(np.mean([u for _, u, _ in log]))
+    out_per_launch = float(np.mean([e for _, _, e in log]))
+    achieved = FLOP_PER_UPDATE * upd_per_launch / (kern_ms * 1e-3) / 1e12
     traffic = None
     if os.path.exists(args.traffic_json):
         with open(args.traffic_json) as f:
             traffic = json.load(f).get("bytes_per_launch")
     roofline = {"bound": "mfma", "pipe": "fp64-vector", "achieved": achieved,
                 "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
-                "traffic": traffic, "kernel": "kh::forward_kernel<W, true>",
-                "kernel_ms": kern_ms, "flop_per_update": FLOP_PER_UPDATE,
-                "algorithmic_bytes": 16 * len(p) * len(omega),
-                "hbm_gbs": 16 * len(p) * len(omega) / (kern_ms * 1e-3) / 1e9}
+                "traffic": traffic, "kernel": "kh::multi_kernel<4>", "launches": len(log),
+                "kernel_ms": kern_ms, "updates_per_launch": upd_per_launch,
+                "flop_per_update": FLOP_PER_UPDATE,
+                "algorithmic_bytes_per_launch": 16.0 * out_per_launch}
 
-    cpu = None
-    if args.cpu_sample_rows > 0:
-        cpu = cpu_baseline(model, p, omega, args.cpu_sample_rows)
-
+    cpu = cpu_baseline(specs)
+    value = updates / t_served
     line = {
-        "metric": "p*omega*layer updates/s (complex128)",
+        "metric": "p*omega*layer updates/s (complex128), served vs plain, with p50/p99 latency",
         "value": value,
-        "unit": "p*omega*layer updates/s",
+        "unit": UNIT,
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": t_served * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "complex128",
-        "data": "synthetic (seeded 20-layer model)",
-        "config": {"workload": "forward reflectivity of the gather workload: 20-layer seeded "
-                               "model (seed 42), free surface, 8192 p in (0,1.2] s/km, 1023 "
-                               "damped omega (T=64 s, nt=2048, gamma=pi/64)",
-                   "n_layers": 20, "n_p": len(p), "n_omega": len(omega),
-                   "updates_per_step": updates, "parallelism": "single GPU"},
+        "data": "synthetic (seeded layered earth models)",
+        "config": {"workload": f"{len(specs)} forward requests of the mixed-size serving "
+                               "stream (seed 42; nl log-uniform 3-100, np log-uniform 16-2048 "
+                               "in (0,0.8] s/km, nw in {256,1024,4096}) served by continuous "
+                               "batching with device-resident results",
+                   "requests": len(specs), "updates_per_step": updates,
+                   "batches_per_step": batches_per_step, "streams": 3,
+                   "launch_shape": [256, 4], "parallelism": "single GPU"},
+        "plain_updates_per_s": updates / t_plain,
+        "served_over_plain": t_plain / t_served,
+        "served_host_results_updates_per_s": updates / t_served_host,
+        "plain_host_results_updates_per_s": updates / t_plain_host,
+        "served_over_plain_host_results": t_plain_host / t_served_host,
+        "open_loop": open_loop,
+        "kernels": kernels,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

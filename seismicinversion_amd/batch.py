@@ -77,8 +77,18 @@ class MultiPlan:
     order: list[int]            # queue slot -> input index
 
 
-def plan_multi(specs: list[ModelSpec], block: int, wpt: int, dev=None) -> MultiPlan:
-    """Pack specs into device buffers and descriptors for one launch shape."""
+def _to_device(arr: np.ndarray, dev, non_blocking: bool) -> torch.Tensor:
+    t = torch.from_numpy(arr)
+    if non_blocking:  # pinned source: the copy is queued on the current stream
+        t = t.pin_memory()
+    return t.to(dev, non_blocking=non_blocking)
+
+
+def plan_multi(specs: list[ModelSpec], block: int, wpt: int, dev=None,
+               non_blocking: bool = False) -> MultiPlan:
+    """Pack specs into device buffers and descriptors for one launch shape.
+    With ``non_blocking`` the host->device copies go through pinned memory and
+    are only queued on the current stream (the host does not wait)."""
     dev = _abi.device(dev)
     tabs = [s.table() for s in specs]
     for t in tabs:
@@ -99,11 +109,11 @@ def plan_multi(specs: list[ModelSpec], block: int, wpt: int, dev=None) -> MultiP
                       float(s.gamma), tabs[i].shape[0], len(s.p), s.nw, ntiles,
                       int(bool(s.free_surface)), 0)
         item += len(s.p) * ntiles
-    layers = torch.from_numpy(np.concatenate(tabs, axis=0)).to(dev)
-    p = torch.from_numpy(np.concatenate([np.asarray(s.p, np.float64) for s in specs])
-                         if specs else np.zeros(0)).to(dev)
+    layers = _to_device(np.concatenate(tabs, axis=0), dev, non_blocking)
+    p = _to_device(np.concatenate([np.asarray(s.p, np.float64) for s in specs])
+                   if specs else np.zeros(0), dev, non_blocking)
     R = torch.empty(int(r_off[-1]), dtype=torch.complex128, device=dev)
-    return MultiPlan(desc=torch.from_numpy(desc.view(np.uint8)).to(dev), layers=layers, p=p, R=R,
+    return MultiPlan(desc=_to_device(desc.view(np.uint8), dev, non_blocking), layers=layers, p=p, R=R,
                      r_off=r_off, shapes=[(len(s.p), s.nw) for s in specs], total_items=item,
                      max_nl=max(t.shape[0] for t in tabs) if tabs else 2, block=block, wpt=wpt,
                      counter=torch.zeros(1, dtype=torch.int64, device=dev),

@@ -42,7 +42,8 @@ import numpy as np
 
 from .batch import ModelSpec
 
-__all__ = ["InvertSpec", "Request", "KennettServer", "DeviceBackend", "percentile_stats"]
+__all__ = ["InvertSpec", "Request", "KennettServer", "DeviceBackend", "percentile_stats",
+           "stream_requests"]
 
 
 @dataclass
@@ -75,8 +76,38 @@ class Request:
     future: Future
     arrival: float
     deadline: float | None = None
+    on_device: bool = False
     mem: int = 0
     record: dict = field(default_factory=dict)
+
+
+def stream_requests(items, p_max: float = 0.8, lm_np_max: int = 32, lm_nfreq_max: int = 128,
+                    lm_hessian_mode: str = "gram"):
+    """(kind, spec) for each StreamItem of ``workloads.serving_stream``.
+
+    Forward requests take the drawn sizes: the seeded model with
+    ``n_layers`` layers (seed = the item's seed), ``n_p`` slownesses uniform
+    in (0, p_max] and ``n_omega`` frequencies omega_k = k * 2*pi/64.
+    Inversions (5 Newton-LM iterations with the exact Hessian) keep the drawn
+    layer count and cap slownesses and frequencies at ``lm_np_max`` /
+    ``lm_nfreq_max``: one exact-Hessian iteration costs about
+    4 (nl - 1) tangent-over-adjoint sweeps of the whole (p, omega) grid, so an
+    uncapped draw (up to 2048 x 4096 at nl = 100) would take minutes per
+    request."""
+    from .workloads import seeded_model, uniform_slownesses
+
+    out = []
+    for it in items:
+        model = seeded_model(it.n_layers, it.seed)
+        if it.kind == "forward":
+            out.append(("forward", ModelSpec(model=model, p=uniform_slownesses(it.n_p, p_max),
+                                             nw=it.n_omega)))
+        else:
+            out.append(("invert", InvertSpec(
+                true_model=model, p=uniform_slownesses(min(it.n_p, lm_np_max), p_max),
+                nfreq=min(it.n_omega, lm_nfreq_max), iterations=5, seed=it.seed,
+                hessian_mode=lm_hessian_mode)))
+    return out
 
 
 def percentile_stats(records, kind="forward", n_layers=None) -> dict:
@@ -93,23 +124,29 @@ def percentile_stats(records, kind="forward", n_layers=None) -> dict:
 class DeviceBackend:
     """HIP backend: persistent multi-model launches and device LM inversions.
 
-    Each of the ``n_streams`` stream slots owns a pinned host staging buffer
-    of ``staging_bytes``.  A batch's outputs reach it with one device->host
-    copy on the batch's stream (pinned memory copies at the link rate;
-    pageable memory would be staged through the runtime at a fraction of
-    it), and ``results`` then copies them into one host array per request on
-    ``copy_threads`` threads while later batches run on the other slots.
-    The server caps batches at ``max_batch_bytes`` (the staging size); a
-    single request larger than that gets a one-off pinned buffer."""
+    Batch kernels run back to back on one compute stream (a persistent launch
+    fills the device by itself, so kernels gain nothing from overlapping one
+    another).  Each of the ``n_streams`` slots has its own copy stream and a
+    pinned host staging buffer of ``staging_bytes``: a batch's inputs go up
+    through pinned memory on its slot's copy stream, the compute stream waits
+    for them, and its outputs come back into the staging buffer on the copy
+    stream once the kernel is done (pinned copies run at the link rate;
+    pageable memory would be staged by the runtime at a fraction of it).
+    ``results`` copies them into one host array per request on
+    ``copy_threads`` threads while later batches run.  The server caps
+    batches at ``max_batch_bytes`` (the staging size); a single request larger
+    than that gets a one-off pinned buffer."""
 
     def __init__(self, n_streams: int = 3, shape=(256, 4), device=None,
-                 staging_bytes: int = 512 << 20, copy_threads: int = 8):
+                 staging_bytes: int = 512 << 20, copy_threads: int = 8,
+                 kernel_timing: bool = False):
         import torch
 
         from . import _abi
 
         self.torch = torch
         self.dev = _abi.device(device)
+        self.compute = torch.cuda.Stream(device=self.dev)
         self.streams = [torch.cuda.Stream(device=self.dev) for _ in range(n_streams)]
         self.lm_stream = torch.cuda.Stream(device=self.dev)
         self.slots = n_streams
@@ -121,6 +158,10 @@ class DeviceBackend:
         self._lock = threading.Lock()
         self.pool = ThreadPoolExecutor(max_workers=copy_threads,
                                        thread_name_prefix="kennett-copy")
+        # with kernel_timing, (launch ms from HIP events on the compute
+        # stream, updates, output elements) per completed batch
+        self.kernel_timing = kernel_timing
+        self.kernel_log: list[tuple[float, int, int]] = []
 
     def free_bytes(self) -> int:
         free, _ = self.torch.cuda.mem_get_info(self.dev)
@@ -137,35 +178,64 @@ class DeviceBackend:
         nl, n_p, nw = spec.n_layers, len(spec.p), spec.nfreq
         return min(derivatives.WORKSPACE_LIMIT, 1 << 34) + 64 * n_p * nw * 4 * nl
 
-    def launch_forward(self, specs: list[ModelSpec]):
-        """Enqueue one persistent launch for the batch on a free stream slot,
-        then the device->host copies of its outputs and NaN/Inf counts."""
+    def launch_forward(self, specs: list[ModelSpec], to_host: list[bool] | None = None):
+        """Queue one batch: inputs up on a slot's copy stream, the persistent
+        launch on the compute stream, then the device->host copies of the
+        outputs that go to the host (``to_host[i]``, all by default) and of the
+        NaN/Inf counts on the copy stream.  The host does not wait."""
         from . import batch
 
         torch = self.torch
+        if to_host is None:
+            to_host = [True] * len(specs)
         with self._lock:
             if not self._free:
                 raise RuntimeError("no free stream slot: more batches in flight than streams")
             slot = self._free.pop()
-        s = self.streams[slot]
+        cp = self.streams[slot]
         try:
-            nbytes = 16 * sum(len(x.p) * x.nw for x in specs)
-            with torch.cuda.stream(s):
-                plan = batch.plan_multi(specs, *self.shape, dev=self.dev)
+            with torch.cuda.stream(cp):
+                plan = batch.plan_multi(specs, *self.shape, dev=self.dev, non_blocking=True)
+                inputs_ready = torch.cuda.Event()
+                inputs_ready.record(cp)
+            self.compute.wait_event(inputs_ready)
+            with torch.cuda.stream(self.compute):
+                t_ev = None
+                if self.kernel_timing:
+                    t_ev = (torch.cuda.Event(enable_timing=True),
+                            torch.cuda.Event(enable_timing=True))
+                    t_ev[0].record(self.compute)
                 batch.launch(plan)
-                if nbytes <= self.max_batch_bytes:
-                    host = self.staging[slot][:nbytes].view(torch.complex128)
-                else:
-                    host = torch.empty(plan.R.numel(), dtype=torch.complex128, pin_memory=True)
-                host.copy_(plan.R, non_blocking=True)
+                if t_ev is not None:
+                    t_ev[1].record(self.compute)
+                kernel_done = torch.cuda.Event()
+                kernel_done.record(self.compute)
+            cp.wait_event(kernel_done)
+            with torch.cuda.stream(cp):
+                total = int(plan.r_off[-1])
+                host = None
+                if any(to_host):
+                    if 16 * total <= self.max_batch_bytes:
+                        host = self.staging[slot][:16 * total].view(torch.complex128)
+                    else:
+                        host = torch.empty(total, dtype=torch.complex128, pin_memory=True)
+                    if all(to_host):
+                        host.copy_(plan.R, non_blocking=True)
+                    else:
+                        for i, h in enumerate(to_host):
+                            if h:
+                                a, b = int(plan.r_off[i]), int(plan.r_off[i + 1])
+                                host[a:b].copy_(plan.R[a:b], non_blocking=True)
                 counts = torch.empty(len(specs), dtype=torch.int32, pin_memory=True)
                 counts.copy_(plan.nonfinite[:len(specs)], non_blocking=True)
                 ev = torch.cuda.Event()
-                ev.record(s)
+                ev.record(cp)
         except BaseException:
             self._release_slot(slot)
             raise
-        return {"plan": plan, "host": host, "counts": counts, "event": ev, "slot": slot}
+        return {"plan": plan, "host": host, "to_host": list(to_host), "counts": counts,
+                "event": ev, "slot": slot, "timing": t_ev,
+                "updates": sum(x.updates() for x in specs)}
 
     def _release_slot(self, slot: int) -> None:
         with self._lock:
@@ -175,29 +245,42 @@ class DeviceBackend:
         return handle["event"].query()
 
     def results(self, handle):
-        """(per-request host arrays, per-request NaN/Inf counts); frees the slot."""
+        """(per-request results, per-request NaN/Inf counts); frees the slot.
+        Host results are copied out of the staging buffer by the copy pool;
+        device results are views of the batch output."""
         from . import batch
 
         try:
             handle["event"].synchronize()
-            plan = handle["plan"]
-            flat = handle["host"].numpy()
-            outs = [np.empty(shape, np.complex128) for shape in plan.shapes]
+            plan, to_host = handle["plan"], handle["to_host"]
+            dev_views = batch.split(plan)
+            outs = [np.empty(shape, np.complex128) if h else dev_views[i]
+                    for i, (shape, h) in enumerate(zip(plan.shapes, to_host))]
             jobs = []
             chunk = 1 << 19  # complex elements (8 MiB) per copy job
             for i, (n_p, nw) in enumerate(plan.shapes):
+                if not to_host[i]:
+                    continue
                 off = int(plan.r_off[i])
                 rows = max(1, chunk // nw)
                 for r0 in range(0, n_p, rows):
                     jobs.append((i, off, nw, r0, min(n_p, r0 + rows)))
+            if jobs:
+                flat = handle["host"].numpy()
 
-            def copy(job):
-                i, off, nw, r0, r1 = job
-                np.copyto(outs[i][r0:r1], flat[off + r0 * nw:off + r1 * nw].reshape(r1 - r0, nw))
+                def copy(job):
+                    i, off, nw, r0, r1 = job
+                    np.copyto(outs[i][r0:r1],
+                              flat[off + r0 * nw:off + r1 * nw].reshape(r1 - r0, nw))
 
-            for _ in self.pool.map(copy, jobs):
-                pass
+                for _ in self.pool.map(copy, jobs):
+                    pass
             counts = batch.nonfinite_counts(plan, handle["counts"].numpy())
+            if handle["timing"] is not None:
+                a, b = handle["timing"]
+                with self._lock:
+                    self.kernel_log.append((a.elapsed_time(b), handle["updates"],
+                                            int(plan.r_off[-1])))
         finally:
             self._release_slot(handle["slot"])
         return outs, counts
@@ -236,8 +319,10 @@ class KennettServer:
         self.outstanding = 0  # submitted and not yet finished
         self.records: list[dict] = []
         self.lock = threading.Lock()
+        self.waiting_inv: list[Request] = []
         self.lm_threads: list[threading.Thread] = []
-        self.lm_slots = threading.Semaphore(max_lm_concurrency)
+        self.max_lm = max_lm_concurrency
+        self.lm_running = 0
         self.idle_sleep = idle_sleep
         self._stop = threading.Event()
         self._thread: threading.Thread | None = None
@@ -245,17 +330,26 @@ class KennettServer:
         self.batches = 0
 
     # ------------------------------------------------------------------ client API
-    def submit_forward(self, spec: ModelSpec, deadline_s: float | None = None) -> Future:
-        return self._submit("forward", spec, deadline_s)
+    def submit_forward(self, spec: ModelSpec, deadline_s: float | None = None,
+                       on_device: bool = False) -> Future:
+        """R (np, nw) complex128 as a host array, or with ``on_device`` as a
+        device tensor (a view into its batch's output, which stays allocated
+        while the caller holds it; no device->host copy is made).  The tensor
+        was written on one of the server's streams and is complete when the
+        future resolves; a consumer on another stream calls
+        ``tensor.record_stream(stream)`` before dropping it, as for any tensor
+        shared across streams."""
+        return self._submit("forward", spec, deadline_s, on_device)
 
     def submit_invert(self, spec: InvertSpec, deadline_s: float | None = None) -> Future:
         return self._submit("invert", spec, deadline_s)
 
-    def _submit(self, kind, spec, deadline_s) -> Future:
+    def _submit(self, kind, spec, deadline_s, on_device=False) -> Future:
         fut: Future = Future()
         now = time.perf_counter()
         req = Request(kind=kind, spec=spec, future=fut, arrival=now,
-                      deadline=None if deadline_s is None else now + deadline_s)
+                      deadline=None if deadline_s is None else now + deadline_s,
+                      on_device=bool(on_device))
         req.record = {"kind": kind, "arrival": now, "n_layers": 0, "updates": 0,
                       "status": "queued"}
         try:
@@ -354,6 +448,11 @@ class KennettServer:
             raise ValueError("forward request needs at least one slowness and one frequency")
 
     def _admit(self) -> list[Request]:
+        """Validated requests leave the arrival queue for one of two FIFOs.
+        Forward requests are admitted in arrival order as device memory allows
+        (once one waits, later ones wait too).  An inversion is admitted only
+        when an inversion slot is free and its workspace fits, so inversions
+        waiting for a slot hold no memory and never hold up forward requests."""
         while True:
             try:
                 req = self.incoming.get_nowait()
@@ -367,20 +466,25 @@ class KennettServer:
             except (ValueError, MemoryError) as e:
                 self._finish(req, error=e)
                 continue
-            self.waiting.append(req)
-        admitted, keep = [], []
+            (self.waiting if req.kind == "forward" else self.waiting_inv).append(req)
+        admitted = []
         now = time.perf_counter()
-        for req in self.waiting:
-            if req.deadline is not None and now > req.deadline:
-                self._finish(req, error=TimeoutError("deadline passed before admission"))
-                continue
-            # arrival order is kept: once one request waits for memory, later ones wait too
-            if not keep and self._reserve(req.mem):
-                req.record["admit"] = now
-                admitted.append(req)
-            else:
-                keep.append(req)
-        self.waiting = keep
+        for fifo, is_inv in ((self.waiting, False), (self.waiting_inv, True)):
+            keep = []
+            for req in fifo:
+                if req.deadline is not None and now > req.deadline:
+                    self._finish(req, error=TimeoutError("deadline passed before admission"))
+                    continue
+                if not keep and (not is_inv or self.lm_running < self.max_lm) and \
+                        self._reserve(req.mem):
+                    req.record["admit"] = now
+                    admitted.append(req)
+                    if is_inv:
+                        with self.lock:
+                            self.lm_running += 1
+                else:
+                    keep.append(req)
+            fifo[:] = keep
         return admitted
 
     def _reserve(self, nbytes: int) -> bool:
@@ -411,7 +515,7 @@ class KennettServer:
                 cap = self.max_batch_bytes
                 while pending_fwd:
                     nxt = pending_fwd[0].spec
-                    b = 16 * len(nxt.p) * nxt.nw
+                    b = 0 if pending_fwd[0].on_device else 16 * len(nxt.p) * nxt.nw
                     if batch and (total + nxt.updates() > self.max_batch_updates or
                                   (cap is not None and nbytes + b > cap)):
                         break
@@ -421,7 +525,8 @@ class KennettServer:
                     nbytes += b
                 launched = time.perf_counter()
                 try:
-                    handle = self.backend.launch_forward([r.spec for r in batch])
+                    handle = self.backend.launch_forward([r.spec for r in batch],
+                                                         [not r.on_device for r in batch])
                 except Exception as e:  # launch refused: fail the batch, free its memory
                     for r in batch:
                         self._release(r.mem)
@@ -462,13 +567,14 @@ class KennettServer:
                 self.n_inflight -= 1
 
     def _run_invert(self, req: Request):
-        with self.lm_slots:
-            req.record["launch"] = time.perf_counter()
-            try:
-                res = self.backend.invert(req.spec)
-            except Exception as e:
-                self._release(req.mem)
-                self._finish(req, error=e)
-                return
-            self._release(req.mem)
-            self._finish(req, result=res)
+        req.record["launch"] = time.perf_counter()
+        try:
+            res = self.backend.invert(req.spec)
+        except Exception as e:
+            res, err = None, e
+        else:
+            err = None
+        self._release(req.mem)
+        with self.lock:
+            self.lm_running -= 1
+        self._finish(req, result=res, error=err)

@@ -40,7 +40,7 @@ class HostBackend:
     def invert_bytes(spec):
         return 1000
 
-    def launch_forward(self, specs):
+    def launch_forward(self, specs, to_host=None):
         out = {}
 
         def run():

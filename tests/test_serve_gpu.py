@@ -40,7 +40,7 @@ def test_results_identical_alone_and_in_the_mix():
         def client(idx):
             for i in idx:
                 time.sleep(items[i].t / 4)
-                futs[i] = srv.submit_forward(specs[i])
+                futs[i] = srv.submit_forward(specs[i], on_device=(i % 4 == 0))
 
         ths = [threading.Thread(target=client, args=(list(range(k, len(specs), 3)),))
                for k in range(3)]
@@ -59,6 +59,9 @@ def test_results_identical_alone_and_in_the_mix():
         alone = engine.forward_grid(engine.layers_tensor(s.model), torch.from_numpy(s.p), s.k0,
                                     s.dw, s.gamma, s.nw, s.free_surface, block=256,
                                     omega_per_item=4).cpu().numpy()
+        if isinstance(R, torch.Tensor):  # device-resident result
+            assert R.is_cuda
+            R = R.cpu().numpy()
         np.testing.assert_array_equal(R, alone)
     solo = invert_taup(seeded_model(5, 3), [0.1, 0.2, 0.3], nfreq=32, max_iter=3, tol=0.0,
                        hessian_mode="hvp", perturbation=0.05)
