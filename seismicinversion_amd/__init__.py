@@ -41,7 +41,9 @@ _EXPORTS = {
     "invert_taup": "taup_inversion",
     "taup_misfit_grad_hessian": "taup_inversion",
     "reflectivity_multi": "batch",
+    "ModelSpec": "batch",
     "KennettServer": "serve",
+    "InvertSpec": "serve",
 }
 
 __all__ = sorted(_EXPORTS)
