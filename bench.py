@@ -21,8 +21,10 @@ Beside it, on the same requests:
   requests, for 20-layer forward requests and for inversions;
 * kernels: the gather workload's forward kernel (20 layers, 8192 p x 1023
   damped omega, free surface), the full gather (forward + J0 wavenumber sum +
-  inverse FFT, 100 offsets) and the 256-model uneven batch (persistent
-  launch vs one launch per model).
+  inverse FFT, 100 offsets), the 256-model uneven batch (persistent
+  launch vs one launch per model) and the 20-layer Newton-LM inversion
+  (75 parameters, 128 p x 512 omega, 10 iterations, eager and from HIP
+  graphs).
 
 ``roofline``: the persistent multi-model kernel of the served steps, FP64
 algorithmic FLOPs (340 per update) per launch over the average launch
@@ -155,7 +157,47 @@ def kernel_figures(torch, dev):
                                 "plain_ms": t_plain * 1e3, "multi_ms": t_multi * 1e3,
                                 "multi_updates_per_s": upd / t_multi,
                                 "speedup": t_plain / t_multi}
+    out["lm_20_layers"] = lm_figures(torch, dev)
     return out
+
+
+def lm_figures(torch, dev):
+    """20-layer Newton-LM: P = 75 packed parameters, 128 p in (0, 0.6] x 512
+    omega, start model perturbed 5 %, 10 iterations; eager and graph-replayed,
+    plus one misfit/gradient/exact-Hessian evaluation timed on its own."""
+    from seismicinversion_amd import engine, kennett_torch as kt
+    from seismicinversion_amd.taup_inversion import _Problem, invert_taup
+    from seismicinversion_amd.workloads import seeded_model, uniform_slownesses
+
+    m = seeded_model(20, 42)
+    p = list(uniform_slownesses(128, 0.6))
+    invert_taup(m, p[:4], nfreq=16, max_iter=1, tol=0.0)  # warm-up
+    runs = {}
+    for graphs in (False, True):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = invert_taup(m, p, nfreq=512, perturbation=0.05, max_iter=10, tol=0.0,
+                          use_graphs=graphs)
+        torch.cuda.synchronize()
+        runs[graphs] = (time.perf_counter() - t0, res)
+    tens = kt.model_to_tensors(m)
+    refs = [tens[k] for k in ("alpha", "beta", "rho", "thickness", "Q_alpha", "Q_beta")]
+    w = torch.arange(1, 513, dtype=torch.float64) * (2 * np.pi / 64)
+    obs = engine.forward(engine.layers_tensor(m, dev), torch.tensor(p), w.to(torch.complex128))
+    prob = _Problem(*refs, p, w, obs)
+    phys = kt._pack_params(*refs[:4]).numpy() * 1.02
+    prob.grad_hessian(phys)
+    t_gh = timed(lambda: prob.grad_hessian(phys), 3, torch)
+    res = runs[False][1]
+    return {"P": prob.P, "n_p": 128, "n_omega": 512, "iterations": res.n_iterations,
+            "eager_s": runs[False][0], "graphs_s": runs[True][0],
+            "eager_ms_per_iteration": runs[False][0] * 1e3 / res.n_iterations,
+            "graphs_ms_per_iteration": runs[True][0] * 1e3 / runs[True][1].n_iterations,
+            "same_trajectory": runs[True][1].misfit_history == res.misfit_history,
+            "misfit_first_last": [res.misfit_history[0], res.misfit_history[-1]],
+            "param_error_last": res.param_error_history[-1],
+            "grad_hessian_ms": t_gh * 1e3,
+            "tangent_over_adjoint_lane_updates": prob.P * 128 * 512 * 19}
 
 
 def main():

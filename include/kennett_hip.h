@@ -165,7 +165,8 @@ int kh_bessel_sum(const double* R, int np, int nw, const double* p, double dp,
 
 /* Real traces (nr, nt = 2*nw) float64 from U (nr, nw-1) complex128:
  * trace_r[n] = Re(FFT_nt([0, U_r, 0, conj(reversed U_r)])[n]) * exp(-gamma n T/nt),
- * nw a power of two with nt*16 bytes <= the device's LDS per workgroup.
+ * nw a power of two with nw*16 bytes <= the device's LDS per workgroup (the
+ * Hermitian input is transformed as one complex FFT of length nt/2).
  * Replaces the per-trace np.fft.fft loop of
  * Kennett_Reflectivity/kennett_gather_gpu.py:273-282 (and kennett_gather.py:295-303). */
 int kh_hermitian_traces(const double* U, int nr, int nw, double gamma, double T,
@@ -245,6 +246,14 @@ size_t kh_gram_workspace_bytes(int P, long K);
  * torch.func.hessian. */
 int kh_gram(const double* X, int P, long K, double scale, double* G, void* workspace,
             size_t workspace_bytes, void* stream);
+
+/* Misfit per slowness chi (np,) float64: chi[j] = sum_w |R[j, w] - R_obs[j, w]|^2
+ * for R, R_obs (np, nw) complex128, in a fixed summation order.  The trial
+ * misfit of the LM damping loop (_eval_misfit_only,
+ * Kennett_Reflectivity/taup_inversion.py:190-216) after a kh_forward of the
+ * trial model. */
+int kh_residual_norms(const double* R, const double* R_obs, int np, int nw, double* chi,
+                      void* stream);
 
 #ifdef __cplusplus
 }

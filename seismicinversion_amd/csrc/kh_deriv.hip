@@ -191,8 +191,13 @@ __device__ __forceinline__ double tangent_re(const dcd& z) { return z.d.r; }
 
 // grid-stride over (dir, p, l).  grad (np, nl, 6) from dir 0; hvp (ndir, np, nl, 6)
 // for dual Z; chi (ndir, np, 2) [value, tangent] when chi_part is given.
+// One wave per workgroup: the chain rule through the interface formulas keeps
+// dual-of-dual temporaries live, so the kernel takes the whole register file
+// of a wave (a 1024-thread bound would cap it at 128 VGPRs and spill).
+constexpr int CH_BLOCK = 64;
+
 template <class Z>
-__global__ void chain_kernel(const double* __restrict__ layers, const double* __restrict__ dlayers,
+__global__ void __launch_bounds__(CH_BLOCK) chain_kernel(const double* __restrict__ layers, const double* __restrict__ dlayers,
                              int nl, const double* __restrict__ p, int np, int ndir,
                              const IfaceZ<Z>* __restrict__ cot_if,
                              const LayerCot<Z>* __restrict__ cot_ly,
@@ -288,7 +293,8 @@ static int run_adjoint(const char* what, const double* layers, const double* dla
                      reinterpret_cast<cd*>(R), reinterpret_cast<cd*>(dR), d.ntiles);
   KH_TRY(check_launch(what));
   const long n = (long)ndir * np * nl;
-  hipLaunchKernelGGL((chain_kernel<Z>), dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st,
+  hipLaunchKernelGGL((chain_kernel<Z>), dim3((unsigned)((n + CH_BLOCK - 1) / CH_BLOCK)),
+                     dim3(CH_BLOCK), 0, st,
                      layers, dlayers, nl, p, np, ndir, cif, cly, MISFIT ? chip : nullptr,
                      d.ntiles, grad, hvp, chi);
   return check_launch(what);

@@ -94,48 +94,55 @@ __global__ void bessel_reduce_kernel(const cd* __restrict__ partial, int nchunks
   U[idx] = s;
 }
 
-// One workgroup per trace: in-place radix-2 decimation-in-time FFT of the
-// length-nt Hermitian vector in LDS (loaded in bit-reversed order), then
-// trace[n] = Re(X^[n]) * exp(-gamma n dt).
+// One workgroup per trace.  The length-nt vector X is Hermitian
+// (X[nt-k] = conj(X[k])), so x[n] = sum_k X[k] exp(-2 pi i k n / nt) is real
+// and half-length transform gives all of it: with M = nt/2 and
+// w = exp(-2 pi i / nt),
+//   Z[k] = (X[k] + X[k+M]) + i (X[k] - X[k+M]) w^k,   k < M,
+//   z = FFT_M(Z)  =>  z[m] = x[2m] + i x[2m+1],
+// because w^(2M) = 1 and w^M = -1 split x into its even and odd samples.
+// Z is built in LDS in bit-reversed order and transformed in place by
+// radix-2 decimation in time; then trace[n] = x[n] * exp(-gamma n dt).
+// Here X[0] = X[M] = 0, X[k] = U[k-1] and X[M+k] = conj(U[M-k-1]) (M = nw).
 __global__ void __launch_bounds__(1024) hermitian_trace_kernel(const cd* __restrict__ U, int nw,
-                                                               int log2nt, double gamma, double dt,
+                                                               int log2m, double gamma, double dt,
                                                                double* __restrict__ traces) {
   extern __shared__ double smem_f[];
-  cd* X = reinterpret_cast<cd*>(smem_f);
-  const int nt = 1 << log2nt;
-  const int nwm = nw - 1;
+  cd* Z = reinterpret_cast<cd*>(smem_f);
+  const int m = 1 << log2m;  // == nw
   const int r = blockIdx.x;
-  const cd* Ur = U + (size_t)r * nwm;
-  for (int k = threadIdx.x; k < nt; k += blockDim.x) {
-    cd v{0.0, 0.0};
-    if (k >= 1 && k < nw) v = Ur[k - 1];
-    else if (k > nw) v = conj(Ur[nt - k - 1]);
-    const int rev = (int)(__brev((unsigned)k) >> (32 - log2nt));
-    X[rev] = v;
+  const cd* Ur = U + (size_t)r * (nw - 1);
+  for (int k = threadIdx.x; k < m; k += blockDim.x) {
+    const cd a = k >= 1 ? Ur[k - 1] : cd{0.0, 0.0};
+    const cd b = k >= 1 ? conj(Ur[m - k - 1]) : cd{0.0, 0.0};
+    double sn, cs;
+    sincospi(-(double)k / (double)m, &sn, &cs);  // w^k
+    const cd v = (a + b) + mul_i((a - b) * cd{cs, sn});
+    const int rev = log2m > 0 ? (int)(__brev((unsigned)k) >> (32 - log2m)) : 0;
+    Z[rev] = v;
   }
   __syncthreads();
-  for (int s = 1; s <= log2nt; ++s) {
+  for (int s = 1; s <= log2m; ++s) {
     const int half = 1 << (s - 1);
-    for (int t = threadIdx.x; t < nt / 2; t += blockDim.x) {
+    for (int t = threadIdx.x; t < m / 2; t += blockDim.x) {
       const int grp = t >> (s - 1);
       const int pos = t & (half - 1);
       const int i0 = (grp << s) + pos;
       const int i1 = i0 + half;
       double sn, cs;
-      // twiddle exp(-2 pi i pos / 2^s)
-      sincospi(-(double)pos / (double)half, &sn, &cs);
-      const cd tw{cs, sn};
-      const cd a = X[i0];
-      const cd bb = X[i1] * tw;
-      X[i0] = a + bb;
-      X[i1] = a - bb;
+      sincospi(-(double)pos / (double)half, &sn, &cs);  // exp(-2 pi i pos / 2^s)
+      const cd a = Z[i0];
+      const cd bb = Z[i1] * cd{cs, sn};
+      Z[i0] = a + bb;
+      Z[i1] = a - bb;
     }
     __syncthreads();
   }
-  double* tr = traces + (size_t)r * nt;
-  for (int n = threadIdx.x; n < nt; n += blockDim.x) {
-    const double t = (double)n * dt;
-    tr[n] = X[n].r * exp(-gamma * t);
+  double* tr = traces + (size_t)r * 2 * m;
+  for (int q = threadIdx.x; q < m; q += blockDim.x) {
+    const double t0 = (double)(2 * q) * dt, t1 = (double)(2 * q + 1) * dt;
+    tr[2 * q] = Z[q].r * exp(-gamma * t0);
+    tr[2 * q + 1] = Z[q].i * exp(-gamma * t1);
   }
 }
 
@@ -205,17 +212,19 @@ extern "C" int kh_hermitian_traces(const double* U, int nr, int nw, double gamma
   clear_error();
   KH_REQUIRE(nr >= 0 && nw >= 1, KH_ERR_INVALID_ARGUMENT, "kh_hermitian_traces: bad sizes");
   if (nr == 0) return KH_OK;
-  KH_REQUIRE(U && traces, KH_ERR_INVALID_ARGUMENT, "kh_hermitian_traces: NULL pointer");
+  // nw = 1 has no frequencies to read (U is empty and may be NULL): zero traces
+  KH_REQUIRE(traces && (U || nw == 1), KH_ERR_INVALID_ARGUMENT,
+             "kh_hermitian_traces: NULL pointer");
   const long nt = 2L * nw;
-  const int log2nt = ilog2_exact(nt);
-  KH_REQUIRE(log2nt >= 1, KH_ERR_INVALID_ARGUMENT, "kh_hermitian_traces: nw=%d is not a power of 2",
+  const int log2m = ilog2_exact(nw);
+  KH_REQUIRE(log2m >= 0, KH_ERR_INVALID_ARGUMENT, "kh_hermitian_traces: nw=%d is not a power of 2",
              nw);
-  const size_t lds = (size_t)nt * sizeof(cd);
+  const size_t lds = (size_t)nw * sizeof(cd);
   KH_REQUIRE(lds <= (size_t)max_lds_bytes(), KH_ERR_UNSUPPORTED,
              "kh_hermitian_traces: nt=%ld needs %zu bytes of LDS per workgroup", nt, lds);
-  const int block = (int)(nt / 2 < 1024 ? (nt / 2 < 64 ? 64 : nt / 2) : 1024);
+  const int block = nw < 1024 ? (nw < 64 ? 64 : nw) : 1024;
   const double dt = T / (double)nt;
   hipLaunchKernelGGL(hermitian_trace_kernel, dim3(nr), dim3(block), lds, as_stream(stream),
-                     reinterpret_cast<const cd*>(U), nw, log2nt, gamma, dt, traces);
+                     reinterpret_cast<const cd*>(U), nw, log2m, gamma, dt, traces);
   return check_launch("kh_hermitian_traces");
 }

@@ -12,6 +12,7 @@
 // Operand layout of v_mfma_f64_16x16x4f64 (wave64, measured on gfx950 with
 // single-nonzero inputs): lane l supplies A[l % 16][l / 16] and
 // B[l / 16][l % 16]; result j (0..3) of lane l is D[4 * j + l / 16][l % 16].
+#include "kh_complex.hpp"
 #include "kh_runtime.hpp"
 
 namespace kh {
@@ -74,9 +75,48 @@ __global__ void gram_reduce_kernel(const double* __restrict__ partial, int P, in
   G[idx] = scale * s;
 }
 
+// chi[j] = sum_w |R[j, w] - R_obs[j, w]|^2, one workgroup per slowness row:
+// each work-item sums its strided frequencies in ascending order, then a
+// fixed LDS tree adds the 256 partial sums, so the value does not depend on
+// timing (and a captured graph replays it exactly).
+constexpr int RN_BLOCK = 256;
+
+__global__ void __launch_bounds__(RN_BLOCK) residual_norm_kernel(const cd* __restrict__ R,
+                                                                 const cd* __restrict__ R_obs,
+                                                                 int nw, double* __restrict__ chi) {
+  __shared__ double s[RN_BLOCK];
+  const size_t row = blockIdx.x;
+  const cd* a = R + row * nw;
+  const cd* b = R_obs + row * nw;
+  double acc = 0.0;
+  for (int w = threadIdx.x; w < nw; w += RN_BLOCK) {
+    const double dr = a[w].r - b[w].r, di = a[w].i - b[w].i;
+    acc += dr * dr + di * di;
+  }
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = RN_BLOCK / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) chi[row] = s[0];
+}
+
 }  // namespace kh
 
 using namespace kh;
+
+extern "C" int kh_residual_norms(const double* R, const double* R_obs, int np, int nw,
+                                 double* chi, void* stream) {
+  clear_error();
+  KH_REQUIRE(np >= 0 && nw >= 0, KH_ERR_INVALID_ARGUMENT, "kh_residual_norms: negative size");
+  if (np == 0) return KH_OK;
+  KH_REQUIRE(R && R_obs && chi, KH_ERR_INVALID_ARGUMENT, "kh_residual_norms: NULL pointer");
+  hipLaunchKernelGGL(residual_norm_kernel, dim3((unsigned)np), dim3(RN_BLOCK), 0,
+                     as_stream(stream), reinterpret_cast<const cd*>(R),
+                     reinterpret_cast<const cd*>(R_obs), nw, chi);
+  return check_launch("kh_residual_norms");
+}
 
 extern "C" size_t kh_gram_workspace_bytes(int P, long K) {
   if (P <= 0 || K <= 0) return 0;

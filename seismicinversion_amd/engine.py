@@ -20,6 +20,11 @@ __all__ = [
     "forward_grid",
     "vertical_slowness",
     "interface_coefficients",
+    "bessel_sum",
+    "hermitian_traces",
+    "interfaces_direct",
+    "residual_norms",
+    "gram",
 ]
 
 
@@ -213,6 +218,25 @@ def interfaces_direct(p, eta1, neta1, rho1, beta1, eta2, neta2, rho2, beta2, oce
         ptr(c[4]), ptr(f[2]), ptr(c[5]), *[_abi.ptr(o) for o in outs], _abi.stream_handle(dev))
     _abi.check(rc, "kh_interfaces_direct")
     return tuple(outs)
+
+
+def residual_norms(R: torch.Tensor, R_obs: torch.Tensor,
+                   out: torch.Tensor | None = None) -> torch.Tensor:
+    """chi (np,) float64: sum over frequency of |R - R_obs|^2 per slowness row
+    (kh_residual_norms, fixed summation order)."""
+    check_precision(R, "R")
+    check_precision(R_obs, "R_obs")
+    if R.shape != R_obs.shape or R.dim() != 2:
+        raise ValueError("R and R_obs must both be (np, nw)")
+    R = R.to(torch.complex128).contiguous()
+    R_obs = R_obs.to(device=R.device, dtype=torch.complex128).contiguous()
+    n_p, nw = R.shape
+    if out is None:
+        out = torch.empty(n_p, dtype=torch.float64, device=R.device)
+    rc = _abi.lib().kh_residual_norms(_abi.ptr(R), _abi.ptr(R_obs), n_p, nw, _abi.ptr(out),
+                                      _abi.stream_handle(R.device))
+    _abi.check(rc, "kh_residual_norms")
+    return out
 
 
 def gram(X: torch.Tensor, scale: float = 1.0, out: torch.Tensor | None = None) -> torch.Tensor:

@@ -60,6 +60,7 @@ class InvertSpec:
     seed: int = 42
     hessian_mode: str = "hvp"
     max_trials: int = 10
+    use_graphs: bool = False  # replay each iteration's device work from HIP graphs
 
     @property
     def n_layers(self) -> int:
@@ -292,7 +293,7 @@ class DeviceBackend:
             res = invert_taup(spec.true_model, list(spec.p), nfreq=spec.nfreq,
                               perturbation=spec.perturbation, max_iter=spec.iterations,
                               seed=spec.seed, tol=0.0, hessian_mode=spec.hessian_mode,
-                              max_trials=spec.max_trials)
+                              max_trials=spec.max_trials, use_graphs=spec.use_graphs)
             self.lm_stream.synchronize()
         return res
 

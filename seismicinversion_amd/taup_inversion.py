@@ -57,12 +57,55 @@ class InversionResult:
     converged: bool = False
 
 
+class _Graphed:
+    """A device computation of a (nl, 6) layer table, captured once as a HIP
+    graph (torch.cuda.CUDAGraph) and replayed for every new table.
+
+    The table is copied into a static input before each replay; the outputs
+    are static tensors overwritten by the next replay, so callers copy them
+    out first.  Capture is thread-local (other threads may use the device
+    meanwhile).  The captured work is this library's kernels and small torch
+    reductions along short axes; none of it records a memset node."""
+
+    def __init__(self, fn, layers: torch.Tensor):
+        dev = layers.device
+        self.inp = layers.detach().clone()
+        cur = torch.cuda.current_stream(dev)
+        side = torch.cuda.Stream(device=dev)
+        side.wait_stream(cur)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.stream(side):
+            for _ in range(2):  # settle allocations before capture
+                fn(self.inp)
+            # capture on the side stream; unlike the torch.cuda.graph context
+            # this neither synchronises the whole device nor empties the
+            # allocator cache, so other threads' work is not held up
+            self.graph.capture_begin(capture_error_mode="thread_local")
+            try:
+                self.out = fn(self.inp)
+            finally:
+                self.graph.capture_end()
+        cur.wait_stream(side)
+
+    def __call__(self, layers: torch.Tensor):
+        self.inp.copy_(layers)
+        self.graph.replay()
+        return self.out
+
+
 class _Problem:
     """Device-resident inversion data: reference layer table, slownesses,
-    frequencies, observed reflectivity and the packed-parameter map."""
+    frequencies, observed reflectivity and the packed-parameter map.
+
+    With ``use_graphs`` the two fixed-shape device computations of an LM
+    iteration — misfit, gradient and Hessian of a model, and the misfit of a
+    trial model — are captured as HIP graphs on first use and replayed; the
+    host keeps the data-dependent control (solve, accept/reject, damping).
+    Replays run exactly the kernels of the eager path, so both give
+    identical numbers."""
 
     def __init__(self, alpha_ref, beta_ref, rho_ref, thickness_ref, Q_alpha, Q_beta, p_values,
-                 omega, R_obs, free_surface=False):
+                 omega, R_obs, free_surface=False, use_graphs: bool = False):
         self.dev = _abi.device()
         cols = [torch.as_tensor(x, dtype=_F).detach().cpu() for x in
                 (alpha_ref, beta_ref, rho_ref, thickness_ref, Q_alpha, Q_beta)]
@@ -83,6 +126,8 @@ class _Problem:
         for k, (l, c) in enumerate(idx):
             basis[k, l, c] = 1.0
         self.basis = basis.to(self.dev)
+        self.use_graphs = use_graphs
+        self._graphs: dict[str, _Graphed] = {}
 
     def table(self, phys: np.ndarray) -> torch.Tensor:
         a, b, r, h = _unpack_params(torch.as_tensor(phys, dtype=_F), *self.refs[:4])
@@ -90,13 +135,23 @@ class _Problem:
         tab = torch.stack([a, b, r, h, self.refs[4], self.refs[5]], dim=1)
         return engine.layers_tensor(tab, self.dev)
 
-    def misfit(self, phys: np.ndarray) -> float:
-        R = engine.forward(self.table(phys), self.p, self.omega, free_surface=self.free_surface)
-        res = R - self.R_obs
-        return float((res.real ** 2 + res.imag ** 2).sum().item())
+    def _run(self, key, fn, layers):
+        if not self.use_graphs:
+            return fn(layers)
+        g = self._graphs.get(key)
+        if g is None:
+            g = self._graphs[key] = _Graphed(fn, layers)
+        return g(layers)
 
-    def grad_hessian(self, phys: np.ndarray, mode: str = "gram"):
-        layers = self.table(phys)
+    def _misfit_rows(self, layers: torch.Tensor) -> torch.Tensor:
+        R = engine.forward(layers, self.p, self.omega, free_surface=self.free_surface)
+        return engine.residual_norms(R, self.R_obs)
+
+    def misfit(self, phys: np.ndarray) -> float:
+        chi = self._run("misfit", self._misfit_rows, self.table(phys))
+        return float(np.sum(chi.cpu().numpy()))
+
+    def _grad_hessian_dev(self, layers: torch.Tensor, mode: str):
         if mode == "hvp":
             chi, grad, hvp, _ = derivatives.misfit_hvp(layers, self.basis, self.p, self.omega,
                                                        self.R_obs, self.free_surface)
@@ -110,7 +165,14 @@ class _Problem:
         else:
             raise ValueError(f"unknown hessian mode {mode!r}")
         g = grad.sum(0)[self.li, self.ci]
-        return float(chi.sum().item()), g.cpu().numpy(), H.cpu().numpy()
+        return chi, g, H
+
+    def grad_hessian(self, phys: np.ndarray, mode: str = "gram"):
+        if mode not in ("hvp", "gram"):
+            raise ValueError(f"unknown hessian mode {mode!r}")
+        chi, g, H = self._run("gh_" + mode, lambda L: self._grad_hessian_dev(L, mode),
+                              self.table(phys))
+        return float(np.sum(chi.cpu().numpy())), g.cpu().numpy(), H.cpu().numpy()
 
 
 def _log_transform(m, g, H):
@@ -153,8 +215,10 @@ def _to_model(tab_cols, ref_model: LayerModel) -> LayerModel:
 def invert_taup(true_model: LayerModel, p_values: list[float], nfreq: int = 64,
                 perturbation: float = 0.15, max_iter: int = 100, seed: int = 42,
                 tol: float = 1e-8, hessian_mode: str = "gram",
-                max_trials: int = 10) -> InversionResult:
-    """Newton-LM inversion from a perturbed start model (taup_inversion.py:262-470)."""
+                max_trials: int = 10, use_graphs: bool = False) -> InversionResult:
+    """Newton-LM inversion from a perturbed start model (taup_inversion.py:262-470).
+    ``use_graphs`` replays the iteration's device work from captured HIP
+    graphs (same numbers, fewer launches from the host)."""
     t = model_to_tensors(true_model)
     refs = (t["alpha"], t["beta"], t["rho"], t["thickness"], t["Q_alpha"], t["Q_beta"])
     dw = 2.0 * np.pi / 64.0
@@ -162,7 +226,7 @@ def invert_taup(true_model: LayerModel, p_values: list[float], nfreq: int = 64,
     dev = _abi.device()
     obs = engine.forward(engine.layers_tensor(true_model, dev),
                          torch.tensor([float(p) for p in p_values], dtype=_F), omega.to(_C))
-    prob = _Problem(*refs, p_values, omega, obs)
+    prob = _Problem(*refs, p_values, omega, obs, use_graphs=use_graphs)
 
     true_params = _pack_params(*refs[:4]).numpy()
     n_params = len(true_params)

@@ -70,12 +70,43 @@ def test_j0_against_scipy():
 
 
 def test_hermitian_traces_vs_numpy_fft():
+    """Half-length complex FFT of the Hermitian vector vs numpy.fft on the full
+    one, up to nt = 8192 (and 16384 where the device allows 128 KiB of LDS
+    per workgroup)."""
     rng = np.random.default_rng(3)
-    for nw in (2, 64, 1024, 2048):
+    sizes = [1, 2, 4, 64, 1024, 2048, 4096]
+    if torch.cuda.get_device_properties(0).shared_memory_per_block >= 8192 * 16:
+        sizes.append(8192)
+    for nw in sizes:
         U = rng.standard_normal((5, nw - 1)) + 1j * rng.standard_normal((5, nw - 1))
         tr = engine.hermitian_traces(torch.from_numpy(U).cuda(), nw, 0.049, 64.0).cpu().numpy()
         _, expect = K.hermitian_ifft(U, nw, 0.049, 64.0)
+        assert tr.shape == (5, 2 * nw)
+        if nw == 1:
+            assert np.all(tr == 0.0)
+            continue
         assert rel(tr, expect) < 1e-13, nw
+
+
+def test_gather_at_reference_defaults():
+    """compute_gather_gpu's own defaults (nw = 2048 -> nt = 4096, 4096
+    slownesses up to 1.2 s/km): traces against the oracle's Bessel sum and
+    FFT of the same device reflectivity."""
+    m = seeded_model(5, 42)
+    offsets = np.array([1.0, 7.5])
+    t, off, g = compute_gather_gpu(m, offsets)
+    assert g.shape == (2, 4096) and np.all(np.isfinite(g))
+    dev = torch.device("cuda")
+    gamma = np.pi / 64.0
+    w = np.arange(1, 2048) * (2 * np.pi / 64.0)
+    S = K.ricker_spectrum(w, 2048 * 2 * np.pi / 64.0)  # the reference's default source
+    _, R, U = gather_device(engine.layers_tensor(m, dev), torch.from_numpy(offsets).to(dev), 64.0,
+                            2048, 4096, 1.2, gamma, torch.from_numpy(S).to(dev), False, keep=True)
+    pn = np.arange(1, 4097) * (1.2 / 4096)
+    Uo = K.bessel_sum(R.cpu().numpy(), pn, w, offsets, 1.2 / 4096, p_max=pn[-1]) * S[None, :]
+    assert rel(U.cpu().numpy(), Uo) < 1e-12
+    _, tro = K.hermitian_ifft(Uo, 2048, gamma, 64.0)
+    assert rel(g, tro) < 1e-12
 
 
 def test_bessel_sum_linearity_and_zero_offset():

@@ -190,3 +190,56 @@ def test_compute_taup_traces_vs_oracle():
         R = K.reflectivity(m, pv, w)
         _, expect = K.hermitian_ifft((R * S)[None], 512, 0.0, 64.0)
         assert rel(tr[pv], expect[0]) < 1e-11
+
+
+def test_residual_norms_kernel():
+    rng = np.random.default_rng(3)
+    for n_p, nw in ((1, 1), (7, 300), (33, 1023)):
+        R = torch.tensor(rng.standard_normal((n_p, nw)) + 1j * rng.standard_normal((n_p, nw)),
+                         device="cuda")
+        Ro = torch.tensor(rng.standard_normal((n_p, nw)) + 1j * rng.standard_normal((n_p, nw)),
+                          device="cuda")
+        chi = engine.residual_norms(R, Ro).cpu().numpy()
+        want = (np.abs(R.cpu().numpy() - Ro.cpu().numpy()) ** 2).sum(1)
+        assert np.max(np.abs(chi - want) / want) < 1e-13
+        assert torch.equal(engine.residual_norms(R, Ro), engine.residual_norms(R, Ro))
+
+
+@pytest.mark.parametrize("mode", ["gram", "hvp"])
+def test_lm_graph_replays_match_eager(mode):
+    """Each replay of the captured misfit / gradient / Hessian graphs, with
+    eager work and synchronisations in between, equals the eager result bit
+    for bit."""
+    from seismicinversion_amd.taup_inversion import _Problem
+
+    m = seeded_model(7, 5)
+    tens = kt.model_to_tensors(m)
+    refs = (tens["alpha"], tens["beta"], tens["rho"], tens["thickness"], tens["Q_alpha"],
+            tens["Q_beta"])
+    p_values = list(np.linspace(0.05, 0.5, 12))
+    w = torch.arange(1, 97, dtype=torch.float64) * (2 * np.pi / 64)
+    obs = engine.forward(engine.layers_tensor(m), torch.tensor(p_values), w.to(torch.complex128))
+    eager = _Problem(*refs, p_values, w, obs)
+    graphed = _Problem(*refs, p_values, w, obs, use_graphs=True)
+    base = kt._pack_params(*refs[:4]).numpy()
+    rng = np.random.default_rng(11)
+    for it in range(4):
+        phys = base * rng.uniform(0.95, 1.05, base.size)
+        chi_e, g_e, H_e = eager.grad_hessian(phys, mode)
+        mis_e = eager.misfit(phys * 1.01)
+        torch.cuda.synchronize()
+        chi_r, g_r, H_r = graphed.grad_hessian(phys, mode)
+        mis_r = graphed.misfit(phys * 1.01)
+        assert chi_r == chi_e and mis_r == mis_e, f"replay {it}"
+        assert np.array_equal(g_r, g_e) and np.array_equal(H_r, H_e), f"replay {it}"
+    assert set(graphed._graphs) == {"gh_" + mode, "misfit"}
+
+
+def test_invert_taup_with_graphs_equals_eager():
+    m = seeded_model(6, 8)
+    p = list(np.linspace(0.05, 0.45, 10))
+    a = invert_taup(m, p, nfreq=48, perturbation=0.05, max_iter=4, tol=0.0)
+    b = invert_taup(m, p, nfreq=48, perturbation=0.05, max_iter=4, tol=0.0, use_graphs=True)
+    assert a.misfit_history == b.misfit_history
+    assert a.grad_norm_history == b.grad_norm_history
+    assert np.array_equal(a.recovered_model.alpha, b.recovered_model.alpha)
