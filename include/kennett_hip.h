@@ -89,7 +89,7 @@ int kh_forward(const double* layers, int nl, const double* p, int np,
 /* kh_forward with an explicit launch shape: `block` work-items per
  * workgroup (64, 128 or 256) and `omega_per_item` frequencies
  * walked by each work-item (1, 2, 3, 4 or 8).  Results are identical to
- * kh_forward's for every shape. */
+ * kh_forward's for every shape (every phase factor is evaluated directly). */
 int kh_forward_ex(const double* layers, int nl, const double* p, int np,
                   const double* omega, int nw, int free_surface, double* R,
                   int block, int omega_per_item, void* stream);
@@ -104,7 +104,10 @@ int kh_forward_grid(const double* layers, int nl, const double* p, int np,
                     double k0, double dw, double gamma, int nw, int free_surface,
                     double* R, void* stream);
 
-/* kh_forward_grid with an explicit launch shape (as kh_forward_ex). */
+/* kh_forward_grid with an explicit launch shape (as kh_forward_ex).  The
+ * phase recurrence steps by block*dw, so results are bitwise identical for
+ * equal `block` (any omega_per_item) and agree to rounding (about 1e-14
+ * relative) across block sizes. */
 int kh_forward_grid_ex(const double* layers, int nl, const double* p, int np,
                        double k0, double dw, double gamma, int nw, int free_surface,
                        double* R, int block, int omega_per_item, void* stream);
@@ -135,7 +138,7 @@ long long kh_multi_items(int np, int nw, int block, int omega_per_item);
  * nonfinite = NULL or a device int array (nmodels,) that receives, for each
  * desc entry k, the number of NaN/Inf outputs of that model (reset by the call);
  * workgroups <= 0 picks CUs x resident workgroups.  block in {64, 128, 256},
- * omega_per_item in {1, 2, 4}.  Each model's output equals kh_forward_grid_ex
+ * omega_per_item in {1, 2, 4, 8}.  Each model's output equals kh_forward_grid_ex
  * of that model alone with the same block and omega_per_item, bit for bit.
  * Replaces a loop of kennett_reflectivity_batch calls, one per model
  * (Kennett_Reflectivity/kennett_reflectivity_gpu.py:88). */

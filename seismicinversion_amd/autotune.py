@@ -19,7 +19,7 @@ import torch
 
 __all__ = ["CANDIDATES", "multi_shape", "forward_shape", "cache", "save", "load", "clear"]
 
-CANDIDATES = [(256, 4), (256, 2), (128, 2), (256, 1), (64, 2), (64, 1)]
+CANDIDATES = [(64, 8), (128, 8), (64, 4), (128, 4), (256, 4), (256, 2), (64, 2), (64, 1)]
 _LOCK = threading.Lock()
 _CACHE: dict[str, tuple[int, int]] = {}
 

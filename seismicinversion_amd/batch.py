@@ -116,8 +116,9 @@ def plan_multi(specs: list[ModelSpec], block: int, wpt: int, dev=None,
     return MultiPlan(desc=_to_device(desc.view(np.uint8), dev, non_blocking), layers=layers, p=p, R=R,
                      r_off=r_off, shapes=[(len(s.p), s.nw) for s in specs], total_items=item,
                      max_nl=max(t.shape[0] for t in tabs) if tabs else 2, block=block, wpt=wpt,
-                     counter=torch.zeros(1, dtype=torch.int64, device=dev),
-                     nonfinite=torch.zeros(max(len(specs), 1), dtype=torch.int32, device=dev),
+                     # both are reset by the launch itself (no separate fill kernels)
+                     counter=torch.empty(1, dtype=torch.int64, device=dev),
+                     nonfinite=torch.empty(max(len(specs), 1), dtype=torch.int32, device=dev),
                      order=order)
 
 

@@ -35,7 +35,7 @@ __host__ __device__ __forceinline__ M2<Z> step_fwd(const M2<Z>& r, const Z& ea, 
   const Z a01 = -(f.ru00 * m01 + f.ru01 * m11);
   const Z a10 = -(f.ru01 * m00 + f.ru11 * m10);
   const Z a11 = one - (f.ru01 * m01 + f.ru11 * m11);
-  const Z rdet = recip(a00 * a11 - a01 * a10);
+  const Z rdet = recip_det(a00 * a11 - a01 * a10);
   const Z u00 = a11 * rdet, u01 = -a01 * rdet, u10 = -a10 * rdet, u11 = a00 * rdet;
   const Z x00 = m00 * u00 + m01 * u10, x01 = m00 * u01 + m01 * u11;
   const Z x10 = m10 * u00 + m11 * u10, x11 = m10 * u01 + m11 * u11;
@@ -64,7 +64,7 @@ __host__ __device__ __forceinline__ void step_rev(const M2<Z>& r, const Z& ea, c
   const Z a01 = -(f.ru00 * m01 + f.ru01 * m11);
   const Z a10 = -(f.ru01 * m00 + f.ru11 * m10);
   const Z a11 = one - (f.ru01 * m01 + f.ru11 * m11);
-  const Z rdet = recip(a00 * a11 - a01 * a10);
+  const Z rdet = recip_det(a00 * a11 - a01 * a10);
   const Z u00 = a11 * rdet, u01 = -a01 * rdet, u10 = -a10 * rdet, u11 = a00 * rdet;
   const Z x00 = m00 * u00 + m01 * u10, x01 = m00 * u01 + m01 * u11;
   const Z x10 = m10 * u00 + m11 * u10, x11 = m10 * u01 + m11 * u11;

@@ -48,6 +48,17 @@ __host__ __device__ __forceinline__ cd operator/(cd a, cd b) {
 // 1 / b with the same scaling as the division above.
 __host__ __device__ __forceinline__ cd recip(cd b) { return cd{1.0, 0.0} / b; }
 
+// 1 / det for the 2x2 determinant of the recursion step, det = 1 - O(|Ru MT|):
+// of order one, so |det|^2 neither overflows nor underflows and
+// conj(det) / |det|^2 needs one division and no branch (Smith's scaling,
+// two divisions behind a data-dependent branch, guards ranges this value
+// never reaches).  A singular step (det = 0) gives non-finite values, as the
+// reference's division does.
+__host__ __device__ __forceinline__ cd recip_det(cd b) {
+  const double inv = 1.0 / fma(b.r, b.r, b.i * b.i);
+  return cd{b.r * inv, -b.i * inv};
+}
+
 __host__ __device__ __forceinline__ cd csqrt(cd z) {
   const double x = z.r, y = z.i;
   if (x == 0.0 && y == 0.0) return cd{0.0, y};
@@ -57,9 +68,58 @@ __host__ __device__ __forceinline__ cd csqrt(cd z) {
   return cd{fabs(y) / (2.0 * t), copysign(t, y)};
 }
 
+// (sin x, cos x) for the unit-modulus phase factors exp(i x).
+// Reduction by the nearest multiple k of pi/2, with pi/2 split into three
+// doubles P1 + P2 + P3 (Cody and Waite, "Software Manual for the Elementary
+// Functions", 1980) and each k * Pj removed by one FMA; then the Taylor
+// series of sin and cos on |r| <= pi/4, through r^17 and r^18 (the next
+// terms are below 1e-19).  The absolute error is a few units of 2^-53 for
+// |x| < 1e8, which is what a factor of modulus 1 needs (the argument's own
+// rounding, ulp(x), is far larger); larger or non-finite arguments take the
+// device library's sincos.
+__host__ __device__ __forceinline__ void sincos_phase(double x, double* s, double* c) {
+  if (!(fabs(x) < 1.0e8)) {
+    sincos(x, s, c);
+    return;
+  }
+  const double k = rint(x * 0.6366197723675814);  // 2 / pi
+  double r = fma(-k, 1.5707963267948966, x);
+  r = fma(-k, 6.123233995736766e-17, r);
+  r = fma(-k, -1.4973849048591698e-33, r);
+  const double z = r * r;
+  double ps = 2.8114572543455206e-15;  // (-1)^n / (2n+1)!, n = 8 .. 1
+  ps = fma(ps, z, -7.647163731819816e-13);
+  ps = fma(ps, z, 1.6059043836821613e-10);
+  ps = fma(ps, z, -2.505210838544172e-08);
+  ps = fma(ps, z, 2.7557319223985893e-06);
+  ps = fma(ps, z, -0.0001984126984126984);
+  ps = fma(ps, z, 0.008333333333333333);
+  ps = fma(ps, z, -0.16666666666666666);
+  const double sr = fma(r * z, ps, r);
+  double pc = -1.5619206968586225e-16;  // (-1)^n / (2n)!, n = 9 .. 2
+  pc = fma(pc, z, 4.779477332387385e-14);
+  pc = fma(pc, z, -1.1470745597729725e-11);
+  pc = fma(pc, z, 2.08767569878681e-09);
+  pc = fma(pc, z, -2.755731922398589e-07);
+  pc = fma(pc, z, 2.48015873015873e-05);
+  pc = fma(pc, z, -0.001388888888888889);
+  pc = fma(pc, z, 0.041666666666666664);
+  const double hz = 0.5 * z;
+  const double w = 1.0 - hz;
+  const double cr = w + (((1.0 - w) - hz) + (z * z) * pc);
+  // x = k pi/2 + r: quadrant q = k mod 4 rotates (sin r, cos r)
+  const int q = (int)k & 3;
+  double ss = (q & 1) ? cr : sr;
+  double cc = (q & 1) ? sr : cr;
+  if (q == 1 || q == 2) cc = -cc;
+  if (q >= 2) ss = -ss;
+  *s = (x == 0.0) ? x : ss;  // sin(+-0) = +-0
+  *c = cc;
+}
+
 __host__ __device__ __forceinline__ cd cexp(cd z) {
   double s, c;
-  sincos(z.i, &s, &c);
+  sincos_phase(z.i, &s, &c);
   const double e = exp(z.r);
   return cd{e * c, e * s};
 }

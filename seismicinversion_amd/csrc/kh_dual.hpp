@@ -57,6 +57,11 @@ __host__ __device__ __forceinline__ Dual<T> recip(const Dual<T>& b) {
   return {r, -(r * r) * b.d};
 }
 template <class T>
+__host__ __device__ __forceinline__ Dual<T> recip_det(const Dual<T>& b) {
+  const T r = recip_det(b.v);
+  return {r, -(r * r) * b.d};
+}
+template <class T>
 __host__ __device__ __forceinline__ Dual<T> conj(const Dual<T>& a) {
   return {conj(a.v), conj(a.d)};
 }

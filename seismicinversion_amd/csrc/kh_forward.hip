@@ -134,17 +134,24 @@ size_t forward_lds_bytes(int nl) {
   return (size_t)nl * (sizeof(LayerPt) + sizeof(LayerStep)) + (size_t)(nl - 1) * sizeof(Iface);
 }
 
-// Default launch shape (measured on MI355X, 20-layer 8192 x 1023 gather grid:
-// 4 frequencies per work-item 1.48 ms, 2 per item 1.62 ms, 1 per item 2.06 ms,
-// all at 256 work-items): the most frequencies per work-item (4, 2, 1) that
-// still leaves >= 1024 workgroups.
+// Default launch shape, measured on MI355X (clocks warmed up, interleaved
+// rounds) on the 20-layer gather grid, 8192 p x 1023 omega, free surface
+// (work-items per workgroup x frequencies per work-item):
+//   uniform grid: 128 x 4 1.384 ms, 256 x 4 1.394, 64 x 4 1.399, 64 x 8 1.445,
+//                 128 x 8 1.450, 256 x 2 1.592 ms.
+// Four frequencies per work-item share one phase exponential pair per layer;
+// eight need more registers than the sharing saves.  Picks 128 work-items and
+// the most frequencies per work-item (4, 2, 1) that keeps >= 1024 workgroups
+// and leaves less than a quarter of the last tile's lanes idle.
 static void default_shape(int np, int nw, bool /*grid*/, int* block, int* wpt) {
-  *block = (nw <= 64) ? 64 : 256;
+  *block = nw <= 64 ? 64 : 128;
   *wpt = 1;
   const int cands[2] = {4, 2};
   for (int c : cands) {
-    const long tiles = (nw + (long)c * *block - 1) / ((long)c * *block);
-    if (tiles * (long)np >= 1024) {
+    const long span = (long)c * *block;
+    const long tiles = (nw + span - 1) / span;
+    const long idle = tiles * span - nw;
+    if (tiles * (long)np >= 1024 && 4 * idle < span) {
       *wpt = c;
       break;
     }

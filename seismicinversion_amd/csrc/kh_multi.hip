@@ -146,6 +146,7 @@ extern "C" int kh_forward_multi(const kh_model_desc* desc, int nmodels, long lon
       case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, multi_kernel<1>, block, lds); break;
       case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, multi_kernel<2>, block, lds); break;
       case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, multi_kernel<4>, block, lds); break;
+      case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, multi_kernel<8>, block, lds); break;
       default: break;
     }
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
@@ -161,8 +162,9 @@ extern "C" int kh_forward_multi(const kh_model_desc* desc, int nmodels, long lon
     case 1: return launch_multi<1>(desc, nmodels, total_items, layers_all, p_all, R, block, max_nl, c, nonfinite, wgs, st);
     case 2: return launch_multi<2>(desc, nmodels, total_items, layers_all, p_all, R, block, max_nl, c, nonfinite, wgs, st);
     case 4: return launch_multi<4>(desc, nmodels, total_items, layers_all, p_all, R, block, max_nl, c, nonfinite, wgs, st);
+    case 8: return launch_multi<8>(desc, nmodels, total_items, layers_all, p_all, R, block, max_nl, c, nonfinite, wgs, st);
     default:
-      set_error("kh_forward_multi: omega_per_item %d not in {1, 2, 4}", omega_per_item);
+      set_error("kh_forward_multi: omega_per_item %d not in {1, 2, 4, 8}", omega_per_item);
       return KH_ERR_INVALID_ARGUMENT;
   }
 }

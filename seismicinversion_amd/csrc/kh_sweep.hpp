@@ -60,7 +60,7 @@ __host__ __device__ __forceinline__ void kennett_step(cd& r00, cd& r01, cd& r10,
   const cd a01 = -(f.ru00 * m01 + f.ru01 * m11);
   const cd a10 = -(f.ru01 * m00 + f.ru11 * m10);
   const cd a11 = one - (f.ru01 * m01 + f.ru11 * m11);
-  const cd rdet = recip(a00 * a11 - a01 * a10);
+  const cd rdet = recip_det(a00 * a11 - a01 * a10);
   const cd u00 = a11 * rdet, u01 = -a01 * rdet, u10 = -a10 * rdet, u11 = a00 * rdet;
   // X = MT U
   const cd x00 = m00 * u00 + m01 * u10, x01 = m00 * u01 + m01 * u11;

@@ -138,7 +138,7 @@ class DeviceBackend:
     batches at ``max_batch_bytes`` (the staging size); a single request larger
     than that gets a one-off pinned buffer."""
 
-    def __init__(self, n_streams: int = 3, shape=(256, 4), device=None,
+    def __init__(self, n_streams: int = 3, shape=(64, 8), device=None,
                  staging_bytes: int = 512 << 20, copy_threads: int = 8,
                  kernel_timing: bool = False):
         import torch

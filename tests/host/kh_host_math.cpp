@@ -65,3 +65,8 @@ extern "C" int host_forward_grid(const double* layers, int nl, const double* p, 
   }
   return 0;
 }
+
+// The phase factors' sine and cosine (csrc/kh_complex.hpp) for n arguments.
+extern "C" void host_sincos_phase(const double* x, int n, double* s, double* c) {
+  for (int i = 0; i < n; ++i) kh::sincos_phase(x[i], s + i, c + i);
+}

@@ -21,7 +21,7 @@ def specs_c3(n_models, nw=1024, seed=42):
             for m, p in uneven_batch_models(n_models, seed)]
 
 
-@pytest.mark.parametrize("block,wpt", [(256, 2), (128, 1), (256, 4), (64, 2)])
+@pytest.mark.parametrize("block,wpt", [(256, 2), (128, 1), (256, 4), (64, 2), (64, 8), (128, 8)])
 def test_multi_equals_plain_bitwise(block, wpt):
     specs = specs_c3(20)
     specs[3].free_surface = True

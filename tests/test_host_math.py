@@ -9,6 +9,7 @@ judged per slowness row against the measured 1-ulp sensitivity."""
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 import shutil
 import subprocess
@@ -88,3 +89,38 @@ def test_host_math_uniform_grid_path(host_lib, block):
     w = np.arange(1, nw + 1) * dw + 1j * gamma
     O = K.reflectivity_batch(m, p, w, True)
     assert_rows_close(R, O, row_tolerance(m, p, w, True), "grid path")
+
+
+def test_phase_sincos_vs_libm(host_lib):
+    """The kernels' own sin/cos (Cody-Waite reduction + Taylor series) against
+    the C library's, in absolute terms (the phase factors have modulus 1):
+    random arguments over the kernels' range, neighbourhoods of multiples of
+    pi/2, tiny and huge arguments, signed zeros and non-finite values."""
+    rng = np.random.default_rng(5)
+    k = np.arange(-2000, 2001, dtype=np.float64)
+    x = np.concatenate([
+        rng.uniform(-3000.0, 3000.0, 200000),
+        rng.uniform(-1.0, 1.0, 20000),
+        (k * (np.pi / 2))[:, None] + rng.uniform(-1e-6, 1e-6, (k.size, 4)),
+        rng.uniform(-1e8, 1e8, 20000),          # up to the reduction's bound
+        rng.uniform(1e8, 1e12, 200) * rng.choice([-1, 1], 200),  # library path
+        np.array([0.0, -0.0, 1e-300, -1e-300, 5e-324, np.pi / 4, -np.pi / 4,
+                  np.pi / 2, np.pi, 1e8 - 1.0]),
+    ], axis=None)
+    x = np.ascontiguousarray(x)
+    s = np.empty_like(x)
+    c = np.empty_like(x)
+    vp = ctypes.c_void_p
+    host_lib.host_sincos_phase(vp(x.ctypes.data), x.size, vp(s.ctypes.data), vp(c.ctypes.data))
+    es = np.array([math.sin(v) for v in x])
+    ec = np.array([math.cos(v) for v in x])
+    err = np.hypot(s - es, c - ec)
+    assert err.max() <= 3.5e-16, (err.max(), x[err.argmax()])
+    # small arguments keep full relative accuracy in sin, and the sign of zero
+    small = np.abs(x) < 0.5
+    assert np.all(np.abs(s[small] - es[small]) <= 2.3e-16 * np.abs(es[small]) + 5e-324)
+    assert np.signbit(s[x.size - 9]) and not np.signbit(s[x.size - 10])
+    bad = np.array([np.nan, np.inf, -np.inf])
+    sb, cb = np.empty(3), np.empty(3)
+    host_lib.host_sincos_phase(vp(bad.ctypes.data), 3, vp(sb.ctypes.data), vp(cb.ctypes.data))
+    assert np.all(np.isnan(sb)) and np.all(np.isnan(cb))

@@ -31,7 +31,7 @@ def test_results_identical_alone_and_in_the_mix():
     items = [it for it in serving_stream(2000.0, 60, seed=7, lm_fraction=0.0,
                                          np_range=(16, 256), omegas=(256, 1024))]
     specs = [spec_of(it) for it in items]
-    backend = DeviceBackend(n_streams=2, shape=(256, 4))
+    backend = DeviceBackend(n_streams=2, shape=(64, 8))
     futs = [None] * len(specs)
     with KennettServer(backend=backend) as srv:
         inv = srv.submit_invert(InvertSpec(true_model=seeded_model(5, 3),
@@ -57,8 +57,8 @@ def test_results_identical_alone_and_in_the_mix():
     assert all(r["nonfinite"] == 0 for r in srv.records if r["kind"] == "forward")
     for s, R in zip(specs, outs):
         alone = engine.forward_grid(engine.layers_tensor(s.model), torch.from_numpy(s.p), s.k0,
-                                    s.dw, s.gamma, s.nw, s.free_surface, block=256,
-                                    omega_per_item=4).cpu().numpy()
+                                    s.dw, s.gamma, s.nw, s.free_surface, block=64,
+                                    omega_per_item=8).cpu().numpy()
         if isinstance(R, torch.Tensor):  # device-resident result
             assert R.is_cuda
             R = R.cpu().numpy()
