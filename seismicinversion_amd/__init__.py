@@ -26,6 +26,8 @@ _EXPORTS = {
     "ricker_spectrum": "source",
     "ricker_wavelet": "source",
     "compute_gather": "kennett_gather",
+    "plot_gather": "kennett_gather",
+    "compute_seismogram": "kennett_seismogram",
     "default_ocean_crust_model": "workloads",
     "compute_gather_gpu": "kennett_gather_gpu",
     "kennett_reflectivity_batch": "kennett_reflectivity_gpu",

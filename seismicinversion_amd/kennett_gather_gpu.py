@@ -25,7 +25,7 @@ from .layer_model import LayerModel
 from .source import ricker_spectrum
 from .workloads import default_ocean_crust_model
 
-__all__ = ["compute_gather_gpu", "default_ocean_crust_model", "gather_device"]
+__all__ = ["compute_gather_gpu", "default_ocean_crust_model", "gather_device", "main"]
 
 
 def gather_device(layers: torch.Tensor, offsets: torch.Tensor, T: float, nw: int, np_slow: int,
@@ -81,3 +81,20 @@ def compute_gather_gpu(
     """(time (nt,), offsets (nr,), gather (nr, nt)) with nt = 2*nw."""
     return _gather(model, offsets, T, nw, np_slow, p_max, gamma, source_func, free_surface,
                    device, taper_p_max=None)
+
+
+def main(argv: list[str] | None = None) -> None:
+    """CLI of kennett_gather_gpu.py (defaults: offsets to 50 km, 4096 slownesses up to 1.2 s/km)."""
+    from .kennett_gather import gather_cli
+
+    gather_cli(argv, description="GPU seismogram gather via Kennett reflectivity.",
+               r_max=50.0, np_slow=4096, p_max=1.2, t_max=30.0, output="gather_gpu.png",
+               workers_flag=False,
+               compute=lambda m, off, a: compute_gather_gpu(
+                   m, off, T=a.duration, nw=a.nw, np_slow=a.np_slow, p_max=a.p_max,
+                   gamma=a.gamma, free_surface=a.free_surface,
+                   device=torch.device(a.device) if a.device else None))
+
+
+if __name__ == "__main__":
+    main()
