@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--requests", type=int, default=400)
     ap.add_argument("--rate", type=float, default=200.0, help="open-loop arrivals per second")
     ap.add_argument("--open-seconds", type=float, default=4.0)
+    ap.add_argument("--cpu-workers", type=int, default=min(16, os.cpu_count() or 1),
+                    help="processes of the all-core CPU baseline (the box's share is 16 cores)")
     ap.add_argument("--warmup-seconds", type=float, default=2.0,
                     help="untimed forward launches before anything is timed (clock ramp)")
     ap.add_argument("--traffic-json", default=os.path.join(HERE, "profiles", "traffic_multi_served.json"),
@@ -114,23 +116,90 @@ def event_ms(fn, reps, torch):
     return float(np.mean([a.elapsed_time(b) for a, b in ev]))
 
 
-def cpu_baseline(specs):
-    """The oracle (NumPy, one process) on the middle slowness row of every request."""
+# Work shared with forked CPU workers (set before the pool is created).
+_CPU = {}
+
+
+def _cpu_rows(chunk):
+    """Worker: the oracle on `rows` evenly spaced slowness rows of requests lo..hi."""
     from oracle import kennett_numpy as K
+
+    lo, hi, rows = chunk
+    upd = 0
+    for s in _CPU["specs"][lo:hi]:
+        idx = np.unique(np.linspace(0, len(s.p) - 1, min(rows, len(s.p))).round().astype(int))
+        w = (s.k0 + np.arange(s.nw)) * s.dw + 1j * s.gamma
+        K.reflectivity_batch(s.model, s.p[idx], w, s.free_surface)
+        upd += len(idx) * s.nw * (s.n_layers - 1)
+    return upd
+
+
+def _cpu_gather_part(rng):
+    """Worker: reflectivity and partial wavenumber sum of slownesses j0..j1 of
+    the gather workload (the per-chunk work of kennett_gather.py:246-280)."""
+    from oracle import kennett_numpy as K
+
+    g = _CPU["gather"]
+    j0, j1 = rng
+    ps = g["p"][j0:j1]
+    R = K.reflectivity_batch(g["model"], ps, g["w"] + 1j * g["gamma"], True)
+    return K.bessel_sum(R, ps, g["w"], g["offsets"], g["dp"], p_max=g["p"][-1])
+
+
+def cpu_baselines(specs, workers):
+    """The oracle timed on the host before the GPU is touched (workers are
+    forked): one process on the middle slowness row of every request, all
+    workers on `workers` rows of every request, and the whole gather
+    workload split over the workers by slowness chunks."""
+    import multiprocessing as mp
+
+    from oracle import kennett_numpy as K
+    from seismicinversion_amd.workloads import gather_config_c2
 
     s0 = specs[0]
     K.reflectivity_batch(s0.model, s0.p[:1], (s0.k0 + np.arange(8)) * s0.dw, False)  # warm-up
-    upd = 0
+    upd1 = 0
     t0 = time.perf_counter()
     for s in specs:
         mid = len(s.p) // 2
         w = (s.k0 + np.arange(s.nw)) * s.dw + 1j * s.gamma
         K.reflectivity_batch(s.model, s.p[mid:mid + 1], w, s.free_surface)
-        upd += s.nw * (s.n_layers - 1)
-    dt = time.perf_counter() - t0
-    return {"value": upd / dt, "unit": UNIT, "cores": 1, "kind": "port",
-            "sample": f"middle slowness row of each of the {len(specs)} requests "
-                      f"({upd:.3e} updates), NumPy oracle, 1 process, {dt:.1f} s"}
+        upd1 += s.nw * (s.n_layers - 1)
+    dt1 = time.perf_counter() - t0
+
+    cfg = gather_config_c2()
+    dw = 2 * np.pi / cfg.T
+    p = np.arange(1, cfg.np_slow + 1) * (cfg.p_max / cfg.np_slow)
+    _CPU["specs"] = specs
+    _CPU["gather"] = dict(model=cfg.model, p=p, w=np.arange(1, cfg.nw) * dw, gamma=np.pi / cfg.T,
+                          offsets=cfg.offsets, dp=cfg.p_max / cfg.np_slow)
+    chunks = [(i, min(len(specs), i + 5), workers) for i in range(0, len(specs), 5)]
+    ranges = [(j, min(cfg.np_slow, j + 32)) for j in range(0, cfg.np_slow, 32)]
+    with mp.get_context("fork").Pool(workers) as pool:
+        pool.map(_cpu_rows, [(0, 1, 1)] * workers)  # start-up
+        t0 = time.perf_counter()
+        updn = sum(pool.map(_cpu_rows, chunks, chunksize=1))
+        dtn = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        parts = pool.map(_cpu_gather_part, ranges, chunksize=1)
+        U = parts[0].copy()
+        for part in parts[1:]:  # chunks added in slowness order
+            U += part
+        w = np.arange(1, cfg.nw) * dw
+        S = K.ricker_spectrum(w, cfg.ricker_omega_max)
+        _, traces = K.hermitian_ifft(U * S[None, :], cfg.nw, np.pi / cfg.T, cfg.T)
+        dtg = time.perf_counter() - t0
+    cpu = {"value": updn / dtn, "unit": UNIT, "cores": workers, "kind": "port",
+           "sample": f"{workers} evenly spaced slowness rows of each of the {len(specs)} requests "
+                     f"({updn:.3e} updates), NumPy oracle in {workers} forked processes "
+                     f"(one thread each), {dtn:.1f} s",
+           "single_core": {"value": upd1 / dt1, "cores": 1,
+                           "sample": f"middle slowness row of each request ({upd1:.3e} updates), "
+                                     f"{dt1:.1f} s"}}
+    gather_cpu = {"s": dtg, "cores": workers,
+                  "what": "whole gather workload: reflectivity + wavenumber sum per 32-slowness "
+                          "chunk in the workers, chunks added in order, FFT in the parent"}
+    return cpu, gather_cpu, traces
 
 
 def kernel_figures(torch, dev):
@@ -163,6 +232,7 @@ def kernel_figures(torch, dev):
     gat()
     out["gather"] = {"workload": "20 layers, 8192 p x 1023 omega, 100 offsets, nt 2048",
                      "ms": event_ms(gat, 5, torch)}
+    out["_gather_traces"] = gat().cpu().numpy()
     # 256 uneven models: persistent launch vs one launch per model
     specs = [batch.ModelSpec(model=m, p=pp, nw=1024) for m, pp in uneven_batch_models(256, 42)]
     upd = sum(s.updates() for s in specs)
@@ -242,12 +312,14 @@ def main():
     from seismicinversion_amd.serve import DeviceBackend, KennettServer, stream_requests
     from seismicinversion_amd.workloads import serving_stream
 
-    dev = torch.device("cuda", 0)
-    torch.cuda.set_device(dev)
-    warm_clocks(torch, dev, args.warmup_seconds)
     items = serving_stream(0.0, args.requests, seed=42, lm_fraction=0.0)
     specs = [s for _, s in stream_requests(items)]
     updates = sum(s.updates() for s in specs)
+    cpu, gather_cpu, cpu_traces = cpu_baselines(specs, args.cpu_workers)
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    warm_clocks(torch, dev, args.warmup_seconds)
 
     # ---- served, device-resident results (headline)
     be = DeviceBackend(n_streams=3, kernel_timing=True)
@@ -331,7 +403,11 @@ def main():
                 "flop_per_update": FLOP_PER_UPDATE,
                 "algorithmic_bytes_per_launch": 16.0 * out_per_launch}
 
-    cpu = cpu_baseline(specs)
+    g = kernels["gather"]
+    g["cpu_all_cores"] = gather_cpu
+    g["gpu_over_cpu"] = gather_cpu["s"] / (g["ms"] * 1e-3)
+    g["max_rel_diff_vs_cpu"] = float(np.max(np.abs(kernels.pop("_gather_traces") - cpu_traces))
+                                     / np.max(np.abs(cpu_traces)))
     value = updates / t_served
     line = {
         "metric": "p*omega*layer updates/s (complex128), served vs plain, with p50/p99 latency",
