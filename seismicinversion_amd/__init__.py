@@ -42,6 +42,9 @@ _EXPORTS = {
     "compute_taup_traces": "taup_inversion",
     "invert_taup": "taup_inversion",
     "taup_misfit_grad_hessian": "taup_inversion",
+    "load_config": "inversion_config",
+    "save_config": "inversion_config",
+    "InversionConfig": "inversion_config",
     "reflectivity_multi": "batch",
     "ModelSpec": "batch",
     "KennettServer": "serve",

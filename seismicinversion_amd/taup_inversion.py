@@ -36,6 +36,11 @@ __all__ = [
     "newton_lm_step",
     "invert_taup",
     "compute_taup_traces",
+    "write_model_table_latex",
+    "write_model_profiles_tikz",
+    "plot_taup_traces",
+    "plot_convergence_curves",
+    "main",
 ]
 
 logger = logging.getLogger(__name__)
@@ -302,3 +307,79 @@ def compute_taup_traces(model: LayerModel, p_values: list[float], T: float = 64.
     tr = engine.hermitian_traces(R * S[None, :], nw, 0.0, T).cpu().numpy()
     time = np.arange(2 * nw, dtype=np.float64) * (T / float(2 * nw))
     return time, {p: tr[i] for i, p in enumerate(p_values)}
+
+
+def __getattr__(name):  # the report writers live in taup_report
+    if name in ("write_model_table_latex", "write_model_profiles_tikz", "plot_taup_traces",
+                "plot_convergence_curves"):
+        from . import taup_report
+
+        return getattr(taup_report, name)
+    raise AttributeError(name)
+
+
+def main(argv: list[str] | None = None) -> None:
+    """Inversion CLI (taup_inversion.py:1082-1198): a YAML config (default: the
+    5-layer ocean-crust config shipped with the package), the same override
+    flags, the device LM, then inversion_config.yaml and the configured
+    report formats in the output directory."""
+    import argparse
+    from pathlib import Path
+
+    from . import taup_report
+    from .inversion_config import DEFAULT_CONFIG, load_config, save_config
+
+    ap = argparse.ArgumentParser(description="Tau-p inversion with Newton-Levenberg-Marquardt "
+                                             "on the GPU")
+    ap.add_argument("--config", type=Path, default=None,
+                    help="YAML config (default: the built-in 5-layer ocean-crust model)")
+    ap.add_argument("--max-iter", type=int, default=None, help="Override max Newton iterations")
+    ap.add_argument("--perturbation", type=float, default=None,
+                    help="Override initial model perturbation fraction")
+    ap.add_argument("--seed", type=int, default=None, help="Override random seed")
+    ap.add_argument("--output-dir", type=Path, default=None, help="Override output directory")
+    ap.add_argument("--hessian-mode", choices=("gram", "hvp"), default="gram",
+                    help="exact Hessian as MFMA Gram + second-order term, or as HVPs")
+    ap.add_argument("--graphs", action="store_true",
+                    help="replay each iteration's device work from captured HIP graphs")
+    args = ap.parse_args(argv)
+    logging.basicConfig(level=logging.INFO, format="%(message)s")
+
+    cfg = load_config(args.config if args.config is not None else DEFAULT_CONFIG)
+    if args.max_iter is not None:
+        cfg.max_iter = args.max_iter
+    if args.perturbation is not None:
+        cfg.perturbation = args.perturbation
+    if args.seed is not None:
+        cfg.seed = args.seed
+    if args.output_dir is not None:
+        cfg.output.directory = args.output_dir
+
+    result = invert_taup(cfg.true_model, cfg.p_values, nfreq=cfg.nfreq,
+                         perturbation=cfg.perturbation, max_iter=cfg.max_iter, seed=cfg.seed,
+                         tol=cfg.tol, hessian_mode=args.hessian_mode, use_graphs=args.graphs)
+    logger.info("Converged: %s", result.converged)
+    logger.info("Iterations: %d", result.n_iterations)
+    logger.info("Final misfit: %.2e", result.misfit_history[-1])
+    logger.info("Final param error: %.4e", result.param_error_history[-1])
+
+    out = Path(cfg.output.directory)
+    out.mkdir(parents=True, exist_ok=True)
+    save_config(cfg, out / "inversion_config.yaml")
+    fmts = cfg.output.formats
+    if "table" in fmts:
+        taup_report.write_model_table_latex(result, out / "taup_model_parameters.tex",
+                                            layer_names=cfg.layer_names)
+    if "profiles" in fmts:
+        taup_report.write_model_profiles_tikz(result, out / "taup_model_profiles.tex")
+    if "traces" in fmts:
+        td = cfg.output.trace_display
+        taup_report.plot_taup_traces(result, out / "taup_trace_comparison.pdf", T=64.0,
+                                     nw=td.nw, t_max=td.t_max)
+    if "convergence" in fmts:
+        taup_report.plot_convergence_curves(result, out / "taup_convergence.pdf")
+    logger.info("Outputs in %s", out)
+
+
+if __name__ == "__main__":
+    main()
