@@ -224,6 +224,15 @@ def kernel_figures(torch, dev):
         "updates_per_s": upd / (ms * 1e-3),
         "tflops": FLOP_PER_UPDATE * upd / (ms * 1e-3) / 1e12,
         "frac_fp64_peak": FLOP_PER_UPDATE * upd / (ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS}
+    # the Global Matrix Method's block-Riccati kernel on the same grid
+    gmm = lambda: engine.gmm_forward_grid(L, p, 1.0, dw, gamma, nw, True)  # noqa: E731
+    gms = event_ms(gmm, 5, torch)
+    Rg = gmm()
+    fwd()
+    diff = float(((Rg - R).abs().amax(dim=1) / R.abs().amax(dim=1)).max())
+    out["gmm_gather_workload"] = {"kernel": "kh::gmm_kernel<1, true>", "ms": gms,
+                                  "updates_per_s": upd / (gms * 1e-3),
+                                  "max_row_rel_diff_vs_kennett": diff}
     # full gather, inputs resident
     off = torch.from_numpy(cfg.offsets).to(dev)
     w = np.arange(1, cfg.nw) * dw

@@ -250,6 +250,23 @@ size_t kh_gram_workspace_bytes(int P, long K);
 int kh_gram(const double* X, int P, long K, double scale, double* G, void* workspace,
             size_t workspace_bytes, void* stream);
 
+/* ---- Global Matrix Method ---- */
+
+/* PP reflectivity R (np, nw) complex128 by the Global Matrix Method's
+ * block-Riccati sweep: per (p, omega) lane, a 4x4 complex system with two
+ * right-hand sides per finite layer (LU with partial pivoting), then the
+ * ocean-bottom 2x2 system; arguments as kh_forward.  An independent
+ * formulation of the same boundary-value problem as the Kennett recursion
+ * (they agree to about 1e-10).  Replaces gmm_reflectivity(..., solver="riccati")
+ * (GlobalMatrix/global_matrix.py:297-348, riccati_solver.py:25-194). */
+int kh_gmm_forward(const double* layers, int nl, const double* p, int np,
+                   const double* omega, int nw, int free_surface, double* R, void* stream);
+
+/* kh_gmm_forward on the uniform grid omega_j = (k0 + j)*dw + i*gamma. */
+int kh_gmm_forward_grid(const double* layers, int nl, const double* p, int np, double k0,
+                        double dw, double gamma, int nw, int free_surface, double* R,
+                        void* stream);
+
 /* Misfit per slowness chi (np,) float64: chi[j] = sum_w |R[j, w] - R_obs[j, w]|^2
  * for R, R_obs (np, nw) complex128, in a fixed summation order.  The trial
  * misfit of the LM damping loop (_eval_misfit_only,

@@ -45,6 +45,8 @@ _EXPORTS = {
     "load_config": "inversion_config",
     "save_config": "inversion_config",
     "InversionConfig": "inversion_config",
+    "gmm_reflectivity": "global_matrix",
+    "gmm_reflectivity_batch": "global_matrix",
     "reflectivity_multi": "batch",
     "ModelSpec": "batch",
     "KennettServer": "serve",

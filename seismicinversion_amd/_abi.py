@@ -52,6 +52,9 @@ EXPORTS: dict[str, list] = {
     "kh_gram_workspace_bytes": [_c_int, ctypes.c_long],
     "kh_gram": [_vp, _c_int, ctypes.c_long, _c_double, _vp, _vp, ctypes.c_size_t, _vp],
     "kh_residual_norms": [_vp, _vp, _c_int, _c_int, _vp, _vp],
+    "kh_gmm_forward": [_vp, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _vp, _vp],
+    "kh_gmm_forward_grid": [_vp, _c_int, _vp, _c_int, _c_double, _c_double, _c_double, _c_int,
+                            _c_int, _vp, _vp],
     "kh_multi_items": [_c_int, _c_int, _c_int, _c_int],
     "kh_forward_multi": [_vp, _c_int, ctypes.c_longlong, _c_int, _vp, _vp, _vp, _c_int, _c_int,
                          _c_int, _vp, _vp, _vp],

@@ -25,6 +25,8 @@ __all__ = [
     "interfaces_direct",
     "residual_norms",
     "gram",
+    "gmm_forward",
+    "gmm_forward_grid",
 ]
 
 
@@ -218,6 +220,34 @@ def interfaces_direct(p, eta1, neta1, rho1, beta1, eta2, neta2, rho2, beta2, oce
         ptr(c[4]), ptr(f[2]), ptr(c[5]), *[_abi.ptr(o) for o in outs], _abi.stream_handle(dev))
     _abi.check(rc, "kh_interfaces_direct")
     return tuple(outs)
+
+
+def gmm_forward(layers: torch.Tensor, p: torch.Tensor, omega: torch.Tensor,
+                free_surface: bool = False) -> torch.Tensor:
+    """R (np, nw) by the Global Matrix Method's block-Riccati sweep (kh_gmm_forward)."""
+    dev = layers.device
+    p = _as_dev(p, torch.float64, dev)
+    omega = _as_dev(omega, torch.complex128, dev)
+    out = torch.empty((p.numel(), omega.numel()), dtype=torch.complex128, device=dev)
+    rc = _abi.lib().kh_gmm_forward(_abi.ptr(layers), layers.shape[0], _abi.ptr(p), p.numel(),
+                                   _abi.ptr(omega), omega.numel(), int(bool(free_surface)),
+                                   _abi.ptr(out), _abi.stream_handle(dev))
+    _abi.check(rc, "kh_gmm_forward")
+    return out
+
+
+def gmm_forward_grid(layers: torch.Tensor, p: torch.Tensor, k0: float, dw: float, gamma: float,
+                     n_omega: int, free_surface: bool = False) -> torch.Tensor:
+    """gmm_forward on omega_j = (k0 + j)*dw + i*gamma (kh_gmm_forward_grid)."""
+    dev = layers.device
+    p = _as_dev(p, torch.float64, dev)
+    out = torch.empty((p.numel(), n_omega), dtype=torch.complex128, device=dev)
+    rc = _abi.lib().kh_gmm_forward_grid(_abi.ptr(layers), layers.shape[0], _abi.ptr(p),
+                                        p.numel(), float(k0), float(dw), float(gamma),
+                                        int(n_omega), int(bool(free_surface)), _abi.ptr(out),
+                                        _abi.stream_handle(dev))
+    _abi.check(rc, "kh_gmm_forward_grid")
+    return out
 
 
 def residual_norms(R: torch.Tensor, R_obs: torch.Tensor,

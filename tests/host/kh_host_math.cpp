@@ -1,6 +1,7 @@
 // Host build of the device arithmetic (csrc/kh_physics.hpp, csrc/kh_sweep.hpp):
 // the same source lines the kernels run, compiled for the CPU so the CPU test
 // suite can compare them with the NumPy oracle without a GPU.
+#include "kh_gmm_lanes.hpp"
 #include "kh_sweep.hpp"
 
 #include <vector>
@@ -69,4 +70,24 @@ extern "C" int host_forward_grid(const double* layers, int nl, const double* p, 
 // The phase factors' sine and cosine (csrc/kh_complex.hpp) for n arguments.
 extern "C" void host_sincos_phase(const double* x, int n, double* s, double* c) {
   for (int i = 0; i < n; ++i) kh::sincos_phase(x[i], s + i, c + i);
+}
+
+// The Global Matrix Method lane (csrc/kh_gmm_lanes.hpp), one frequency per
+// lane, explicit frequencies.
+extern "C" int host_gmm_forward(const double* layers, int nl, const double* p, int np,
+                                const double* omega, int nw, int free_surface, double* R) {
+  std::vector<LayerPt> L(nl);
+  std::vector<EigD> E(nl);
+  for (int j = 0; j < np; ++j) {
+    for (int l = 0; l < nl; ++l) L[l] = layer_at(layers + (size_t)l * NCOL, l, p[j]);
+    for (int l = 0; l < nl; ++l) E[l] = eig_d(p[j], L[l]);
+    for (int w = 0; w < nw; ++w) {
+      const cd om[1] = {cd{omega[2 * w], omega[2 * w + 1]}};
+      cd out[1];
+      gmm_lanes<1, false>(L.data(), E.data(), nullptr, nl, om, free_surface, out);
+      R[2 * ((size_t)j * nw + w)] = out[0].r;
+      R[2 * ((size_t)j * nw + w) + 1] = out[0].i;
+    }
+  }
+  return 0;
 }

@@ -1,0 +1,84 @@
+"""The Global Matrix Method oracle (oracle/gmm_numpy.py) pinned on the CPU:
+the block-Riccati sweep against the dense global solve (an independent
+brute-force statement of the same boundary-value problem) and both against
+the Kennett oracle, at the bars of the reference's own tests
+(GlobalMatrix/test_riccati.py:53-104, 212-348)."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import gmm_numpy as G
+from oracle import kennett_numpy as K
+from seismicinversion_amd.layer_model import LayerModel
+from seismicinversion_amd.workloads import default_ocean_crust_model
+
+W = np.arange(1, 256, dtype=np.float64) * (2 * np.pi / 64.0)
+
+
+def close(a, b, rel, abs_=None):
+    mask = np.abs(b) > 1e-15
+    r = np.max(np.abs(a[mask] - b[mask]) / np.abs(b[mask]))
+    assert r < rel, r
+    if abs_ is not None:
+        assert np.max(np.abs(a - b)) < abs_
+
+
+def many_layer_model(n_elastic=20):
+    """20 elastic layers between the ocean and a half-space, velocities
+    increasing with depth (as GlobalMatrix/test_riccati.py:262-307 builds)."""
+    rng = np.random.default_rng(42)
+    n = n_elastic + 2
+    a, b, r, h = (np.zeros(n) for _ in range(4))
+    a[0], b[0], r[0], h[0] = 1.5, 0.0, 1.0, 2.0
+    for i in range(1, n_elastic + 1):
+        a[i] = 2.0 + 0.15 * i + rng.uniform(-0.05, 0.05)
+        b[i] = a[i] * 0.5 + rng.uniform(-0.02, 0.02)
+        r[i] = 2.0 + 0.05 * i + rng.uniform(-0.02, 0.02)
+        h[i] = 0.5 + rng.uniform(0, 0.5)
+    a[-1], b[-1], r[-1], h[-1] = a[-2] + 0.5, b[-2] + 0.3, r[-2] + 0.2, np.inf
+    qa = np.full(n, 100.0)
+    qb = np.full(n, 100.0)
+    qa[0], qb[0] = 20000.0, 1e10
+    return LayerModel.from_arrays(alpha=a, beta=b, rho=r, thickness=h, Q_alpha=qa, Q_beta=qb)
+
+
+@pytest.mark.parametrize("p", [0.1, 0.2, 0.3, 0.4, 0.6])
+def test_riccati_dense_kennett_agree(p):
+    m = default_ocean_crust_model()
+    Rr = G.riccati_reflectivity(m, p, W)
+    close(Rr, G.dense_reflectivity(m, p, W), 1e-10, 1e-12)
+    close(Rr, K.reflectivity(m, p, W), 1e-10, 1e-12)
+
+
+@pytest.mark.parametrize("p", [0.1, 0.3, 0.6])
+def test_free_surface(p):
+    m = default_ocean_crust_model()
+    Rr = G.riccati_reflectivity(m, p, W, free_surface=True)
+    close(Rr, G.dense_reflectivity(m, p, W, free_surface=True), 1e-10)
+    close(Rr, K.reflectivity(m, p, W, True), 1e-10)
+
+
+@pytest.mark.parametrize("layers", [2, 3])
+def test_two_and_three_layer_models(layers):
+    cols = dict(alpha=[1.5, 3.0, 5.0], beta=[0.0, 1.5, 3.0], rho=[1.0, 3.0, 3.0],
+                thickness=[2.0, 1.0, np.inf], Q_alpha=[20000, 100, 100], Q_beta=[1e10, 100, 100])
+    if layers == 2:
+        cols = {k: [v[0], v[2]] for k, v in cols.items()}
+        cols["thickness"] = [2.0, np.inf]
+    m = LayerModel.from_arrays(**cols)
+    Rr = G.riccati_reflectivity(m, 0.2, W)
+    np.testing.assert_allclose(Rr, G.dense_reflectivity(m, 0.2, W), rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(Rr, K.reflectivity(m, 0.2, W), rtol=1e-10, atol=1e-12)
+
+
+def test_many_layers_and_evanescent():
+    m = many_layer_model(20)
+    w = W[:63]
+    Rr = G.riccati_reflectivity(m, 0.15, w)
+    close(Rr, G.dense_reflectivity(m, 0.15, w), 1e-8)
+    close(Rr, K.reflectivity(m, 0.15, w), 1e-8)
+    assert np.all(np.isfinite(G.riccati_reflectivity(m, 0.3, w)))
+    for p in (0.4, 0.6, 0.8):
+        assert np.all(np.isfinite(G.riccati_reflectivity(default_ocean_crust_model(), p, W)))

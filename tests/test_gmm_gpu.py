@@ -1,0 +1,63 @@
+"""The Global Matrix Method kernel (kh_gmm_forward[_grid]) against the GMM
+oracle (itself pinned against the dense global solve and the Kennett oracle,
+tests/test_gmm_cpu.py) and against the Kennett kernel on the device."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import gmm_numpy as G
+from parity import assert_rows_close, row_tolerance
+from seismicinversion_amd import engine
+from seismicinversion_amd.global_matrix import gmm_reflectivity, gmm_reflectivity_batch
+from seismicinversion_amd.layer_model import LayerModel
+from seismicinversion_amd.workloads import default_ocean_crust_model, seeded_model, uniform_slownesses
+from test_gmm_cpu import many_layer_model
+
+pytestmark = pytest.mark.gpu
+W = np.arange(1, 256) * (2 * np.pi / 64.0)
+
+
+@pytest.mark.parametrize("fs", [False, True])
+def test_gmm_kernel_vs_oracle(fs):
+    for m in (default_ocean_crust_model(), many_layer_model(20), seeded_model(12, 3)):
+        p = np.array([0.05, 0.15, 0.3, 0.45, 0.6, 0.8])
+        R = gmm_reflectivity_batch(m, p, W, fs)
+        O = np.stack([G.riccati_reflectivity(m, pv, W, fs) for pv in p])
+        assert_rows_close(R, O, np.maximum(row_tolerance(m, p, W, fs), 1e-11), "gmm vs oracle")
+        assert np.all(np.isfinite(R))
+
+
+def test_gmm_matches_kennett_kernel_on_the_gather_grid():
+    """Damped uniform grid and explicit frequencies, against the Kennett
+    kernel: two formulations of one boundary-value problem."""
+    m = seeded_model(20, 42)
+    p = torch.from_numpy(uniform_slownesses(1024, 1.2)).cuda()
+    dw, gamma, nw = 2 * np.pi / 64.0, np.pi / 64.0, 1023
+    L = engine.layers_tensor(m)
+    Rg = engine.gmm_forward_grid(L, p, 1.0, dw, gamma, nw, True)
+    w = torch.from_numpy(np.arange(1, nw + 1) * dw + 1j * gamma).cuda()
+    Ra = engine.gmm_forward(L, p, w, True)
+    Rk = engine.forward_grid(L, p, 1.0, dw, gamma, nw, True)
+    torch.cuda.synchronize()
+    scale = Rk.abs().amax(dim=1)
+    assert float(((Rg - Rk).abs().amax(dim=1) / scale).max()) < 1e-9
+    assert float(((Ra - Rg).abs().amax(dim=1) / scale).max()) < 1e-12
+
+
+def test_gmm_small_models_and_api():
+    for cols in (dict(alpha=[1.5, 5.0], beta=[0.0, 3.0], rho=[1.0, 3.0], thickness=[2.0, np.inf],
+                      Q_alpha=[20000, 100], Q_beta=[1e10, 100]),
+                 dict(alpha=[1.5, 3.0, 5.0], beta=[0.0, 1.5, 3.0], rho=[1.0, 3.0, 3.0],
+                      thickness=[2.0, 1.0, np.inf], Q_alpha=[20000, 100, 100],
+                      Q_beta=[1e10, 100, 100])):
+        m = LayerModel.from_arrays(**cols)
+        R = gmm_reflectivity(m, 0.2, W)
+        np.testing.assert_allclose(R, G.riccati_reflectivity(m, 0.2, W), rtol=1e-11, atol=1e-13)
+    with pytest.raises(NotImplementedError):
+        gmm_reflectivity(default_ocean_crust_model(), 0.2, W, solver="dense")
+    with pytest.raises(ValueError):
+        gmm_reflectivity(default_ocean_crust_model(), 0.2, W, solver="lu")
+    assert gmm_reflectivity_batch(default_ocean_crust_model(), np.zeros(0), W).shape == (0, 255)

@@ -124,3 +124,26 @@ def test_phase_sincos_vs_libm(host_lib):
     sb, cb = np.empty(3), np.empty(3)
     host_lib.host_sincos_phase(vp(bad.ctypes.data), 3, vp(sb.ctypes.data), vp(cb.ctypes.data))
     assert np.all(np.isnan(sb)) and np.all(np.isnan(cb))
+
+
+@pytest.mark.parametrize("p,fs", [(0.1, False), (0.3, True), (0.6, False), (0.8, True)])
+def test_host_gmm_lane_vs_oracle(host_lib, p, fs):
+    """The block-Riccati lane (csrc/kh_gmm.hpp) built for the host against the
+    GMM oracle and the Kennett oracle, default and 22-layer models."""
+    from oracle import gmm_numpy as G
+    from test_gmm_cpu import many_layer_model
+
+    w = np.arange(1, 256) * (2 * np.pi / 64.0)
+    for m in (seeded_model(5, 42), many_layer_model(20)):
+        tab = pack_layers(m)
+        pp = np.array([p])
+        wc = np.ascontiguousarray(w.astype(np.complex128))
+        R = np.zeros((1, len(w)), np.complex128)
+        vp = ctypes.c_void_p
+        host_lib.host_gmm_forward(vp(tab.ctypes.data), len(tab), vp(pp.ctypes.data), 1,
+                                  vp(wc.ctypes.data), len(w), int(fs), vp(R.ctypes.data))
+        O = G.riccati_reflectivity(m, p, w, fs)
+        tol = max(1e-12, row_tolerance(m, pp, w, fs)[0])
+        assert np.max(np.abs(R[0] - O)) / np.max(np.abs(O)) < max(tol, 1e-11)
+        Kn = K.reflectivity_batch(m, pp, w, fs)[0]
+        assert np.max(np.abs(R[0] - Kn)) / np.max(np.abs(Kn)) < 1e-9
