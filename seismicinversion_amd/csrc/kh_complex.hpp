@@ -28,6 +28,13 @@ __host__ __device__ __forceinline__ cd operator*(cd a, cd b) {
 __host__ __device__ __forceinline__ cd operator*(cd a, double s) { return cd{a.r * s, a.i * s}; }
 __host__ __device__ __forceinline__ cd operator*(double s, cd a) { return cd{a.r * s, a.i * s}; }
 __host__ __device__ __forceinline__ cd conj(cd a) { return cd{a.r, -a.i}; }
+// c + a*b and c - a*b as two fused multiply-adds per component.
+__host__ __device__ __forceinline__ cd cfma(cd a, cd b, cd c) {
+  return cd{fma(a.r, b.r, fma(-a.i, b.i, c.r)), fma(a.r, b.i, fma(a.i, b.r, c.i))};
+}
+__host__ __device__ __forceinline__ cd cfms(cd a, cd b, cd c) {
+  return cd{fma(-a.r, b.r, fma(a.i, b.i, c.r)), fma(-a.r, b.i, fma(-a.i, b.r, c.i))};
+}
 // multiply by i and by -i: exact component swaps
 __host__ __device__ __forceinline__ cd mul_i(cd a) { return cd{-a.i, a.r}; }
 __host__ __device__ __forceinline__ cd mul_mi(cd a) { return cd{a.i, -a.r}; }

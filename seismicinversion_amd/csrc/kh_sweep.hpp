@@ -49,30 +49,34 @@ __host__ __device__ __forceinline__ LayerStep layer_step(const LayerPt& L, doubl
   return s;
 }
 
-// One interface step for one frequency: RRd <- Rd + Tu MT U Td.
+// One interface step for one frequency: RRd <- Rd + Tu MT U Td, with
+// MT = E RRd E, U = (I - Ru MT)^-1, Tu = Td^T.  Every sum of products is one
+// chain of fused multiply-adds seeded with its first product or its
+// constant term (1 - ..., Rd + ...), which keeps the FP64 instruction count
+// near one per two FLOPs.
 __host__ __device__ __forceinline__ void kennett_step(cd& r00, cd& r01, cd& r10, cd& r11, cd ea,
                                                       cd eb, const Iface& f) {
   const cd eaea = ea * ea, ebeb = eb * eb, eaeb = ea * eb;
   const cd m00 = eaea * r00, m01 = eaeb * r01, m10 = eaeb * r10, m11 = ebeb * r11;
   const cd one{1.0, 0.0};
-  // A = I - Ru MT
-  const cd a00 = one - (f.ru00 * m00 + f.ru01 * m10);
-  const cd a01 = -(f.ru00 * m01 + f.ru01 * m11);
-  const cd a10 = -(f.ru01 * m00 + f.ru11 * m10);
-  const cd a11 = one - (f.ru01 * m01 + f.ru11 * m11);
-  const cd rdet = recip_det(a00 * a11 - a01 * a10);
-  const cd u00 = a11 * rdet, u01 = -a01 * rdet, u10 = -a10 * rdet, u11 = a00 * rdet;
+  // A = I - Ru MT (Ru symmetric)
+  const cd a00 = cfms(f.ru01, m10, cfms(f.ru00, m00, one));
+  const cd a01 = cfms(f.ru01, m11, -(f.ru00 * m01));
+  const cd a10 = cfms(f.ru11, m10, -(f.ru01 * m00));
+  const cd a11 = cfms(f.ru11, m11, cfms(f.ru01, m01, one));
+  const cd rdet = recip_det(cfms(a01, a10, a00 * a11));
+  const cd u00 = a11 * rdet, u01 = -(a01 * rdet), u10 = -(a10 * rdet), u11 = a00 * rdet;
   // X = MT U
-  const cd x00 = m00 * u00 + m01 * u10, x01 = m00 * u01 + m01 * u11;
-  const cd x10 = m10 * u00 + m11 * u10, x11 = m10 * u01 + m11 * u11;
+  const cd x00 = cfma(m01, u10, m00 * u00), x01 = cfma(m01, u11, m00 * u01);
+  const cd x10 = cfma(m11, u10, m10 * u00), x11 = cfma(m11, u11, m10 * u01);
   // Y = Tu X with Tu = Td^T
-  const cd y00 = f.td00 * x00 + f.td10 * x10, y01 = f.td00 * x01 + f.td10 * x11;
-  const cd y10 = f.td01 * x00 + f.td11 * x10, y11 = f.td01 * x01 + f.td11 * x11;
+  const cd y00 = cfma(f.td10, x10, f.td00 * x00), y01 = cfma(f.td10, x11, f.td00 * x01);
+  const cd y10 = cfma(f.td11, x10, f.td01 * x00), y11 = cfma(f.td11, x11, f.td01 * x01);
   // RRd = Rd + Y Td
-  r00 = f.rd00 + (y00 * f.td00 + y01 * f.td10);
-  r01 = f.rd01 + (y00 * f.td01 + y01 * f.td11);
-  r10 = f.rd01 + (y10 * f.td00 + y11 * f.td10);
-  r11 = f.rd11 + (y10 * f.td01 + y11 * f.td11);
+  r00 = cfma(y01, f.td10, cfma(y00, f.td00, f.rd00));
+  r01 = cfma(y01, f.td11, cfma(y00, f.td01, f.rd01));
+  r10 = cfma(y11, f.td10, cfma(y10, f.td00, f.rd01));
+  r11 = cfma(y11, f.td11, cfma(y10, f.td01, f.rd11));
 }
 
 // RRd00 just below the water layer for W frequencies of one lane.  With
