@@ -25,27 +25,56 @@ struct M2 {
   Z a00, a01, a10, a11;
 };
 
+// Intermediates of one step (shared by the forward step and its adjoint so
+// that the reverse pass recomputes exactly the forward values).  Sums of
+// products are fused multiply-add chains, as in kennett_step.
+template <class Z>
+struct StepVals {
+  Z eaea, ebeb, eaeb, m00, m01, m10, m11, a00, a01, a10, a11, rdet, u00, u01, u10, u11;
+  Z x00, x01, x10, x11, y00, y01, y10, y11;
+};
+
+template <class Z>
+__host__ __device__ __forceinline__ StepVals<Z> step_vals(const M2<Z>& r, const Z& ea,
+                                                          const Z& eb, const IfaceZ<Z>& f) {
+  StepVals<Z> v;
+  v.eaea = ea * ea;
+  v.ebeb = eb * eb;
+  v.eaeb = ea * eb;
+  v.m00 = v.eaea * r.a00;
+  v.m01 = v.eaeb * r.a01;
+  v.m10 = v.eaeb * r.a10;
+  v.m11 = v.ebeb * r.a11;
+  const Z one = lift<Z>(1.0);
+  v.a00 = cfms(f.ru01, v.m10, cfms(f.ru00, v.m00, one));
+  v.a01 = cfms(f.ru01, v.m11, -(f.ru00 * v.m01));
+  v.a10 = cfms(f.ru11, v.m10, -(f.ru01 * v.m00));
+  v.a11 = cfms(f.ru11, v.m11, cfms(f.ru01, v.m01, one));
+  v.rdet = recip_det(cfms(v.a01, v.a10, v.a00 * v.a11));
+  v.u00 = v.a11 * v.rdet;
+  v.u01 = -(v.a01 * v.rdet);
+  v.u10 = -(v.a10 * v.rdet);
+  v.u11 = v.a00 * v.rdet;
+  v.x00 = cfma(v.m01, v.u10, v.m00 * v.u00);
+  v.x01 = cfma(v.m01, v.u11, v.m00 * v.u01);
+  v.x10 = cfma(v.m11, v.u10, v.m10 * v.u00);
+  v.x11 = cfma(v.m11, v.u11, v.m10 * v.u01);
+  v.y00 = cfma(f.td10, v.x10, f.td00 * v.x00);
+  v.y01 = cfma(f.td10, v.x11, f.td00 * v.x01);
+  v.y10 = cfma(f.td11, v.x10, f.td01 * v.x00);
+  v.y11 = cfma(f.td11, v.x11, f.td01 * v.x01);
+  return v;
+}
+
 template <class Z>
 __host__ __device__ __forceinline__ M2<Z> step_fwd(const M2<Z>& r, const Z& ea, const Z& eb,
                                                    const IfaceZ<Z>& f) {
-  const Z eaea = ea * ea, ebeb = eb * eb, eaeb = ea * eb;
-  const Z m00 = eaea * r.a00, m01 = eaeb * r.a01, m10 = eaeb * r.a10, m11 = ebeb * r.a11;
-  const Z one = lift<Z>(1.0);
-  const Z a00 = one - (f.ru00 * m00 + f.ru01 * m10);
-  const Z a01 = -(f.ru00 * m01 + f.ru01 * m11);
-  const Z a10 = -(f.ru01 * m00 + f.ru11 * m10);
-  const Z a11 = one - (f.ru01 * m01 + f.ru11 * m11);
-  const Z rdet = recip_det(a00 * a11 - a01 * a10);
-  const Z u00 = a11 * rdet, u01 = -a01 * rdet, u10 = -a10 * rdet, u11 = a00 * rdet;
-  const Z x00 = m00 * u00 + m01 * u10, x01 = m00 * u01 + m01 * u11;
-  const Z x10 = m10 * u00 + m11 * u10, x11 = m10 * u01 + m11 * u11;
-  const Z y00 = f.td00 * x00 + f.td10 * x10, y01 = f.td00 * x01 + f.td10 * x11;
-  const Z y10 = f.td01 * x00 + f.td11 * x10, y11 = f.td01 * x01 + f.td11 * x11;
+  const StepVals<Z> v = step_vals(r, ea, eb, f);
   M2<Z> s;
-  s.a00 = f.rd00 + (y00 * f.td00 + y01 * f.td10);
-  s.a01 = f.rd01 + (y00 * f.td01 + y01 * f.td11);
-  s.a10 = f.rd01 + (y10 * f.td00 + y11 * f.td10);
-  s.a11 = f.rd11 + (y10 * f.td01 + y11 * f.td11);
+  s.a00 = cfma(v.y01, f.td10, cfma(v.y00, f.td00, f.rd00));
+  s.a01 = cfma(v.y01, f.td11, cfma(v.y00, f.td01, f.rd01));
+  s.a10 = cfma(v.y11, f.td10, cfma(v.y10, f.td00, f.rd01));
+  s.a11 = cfma(v.y11, f.td11, cfma(v.y10, f.td01, f.rd11));
   return s;
 }
 
@@ -56,84 +85,73 @@ template <class Z>
 __host__ __device__ __forceinline__ void step_rev(const M2<Z>& r, const Z& ea, const Z& eb,
                                                   const IfaceZ<Z>& f, const M2<Z>& sb,
                                                   M2<Z>& rb, IfaceZ<Z>& fb, Z& eab, Z& ebb) {
-  // recompute the forward intermediates
-  const Z eaea = ea * ea, ebeb = eb * eb, eaeb = ea * eb;
-  const Z m00 = eaea * r.a00, m01 = eaeb * r.a01, m10 = eaeb * r.a10, m11 = ebeb * r.a11;
-  const Z one = lift<Z>(1.0);
-  const Z a00 = one - (f.ru00 * m00 + f.ru01 * m10);
-  const Z a01 = -(f.ru00 * m01 + f.ru01 * m11);
-  const Z a10 = -(f.ru01 * m00 + f.ru11 * m10);
-  const Z a11 = one - (f.ru01 * m01 + f.ru11 * m11);
-  const Z rdet = recip_det(a00 * a11 - a01 * a10);
-  const Z u00 = a11 * rdet, u01 = -a01 * rdet, u10 = -a10 * rdet, u11 = a00 * rdet;
-  const Z x00 = m00 * u00 + m01 * u10, x01 = m00 * u01 + m01 * u11;
-  const Z x10 = m10 * u00 + m11 * u10, x11 = m10 * u01 + m11 * u11;
-  const Z y00 = f.td00 * x00 + f.td10 * x10, y01 = f.td00 * x01 + f.td10 * x11;
-  const Z y10 = f.td01 * x00 + f.td11 * x10, y11 = f.td01 * x01 + f.td11 * x11;
+  const StepVals<Z> v = step_vals(r, ea, eb, f);  // recompute the forward intermediates
 
   // S = Rd + Y Td
   fb.rd00 = fb.rd00 + sb.a00;
   fb.rd01 = fb.rd01 + (sb.a01 + sb.a10);
   fb.rd11 = fb.rd11 + sb.a11;
-  const Z yb00 = sb.a00 * conj(f.td00) + sb.a01 * conj(f.td01);
-  const Z yb01 = sb.a00 * conj(f.td10) + sb.a01 * conj(f.td11);
-  const Z yb10 = sb.a10 * conj(f.td00) + sb.a11 * conj(f.td01);
-  const Z yb11 = sb.a10 * conj(f.td10) + sb.a11 * conj(f.td11);
-  Z tdb00 = conj(y00) * sb.a00 + conj(y10) * sb.a10;
-  Z tdb01 = conj(y00) * sb.a01 + conj(y10) * sb.a11;
-  Z tdb10 = conj(y01) * sb.a00 + conj(y11) * sb.a10;
-  Z tdb11 = conj(y01) * sb.a01 + conj(y11) * sb.a11;
-  // Y = Tu X, Tu = Td^T: Tu_bar = Yb X^H (transposed into Td_bar), X_bar = Tu^H Yb
-  tdb00 = tdb00 + (yb00 * conj(x00) + yb01 * conj(x01));
-  tdb10 = tdb10 + (yb00 * conj(x10) + yb01 * conj(x11));
-  tdb01 = tdb01 + (yb10 * conj(x00) + yb11 * conj(x01));
-  tdb11 = tdb11 + (yb10 * conj(x10) + yb11 * conj(x11));
-  const Z xb00 = conj(f.td00) * yb00 + conj(f.td01) * yb10;
-  const Z xb01 = conj(f.td00) * yb01 + conj(f.td01) * yb11;
-  const Z xb10 = conj(f.td10) * yb00 + conj(f.td11) * yb10;
-  const Z xb11 = conj(f.td10) * yb01 + conj(f.td11) * yb11;
+  const Z yb00 = cfma(sb.a01, conj(f.td01), sb.a00 * conj(f.td00));
+  const Z yb01 = cfma(sb.a01, conj(f.td11), sb.a00 * conj(f.td10));
+  const Z yb10 = cfma(sb.a11, conj(f.td01), sb.a10 * conj(f.td00));
+  const Z yb11 = cfma(sb.a11, conj(f.td11), sb.a10 * conj(f.td10));
+  // Td_bar from S = Rd + Y Td, then from Y = Tu X (Tu = Td^T: Tu_bar = Yb X^H transposed)
+  Z tdb00 = cfma(conj(v.y10), sb.a10, conj(v.y00) * sb.a00);
+  Z tdb01 = cfma(conj(v.y10), sb.a11, conj(v.y00) * sb.a01);
+  Z tdb10 = cfma(conj(v.y11), sb.a10, conj(v.y01) * sb.a00);
+  Z tdb11 = cfma(conj(v.y11), sb.a11, conj(v.y01) * sb.a01);
+  tdb00 = cfma(yb01, conj(v.x01), cfma(yb00, conj(v.x00), tdb00));
+  tdb10 = cfma(yb01, conj(v.x11), cfma(yb00, conj(v.x10), tdb10));
+  tdb01 = cfma(yb11, conj(v.x01), cfma(yb10, conj(v.x00), tdb01));
+  tdb11 = cfma(yb11, conj(v.x11), cfma(yb10, conj(v.x10), tdb11));
+  const Z xb00 = cfma(conj(f.td01), yb10, conj(f.td00) * yb00);
+  const Z xb01 = cfma(conj(f.td01), yb11, conj(f.td00) * yb01);
+  const Z xb10 = cfma(conj(f.td11), yb10, conj(f.td10) * yb00);
+  const Z xb11 = cfma(conj(f.td11), yb11, conj(f.td10) * yb01);
   fb.td00 = fb.td00 + tdb00;
   fb.td01 = fb.td01 + tdb01;
   fb.td10 = fb.td10 + tdb10;
   fb.td11 = fb.td11 + tdb11;
   // X = M U
-  Z mb00 = xb00 * conj(u00) + xb01 * conj(u01);
-  Z mb01 = xb00 * conj(u10) + xb01 * conj(u11);
-  Z mb10 = xb10 * conj(u00) + xb11 * conj(u01);
-  Z mb11 = xb10 * conj(u10) + xb11 * conj(u11);
-  const Z ub00 = conj(m00) * xb00 + conj(m10) * xb10;
-  const Z ub01 = conj(m00) * xb01 + conj(m10) * xb11;
-  const Z ub10 = conj(m01) * xb00 + conj(m11) * xb10;
-  const Z ub11 = conj(m01) * xb01 + conj(m11) * xb11;
+  Z mb00 = cfma(xb01, conj(v.u01), xb00 * conj(v.u00));
+  Z mb01 = cfma(xb01, conj(v.u11), xb00 * conj(v.u10));
+  Z mb10 = cfma(xb11, conj(v.u01), xb10 * conj(v.u00));
+  Z mb11 = cfma(xb11, conj(v.u11), xb10 * conj(v.u10));
+  const Z ub00 = cfma(conj(v.m10), xb10, conj(v.m00) * xb00);
+  const Z ub01 = cfma(conj(v.m10), xb11, conj(v.m00) * xb01);
+  const Z ub10 = cfma(conj(v.m11), xb10, conj(v.m01) * xb00);
+  const Z ub11 = cfma(conj(v.m11), xb11, conj(v.m01) * xb01);
   // U = adj(A) rdet
-  const Z crd = conj(rdet);
+  const Z crd = conj(v.rdet);
   Z ab00 = crd * ub11, ab11 = crd * ub00;
   Z ab01 = -(crd * ub01), ab10 = -(crd * ub10);
-  const Z rdetb = conj(a11) * ub00 - conj(a01) * ub01 - conj(a10) * ub10 + conj(a00) * ub11;
+  const Z rdetb = cfma(conj(v.a00), ub11,
+                       cfms(conj(v.a10), ub10, cfms(conj(v.a01), ub01, conj(v.a11) * ub00)));
   // rdet = 1/det, det = a00 a11 - a01 a10
-  const Z detb = -(conj(rdet * rdet) * rdetb);
-  ab00 = ab00 + conj(a11) * detb;
-  ab11 = ab11 + conj(a00) * detb;
-  ab01 = ab01 - conj(a10) * detb;
-  ab10 = ab10 - conj(a01) * detb;
+  const Z detb = -(conj(v.rdet * v.rdet) * rdetb);
+  ab00 = cfma(conj(v.a11), detb, ab00);
+  ab11 = cfma(conj(v.a00), detb, ab11);
+  ab01 = cfms(conj(v.a10), detb, ab01);
+  ab10 = cfms(conj(v.a01), detb, ab10);
   // A = I - Ru M
-  fb.ru00 = fb.ru00 - (ab00 * conj(m00) + ab01 * conj(m01));
-  fb.ru01 = fb.ru01 - (ab00 * conj(m10) + ab01 * conj(m11)) - (ab10 * conj(m00) + ab11 * conj(m01));
-  fb.ru11 = fb.ru11 - (ab10 * conj(m10) + ab11 * conj(m11));
-  mb00 = mb00 - (conj(f.ru00) * ab00 + conj(f.ru01) * ab10);
-  mb01 = mb01 - (conj(f.ru00) * ab01 + conj(f.ru01) * ab11);
-  mb10 = mb10 - (conj(f.ru01) * ab00 + conj(f.ru11) * ab10);
-  mb11 = mb11 - (conj(f.ru01) * ab01 + conj(f.ru11) * ab11);
+  fb.ru00 = cfms(ab01, conj(v.m01), cfms(ab00, conj(v.m00), fb.ru00));
+  fb.ru01 = cfms(ab11, conj(v.m01), cfms(ab10, conj(v.m00),
+                 cfms(ab01, conj(v.m11), cfms(ab00, conj(v.m10), fb.ru01))));
+  fb.ru11 = cfms(ab11, conj(v.m11), cfms(ab10, conj(v.m10), fb.ru11));
+  mb00 = cfms(conj(f.ru01), ab10, cfms(conj(f.ru00), ab00, mb00));
+  mb01 = cfms(conj(f.ru01), ab11, cfms(conj(f.ru00), ab01, mb01));
+  mb10 = cfms(conj(f.ru11), ab10, cfms(conj(f.ru01), ab00, mb10));
+  mb11 = cfms(conj(f.ru11), ab11, cfms(conj(f.ru01), ab01, mb11));
   // M = E2 o r
-  rb.a00 = conj(eaea) * mb00;
-  rb.a01 = conj(eaeb) * mb01;
-  rb.a10 = conj(eaeb) * mb10;
-  rb.a11 = conj(ebeb) * mb11;
+  rb.a00 = conj(v.eaea) * mb00;
+  rb.a01 = conj(v.eaeb) * mb01;
+  rb.a10 = conj(v.eaeb) * mb10;
+  rb.a11 = conj(v.ebeb) * mb11;
   const Z eaeab = conj(r.a00) * mb00;
-  const Z eaebb = conj(r.a01) * mb01 + conj(r.a10) * mb10;
+  const Z eaebb = cfma(conj(r.a10), mb10, conj(r.a01) * mb01);
   const Z ebebb = conj(r.a11) * mb11;
-  eab = conj(ea * 2.0) * eaeab + conj(eb) * eaebb;
-  ebb = conj(eb * 2.0) * ebebb + conj(ea) * eaebb;
+  eab = cfma(conj(eb), eaebb, conj(ea * 2.0) * eaeab);
+  ebb = cfma(conj(ea), eaebb, conj(eb * 2.0) * ebebb);
 }
 
 // Cotangents of eta (complex) and h (real, returned in .r with 0 imaginary

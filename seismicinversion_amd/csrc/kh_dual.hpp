@@ -36,7 +36,18 @@ __host__ __device__ __forceinline__ Dual<T> operator-(const Dual<T>& a) {
 }
 template <class T>
 __host__ __device__ __forceinline__ Dual<T> operator*(const Dual<T>& a, const Dual<T>& b) {
-  return {a.v * b.v, a.v * b.d + a.d * b.v};
+  return {a.v * b.v, cfma(a.v, b.d, a.d * b.v)};
+}
+// c + a*b and c - a*b with fused tangents (see cfma / cfms on cd).
+template <class T>
+__host__ __device__ __forceinline__ Dual<T> cfma(const Dual<T>& a, const Dual<T>& b,
+                                                 const Dual<T>& c) {
+  return {cfma(a.v, b.v, c.v), cfma(a.v, b.d, cfma(a.d, b.v, c.d))};
+}
+template <class T>
+__host__ __device__ __forceinline__ Dual<T> cfms(const Dual<T>& a, const Dual<T>& b,
+                                                 const Dual<T>& c) {
+  return {cfms(a.v, b.v, c.v), cfms(a.v, b.d, cfms(a.d, b.v, c.d))};
 }
 template <class T>
 __host__ __device__ __forceinline__ Dual<T> operator*(const Dual<T>& a, double s) {
