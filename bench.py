@@ -6,7 +6,8 @@ Headline workload (one "step"): N = 400 forward requests drawn from the
 mixed-size serving distribution (seed 42: layer counts round(exp(U[ln 3,
 ln 100])), slownesses round(exp(U[ln 16, ln 2048])) uniform in (0, 0.8] s/km,
 frequency counts uniform over {256, 1024, 4096}, omega_k = k*2*pi/64),
-submitted at once to the continuous-batching server and served with
+submitted at once (``submit_forward_many``) to the continuous-batching
+server and served with
 device-resident results.  ``value`` = p*omega*layer updates per second
 (one update = one interface step of one (p, omega) lane).
 
@@ -335,8 +336,7 @@ def main():
     srv = KennettServer(backend=be).start()
 
     def served(on_device=True):
-        futs = [srv.submit_forward(s, on_device=on_device) for s in specs]
-        for f in futs:
+        for f in srv.submit_forward_many(specs, on_device=on_device):
             f.result()
 
     for _ in range(args.warmup):

@@ -342,10 +342,20 @@ class KennettServer:
         shared across streams."""
         return self._submit("forward", spec, deadline_s, on_device)
 
+    def submit_forward_many(self, specs, deadline_s: float | None = None,
+                            on_device: bool = False) -> list[Future]:
+        """submit_forward for a list of requests, queued as one item: the
+        scheduler sees them together instead of one Python call apart."""
+        reqs = [self._make("forward", s, deadline_s, on_device) for s in specs]
+        self._enqueue([r for r in reqs if not r.future.done()])
+        return [r.future for r in reqs]
+
     def submit_invert(self, spec: InvertSpec, deadline_s: float | None = None) -> Future:
         return self._submit("invert", spec, deadline_s)
 
-    def _submit(self, kind, spec, deadline_s, on_device=False) -> Future:
+    def _make(self, kind, spec, deadline_s, on_device=False) -> Request:
+        """A Request for `spec`; when the spec is unusable its future already
+        holds the error (the request fails alone, the service goes on)."""
         fut: Future = Future()
         now = time.perf_counter()
         req = Request(kind=kind, spec=spec, future=fut, arrival=now,
@@ -356,14 +366,24 @@ class KennettServer:
         try:
             req.record["n_layers"] = spec.n_layers
             req.record["updates"] = spec.updates()
-        except ValueError as e:  # unusable model: this request fails, the service goes on
+        except ValueError as e:
             self._finish(req, error=e)
-            return fut
+        return req
+
+    def _enqueue(self, reqs: list[Request]) -> None:
+        if not reqs:
+            return
         with self.lock:
-            self.outstanding += 1
-        req.record["counted"] = True
-        self.incoming.put(req)
-        return fut
+            self.outstanding += len(reqs)
+        for r in reqs:
+            r.record["counted"] = True
+        self.incoming.put(reqs if len(reqs) > 1 else reqs[0])
+
+    def _submit(self, kind, spec, deadline_s, on_device=False) -> Future:
+        req = self._make(kind, spec, deadline_s, on_device)
+        if not req.future.done():
+            self._enqueue([req])
+        return req.future
 
     def start(self) -> "KennettServer":
         self._stop.clear()
@@ -439,14 +459,19 @@ class KennettServer:
         return self.backend.invert_bytes(req.spec)
 
     def _validate(self, req: Request):
+        """Raises ValueError for an unusable request.  A forward spec's layer
+        table is validated once and cached on the spec (ModelSpec.table), so
+        resubmitted specs cost nothing here."""
         spec = req.spec
-        model = spec.model if req.kind == "forward" else spec.true_model
-        if not isinstance(model, np.ndarray):
+        if req.kind == "forward":
+            if not isinstance(spec.model, np.ndarray):
+                spec.table()
+            if spec.nw <= 0 or len(spec.p) == 0:
+                raise ValueError("forward request needs at least one slowness and one frequency")
+        elif not isinstance(spec.true_model, np.ndarray):
             from .layer_model import pack_layers
 
-            pack_layers(model)  # raises ValueError for an unusable model
-        if req.kind == "forward" and (spec.nw <= 0 or len(spec.p) == 0):
-            raise ValueError("forward request needs at least one slowness and one frequency")
+            pack_layers(spec.true_model)
 
     def _admit(self) -> list[Request]:
         """Validated requests leave the arrival queue for one of two FIFOs.
@@ -456,18 +481,20 @@ class KennettServer:
         waiting for a slot hold no memory and never hold up forward requests."""
         while True:
             try:
-                req = self.incoming.get_nowait()
+                item = self.incoming.get_nowait()
             except queue.Empty:
                 break
-            try:
-                self._validate(req)
-                req.mem = self._mem_of(req)
-                if req.mem > self.budget:
-                    raise MemoryError(f"request needs {req.mem} bytes, budget is {self.budget}")
-            except (ValueError, MemoryError) as e:
-                self._finish(req, error=e)
-                continue
-            (self.waiting if req.kind == "forward" else self.waiting_inv).append(req)
+            for req in (item if isinstance(item, list) else (item,)):
+                try:
+                    self._validate(req)
+                    req.mem = self._mem_of(req)
+                    if req.mem > self.budget:
+                        raise MemoryError(f"request needs {req.mem} bytes, "
+                                          f"budget is {self.budget}")
+                except (ValueError, MemoryError) as e:
+                    self._finish(req, error=e)
+                    continue
+                (self.waiting if req.kind == "forward" else self.waiting_inv).append(req)
         admitted = []
         now = time.perf_counter()
         for fifo, is_inv in ((self.waiting, False), (self.waiting_inv, True)):

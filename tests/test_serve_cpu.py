@@ -151,3 +151,21 @@ def test_inversions_run_beside_forward_batches():
         assert fi.result(timeout=30) == {"iterations": 3}
         ff.result(timeout=30)
     assert srv.stats()["latency_invert"]["count"] == 1
+
+
+def test_submit_many_batches_and_fails_bad_requests_alone():
+    be = HostBackend(delay=0.02)
+    bad = ModelSpec(model=LayerModel.from_arrays(alpha=[1.5], beta=[0.0], rho=[1.0],
+                                                 thickness=[np.inf], Q_alpha=[1e4],
+                                                 Q_beta=[1e4]), p=np.array([0.1]), nw=8)
+    specs = [fwd(3 + i % 4, 3, 16, seed=i) for i in range(12)]
+    with KennettServer(backend=be, hbm_budget=1 << 30) as srv:
+        futs = srv.submit_forward_many(specs[:6] + [bad] + specs[6:])
+        outs = [f.result(timeout=60) if i != 6 else None for i, f in enumerate(futs)]
+        with pytest.raises(ValueError):
+            futs[6].result(timeout=30)
+    assert len(futs) == 13 and len(be.launches) <= 2 and sum(be.launches) == 12
+    for s, R in zip(specs, [o for i, o in enumerate(outs) if i != 6]):
+        w = (s.k0 + np.arange(s.nw)) * s.dw
+        np.testing.assert_array_equal(R, K.reflectivity_batch(s.model, s.p, w, s.free_surface))
+    assert srv.outstanding == 0
