@@ -35,6 +35,20 @@ __host__ __device__ __forceinline__ cd cfma(cd a, cd b, cd c) {
 __host__ __device__ __forceinline__ cd cfms(cd a, cd b, cd c) {
   return cd{fma(-a.r, b.r, fma(a.i, b.i, c.r)), fma(-a.r, b.i, fma(-a.i, b.r, c.i))};
 }
+// fma(a, b, c) for a constant c (a polynomial coefficient).  On the device
+// the three-operand v_fma_f64 with c read from scalar registers: left to
+// itself the compiler keeps such constants in vector registers and emits
+// the two-operand v_fmac_f64, whose accumulator is its destination, so every
+// Horner step paid a 64-bit register copy of the coefficient first.
+__host__ __device__ __forceinline__ double fma_k(double a, double b, double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double d;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
+  return d;
+#else
+  return fma(a, b, c);
+#endif
+}
 // multiply by i and by -i: exact component swaps
 __host__ __device__ __forceinline__ cd mul_i(cd a) { return cd{-a.i, a.r}; }
 __host__ __device__ __forceinline__ cd mul_mi(cd a) { return cd{a.i, -a.r}; }
@@ -75,12 +89,45 @@ __host__ __device__ __forceinline__ cd csqrt(cd z) {
   return cd{fabs(y) / (2.0 * t), copysign(t, y)};
 }
 
+// (sin, cos) of k pi/2 + r for |r| <= pi/4 (slightly more is harmless): the
+// Taylor series of sin r and cos r through r^17 and r^18 (the next terms are
+// below 1e-19), then the quadrant q = k mod 4 rotates the pair.
+__host__ __device__ __forceinline__ void sincos_quadrant(double r, int k, double* s, double* c) {
+  const double z = r * r;
+  double ps = 2.8114572543455206e-15;  // (-1)^n / (2n+1)!, n = 8 .. 1
+  ps = fma_k(ps, z, -7.647163731819816e-13);
+  ps = fma_k(ps, z, 1.6059043836821613e-10);
+  ps = fma_k(ps, z, -2.505210838544172e-08);
+  ps = fma_k(ps, z, 2.7557319223985893e-06);
+  ps = fma_k(ps, z, -0.0001984126984126984);
+  ps = fma_k(ps, z, 0.008333333333333333);
+  ps = fma_k(ps, z, -0.16666666666666666);
+  const double sr = fma(r * z, ps, r);
+  double pc = -1.5619206968586225e-16;  // (-1)^n / (2n)!, n = 9 .. 2
+  pc = fma_k(pc, z, 4.779477332387385e-14);
+  pc = fma_k(pc, z, -1.1470745597729725e-11);
+  pc = fma_k(pc, z, 2.08767569878681e-09);
+  pc = fma_k(pc, z, -2.755731922398589e-07);
+  pc = fma_k(pc, z, 2.48015873015873e-05);
+  pc = fma_k(pc, z, -0.001388888888888889);
+  pc = fma_k(pc, z, 0.041666666666666664);
+  const double hz = 0.5 * z;
+  const double w = 1.0 - hz;
+  const double cr = w + (((1.0 - w) - hz) + (z * z) * pc);
+  const int q = k & 3;
+  double ss = (q & 1) ? cr : sr;
+  double cc = (q & 1) ? sr : cr;
+  if (q == 1 || q == 2) cc = -cc;
+  if (q >= 2) ss = -ss;
+  *s = ss;
+  *c = cc;
+}
+
 // (sin x, cos x) for the unit-modulus phase factors exp(i x).
 // Reduction by the nearest multiple k of pi/2, with pi/2 split into three
 // doubles P1 + P2 + P3 (Cody and Waite, "Software Manual for the Elementary
-// Functions", 1980) and each k * Pj removed by one FMA; then the Taylor
-// series of sin and cos on |r| <= pi/4, through r^17 and r^18 (the next
-// terms are below 1e-19).  The absolute error is a few units of 2^-53 for
+// Functions", 1980) and each k * Pj removed by one FMA; then
+// sincos_quadrant.  The absolute error is a few units of 2^-53 for
 // |x| < 1e8, which is what a factor of modulus 1 needs (the argument's own
 // rounding, ulp(x), is far larger); larger or non-finite arguments take the
 // device library's sincos.
@@ -93,35 +140,9 @@ __host__ __device__ __forceinline__ void sincos_phase(double x, double* s, doubl
   double r = fma(-k, 1.5707963267948966, x);
   r = fma(-k, 6.123233995736766e-17, r);
   r = fma(-k, -1.4973849048591698e-33, r);
-  const double z = r * r;
-  double ps = 2.8114572543455206e-15;  // (-1)^n / (2n+1)!, n = 8 .. 1
-  ps = fma(ps, z, -7.647163731819816e-13);
-  ps = fma(ps, z, 1.6059043836821613e-10);
-  ps = fma(ps, z, -2.505210838544172e-08);
-  ps = fma(ps, z, 2.7557319223985893e-06);
-  ps = fma(ps, z, -0.0001984126984126984);
-  ps = fma(ps, z, 0.008333333333333333);
-  ps = fma(ps, z, -0.16666666666666666);
-  const double sr = fma(r * z, ps, r);
-  double pc = -1.5619206968586225e-16;  // (-1)^n / (2n)!, n = 9 .. 2
-  pc = fma(pc, z, 4.779477332387385e-14);
-  pc = fma(pc, z, -1.1470745597729725e-11);
-  pc = fma(pc, z, 2.08767569878681e-09);
-  pc = fma(pc, z, -2.755731922398589e-07);
-  pc = fma(pc, z, 2.48015873015873e-05);
-  pc = fma(pc, z, -0.001388888888888889);
-  pc = fma(pc, z, 0.041666666666666664);
-  const double hz = 0.5 * z;
-  const double w = 1.0 - hz;
-  const double cr = w + (((1.0 - w) - hz) + (z * z) * pc);
-  // x = k pi/2 + r: quadrant q = k mod 4 rotates (sin r, cos r)
-  const int q = (int)k & 3;
-  double ss = (q & 1) ? cr : sr;
-  double cc = (q & 1) ? sr : cr;
-  if (q == 1 || q == 2) cc = -cc;
-  if (q >= 2) ss = -ss;
+  double ss;
+  sincos_quadrant(r, (int)k, &ss, c);
   *s = (x == 0.0) ? x : ss;  // sin(+-0) = +-0
-  *c = cc;
 }
 
 __host__ __device__ __forceinline__ cd cexp(cd z) {

@@ -11,9 +11,12 @@
 // float64 twin is the parity target).  Stage 1: a one-wave workgroup owns 64
 // consecutive frequencies x a group of offsets x a chunk of slownesses and
 // writes one partial sum per (chunk, r, k); stage 2 adds the chunks in a fixed
-// order and applies S_k, so U does not depend on scheduling.  J0 is the ROCm
-// device library's double-precision j0.
+// order and applies S_k, so U does not depend on scheduling.  J0 is kh_j0.hpp:
+// below x = 12 an interval polynomial (its table staged in LDS), above it
+// Hankel's form with 1/x and sqrt(2/(pi x)) formed as products of per-lane
+// factors 1/(w r), sqrt(2/(pi w r)) and per-slowness factors 1/p, sqrt(1/p).
 #include "kh_complex.hpp"
+#include "kh_j0.hpp"
 #include "kh_runtime.hpp"
 
 namespace kh {
@@ -28,6 +31,9 @@ __global__ void __launch_bounds__(BS_WAVE) bessel_partial_kernel(
     cd* __restrict__ partial, int n_rgroups, int n_wtiles) {
   __shared__ double s_tp[BS_PCHUNK];  // taper_j * p_j
   __shared__ double s_p[BS_PCHUNK];
+  __shared__ double s_ip[BS_PCHUNK];   // 1 / |p_j|
+  __shared__ double s_sip[BS_PCHUNK];  // sqrt(1 / |p_j|)
+  __shared__ double s_j0[J0_XA * (J0_DEG_SMALL + 1)];
   int b = blockIdx.x;
   const int wtile = b % n_wtiles;
   b /= n_wtiles;
@@ -38,9 +44,14 @@ __global__ void __launch_bounds__(BS_WAVE) bessel_partial_kernel(
   for (int t = threadIdx.x; t < nj; t += blockDim.x) {
     const double pj = p[j0_ + t];
     const double taper = 0.5 * (1.0 + cos(M_PI * pj / p_max));
+    const double ip = 1.0 / fabs(pj);
     s_p[t] = pj;
     s_tp[t] = taper * pj;
+    s_ip[t] = ip;
+    s_sip[t] = sqrt(ip);
   }
+  for (int t = threadIdx.x; t < J0_XA * (J0_DEG_SMALL + 1); t += blockDim.x)
+    s_j0[t] = (&J0_SMALL[0][0])[t];
   __syncthreads();
 
   const int k = wtile * BS_WAVE + threadIdx.x;
@@ -48,9 +59,15 @@ __global__ void __launch_bounds__(BS_WAVE) bessel_partial_kernel(
   const double w = (k0 + (double)kc) * dw;
   const int r0 = rgroup * BS_RG;
   const int nrg = min(BS_RG, nr - r0);
-  double rr[BS_RG];
+  double rr[BS_RG], cu[BS_RG], ca[BS_RG];
 #pragma unroll
-  for (int q = 0; q < BS_RG; ++q) rr[q] = offsets[min(r0 + q, nr - 1)];
+  for (int q = 0; q < BS_RG; ++q) {
+    rr[q] = offsets[min(r0 + q, nr - 1)];
+    // 1/x = cu / p and sqrt(2/(pi x)) = ca sqrt(1/p) for x = w p r; inf when
+    // w r = 0, where x = 0 takes the interval polynomial
+    cu[q] = 1.0 / fabs(w * rr[q]);
+    ca[q] = sqrt(0.6366197723675814 * cu[q]);
+  }
   cd acc[BS_RG];
 #pragma unroll
   for (int q = 0; q < BS_RG; ++q) acc[q] = cd{0.0, 0.0};
@@ -62,9 +79,11 @@ __global__ void __launch_bounds__(BS_WAVE) bessel_partial_kernel(
     const cd Rv = Rcol[(size_t)t * nwm];
     const cd Wv = Rv * wgt;
     const double x0 = w * pj;
+    const double ip = s_ip[t], sip = s_sip[t];
 #pragma unroll
     for (int q = 0; q < BS_RG; ++q) {
-      const double J = j0(x0 * rr[q]);
+      const double x = fabs(x0 * rr[q]);
+      const double J = x < (double)J0_XA ? j0_small(x, s_j0) : j0_large(x, cu[q] * ip, ca[q] * sip);
       acc[q].r += J * Wv.r;
       acc[q].i += J * Wv.i;
     }

@@ -2,6 +2,7 @@
 // the same source lines the kernels run, compiled for the CPU so the CPU test
 // suite can compare them with the NumPy oracle without a GPU.
 #include "kh_gmm_lanes.hpp"
+#include "kh_j0.hpp"
 #include "kh_sweep.hpp"
 
 #include <vector>
@@ -70,6 +71,21 @@ extern "C" int host_forward_grid(const double* layers, int nl, const double* p, 
 // The phase factors' sine and cosine (csrc/kh_complex.hpp) for n arguments.
 extern "C" void host_sincos_phase(const double* x, int n, double* s, double* c) {
   for (int i = 0; i < n; ++i) kh::sincos_phase(x[i], s + i, c + i);
+}
+
+// J0 (csrc/kh_j0.hpp) for n arguments: the stand-alone form, and the large
+// regime with u = 1/x and amp = sqrt(2/(pi x)) passed in as the gather
+// kernel passes its per-lane and per-slowness products.
+extern "C" void host_j0(const double* x, int n, double* out) {
+  for (int i = 0; i < n; ++i) out[i] = kh::j0_kh(x[i]);
+}
+
+extern "C" void host_j0_split(const double* x, const double* u, const double* amp, int n,
+                              double* out) {
+  for (int i = 0; i < n; ++i) {
+    const double a = fabs(x[i]);
+    out[i] = a < kh::J0_XA ? kh::j0_small(a, &kh::J0_SMALL[0][0]) : kh::j0_large(a, u[i], amp[i]);
+  }
 }
 
 // The Global Matrix Method lane (csrc/kh_gmm_lanes.hpp), one frequency per

@@ -147,3 +147,83 @@ def test_host_gmm_lane_vs_oracle(host_lib, p, fs):
         assert np.max(np.abs(R[0] - O)) / np.max(np.abs(O)) < max(tol, 1e-11)
         Kn = K.reflectivity_batch(m, pp, w, fs)[0]
         assert np.max(np.abs(R[0] - Kn)) / np.max(np.abs(Kn)) < 1e-9
+
+
+def _j0_args():
+    rng = np.random.default_rng(11)
+    import mpmath as mp
+
+    zeros = np.array([float(mp.besseljzero(0, k)) for k in range(1, 41)])
+    return np.ascontiguousarray(np.concatenate([
+        rng.uniform(0.0, 12.0, 100000),
+        rng.uniform(12.0, 200.0, 100000),
+        rng.uniform(200.0, 8000.0, 100000),
+        12.0 + rng.uniform(-1e-9, 1e-9, 1000),
+        np.arange(0.0, 13.0, 0.5),                                  # interval edges
+        (zeros[:, None] + rng.uniform(-1e-4, 1e-4, (40, 8))).ravel(),
+        np.array([1e-300, 5e-324, 1e-8, 12.0 - 2e-15, 1e5, 9.99e7]),
+    ]))
+
+
+def test_j0_vs_scipy_and_mpmath(host_lib):
+    """The gather's own J0 (interval polynomials below 12, Hankel's form
+    above) against 40-digit mpmath on a subset and against scipy.special.j0
+    (the reference's J0) everywhere, in absolute terms scaled by the envelope
+    max(|J0|, sqrt(2/(pi x))) near which both regimes work.  Measured against
+    mpmath this J0 is within 3.5e-16 of the envelope; scipy (Cephes) is
+    within 1e-15 below 12 and 3e-15 above, plus the rounding of its own
+    x - pi/4 (up to half an ulp of x in phase), which the comparison with
+    scipy allows for."""
+    import mpmath as mp
+    from scipy.special import j0 as scipy_j0
+
+    mp.mp.dps = 40
+    x = _j0_args()
+    out = np.empty_like(x)
+    vp = ctypes.c_void_p
+    host_lib.host_j0(vp(x.ctypes.data), x.size, vp(out.ctypes.data))
+    amp = np.sqrt(2.0 / (np.pi * np.maximum(x, 1e-300)))
+    env = np.minimum(np.maximum(amp, np.abs(scipy_j0(x))), 1.0)
+    sub = np.concatenate([np.arange(0, x.size, 97), np.arange(x.size - 250, x.size)])
+    exact = np.array([float(mp.besselj(0, mp.mpf(v))) for v in x[sub]])
+    err_mp = np.abs(out[sub] - exact) / env[sub]
+    assert err_mp.max() < 4e-16, (err_mp.max(), x[sub][err_mp.argmax()])
+    err_scipy = np.abs(out - scipy_j0(x)) - amp * 0.5 * np.spacing(x)
+    assert np.max(err_scipy / env) < 3.5e-15, (np.max(err_scipy / env), x[np.argmax(err_scipy / env)])
+    # even, and the special values; beyond 1e8 the library's sincos is used
+    neg = -x[:1000].copy()
+    on = np.empty_like(neg)
+    host_lib.host_j0(vp(neg.ctypes.data), neg.size, vp(on.ctypes.data))
+    assert np.array_equal(on, out[:1000])
+    sp = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 1e9, 3.3e12])
+    so = np.empty_like(sp)
+    host_lib.host_j0(vp(sp.ctypes.data), sp.size, vp(so.ctypes.data))
+    assert so[0] == 1.0 and so[1] == 1.0 and so[2] == 0.0 and so[3] == 0.0 and np.isnan(so[4])
+    big = np.array([float(mp.besselj(0, mp.mpf(v))) for v in sp[5:]])
+    assert np.all(np.abs(so[5:] - big) < 1e-20)
+
+
+def test_j0_split_form_as_the_gather_kernel_forms_it(host_lib):
+    """The large regime with u = 1/x and sqrt(2/(pi x)) built as the gather
+    kernel builds them: per-lane 1/(w r) and sqrt(2/(pi w r)) times
+    per-slowness 1/p and sqrt(1/p), at x = (w p) r."""
+    from scipy.special import j0 as scipy_j0
+
+    rng = np.random.default_rng(3)
+    w = rng.uniform(0.05, 200.0, 50000)
+    p = rng.uniform(1e-3, 1.2, 50000)
+    r = rng.uniform(0.0, 60.0, 50000)
+    x = np.ascontiguousarray((w * p) * r)
+    cu = 1.0 / (w * r)
+    u = np.ascontiguousarray(cu * (1.0 / p))
+    amp = np.ascontiguousarray(np.sqrt(0.6366197723675814 * cu) * np.sqrt(1.0 / p))
+    out = np.empty_like(x)
+    vp = ctypes.c_void_p
+    host_lib.host_j0_split(vp(x.ctypes.data), vp(u.ctypes.data), vp(amp.ctypes.data), x.size,
+                           vp(out.ctypes.data))
+    env = np.minimum(np.maximum(np.sqrt(2.0 / (np.pi * np.maximum(x, 1e-300))),
+                                np.abs(scipy_j0(x))), 1.0)
+    ampx = np.sqrt(2.0 / (np.pi * np.maximum(x, 1e-300)))
+    err = (np.abs(out - scipy_j0(x)) - ampx * 0.5 * np.spacing(x)) / env
+    assert err.max() < 3.5e-15, (err.max(), x[err.argmax()])
+    assert np.sum(x >= 12.0) > 40000
