@@ -35,27 +35,74 @@ template <class T>
 __device__ __forceinline__ Dual<T> wave_sum(const Dual<T>& z) {
   return {wave_sum(z.v), wave_sum(z.d)};
 }
+// Flat access to the doubles of a scalar (compile-time component index).
 template <class Z>
-__device__ __forceinline__ IfaceZ<Z> wave_sum(const IfaceZ<Z>& f) {
-  IfaceZ<Z> o;
-  o.rd00 = wave_sum(f.rd00); o.rd01 = wave_sum(f.rd01); o.rd11 = wave_sum(f.rd11);
-  o.ru00 = wave_sum(f.ru00); o.ru01 = wave_sum(f.ru01); o.ru11 = wave_sum(f.ru11);
-  o.td00 = wave_sum(f.td00); o.td01 = wave_sum(f.td01); o.td10 = wave_sum(f.td10);
-  o.td11 = wave_sum(f.td11);
-  return o;
+struct Flat;
+template <>
+struct Flat<cd> {
+  static constexpr int N = 2;
+  __device__ static double get(const cd& z, int k) { return k == 0 ? z.r : z.i; }
+};
+template <>
+struct Flat<dcd> {
+  static constexpr int N = 4;
+  __device__ static double get(const dcd& z, int k) {
+    return k == 0 ? z.v.r : (k == 1 ? z.v.i : (k == 2 ? z.d.r : z.d.i));
+  }
+};
+
+// Row stride (doubles) of the reduction buffer: one pad element spreads the
+// rows that different lanes sum over the LDS banks.
+constexpr int RED_STRIDE = AD_WAVE + 1;
+template <class Z>
+constexpr int red_rows() {
+  return 10 * Flat<Z>::N;  // the largest record reduced: IfaceZ<Z>
 }
 
+// Per-wave sums of the per-lane cotangent records, written per tile.  Each
+// lane writes its record as a column of an LDS matrix (one row per double of
+// the record), then lane e adds row e in a fixed order and stores it: about a
+// tenth of the LDS operations of a shuffle butterfly per double, and the
+// same order on every run.
 template <class Z>
 struct WaveSink {
   IfaceZ<Z>* cif;
   LayerCot<Z>* cly;
+  double* red;  // LDS, red_rows<Z>() x RED_STRIDE
+
+  template <int NF>
+  __device__ void reduce_store(const Z (&v)[NF], double* __restrict__ dst) {
+    constexpr int N = Flat<Z>::N;
+    constexpr int ROWS = NF * N;
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) red[(f * N + k) * RED_STRIDE + lane] = Flat<Z>::get(v[f], k);
+    }
+    __syncthreads();
+    if (lane < ROWS) {
+      const double* row = red + lane * RED_STRIDE;
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll 4
+      for (int j = 0; j < AD_WAVE; j += 4) {
+        s0 += row[j];
+        s1 += row[j + 1];
+        s2 += row[j + 2];
+        s3 += row[j + 3];
+      }
+      dst[lane] = (s0 + s1) + (s2 + s3);
+    }
+    __syncthreads();  // the buffer is rewritten by the next record
+  }
   __device__ void iface(int i, const IfaceZ<Z>& f) {
-    const IfaceZ<Z> r = wave_sum(f);
-    if (threadIdx.x == 0) cif[i] = r;
+    const Z v[10] = {f.rd00, f.rd01, f.rd11, f.ru00, f.ru01,
+                     f.ru11, f.td00, f.td01, f.td10, f.td11};
+    reduce_store<10>(v, reinterpret_cast<double*>(cif + i));
   }
   __device__ void layer(int l, const LayerCot<Z>& c) {
-    const LayerCot<Z> r{wave_sum(c.eta), wave_sum(c.neta), wave_sum(c.h)};
-    if (threadIdx.x == 0) cly[l] = r;
+    const Z v[3] = {c.eta, c.neta, c.h};
+    reduce_store<3>(v, reinterpret_cast<double*>(cly + l));
   }
 };
 
@@ -129,6 +176,7 @@ __global__ void __launch_bounds__(AD_WAVE) adjoint_kernel(
   extern __shared__ double smem_a[];
   LayerZ<Z>* s_layer = reinterpret_cast<LayerZ<Z>*>(smem_a);
   IfaceZ<Z>* s_if = reinterpret_cast<IfaceZ<Z>*>(s_layer + nl);
+  double* s_red = reinterpret_cast<double*>(s_if + (nl - 1));
   int b = blockIdx.x;
   const int tile = b % ntiles;
   b /= ntiles;
@@ -142,7 +190,7 @@ __global__ void __launch_bounds__(AD_WAVE) adjoint_kernel(
   const cd om = omega[wc];
   const size_t slot = (size_t)blockIdx.x;  // (dir, p, tile)
   GlobalStore<Z> st{ws + (long)blockIdx.x * AD_WAVE + threadIdx.x, nlanes};
-  WaveSink<Z> sink{cot_if + slot * (nl - 1), cot_ly + slot * nl};
+  WaveSink<Z> sink{cot_if + slot * (nl - 1), cot_ly + slot * nl, s_red};
   const cd gin = Gin[(size_t)ip * nw + wc];
   const cd dgin = dGin ? dGin[((size_t)dir * np + ip) * nw + wc] : cd{0.0, 0.0};
   Z chi = lift<Z>(0.0);
@@ -282,7 +330,8 @@ static int run_adjoint(const char* what, const double* layers, const double* dla
   char* wsb = reinterpret_cast<char*>(workspace);
   const long nblocks = (long)ndir * np * d.ntiles;
   KH_REQUIRE(nblocks <= 0x7fffffffL, KH_ERR_UNSUPPORTED, "%s: grid too large", what);
-  const size_t lds = (size_t)nl * sizeof(LayerZ<Z>) + (size_t)(nl - 1) * sizeof(IfaceZ<Z>);
+  const size_t lds = (size_t)nl * sizeof(LayerZ<Z>) + (size_t)(nl - 1) * sizeof(IfaceZ<Z>) +
+                     (size_t)red_rows<Z>() * RED_STRIDE * sizeof(double);
   auto* cif = reinterpret_cast<IfaceZ<Z>*>(wsb + d.cif);
   auto* cly = reinterpret_cast<LayerCot<Z>*>(wsb + d.cly);
   auto* chip = reinterpret_cast<Z*>(wsb + d.chi);
