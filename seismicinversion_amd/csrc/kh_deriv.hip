@@ -13,9 +13,9 @@
 //                         cotangent) or misfit gradient (M = 1);
 //                         Z = Dual<cd>, M = 1: tangent-over-adjoint, i.e. the
 //                         misfit gradient and the Hessian-vector product H dm
-//   chain_kernel<Z>       per (direction, p, layer): sums the tiles in order
-//                         and chains coefficient/slowness cotangents to the six
-//                         layer parameters
+//   chain_kernel<Z>       per (direction, p): sums the tiles in order, then
+//                         one thread per (layer, parameter) chains coefficient
+//                         and slowness cotangents to that layer parameter
 #include <type_traits>
 
 #include "kh_deriv.hpp"
@@ -224,25 +224,18 @@ __global__ void __launch_bounds__(AD_WAVE) adjoint_kernel(
   }
 }
 
-template <class Z>
-__device__ __forceinline__ IfaceZ<Z> iface_add(const IfaceZ<Z>& a, const IfaceZ<Z>& b) {
-  IfaceZ<Z> o;
-  o.rd00 = a.rd00 + b.rd00; o.rd01 = a.rd01 + b.rd01; o.rd11 = a.rd11 + b.rd11;
-  o.ru00 = a.ru00 + b.ru00; o.ru01 = a.ru01 + b.ru01; o.ru11 = a.ru11 + b.ru11;
-  o.td00 = a.td00 + b.td00; o.td01 = a.td01 + b.td01; o.td10 = a.td10 + b.td10;
-  o.td11 = a.td11 + b.td11;
-  return o;
-}
-
 __device__ __forceinline__ double tangent_re(const cd&) { return 0.0; }
 __device__ __forceinline__ double tangent_re(const dcd& z) { return z.d.r; }
 
-// grid-stride over (dir, p, l).  grad (np, nl, 6) from dir 0; hvp (ndir, np, nl, 6)
-// for dual Z; chi (ndir, np, 2) [value, tangent] when chi_part is given.
-// One wave per workgroup: the chain rule through the interface formulas keeps
-// dual-of-dual temporaries live, so the kernel takes the whole register file
-// of a wave (a 1024-thread bound would cap it at 128 VGPRs and spill).
-constexpr int CH_BLOCK = 64;
+// One workgroup per (dir, p).  grad (np, nl, 6) from dir 0; hvp (ndir, np,
+// nl, 6) for dual Z; chi (ndir, np, 2) [value, tangent] when chi_part is
+// given.  The workgroup first adds the per-tile cotangents of every interface
+// and layer in tile order (one double per thread at a time) into LDS, then
+// each thread chains one (layer, parameter column) pair: the chain rule
+// through the interface formulas runs on dual-of-dual values and takes a
+// wave's whole register file, so spreading the columns over threads is what
+// fills the device.
+constexpr int CH_BLOCK = 128;
 
 template <class Z>
 __global__ void __launch_bounds__(CH_BLOCK) chain_kernel(const double* __restrict__ layers, const double* __restrict__ dlayers,
@@ -251,34 +244,44 @@ __global__ void __launch_bounds__(CH_BLOCK) chain_kernel(const double* __restric
                              const LayerCot<Z>* __restrict__ cot_ly,
                              const Z* __restrict__ chi_part, int ntiles, double* __restrict__ grad,
                              double* __restrict__ hvp, double* __restrict__ chi) {
-  const long n = (long)ndir * np * nl;
-  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n) return;
-  const int l = (int)(idx % nl);
-  const int ip = (int)((idx / nl) % np);
-  const int dir = (int)(idx / ((long)nl * np));
+  extern __shared__ double smem_c[];
+  IfaceZ<Z>* s_if = reinterpret_cast<IfaceZ<Z>*>(smem_c);
+  LayerCot<Z>* s_ly = reinterpret_cast<LayerCot<Z>*>(s_if + (nl - 1));
+  const int ip = (int)(blockIdx.x % (unsigned)np);
+  const int dir = (int)(blockIdx.x / (unsigned)np);
   const size_t base = ((size_t)dir * np + ip) * ntiles;
-  const Z zero = lift<Z>(0.0);
-  IfaceZ<Z> up{zero, zero, zero, zero, zero, zero, zero, zero, zero, zero}, dn = up;
-  LayerCot<Z> lc{zero, zero, zero};
-  for (int t = 0; t < ntiles; ++t) {
-    const size_t s = base + t;
-    if (l >= 1) up = iface_add(up, cot_if[s * (nl - 1) + (l - 1)]);
-    if (l <= nl - 2) dn = iface_add(dn, cot_if[s * (nl - 1) + l]);
-    const LayerCot<Z>& c = cot_ly[s * nl + l];
-    lc.eta = lc.eta + c.eta;
-    lc.neta = lc.neta + c.neta;
-    lc.h = lc.h + c.h;
+  constexpr int NI = (int)(sizeof(IfaceZ<Z>) / sizeof(double));
+  constexpr int NLY = (int)(sizeof(LayerCot<Z>) / sizeof(double));
+  const int nif = (nl - 1) * NI;
+  const int nall = nif + nl * NLY;
+  const double* gif = reinterpret_cast<const double*>(cot_if + base * (nl - 1));
+  const double* gly = reinterpret_cast<const double*>(cot_ly + base * nl);
+  double* sif = reinterpret_cast<double*>(s_if);
+  double* sly = reinterpret_cast<double*>(s_ly);
+  for (int e = threadIdx.x; e < nall; e += blockDim.x) {
+    double acc = 0.0;
+    if (e < nif) {
+      for (int t = 0; t < ntiles; ++t) acc += gif[(size_t)t * nif + e];
+      sif[e] = acc;
+    } else {
+      const int e2 = e - nif;
+      for (int t = 0; t < ntiles; ++t) acc += gly[(size_t)t * (nall - nif) + e2];
+      sly[e2] = acc;
+    }
   }
-  Z out[NCOL];
-  chain_layer<Z>(layers, dlayers ? dlayers + (size_t)dir * nl * NCOL : nullptr, nl, l, p[ip], up,
-                 dn, lc, out);
-  const size_t g = ((size_t)ip * nl + l) * NCOL;
-  if (grad && dir == 0)
-    for (int c = 0; c < NCOL; ++c) grad[g + c] = prim(out[c]).r;
-  if (hvp)
-    for (int c = 0; c < NCOL; ++c) hvp[(size_t)dir * np * nl * NCOL + g + c] = tangent_re(out[c]);
-  if (chi && chi_part && l == 0) {
+  __syncthreads();
+  const Z zero = lift<Z>(0.0);
+  const IfaceZ<Z> none{zero, zero, zero, zero, zero, zero, zero, zero, zero, zero};
+  const double* dl = dlayers ? dlayers + (size_t)dir * nl * NCOL : nullptr;
+  for (int k = threadIdx.x; k < nl * NCOL; k += blockDim.x) {
+    const int l = k / NCOL, col = k - (k / NCOL) * NCOL;
+    const Z v = chain_column<Z>(layers, dl, nl, l, p[ip], l >= 1 ? s_if[l - 1] : none,
+                                l <= nl - 2 ? s_if[l] : none, s_ly[l], col);
+    const size_t g = ((size_t)ip * nl + l) * NCOL + col;
+    if (grad && dir == 0) grad[g] = prim(v).r;
+    if (hvp) hvp[(size_t)dir * np * nl * NCOL + g] = tangent_re(v);
+  }
+  if (chi && chi_part && threadIdx.x == 0) {
     Z s = zero;
     for (int t = 0; t < ntiles; ++t) s = s + chi_part[base + t];
     chi[((size_t)dir * np + ip) * 2] = prim(s).r;
@@ -341,9 +344,8 @@ static int run_adjoint(const char* what, const double* layers, const double* dla
                      reinterpret_cast<Z*>(wsb + d.ws), d.nlanes, cif, cly, chip,
                      reinterpret_cast<cd*>(R), reinterpret_cast<cd*>(dR), d.ntiles);
   KH_TRY(check_launch(what));
-  const long n = (long)ndir * np * nl;
-  hipLaunchKernelGGL((chain_kernel<Z>), dim3((unsigned)((n + CH_BLOCK - 1) / CH_BLOCK)),
-                     dim3(CH_BLOCK), 0, st,
+  const size_t lds_c = (size_t)(nl - 1) * sizeof(IfaceZ<Z>) + (size_t)nl * sizeof(LayerCot<Z>);
+  hipLaunchKernelGGL((chain_kernel<Z>), dim3((unsigned)((long)ndir * np)), dim3(CH_BLOCK), lds_c, st,
                      layers, dlayers, nl, p, np, ndir, cif, cly, MISFIT ? chip : nullptr,
                      d.ntiles, grad, hvp, chi);
   return check_launch(what);

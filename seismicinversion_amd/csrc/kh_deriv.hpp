@@ -218,42 +218,50 @@ __host__ __device__ __forceinline__ IfaceZ<Z> outer_iface(const IfaceZ<W>& f) {
   return o;
 }
 
-// Parameter cotangents of layer l (out[c] for c in COL_*), given the summed
-// cotangents: cb_up = interface l-1 (ignored for l = 0), cb_dn = interface l
-// (ignored for l = nl-1), lc = layer l.  `layers`/`dlayers` are the row-major
-// (nl, 6) parameter and direction tables (dlayers only for dual Z).
+// Cotangent of parameter column col (a COL_* index) of layer l, given the
+// summed cotangents: cb_up = interface l-1 (ignored for l = 0), cb_dn =
+// interface l (ignored for l = nl-1), lc = layer l.  `layers`/`dlayers` are
+// the row-major (nl, 6) parameter and direction tables (dlayers only for
+// dual Z).  The thickness enters only through the phase, whose cotangent
+// is lc.h.
+template <class Z>
+__host__ __device__ inline Z chain_column(const double* layers, const double* dlayers, int nl,
+                                          int l, double p, const IfaceZ<Z>& cb_up,
+                                          const IfaceZ<Z>& cb_dn, const LayerCot<Z>& lc,
+                                          int col) {
+  using W = typename Up<Z>::W;
+  if (col == COL_H) return lc.h;
+  LayerZ<W> Ls[3];
+  for (int q = 0; q < 3; ++q) {
+    const int ll = l - 1 + q;
+    if (ll < 0 || ll >= nl) continue;
+    W x[NCOL];
+    for (int c = 0; c < NCOL; ++c) {
+      const double dm = dlayers ? dlayers[(size_t)ll * NCOL + c] : 0.0;
+      x[c] = Up<Z>::seed(layers[(size_t)ll * NCOL + c], dm, (ll == l && c == col) ? 1.0 : 0.0);
+    }
+    Ls[q] = layer_t<W>(x, ll, p);
+  }
+  Z acc = re_dot(Up<Z>::outer(Ls[1].eta), lc.eta) + re_dot(Up<Z>::outer(Ls[1].neta), lc.neta);
+  if (l >= 1) {
+    const IfaceZ<W> f = interface_t<W>(l - 1, p, Ls[0], Ls[1]);
+    acc = acc + iface_dot(outer_iface<W, Z>(f), cb_up);
+  }
+  if (l <= nl - 2) {
+    const IfaceZ<W> f = interface_t<W>(l, p, Ls[1], Ls[2]);
+    acc = acc + iface_dot(outer_iface<W, Z>(f), cb_dn);
+  }
+  return acc;
+}
+
+// All six parameter cotangents of layer l (out[c] for c in COL_*).
 template <class Z>
 __host__ __device__ inline void chain_layer(const double* layers, const double* dlayers, int nl,
                                             int l, double p, const IfaceZ<Z>& cb_up,
                                             const IfaceZ<Z>& cb_dn, const LayerCot<Z>& lc,
                                             Z (&out)[NCOL]) {
-  using W = typename Up<Z>::W;
-  const int cols[5] = {COL_ALPHA, COL_BETA, COL_RHO, COL_QA, COL_QB};
-  out[COL_H] = lc.h;
-  for (int t = 0; t < 5; ++t) {
-    const int col = cols[t];
-    LayerZ<W> Ls[3];
-    for (int q = 0; q < 3; ++q) {
-      const int ll = l - 1 + q;
-      if (ll < 0 || ll >= nl) continue;
-      W x[NCOL];
-      for (int c = 0; c < NCOL; ++c) {
-        const double dm = dlayers ? dlayers[(size_t)ll * NCOL + c] : 0.0;
-        x[c] = Up<Z>::seed(layers[(size_t)ll * NCOL + c], dm, (ll == l && c == col) ? 1.0 : 0.0);
-      }
-      Ls[q] = layer_t<W>(x, ll, p);
-    }
-    Z acc = re_dot(Up<Z>::outer(Ls[1].eta), lc.eta) + re_dot(Up<Z>::outer(Ls[1].neta), lc.neta);
-    if (l >= 1) {
-      const IfaceZ<W> f = interface_t<W>(l - 1, p, Ls[0], Ls[1]);
-      acc = acc + iface_dot(outer_iface<W, Z>(f), cb_up);
-    }
-    if (l <= nl - 2) {
-      const IfaceZ<W> f = interface_t<W>(l, p, Ls[1], Ls[2]);
-      acc = acc + iface_dot(outer_iface<W, Z>(f), cb_dn);
-    }
-    out[col] = acc;
-  }
+  for (int col = 0; col < NCOL; ++col)
+    out[col] = chain_column<Z>(layers, dlayers, nl, l, p, cb_up, cb_dn, lc, col);
 }
 
 }  // namespace kh
