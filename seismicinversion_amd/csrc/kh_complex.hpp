@@ -69,14 +69,32 @@ __host__ __device__ __forceinline__ cd operator/(cd a, cd b) {
 // 1 / b with the same scaling as the division above.
 __host__ __device__ __forceinline__ cd recip(cd b) { return cd{1.0, 0.0} / b; }
 
+// 1 / d for a positive d of order one (far from the overflow and underflow
+// thresholds).  On the device: the hardware reciprocal estimate refined by
+// two Newton steps r <- r + r (1 - d r), each of which squares the relative
+// error, so the result is within an ulp of the rounded quotient; IEEE
+// division adds the range scaling and fix-up steps this operand never
+// needs.  d = 0 or inf gives NaN.
+__host__ __device__ __forceinline__ double recip_pos(double d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+#else
+  return 1.0 / d;
+#endif
+}
+
 // 1 / det for the 2x2 determinant of the recursion step, det = 1 - O(|Ru MT|):
 // of order one, so |det|^2 neither overflows nor underflows and
 // conj(det) / |det|^2 needs one division and no branch (Smith's scaling,
 // two divisions behind a data-dependent branch, guards ranges this value
 // never reaches).  A singular step (det = 0) gives non-finite values, as the
-// reference's division does.
+// reference's division does (NaN here where the division gives inf).
 __host__ __device__ __forceinline__ cd recip_det(cd b) {
-  const double inv = 1.0 / fma(b.r, b.r, b.i * b.i);
+  const double inv = recip_pos(fma(b.r, b.r, b.i * b.i));
   return cd{b.r * inv, -b.i * inv};
 }
 

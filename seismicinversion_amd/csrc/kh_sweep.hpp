@@ -50,13 +50,14 @@ __host__ __device__ __forceinline__ LayerStep layer_step(const LayerPt& L, doubl
 }
 
 // One interface step for one frequency: RRd <- Rd + Tu MT U Td, with
-// MT = E RRd E, U = (I - Ru MT)^-1, Tu = Td^T.  Every sum of products is one
-// chain of fused multiply-adds seeded with its first product or its
-// constant term (1 - ..., Rd + ...), which keeps the FP64 instruction count
-// near one per two FLOPs.
-__host__ __device__ __forceinline__ void kennett_step(cd& r00, cd& r01, cd& r10, cd& r11, cd ea,
-                                                      cd eb, const Iface& f) {
-  const cd eaea = ea * ea, ebeb = eb * eb, eaeb = ea * eb;
+// MT = E RRd E, U = (I - Ru MT)^-1, Tu = Td^T, given the phase products
+// eaea = ea^2, ebeb = eb^2, eaeb = ea eb of E = diag(ea, eb).  Every sum of
+// products is one chain of fused multiply-adds seeded with its first product
+// or its constant term (1 - ..., Rd + ...), which keeps the FP64 instruction
+// count near one per two FLOPs.
+__host__ __device__ __forceinline__ void kennett_step_e2(cd& r00, cd& r01, cd& r10, cd& r11,
+                                                         cd eaea, cd ebeb, cd eaeb,
+                                                         const Iface& f) {
   const cd m00 = eaea * r00, m01 = eaeb * r01, m10 = eaeb * r10, m11 = ebeb * r11;
   const cd one{1.0, 0.0};
   // A = I - Ru MT (Ru symmetric)
@@ -79,9 +80,18 @@ __host__ __device__ __forceinline__ void kennett_step(cd& r00, cd& r01, cd& r10,
   r11 = cfma(y11, f.td11, cfma(y10, f.td01, f.rd11));
 }
 
+// The same step from the phase factors themselves.
+__host__ __device__ __forceinline__ void kennett_step(cd& r00, cd& r01, cd& r10, cd& r11, cd ea,
+                                                      cd eb, const Iface& f) {
+  kennett_step_e2(r00, r01, r10, r11, ea * ea, eb * eb, ea * eb, f);
+}
+
 // RRd00 just below the water layer for W frequencies of one lane.  With
 // GRID, om[k] = om[0] + k * shift (real shift) and `steps` holds the
-// per-layer factors for that shift; otherwise every phase is evaluated.
+// per-layer factors for that shift: the phase products of frequency k are
+// those of frequency k-1 times the products of the factors (ea^2 steps by
+// a^2, eb^2 by b^2, ea eb by a b), evaluated once per layer and lane.
+// Otherwise every phase is evaluated.
 template <int W, bool GRID>
 __host__ __device__ __forceinline__ void sweep_rrd00(const LayerPt* __restrict__ L,
                                                      const Iface* __restrict__ F,
@@ -98,31 +108,42 @@ __host__ __device__ __forceinline__ void sweep_rrd00(const LayerPt* __restrict__
       q11[k] = fb.rd11;
     }
   }
+  const cd one{1.0, 0.0};
   for (int i = nl - 3; i >= 0; --i) {
     const LayerPt& lb = L[i + 1];
     const Iface f = F[i];
-    cd ea[W], eb[W];
-    if (lb.halfspace) {
-#pragma unroll
-      for (int k = 0; k < W; ++k) ea[k] = eb[k] = cd{1.0, 0.0};
-    } else if (GRID) {
-      ea[0] = phase(lb.eta, om[0], lb.h);
-      eb[0] = phase(lb.neta, om[0], lb.h);
-      const LayerStep s = steps[i + 1];
-#pragma unroll
-      for (int k = 1; k < W; ++k) {
-        ea[k] = ea[k - 1] * s.a;
-        eb[k] = eb[k - 1] * s.b;
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < W; ++k) {
-        ea[k] = phase(lb.eta, om[k], lb.h);
-        eb[k] = phase(lb.neta, om[k], lb.h);
+    const bool hs = lb.halfspace;  // unit phase factors
+    cd aa = one, bb = one, ab = one, sa2 = one, sb2 = one, sab = one;
+    if (GRID && !hs) {
+      const cd ea = phase(lb.eta, om[0], lb.h);
+      const cd eb = phase(lb.neta, om[0], lb.h);
+      aa = ea * ea;
+      bb = eb * eb;
+      ab = ea * eb;
+      if (W > 1) {
+        const LayerStep s = steps[i + 1];
+        sa2 = s.a * s.a;
+        sb2 = s.b * s.b;
+        sab = s.a * s.b;
       }
     }
 #pragma unroll
-    for (int k = 0; k < W; ++k) kennett_step(q00[k], q01[k], q10[k], q11[k], ea[k], eb[k], f);
+    for (int k = 0; k < W; ++k) {
+      if (GRID) {
+        if (k > 0) {
+          aa = aa * sa2;
+          bb = bb * sb2;
+          ab = ab * sab;
+        }
+      } else if (!hs) {
+        const cd ea = phase(lb.eta, om[k], lb.h);
+        const cd eb = phase(lb.neta, om[k], lb.h);
+        aa = ea * ea;
+        bb = eb * eb;
+        ab = ea * eb;
+      }
+      kennett_step_e2(q00[k], q01[k], q10[k], q11[k], aa, bb, ab, f);
+    }
   }
 #pragma unroll
   for (int k = 0; k < W; ++k) out[k] = q00[k];
