@@ -373,6 +373,14 @@ def main():
     if args.open_seconds > 0:
         n_open = max(1, int(args.rate * args.open_seconds))
         stream = serving_stream(args.rate, n_open, seed=42, lm_fraction=0.2)
+        # 20-layer forward requests (slowness and frequency counts from the
+        # same distributions) arriving beside the mix at a tenth of its rate,
+        # so that the 20-layer latency percentiles rest on more than the few
+        # 20-layer draws of the mix itself
+        probe_rate = 0.1 * args.rate
+        probe = serving_stream(probe_rate, max(1, int(probe_rate * args.open_seconds)), seed=7,
+                               lm_fraction=0.0, nl_range=(20, 20))
+        stream = sorted(stream + probe, key=lambda it: it.t)
         reqs = stream_requests(stream)
         with KennettServer(backend=DeviceBackend(n_streams=3)) as osrv:
             t0 = time.perf_counter()
@@ -387,6 +395,7 @@ def main():
                 f.result()
         st = osrv.stats()
         open_loop = {"rate_req_per_s": args.rate, "requests": n_open,
+                     "probe_20_layer_rate_req_per_s": probe_rate, "probe_requests": len(probe),
                      "inversions": sum(1 for k, _ in reqs if k == "invert"),
                      "forward_updates_per_s": st["forward_updates_per_s"],
                      "batches": st["batches"], "failed": st["failed"],
@@ -401,10 +410,16 @@ def main():
     upd_per_launch = float(np.mean([u for _, u, _ in log]))
     out_per_launch = float(np.mean([e for _, _, e in log]))
     achieved = FLOP_PER_UPDATE * upd_per_launch / (kern_ms * 1e-3) / 1e12
+    # HBM bytes per launch: the ratio of measured (rocprofv3 FETCH_SIZE and
+    # WRITE_SIZE, calibrated on a device copy) to algorithmic bytes over the
+    # served dispatches of the counter runs, times this run's algorithmic
+    # bytes per launch
     traffic = None
     if os.path.exists(args.traffic_json):
         with open(args.traffic_json) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            ratio = json.load(f).get("traffic_over_algorithmic")
+        if ratio:
+            traffic = float(ratio) * 16.0 * out_per_launch
     roofline = {"bound": "mfma", "pipe": "fp64-vector", "achieved": achieved,
                 "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
                 "traffic": traffic, "kernel": f"kh::multi_kernel<{be.shape[1]}>", "launches": len(log),
