@@ -137,21 +137,32 @@ size_t forward_lds_bytes(int nl) {
 // Default launch shape, measured on MI355X (clocks warmed up, interleaved
 // rounds) on the 20-layer gather grid, 8192 p x 1023 omega, free surface
 // (work-items per workgroup x frequencies per work-item):
-//   uniform grid: 128 x 4 1.384 ms, 256 x 4 1.394, 64 x 4 1.399, 64 x 8 1.445,
-//                 128 x 8 1.450, 256 x 2 1.592 ms.
-// Four frequencies per work-item share one phase exponential pair per layer;
-// eight need more registers than the sharing saves.  Picks 128 work-items and
-// the most frequencies per work-item (4, 2, 1) that keeps >= 1024 workgroups
-// and leaves less than a quarter of the last tile's lanes idle.
-static void default_shape(int np, int nw, bool /*grid*/, int* block, int* wpt) {
+//   uniform grid: 64 x 8 1.021 ms, 128 x 8 1.024, 64 x 4 1.178, 128 x 4 1.183,
+//                 256 x 4 1.200, 256 x 8 2.009 ms.
+// On the uniform grid eight frequencies per work-item share one phase
+// exponential pair per layer and the step factors; the kernel then takes one
+// wave per SIMD (276 registers) and still keeps the FP64 pipes busier than
+// two waves of four frequencies.  Picks 64 x 8 on the uniform grid when that
+// keeps >= 1024 workgroups and leaves less than a quarter of the last tile's
+// lanes idle; otherwise 128 work-items and the most frequencies per
+// work-item (4, 2, 1) that meets the same two conditions.
+static void default_shape(int np, int nw, bool grid, int* block, int* wpt) {
+  auto fits = [&](int blk, int c) {
+    const long span = (long)c * blk;
+    const long tiles = (nw + span - 1) / span;
+    const long idle = tiles * span - nw;
+    return tiles * (long)np >= 1024 && 4 * idle < span;
+  };
+  if (grid && fits(64, 8)) {
+    *block = 64;
+    *wpt = 8;
+    return;
+  }
   *block = nw <= 64 ? 64 : 128;
   *wpt = 1;
   const int cands[2] = {4, 2};
   for (int c : cands) {
-    const long span = (long)c * *block;
-    const long tiles = (nw + span - 1) / span;
-    const long idle = tiles * span - nw;
-    if (tiles * (long)np >= 1024 && 4 * idle < span) {
+    if (fits(*block, c)) {
       *wpt = c;
       break;
     }
