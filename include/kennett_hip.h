@@ -119,7 +119,7 @@ typedef struct {
   long long layer_off; /* first row of this model in layers_all ((rows, 6) float64) */
   long long p_off;     /* first slowness of this model in p_all */
   long long r_off;     /* first complex element of this model's (np, nw) block in R_all */
-  long long item_off;  /* first work item of this model (prefix sum of np * ntiles) */
+  long long item_off;  /* first work item of this model (prefix sum of np) */
   double k0, dw, gamma; /* frequencies omega_j = (k0 + j)*dw + i*gamma, j < nw */
   int nl, np, nw;
   int ntiles;          /* ceil(nw / (block * omega_per_item)) for the launch shape */
@@ -127,12 +127,13 @@ typedef struct {
   int pad;
 } kh_model_desc;
 
-/* Work items (slowness x frequency tiles) of one model for a launch shape. */
+/* Work items of one model for a launch shape: one per slowness (the item
+ * sweeps all ntiles frequency tiles), i.e. np; 0 if any argument is <= 0. */
 long long kh_multi_items(int np, int nw, int block, int omega_per_item);
 
 /* Reflectivity of nmodels models in one persistent launch: workgroups take
- * (model, slowness, frequency tile) items from a device counter until all
- * total_items are done.  desc (nmodels,) is a DEVICE array sorted by item_off
+ * (model, slowness) items from a device counter until all total_items are
+ * done, staging the model's tables at that slowness once per item.  desc (nmodels,) is a DEVICE array sorted by item_off
  * (list the costliest models first for the best drain order); max_nl = the
  * largest nl; counter = 8 bytes of device memory (reset by the call);
  * nonfinite = NULL or a device int array (nmodels,) that receives, for each

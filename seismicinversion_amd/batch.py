@@ -108,7 +108,7 @@ def plan_multi(specs: list[ModelSpec], block: int, wpt: int, dev=None,
         desc[slot] = (layer_off[i], p_off[i], r_off[i], item, float(s.k0), float(s.dw),
                       float(s.gamma), tabs[i].shape[0], len(s.p), s.nw, ntiles,
                       int(bool(s.free_surface)), 0)
-        item += len(s.p) * ntiles
+        item += len(s.p)  # one work item per slowness
     layers = _to_device(np.concatenate(tabs, axis=0), dev, non_blocking)
     p = _to_device(np.concatenate([np.asarray(s.p, np.float64) for s in specs])
                    if specs else np.zeros(0), dev, non_blocking)
