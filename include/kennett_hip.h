@@ -133,8 +133,10 @@ long long kh_multi_items(int np, int nw, int block, int omega_per_item);
 
 /* Reflectivity of nmodels models in one persistent launch: workgroups take
  * (model, slowness) items from a device counter until all total_items are
- * done, staging the model's tables at that slowness once per item.  desc (nmodels,) is a DEVICE array sorted by item_off
- * (list the costliest models first for the best drain order); max_nl = the
+ * done, staging the model's tables at that slowness once per item.
+ * desc (nmodels,) is a DEVICE array sorted by item_off (list the models
+ * with the costliest items, ntiles * (nl - 1), first for the best drain
+ * order); max_nl = the
  * largest nl; counter = 8 bytes of device memory (reset by the call);
  * nonfinite = NULL or a device int array (nmodels,) that receives, for each
  * desc entry k, the number of NaN/Inf outputs of that model (reset by the call);

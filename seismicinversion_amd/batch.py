@@ -2,9 +2,9 @@
 
 ``reflectivity_multi`` takes models of different depths with their own
 slownesses and uniform frequency grids and returns each model's (np, nw)
-complex128 reflectivity.  Models are queued costliest first (interfaces x
-frequencies per work item) so the persistent workgroups drain the long items
-early; results are bitwise identical to one ``kh_forward_grid_ex`` launch
+complex128 reflectivity.  Models are queued by the cost of one work item
+(interfaces x frequency tiles of one slowness), longest first, so the
+persistent workgroups take the long items early; results are bitwise identical to one ``kh_forward_grid_ex`` launch
 per model with the same launch shape.
 """
 
@@ -95,8 +95,13 @@ def plan_multi(specs: list[ModelSpec], block: int, wpt: int, dev=None,
         if t.shape[0] < 2:
             raise ValueError("every model needs at least 2 layers")
     span = block * wpt
-    cost = [(len(s.p) * -(-s.nw // span)) * (t.shape[0] - 1) * span for s, t in zip(specs, tabs)]
-    order = sorted(range(len(specs)), key=lambda i: -cost[i])
+    # Longest items first: an item sweeps all frequency tiles of one slowness,
+    # so its cost is ntiles * (nl - 1) tile steps; ordering the queue by that
+    # (then by the model's total) leaves the smallest items for the drain at
+    # the end of the launch.
+    item_cost = [-(-s.nw // span) * (t.shape[0] - 1) for s, t in zip(specs, tabs)]
+    total_cost = [len(s.p) * c for s, c in zip(specs, item_cost)]
+    order = sorted(range(len(specs)), key=lambda i: (-item_cost[i], -total_cost[i]))
     desc = np.zeros(len(specs), DESC_DTYPE)
     layer_off = np.cumsum([0] + [t.shape[0] for t in tabs])
     p_off = np.cumsum([0] + [len(s.p) for s in specs])

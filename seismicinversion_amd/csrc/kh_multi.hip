@@ -2,8 +2,8 @@
 // grids in one persistent launch.
 //
 // The work is a list of items (model, slowness).  Items are numbered model
-// by model (callers list the costliest models first, so the queue drains
-// longest-first); each workgroup repeatedly takes the next item number from
+// by model (callers list the models with the costliest items first, so the
+// queue drains longest-first); each workgroup repeatedly takes the next item number from
 // a device-wide counter (one atomic per item), stages that model's layers
 // and interfaces at that slowness in LDS once and sweeps all of the model's
 // frequency tiles with them, until the counter passes the item count.
