@@ -52,11 +52,15 @@ struct Flat<dcd> {
 };
 
 // Row stride (doubles) of the reduction buffer: one pad element spreads the
-// rows that different lanes sum over the LDS banks.
+// rows that different lanes sum over the LDS banks.  A record of up to 10
+// scalars is reduced in one pass (one row per double), or, when `split` is
+// set, 5 scalars at a time: with deep models the LDS per workgroup, not the
+// registers, decides how many workgroups share a CU, and the split halves
+// the buffer at the cost of a second pass.
 constexpr int RED_STRIDE = AD_WAVE + 1;
 template <class Z>
-constexpr int red_rows() {
-  return 10 * Flat<Z>::N;  // the largest record reduced: IfaceZ<Z>
+constexpr int red_rows(bool split) {
+  return (split ? 5 : 10) * Flat<Z>::N;
 }
 
 // Per-wave sums of the per-lane cotangent records, written per tile.  Each
@@ -68,20 +72,21 @@ template <class Z>
 struct WaveSink {
   IfaceZ<Z>* cif;
   LayerCot<Z>* cly;
-  double* red;  // LDS, red_rows<Z>() x RED_STRIDE
+  double* red;  // LDS, red_rows<Z>(split) x RED_STRIDE
+  bool split;
 
-  template <int NF>
-  __device__ void reduce_store(const Z (&v)[NF], double* __restrict__ dst) {
+  // one pass over scalars [F0, F1) of the record
+  template <int F0, int F1, int NF>
+  __device__ void reduce_pass(const Z (&v)[NF], double* __restrict__ dst) {
     constexpr int N = Flat<Z>::N;
-    constexpr int ROWS = NF * N;
     const int lane = threadIdx.x;
 #pragma unroll
-    for (int f = 0; f < NF; ++f) {
+    for (int f = F0; f < F1; ++f) {
 #pragma unroll
-      for (int k = 0; k < N; ++k) red[(f * N + k) * RED_STRIDE + lane] = Flat<Z>::get(v[f], k);
+      for (int k = 0; k < N; ++k) red[((f - F0) * N + k) * RED_STRIDE + lane] = Flat<Z>::get(v[f], k);
     }
     __syncthreads();
-    if (lane < ROWS) {
+    if (lane < (F1 - F0) * N) {
       const double* row = red + lane * RED_STRIDE;
       double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll 4
@@ -91,18 +96,24 @@ struct WaveSink {
         s2 += row[j + 2];
         s3 += row[j + 3];
       }
-      dst[lane] = (s0 + s1) + (s2 + s3);
+      dst[F0 * N + lane] = (s0 + s1) + (s2 + s3);
     }
-    __syncthreads();  // the buffer is rewritten by the next record
+    __syncthreads();  // the buffer is rewritten by the next pass
   }
   __device__ void iface(int i, const IfaceZ<Z>& f) {
     const Z v[10] = {f.rd00, f.rd01, f.rd11, f.ru00, f.ru01,
                      f.ru11, f.td00, f.td01, f.td10, f.td11};
-    reduce_store<10>(v, reinterpret_cast<double*>(cif + i));
+    double* dst = reinterpret_cast<double*>(cif + i);
+    if (split) {
+      reduce_pass<0, 5>(v, dst);
+      reduce_pass<5, 10>(v, dst);
+    } else {
+      reduce_pass<0, 10>(v, dst);
+    }
   }
   __device__ void layer(int l, const LayerCot<Z>& c) {
     const Z v[3] = {c.eta, c.neta, c.h};
-    reduce_store<3>(v, reinterpret_cast<double*>(cly + l));
+    reduce_pass<0, 3>(v, reinterpret_cast<double*>(cly + l));
   }
 };
 
@@ -164,6 +175,27 @@ __global__ void __launch_bounds__(256) jvp_kernel(const double* __restrict__ lay
   if (R && dir == 0) R[o] = r.v;
 }
 
+// Stages, for the adjoint sweep, each layer's (eta, nu, h) and each
+// interface's coefficients at slowness pv.  Thread l computes layer l and,
+// for the interface below it, layer l + 1 again, so the full LayerZ records
+// never need LDS of their own.
+template <class Z>
+__device__ __forceinline__ void stage_ph(const double* __restrict__ layers,
+                                         const double* __restrict__ drow_base, int nl, double pv,
+                                         LayerPh<Z>* s_ph, IfaceZ<Z>* s_if) {
+  auto lay = [&](int l) {
+    Z x[NCOL];
+    row_z(layers + (size_t)l * NCOL, drow_base ? drow_base + (size_t)l * NCOL : nullptr, x);
+    return layer_t<Z>(x, l, pv);
+  };
+  for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+    const LayerZ<Z> a = lay(l);
+    s_ph[l] = layer_ph(a);
+    if (l < nl - 1) s_if[l] = interface_t<Z>(l, pv, a, lay(l + 1));
+  }
+  __syncthreads();
+}
+
 // grid.x = ndir * np * ntiles, blockDim = 64 (one wave per block).
 template <class Z, bool MISFIT>
 __global__ void __launch_bounds__(AD_WAVE) adjoint_kernel(
@@ -172,9 +204,9 @@ __global__ void __launch_bounds__(AD_WAVE) adjoint_kernel(
     const cd* __restrict__ Gin, const cd* __restrict__ dGin, int second_only, Z* __restrict__ ws,
     long nlanes,
     IfaceZ<Z>* __restrict__ cot_if, LayerCot<Z>* __restrict__ cot_ly, Z* __restrict__ chi_part,
-    cd* __restrict__ R, cd* __restrict__ dR, int ntiles) {
+    cd* __restrict__ R, cd* __restrict__ dR, int ntiles, int red_split) {
   extern __shared__ double smem_a[];
-  LayerZ<Z>* s_layer = reinterpret_cast<LayerZ<Z>*>(smem_a);
+  LayerPh<Z>* s_layer = reinterpret_cast<LayerPh<Z>*>(smem_a);
   IfaceZ<Z>* s_if = reinterpret_cast<IfaceZ<Z>*>(s_layer + nl);
   double* s_red = reinterpret_cast<double*>(s_if + (nl - 1));
   int b = blockIdx.x;
@@ -182,15 +214,15 @@ __global__ void __launch_bounds__(AD_WAVE) adjoint_kernel(
   b /= ntiles;
   const int ip = b % np;
   const int dir = b / np;
-  stage_z<Z>(layers, dlayers ? dlayers + (size_t)dir * nl * NCOL : nullptr, nl, p[ip], s_layer,
-             s_if);
+  stage_ph<Z>(layers, dlayers ? dlayers + (size_t)dir * nl * NCOL : nullptr, nl, p[ip], s_layer,
+              s_if);
   const int w = tile * AD_WAVE + threadIdx.x;
   const bool valid = w < nw;
   const int wc = valid ? w : nw - 1;
   const cd om = omega[wc];
   const size_t slot = (size_t)blockIdx.x;  // (dir, p, tile)
   GlobalStore<Z> st{ws + (long)blockIdx.x * AD_WAVE + threadIdx.x, nlanes};
-  WaveSink<Z> sink{cot_if + slot * (nl - 1), cot_ly + slot * nl, s_red};
+  WaveSink<Z> sink{cot_if + slot * (nl - 1), cot_ly + slot * nl, s_red, red_split != 0};
   const cd gin = Gin[(size_t)ip * nw + wc];
   const cd dgin = dGin ? dGin[((size_t)dir * np + ip) * nw + wc] : cd{0.0, 0.0};
   Z chi = lift<Z>(0.0);
@@ -333,8 +365,13 @@ static int run_adjoint(const char* what, const double* layers, const double* dla
   char* wsb = reinterpret_cast<char*>(workspace);
   const long nblocks = (long)ndir * np * d.ntiles;
   KH_REQUIRE(nblocks <= 0x7fffffffL, KH_ERR_UNSUPPORTED, "%s: grid too large", what);
-  const size_t lds = (size_t)nl * sizeof(LayerZ<Z>) + (size_t)(nl - 1) * sizeof(IfaceZ<Z>) +
-                     (size_t)red_rows<Z>() * RED_STRIDE * sizeof(double);
+  // split the reductions only when the whole-record buffer would keep four
+  // one-wave workgroups (the register limit of the dual kernel) off a CU's
+  // 160 KB of LDS
+  const size_t tables = (size_t)nl * sizeof(LayerPh<Z>) + (size_t)(nl - 1) * sizeof(IfaceZ<Z>);
+  const size_t red_full = (size_t)red_rows<Z>(false) * RED_STRIDE * sizeof(double);
+  const bool split = 4 * (tables + red_full) > (size_t)160 * 1024;
+  const size_t lds = tables + (size_t)red_rows<Z>(split) * RED_STRIDE * sizeof(double);
   auto* cif = reinterpret_cast<IfaceZ<Z>*>(wsb + d.cif);
   auto* cly = reinterpret_cast<LayerCot<Z>*>(wsb + d.cly);
   auto* chip = reinterpret_cast<Z*>(wsb + d.chi);
@@ -342,7 +379,7 @@ static int run_adjoint(const char* what, const double* layers, const double* dla
                      layers, dlayers, nl, p, np, reinterpret_cast<const cd*>(omega), nw, fs,
                      reinterpret_cast<const cd*>(Gin), reinterpret_cast<const cd*>(dGin), second_only,
                      reinterpret_cast<Z*>(wsb + d.ws), d.nlanes, cif, cly, chip,
-                     reinterpret_cast<cd*>(R), reinterpret_cast<cd*>(dR), d.ntiles);
+                     reinterpret_cast<cd*>(R), reinterpret_cast<cd*>(dR), d.ntiles, split ? 1 : 0);
   KH_TRY(check_launch(what));
   const size_t lds_c = (size_t)(nl - 1) * sizeof(IfaceZ<Z>) + (size_t)nl * sizeof(LayerCot<Z>);
   hipLaunchKernelGGL((chain_kernel<Z>), dim3((unsigned)((long)ndir * np)), dim3(CH_BLOCK), lds_c, st,

@@ -50,8 +50,23 @@ __host__ __device__ __forceinline__ Z zero_z() {
   return lift<Z>(0.0);
 }
 
+// What the sweep reads of a layer: its vertical slownesses and thickness
+// (the interface coefficients are precomputed).  Staging this instead of
+// the whole LayerZ keeps deep models within the LDS that four workgroups
+// per CU can share.
 template <class Z>
-__host__ __device__ __forceinline__ void phases_of(const LayerZ<Z>& lb, cd om, Z& ea, Z& eb) {
+struct LayerPh {
+  Z eta, neta, h;
+  int halfspace;
+};
+
+template <class Z>
+__host__ __device__ __forceinline__ LayerPh<Z> layer_ph(const LayerZ<Z>& L) {
+  return LayerPh<Z>{L.eta, L.neta, L.h, L.halfspace};
+}
+
+template <class Z, class LZ>
+__host__ __device__ __forceinline__ void phases_of(const LZ& lb, cd om, Z& ea, Z& eb) {
   if (lb.halfspace) {
     ea = lift<Z>(1.0);
     eb = lift<Z>(1.0);
@@ -64,8 +79,10 @@ __host__ __device__ __forceinline__ void phases_of(const LayerZ<Z>& lb, cd om, Z
 // Storage: st(e) returns a reference to Z-slot e of this lane.
 // Sink: sink.iface(i, IfaceZ<Z>), sink.layer(l, LayerCot<Z>).
 // G_of(R) -> cotangent of R.  Returns R.
-template <class Z, class Store, class Sink, class GFn>
-__host__ __device__ inline Z lane_adjoint(const LayerZ<Z>* __restrict__ L,
+// L: per-layer records with eta, neta, h and halfspace (LayerZ<Z> or
+// LayerPh<Z>).
+template <class Z, class LZ, class Store, class Sink, class GFn>
+__host__ __device__ inline Z lane_adjoint(const LZ* __restrict__ L,
                                           const IfaceZ<Z>* __restrict__ F, int nl, cd om,
                                           int free_surface, Store& st, Sink& sink, GFn&& G_of) {
   const int K = ckpt_stride(nl);
@@ -93,7 +110,7 @@ __host__ __device__ inline Z lane_adjoint(const LayerZ<Z>* __restrict__ L,
   }
 
   // 2. surface value and its cotangent
-  const LayerZ<Z>& water = L[0];
+  const LZ& water = L[0];
   Z ea0 = lift<Z>(1.0);
   if (!water.halfspace) ea0 = phase_t(water.eta, om, water.h);
   const Z e2 = ea0 * ea0;
@@ -156,7 +173,7 @@ __host__ __device__ inline Z lane_adjoint(const LayerZ<Z>* __restrict__ L,
       step_rev(Sin, ea, eb, F[i], sb, rb, fbar, eab, ebb);
       sink.iface(i, fbar);
       LayerCot<Z> c{zero_z<Z>(), zero_z<Z>(), zero_z<Z>()};
-      const LayerZ<Z>& lb = L[i + 1];
+      const LZ& lb = L[i + 1];
       if (!lb.halfspace) {
         phase_rev(lb.eta, om, lb.h, ea, eab, c.eta, c.h);
         Z hb2 = zero_z<Z>();
