@@ -18,7 +18,13 @@ thread through ``submit_forward`` / ``submit_invert`` and get a
    outputs from pinned staging into per-request host arrays on a pool of
    copy threads, resolves the futures and returns the memory and the stream
    slot, while the scheduler keeps forming and launching batches;
-4. inversions run their LM iterations on their own stream, concurrently.
+4. inversions run their LM iterations on a stream of their own, beside the
+   batches; by default one inversion at a time (``max_lm_concurrency``):
+   an inversion's derivative kernels fill the device by themselves, so a
+   second concurrent one adds little throughput (open loop at 200 req/s with
+   20 % inversions: the queue drains in 7.3 s with one, 6.8 s with four) but
+   delays forward batches behind two streams of long kernels (forward p99
+   13 ms with one, 82 ms with two).
 
 Each request keeps a record (arrival, admission, launch, completion times,
 updates, status) from which ``stats()`` reports p50/p99 latency and
@@ -140,7 +146,7 @@ class DeviceBackend:
 
     def __init__(self, n_streams: int = 3, shape=(64, 8), device=None,
                  staging_bytes: int = 512 << 20, copy_threads: int = 8,
-                 kernel_timing: bool = False):
+                 kernel_timing: bool = False, lm_streams: int = 4):
         import torch
 
         from . import _abi
@@ -149,7 +155,12 @@ class DeviceBackend:
         self.dev = _abi.device(device)
         self.compute = torch.cuda.Stream(device=self.dev)
         self.streams = [torch.cuda.Stream(device=self.dev) for _ in range(n_streams)]
-        self.lm_stream = torch.cuda.Stream(device=self.dev)
+        # one stream per concurrent inversion, so that one inversion's host
+        # synchronisations do not wait for another's kernels
+        self.lm_pool: queue.Queue = queue.Queue()
+        for _ in range(max(1, lm_streams)):
+            self.lm_pool.put(torch.cuda.Stream(device=self.dev))
+        self.lm_streams = max(1, lm_streams)
         self.slots = n_streams
         self.shape = shape
         self.max_batch_bytes = int(staging_bytes)
@@ -289,18 +300,22 @@ class DeviceBackend:
     def invert(self, spec: InvertSpec):
         from .taup_inversion import invert_taup
 
-        with self.torch.cuda.stream(self.lm_stream):
-            res = invert_taup(spec.true_model, list(spec.p), nfreq=spec.nfreq,
-                              perturbation=spec.perturbation, max_iter=spec.iterations,
-                              seed=spec.seed, tol=0.0, hessian_mode=spec.hessian_mode,
-                              max_trials=spec.max_trials, use_graphs=spec.use_graphs)
-            self.lm_stream.synchronize()
+        stream = self.lm_pool.get()
+        try:
+            with self.torch.cuda.stream(stream):
+                res = invert_taup(spec.true_model, list(spec.p), nfreq=spec.nfreq,
+                                  perturbation=spec.perturbation, max_iter=spec.iterations,
+                                  seed=spec.seed, tol=0.0, hessian_mode=spec.hessian_mode,
+                                  max_trials=spec.max_trials, use_graphs=spec.use_graphs)
+                stream.synchronize()
+        finally:
+            self.lm_pool.put(stream)
         return res
 
 
 class KennettServer:
     def __init__(self, backend=None, hbm_budget: int | None = None, max_batch_updates: float = 1e9,
-                 max_lm_concurrency: int = 2, idle_sleep: float = 2e-4,
+                 max_lm_concurrency: int = 1, idle_sleep: float = 2e-4,
                  max_inflight: int | None = None, max_batch_bytes: int | None = None):
         self.backend = backend if backend is not None else DeviceBackend()
         # batches in flight at once (one per stream): further arrivals wait for
