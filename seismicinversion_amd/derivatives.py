@@ -9,12 +9,15 @@ sum over slowness themselves).  Directions are (ndir, nl, 6).
 
 from __future__ import annotations
 
+import threading
+from contextlib import contextmanager
+
 import torch
 
 from . import _abi
 from .engine import check_precision
 
-__all__ = ["jvp", "vjp", "misfit_grad", "misfit_hvp", "vjp_hvp", "WORKSPACE_LIMIT"]
+__all__ = ["jvp", "vjp", "misfit_grad", "misfit_hvp", "vjp_hvp", "paced", "WORKSPACE_LIMIT"]
 
 # Directions per launch are capped so that one launch's workspace stays below this.
 WORKSPACE_LIMIT = 4 << 30
@@ -35,9 +38,37 @@ def _workspace(nl, n_p, nw, ndir, dual, dev):
     return torch.empty(max(nbytes, 1), dtype=torch.uint8, device=dev)
 
 
+_pacing = threading.local()
+
+
+@contextmanager
+def paced(max_lane_updates: float, gate=None):
+    """On this thread, within the block: split direction batches so that one
+    launch covers at most ``max_lane_updates`` (direction, p, omega,
+    interface) steps, and call ``gate()`` before each launch.  A server uses
+    it to keep an inversion's derivative kernels short and to let the forward
+    batches queued before each one run first (``gate`` makes the current
+    stream wait for them)."""
+    prev = getattr(_pacing, "cfg", None)
+    _pacing.cfg = (float(max_lane_updates), gate)
+    try:
+        yield
+    finally:
+        _pacing.cfg = prev
+
+
+def _gate():
+    cfg = getattr(_pacing, "cfg", None)
+    if cfg is not None and cfg[1] is not None:
+        cfg[1]()
+
+
 def _dir_chunks(nl, n_p, nw, ndir):
     one = int(_abi.lib().kh_deriv_workspace_bytes(nl, n_p, nw, 1, 1)) or 1
     step = max(1, min(ndir, WORKSPACE_LIMIT // one))
+    cfg = getattr(_pacing, "cfg", None)
+    if cfg is not None:
+        step = max(1, min(step, int(cfg[0] // max(1, n_p * nw * (nl - 1)))))
     return [(s, min(ndir, s + step)) for s in range(0, ndir, step)]
 
 
@@ -114,6 +145,7 @@ def misfit_hvp(layers, dlayers, p, omega, R_obs, free_surface=False, second_only
         nd = e - s
         ws = _workspace(nl, n_p, nw, nd, True, dev)
         chi_c = torch.empty((nd, n_p, 2), dtype=torch.float64, device=dev)
+        _gate()
         rc = _abi.lib().kh_misfit_hvp(
             _abi.ptr(layers), _abi.ptr(dlayers[s:e]), nd, nl, _abi.ptr(p), n_p, _abi.ptr(omega),
             nw, int(bool(free_surface)), _abi.ptr(R_obs), int(bool(second_only)),
@@ -143,6 +175,7 @@ def vjp_hvp(layers, dlayers, p, omega, G, dG=None, free_surface=False):
     for s, e in _dir_chunks(nl, n_p, nw, ndir):
         nd = e - s
         ws = _workspace(nl, n_p, nw, nd, True, dev)
+        _gate()
         rc = _abi.lib().kh_vjp_hvp(
             _abi.ptr(layers), _abi.ptr(dlayers[s:e]), nd, nl, _abi.ptr(p), n_p, _abi.ptr(omega),
             nw, int(bool(free_surface)), _abi.ptr(G), _abi.ptr(dG[s:e]) if dG is not None else None,

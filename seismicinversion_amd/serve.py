@@ -24,7 +24,11 @@ thread through ``submit_forward`` / ``submit_invert`` and get a
    second concurrent one adds little throughput (open loop at 200 req/s with
    20 % inversions: the queue drains in 7.3 s with one, 6.8 s with four) but
    delays forward batches behind two streams of long kernels (forward p99
-   13 ms with one, 82 ms with two).
+   13 ms with one, 82 ms with two).  An inversion's derivative launches are
+   split to at most ``lm_launch_updates`` (direction, p, omega, interface)
+   steps each, and each waits for the forward batches already queued, so a
+   forward batch never shares the device with more than one such short
+   launch.
 
 Each request keeps a record (arrival, admission, launch, completion times,
 updates, status) from which ``stats()`` reports p50/p99 latency and
@@ -146,7 +150,8 @@ class DeviceBackend:
 
     def __init__(self, n_streams: int = 3, shape=(64, 8), device=None,
                  staging_bytes: int = 512 << 20, copy_threads: int = 8,
-                 kernel_timing: bool = False, lm_streams: int = 4):
+                 kernel_timing: bool = False, lm_streams: int = 1,
+                 lm_launch_updates: float = 2e7):
         import torch
 
         from . import _abi
@@ -155,12 +160,18 @@ class DeviceBackend:
         self.dev = _abi.device(device)
         self.compute = torch.cuda.Stream(device=self.dev)
         self.streams = [torch.cuda.Stream(device=self.dev) for _ in range(n_streams)]
-        # one stream per concurrent inversion, so that one inversion's host
-        # synchronisations do not wait for another's kernels
+        # one stream per concurrent inversion (match the server's
+        # max_lm_concurrency), so that one inversion's host synchronisations do
+        # not wait for another's kernels; each stream keeps its own cached
+        # workspace blocks in torch's allocator
         self.lm_pool: queue.Queue = queue.Queue()
         for _ in range(max(1, lm_streams)):
             self.lm_pool.put(torch.cuda.Stream(device=self.dev))
         self.lm_streams = max(1, lm_streams)
+        # the last batch launched on the compute stream (an inversion's
+        # derivative launches wait for it: forward batches go first)
+        self.last_batch_done = None
+        self.lm_launch_updates = lm_launch_updates
         self.slots = n_streams
         self.shape = shape
         self.max_batch_bytes = int(staging_bytes)
@@ -222,6 +233,7 @@ class DeviceBackend:
                     t_ev[1].record(self.compute)
                 kernel_done = torch.cuda.Event()
                 kernel_done.record(self.compute)
+                self.last_batch_done = kernel_done
             cp.wait_event(kernel_done)
             with torch.cuda.stream(cp):
                 total = int(plan.r_off[-1])
@@ -300,9 +312,18 @@ class DeviceBackend:
     def invert(self, spec: InvertSpec):
         from .taup_inversion import invert_taup
 
+        from . import derivatives
+
         stream = self.lm_pool.get()
+
+        def forward_first():
+            ev = self.last_batch_done
+            if ev is not None and not self.torch.cuda.is_current_stream_capturing():
+                stream.wait_event(ev)
+
         try:
-            with self.torch.cuda.stream(stream):
+            with self.torch.cuda.stream(stream), \
+                    derivatives.paced(self.lm_launch_updates, forward_first):
                 res = invert_taup(spec.true_model, list(spec.p), nfreq=spec.nfreq,
                                   perturbation=spec.perturbation, max_iter=spec.iterations,
                                   seed=spec.seed, tol=0.0, hessian_mode=spec.hessian_mode,
@@ -317,7 +338,8 @@ class KennettServer:
     def __init__(self, backend=None, hbm_budget: int | None = None, max_batch_updates: float = 1e9,
                  max_lm_concurrency: int = 1, idle_sleep: float = 2e-4,
                  max_inflight: int | None = None, max_batch_bytes: int | None = None):
-        self.backend = backend if backend is not None else DeviceBackend()
+        self.backend = (backend if backend is not None
+                        else DeviceBackend(lm_streams=max(1, max_lm_concurrency)))
         # batches in flight at once (one per stream): further arrivals wait for
         # a free slot and then launch together
         self.max_inflight = max_inflight or getattr(self.backend, "slots", 2)
@@ -430,6 +452,12 @@ class KennettServer:
 
     def __exit__(self, *exc):
         self.stop()
+
+    def reset_stats(self) -> None:
+        """Forget the records of completed requests (e.g. after a warm-up)."""
+        with self.lock:
+            self.records = []
+            self.batches = 0
 
     def stats(self) -> dict:
         with self.lock:
