@@ -383,12 +383,21 @@ def main():
         stream = sorted(stream + probe, key=lambda it: it.t)
         reqs = stream_requests(stream)
         with KennettServer(backend=DeviceBackend(n_streams=3)) as osrv:
-            # untimed warm-up on the same server: the largest inversions and
-            # forward requests of the stream populate the allocator's cached
-            # blocks of its streams, so the timed run sees steady state
-            big_inv = sorted((s for k, s in reqs if k == "invert"), key=lambda s: -s.updates())[:2]
-            big_fwd = sorted((s for k, s in reqs if k == "forward"), key=lambda s: -s.updates())[:16]
-            warm = [osrv.submit_invert(s) for s in big_inv] + [osrv.submit_forward(s) for s in big_fwd]
+            # untimed warm-up on the same server: the stream's first second,
+            # replayed at its own pace, populates the allocator's cached
+            # blocks of the server's streams, so the timed run sees steady
+            # state (a cold server's first second runs forward p99 several
+            # times higher)
+            t0 = time.perf_counter()
+            warm = []
+            for it, (kind, spec) in zip(stream, reqs):
+                if it.t > 1.0:
+                    break
+                d = it.t - (time.perf_counter() - t0)
+                if d > 0:
+                    time.sleep(d)
+                warm.append(osrv.submit_forward(spec) if kind == "forward"
+                            else osrv.submit_invert(spec))
             for f in warm:
                 f.result()
             osrv.reset_stats()
