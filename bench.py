@@ -200,7 +200,58 @@ def cpu_baselines(specs, workers):
     gather_cpu = {"s": dtg, "cores": workers,
                   "what": "whole gather workload: reflectivity + wavenumber sum per 32-slowness "
                           "chunk in the workers, chunks added in order, FFT in the parent"}
-    return cpu, gather_cpu, traces
+    # the reference's CPU case: 5 layers, 64 slownesses in (0, 0.6], 1023
+    # frequencies, one process (repeated for at least a second)
+    m1, p1, w1 = c1_inputs()
+    K.reflectivity_batch(m1, p1, w1, False)
+    n1, t0 = 0, time.perf_counter()
+    while True:
+        R1 = K.reflectivity_batch(m1, p1, w1, False)
+        n1 += 1
+        if time.perf_counter() - t0 > 1.0:
+            break
+    c1_cpu = {"ms": (time.perf_counter() - t0) / n1 * 1e3, "cores": 1, "R": R1}
+    return cpu, gather_cpu, traces, c1_cpu
+
+
+def c1_inputs():
+    from seismicinversion_amd.workloads import omega_grid, seeded_model, uniform_slownesses
+
+    return seeded_model(5, 42), uniform_slownesses(64, 0.6), omega_grid(1023)
+
+
+def c1_figure(torch, dev, c1_cpu):
+    """The 5-layer, 64 p x 1023 omega case on the device: the
+    kennett_reflectivity_batch drop-in (host arrays in and out) and the
+    device-resident launch, against the NumPy oracle's time on one core."""
+    from seismicinversion_amd import engine
+    from seismicinversion_amd.kennett_reflectivity_gpu import kennett_reflectivity_batch
+
+    m1, p1, w1 = c1_inputs()
+    upd = len(p1) * len(w1) * (m1.n_layers - 1)
+    drop = lambda: kennett_reflectivity_batch(m1, p1, w1, False)  # noqa: E731
+    R = drop()
+    for _ in range(20):
+        drop()
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < 0.5:
+        R = drop()
+        n += 1
+    t_drop = (time.perf_counter() - t0) / n
+    L = engine.layers_tensor(m1, dev)
+    P = torch.from_numpy(p1).to(dev)
+    Rd = torch.empty((len(p1), len(w1)), dtype=torch.complex128, device=dev)
+    dev_call = lambda: engine.forward_grid(L, P, 1.0, 2 * np.pi / 64.0, 0.0, len(w1), False, Rd)  # noqa: E731
+    dev_call()
+    ms = event_ms(dev_call, 20, torch)
+    ref = c1_cpu["R"]
+    diff = float(np.max(np.abs(R - ref).max(axis=1) / np.abs(ref).max(axis=1)))
+    return {"workload": "5 layers (seed 42), 64 p in (0, 0.6], 1023 omega, free surface off",
+            "updates": upd, "cpu_oracle_one_core_ms": c1_cpu["ms"],
+            "dropin_host_arrays_ms": t_drop * 1e3, "device_resident_ms": ms,
+            "dropin_over_cpu": c1_cpu["ms"] / (t_drop * 1e3),
+            "max_row_rel_diff_vs_oracle": diff}
 
 
 def kernel_figures(torch, dev):
@@ -325,7 +376,7 @@ def main():
     items = serving_stream(0.0, args.requests, seed=42, lm_fraction=0.0)
     specs = [s for _, s in stream_requests(items)]
     updates = sum(s.updates() for s in specs)
-    cpu, gather_cpu, cpu_traces = cpu_baselines(specs, args.cpu_workers)
+    cpu, gather_cpu, cpu_traces, c1_cpu = cpu_baselines(specs, args.cpu_workers)
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
@@ -422,6 +473,7 @@ def main():
                      "latency_invert_ms": st["latency_invert"]}
 
     kernels = kernel_figures(torch, dev)
+    kernels["c1_plain_5_layers"] = c1_figure(torch, dev, c1_cpu)
 
     # ---- roofline of the persistent multi-model kernel over the timed served steps
     kern_ms = float(np.mean([m for m, _, _ in log]))
