@@ -24,6 +24,10 @@
 namespace kh {
 
 constexpr int AD_WAVE = 64;
+// Waves per adjoint workgroup (frequency tiles sharing one staged table):
+// two when a (direction, slowness) has at least two tiles.
+constexpr int AD_WAVES_MAX = 2;
+constexpr int AD_BLOCK_MAX = AD_WAVE * AD_WAVES_MAX;
 
 __device__ __forceinline__ double wave_sum(double x) {
 #pragma unroll
@@ -72,21 +76,24 @@ template <class Z>
 struct WaveSink {
   IfaceZ<Z>* cif;
   LayerCot<Z>* cly;
-  double* red;  // LDS, red_rows<Z>(split) x RED_STRIDE
+  double* red;  // LDS, red_rows<Z>(split) x RED_STRIDE, this wave's own
   bool split;
+  bool store;  // false for a padding wave (no tile): it only keeps the barriers
 
-  // one pass over scalars [F0, F1) of the record
+  // one pass over scalars [F0, F1) of the record (every wave of the
+  // workgroup reduces its own record at the same level: the barriers are
+  // shared)
   template <int F0, int F1, int NF>
   __device__ void reduce_pass(const Z (&v)[NF], double* __restrict__ dst) {
     constexpr int N = Flat<Z>::N;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x % AD_WAVE;
 #pragma unroll
     for (int f = F0; f < F1; ++f) {
 #pragma unroll
       for (int k = 0; k < N; ++k) red[((f - F0) * N + k) * RED_STRIDE + lane] = Flat<Z>::get(v[f], k);
     }
     __syncthreads();
-    if (lane < (F1 - F0) * N) {
+    if (store && lane < (F1 - F0) * N) {
       const double* row = red + lane * RED_STRIDE;
       double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll 4
@@ -197,32 +204,43 @@ __device__ __forceinline__ void stage_ph(const double* __restrict__ layers,
 }
 
 // grid.x = ndir * np * ntiles, blockDim = 64 (one wave per block).
+// grid.x = ndir * np * ngroups, blockDim = 64 * wpb: wave k of group g
+// sweeps frequency tile g * wpb + k of (dir, p), all waves sharing one staged
+// table.  ntp = ngroups * wpb tiles have workspace and cotangent slots; a
+// wave whose tile is >= ntiles (padding) runs with every lane invalid and
+// stores nothing.
 template <class Z, bool MISFIT>
-__global__ void __launch_bounds__(AD_WAVE) adjoint_kernel(
+__global__ void __launch_bounds__(AD_BLOCK_MAX) adjoint_kernel(
     const double* __restrict__ layers, const double* __restrict__ dlayers, int nl,
     const double* __restrict__ p, int np, const cd* __restrict__ omega, int nw, int free_surface,
     const cd* __restrict__ Gin, const cd* __restrict__ dGin, int second_only, Z* __restrict__ ws,
     long nlanes,
     IfaceZ<Z>* __restrict__ cot_if, LayerCot<Z>* __restrict__ cot_ly, Z* __restrict__ chi_part,
-    cd* __restrict__ R, cd* __restrict__ dR, int ntiles, int red_split) {
+    cd* __restrict__ R, cd* __restrict__ dR, int ntiles, int ntp, int red_split) {
   extern __shared__ double smem_a[];
   LayerPh<Z>* s_layer = reinterpret_cast<LayerPh<Z>*>(smem_a);
   IfaceZ<Z>* s_if = reinterpret_cast<IfaceZ<Z>*>(s_layer + nl);
-  double* s_red = reinterpret_cast<double*>(s_if + (nl - 1));
+  const int wpb = (int)(blockDim.x / AD_WAVE);
+  const int wave = (int)(threadIdx.x / AD_WAVE), lane = (int)(threadIdx.x % AD_WAVE);
+  double* s_red = reinterpret_cast<double*>(s_if + (nl - 1)) +
+                  (size_t)wave * red_rows<Z>(red_split != 0) * RED_STRIDE;
+  const int ngroups = ntp / wpb;
   int b = blockIdx.x;
-  const int tile = b % ntiles;
-  b /= ntiles;
+  const int grp = b % ngroups;
+  b /= ngroups;
   const int ip = b % np;
   const int dir = b / np;
   stage_ph<Z>(layers, dlayers ? dlayers + (size_t)dir * nl * NCOL : nullptr, nl, p[ip], s_layer,
               s_if);
-  const int w = tile * AD_WAVE + threadIdx.x;
-  const bool valid = w < nw;
+  const int tile = grp * wpb + wave;
+  const bool wave_ok = tile < ntiles;
+  const int w = tile * AD_WAVE + lane;
+  const bool valid = wave_ok && w < nw;
   const int wc = valid ? w : nw - 1;
   const cd om = omega[wc];
-  const size_t slot = (size_t)blockIdx.x;  // (dir, p, tile)
-  GlobalStore<Z> st{ws + (long)blockIdx.x * AD_WAVE + threadIdx.x, nlanes};
-  WaveSink<Z> sink{cot_if + slot * (nl - 1), cot_ly + slot * nl, s_red, red_split != 0};
+  const size_t slot = ((size_t)dir * np + ip) * ntp + tile;  // (dir, p, tile)
+  GlobalStore<Z> st{ws + (long)slot * AD_WAVE + lane, nlanes};
+  WaveSink<Z> sink{cot_if + slot * (nl - 1), cot_ly + slot * nl, s_red, red_split != 0, wave_ok};
   const cd gin = Gin[(size_t)ip * nw + wc];
   const cd dgin = dGin ? dGin[((size_t)dir * np + ip) * nw + wc] : cd{0.0, 0.0};
   Z chi = lift<Z>(0.0);
@@ -239,13 +257,13 @@ __global__ void __launch_bounds__(AD_WAVE) adjoint_kernel(
     if constexpr (!std::is_same<Z, cd>::value) g.d = dgin;
     return g;
   });
-  if (threadIdx.x == 0) {
+  if (wave_ok && lane == 0) {
     const Z zero = lift<Z>(0.0);
     sink.cly[nl - 1] = LayerCot<Z>{zero, zero, zero};
   }
   if (MISFIT) {
     const Z c = wave_sum(chi);
-    if (threadIdx.x == 0) chi_part[slot] = c;
+    if (wave_ok && lane == 0) chi_part[slot] = c;
   }
   if (valid) {
     const size_t o = (size_t)ip * nw + w;
@@ -274,14 +292,15 @@ __global__ void __launch_bounds__(CH_BLOCK) chain_kernel(const double* __restric
                              int nl, const double* __restrict__ p, int np, int ndir,
                              const IfaceZ<Z>* __restrict__ cot_if,
                              const LayerCot<Z>* __restrict__ cot_ly,
-                             const Z* __restrict__ chi_part, int ntiles, double* __restrict__ grad,
-                             double* __restrict__ hvp, double* __restrict__ chi) {
+                             const Z* __restrict__ chi_part, int ntiles, int ntp,
+                             double* __restrict__ grad, double* __restrict__ hvp,
+                             double* __restrict__ chi) {
   extern __shared__ double smem_c[];
   IfaceZ<Z>* s_if = reinterpret_cast<IfaceZ<Z>*>(smem_c);
   LayerCot<Z>* s_ly = reinterpret_cast<LayerCot<Z>*>(s_if + (nl - 1));
   const int ip = (int)(blockIdx.x % (unsigned)np);
   const int dir = (int)(blockIdx.x / (unsigned)np);
-  const size_t base = ((size_t)dir * np + ip) * ntiles;
+  const size_t base = ((size_t)dir * np + ip) * ntp;  // tiles >= ntiles are padding
   constexpr int NI = (int)(sizeof(IfaceZ<Z>) / sizeof(double));
   constexpr int NLY = (int)(sizeof(LayerCot<Z>) / sizeof(double));
   const int nif = (nl - 1) * NI;
@@ -326,14 +345,18 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 struct DerivLayout {
   size_t ws, cif, cly, chi, total;
   long nlanes;
-  int ntiles;
+  int ntiles;  // frequency tiles of AD_WAVE lanes
+  int wpb;     // waves (tiles) per workgroup
+  int ntp;     // tiles with storage: ntiles rounded up to a multiple of wpb
 };
 
 template <class Z>
 static DerivLayout deriv_layout(int nl, int np, int nw, int ndir) {
   DerivLayout d;
   d.ntiles = (nw + AD_WAVE - 1) / AD_WAVE;
-  const long nblocks = (long)ndir * np * d.ntiles;
+  d.wpb = d.ntiles >= 2 ? AD_WAVES_MAX : 1;
+  d.ntp = (d.ntiles + d.wpb - 1) / d.wpb * d.wpb;
+  const long nblocks = (long)ndir * np * d.ntp;  // (dir, p, tile) slots
   d.nlanes = nblocks * AD_WAVE;
   d.ws = 0;
   d.cif = align256(d.ws + (size_t)d.nlanes * lane_slots(nl) * sizeof(Z));
@@ -363,28 +386,30 @@ static int run_adjoint(const char* what, const double* layers, const double* dla
   KH_REQUIRE(workspace && ws_bytes >= d.total, KH_ERR_INVALID_ARGUMENT,
              "%s: workspace of %zu bytes needed, %zu given", what, d.total, ws_bytes);
   char* wsb = reinterpret_cast<char*>(workspace);
-  const long nblocks = (long)ndir * np * d.ntiles;
+  const long nblocks = (long)ndir * np * (d.ntp / d.wpb);
   KH_REQUIRE(nblocks <= 0x7fffffffL, KH_ERR_UNSUPPORTED, "%s: grid too large", what);
-  // split the reductions only when the whole-record buffer would keep four
-  // one-wave workgroups (the register limit of the dual kernel) off a CU's
+  // split the reductions only when whole-record buffers would keep four
+  // waves (the register limit of the dual kernel: one per SIMD) off a CU's
   // 160 KB of LDS
   const size_t tables = (size_t)nl * sizeof(LayerPh<Z>) + (size_t)(nl - 1) * sizeof(IfaceZ<Z>);
   const size_t red_full = (size_t)red_rows<Z>(false) * RED_STRIDE * sizeof(double);
-  const bool split = 4 * (tables + red_full) > (size_t)160 * 1024;
-  const size_t lds = tables + (size_t)red_rows<Z>(split) * RED_STRIDE * sizeof(double);
+  const int blocks_per_cu = 4 / d.wpb;
+  const bool split = (size_t)blocks_per_cu * (tables + d.wpb * red_full) > (size_t)160 * 1024;
+  const size_t lds = tables + (size_t)d.wpb * red_rows<Z>(split) * RED_STRIDE * sizeof(double);
   auto* cif = reinterpret_cast<IfaceZ<Z>*>(wsb + d.cif);
   auto* cly = reinterpret_cast<LayerCot<Z>*>(wsb + d.cly);
   auto* chip = reinterpret_cast<Z*>(wsb + d.chi);
-  hipLaunchKernelGGL((adjoint_kernel<Z, MISFIT>), dim3((unsigned)nblocks), dim3(AD_WAVE), lds, st,
+  hipLaunchKernelGGL((adjoint_kernel<Z, MISFIT>), dim3((unsigned)nblocks), dim3(AD_WAVE * d.wpb), lds, st,
                      layers, dlayers, nl, p, np, reinterpret_cast<const cd*>(omega), nw, fs,
                      reinterpret_cast<const cd*>(Gin), reinterpret_cast<const cd*>(dGin), second_only,
                      reinterpret_cast<Z*>(wsb + d.ws), d.nlanes, cif, cly, chip,
-                     reinterpret_cast<cd*>(R), reinterpret_cast<cd*>(dR), d.ntiles, split ? 1 : 0);
+                     reinterpret_cast<cd*>(R), reinterpret_cast<cd*>(dR), d.ntiles, d.ntp,
+                     split ? 1 : 0);
   KH_TRY(check_launch(what));
   const size_t lds_c = (size_t)(nl - 1) * sizeof(IfaceZ<Z>) + (size_t)nl * sizeof(LayerCot<Z>);
   hipLaunchKernelGGL((chain_kernel<Z>), dim3((unsigned)((long)ndir * np)), dim3(CH_BLOCK), lds_c, st,
                      layers, dlayers, nl, p, np, ndir, cif, cly, MISFIT ? chip : nullptr,
-                     d.ntiles, grad, hvp, chi);
+                     d.ntiles, d.ntp, grad, hvp, chi);
   return check_launch(what);
 }
 

@@ -152,7 +152,7 @@ class DeviceBackend:
     def __init__(self, n_streams: int = 3, shape=(64, 8), device=None,
                  staging_bytes: int = 512 << 20, copy_threads: int = 8,
                  kernel_timing: bool = False, lm_streams: int = 1,
-                 lm_launch_updates: float = 2e7):
+                 lm_launch_updates: float = 5e7):
         import torch
 
         from . import _abi
