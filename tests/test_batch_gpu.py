@@ -63,13 +63,32 @@ def test_full_uneven_batch_and_autotuner():
     assert all(torch.isfinite(torch.view_as_real(R)).all() for R in outs)
 
 
-def test_nonfinite_counts_per_model():
+def test_multi_mixed_grids_bitwise():
+    """Models with their own frequency counts (one partial tile up to eight
+    full ones per slowness at 64 x 8) and start frequencies in one launch:
+    each equals its own plain launch bit for bit."""
+    specs = specs_c3(9)
+    for s, nw, k0 in zip(specs, (100, 700, 4096, 512, 513, 1, 2048, 3000, 64),
+                         (1.0, 3.0, 1.0, 7.0, 1.0, 2.0, 1.0, 5.0, 1.0)):
+        s.nw, s.k0 = nw, k0
+    outs = batch.reflectivity_multi(specs, block=64, omega_per_item=8)
+    for s, R in zip(specs, outs):
+        L = engine.layers_tensor(s.model)
+        Rp = engine.forward_grid(L, torch.from_numpy(s.p), s.k0, s.dw, s.gamma, s.nw,
+                                 s.free_surface, block=64, omega_per_item=8)
+        assert R.shape == (len(s.p), s.nw)
+        assert torch.equal(R, Rp), f"nw {s.nw}: persistent launch differs from the plain launch"
+
+
+@pytest.mark.parametrize("block,wpt", [(256, 2), (64, 2)])
+def test_nonfinite_counts_per_model(block, wpt):
     """A NaN slowness makes its whole row NaN; the launch counts the non-finite
-    outputs of each model and the others stay at zero."""
+    outputs of each model (over all frequency tiles of the row: three at
+    64 x 2) and the others stay at zero."""
     specs = specs_c3(6, nw=300)
     specs[2].p = specs[2].p.copy()
     specs[2].p[1] = np.nan
-    plan = batch.plan_multi(specs, 256, 2)
+    plan = batch.plan_multi(specs, block, wpt)
     batch.launch(plan)
     counts = batch.nonfinite_counts(plan, plan.nonfinite.cpu().numpy())
     assert counts == [300 if i == 2 else 0 for i in range(6)]

@@ -113,6 +113,36 @@ def test_jvp_and_hvp_vs_torch_forward_over_reverse(name, model, p, fs):
         assert rel(grad.cpu().numpy()[0], gt) < TOL
 
 
+def test_hvp_odd_tile_count_and_several_slownesses():
+    """150 frequencies are three 64-lane tiles: the adjoint runs two tiles per
+    workgroup, so the last workgroup carries a padding wave with no tile of
+    its own, which must leave every stored sum untouched.  Two slownesses,
+    so (direction, slowness) workgroups of both are checked."""
+    name, model, p, fs = CASES[0]
+    tab = pack_layers(model)
+    nl = len(tab)
+    w = np.arange(1, 151) * 2 * np.pi / 64.0
+    ps = [p, 0.5 * p]
+    rng = np.random.default_rng(4)
+    dm = rng.standard_normal((2, nl, 6)) * np.array([0.1, 0.1, 0.1, 0.1, 5.0, 5.0])
+    dm[:, ~np.isfinite(tab[:, 3]), 3] = 0.0
+    layers = engine.layers_tensor(model)
+    pt, wt = torch.tensor(ps, dtype=torch.float64), torch.from_numpy(w.astype(complex))
+    t0 = torch_cols(tab)
+    R_obs = torch.stack([(ref_R(t0, q, torch.tensor(w), fs) * (1 - 0.04j)).detach() for q in ps])
+    chi, grad, hvp, _ = D.misfit_hvp(layers, torch.from_numpy(dm), pt, wt, R_obs, free_surface=fs)
+    for j, q in enumerate(ps):
+        def chi_f(tt, q=q, j=j):
+            r = ref_R(tt, q, torch.tensor(w), fs) - R_obs[j]
+            return (r.real ** 2 + r.imag ** 2).sum()
+
+        assert abs(chi.cpu().numpy()[j] - chi_f(t0).item()) / chi_f(t0).item() < 1e-11
+        for k in range(2):
+            gt, hvt = torch.func.jvp(torch.func.grad(chi_f), (t0,), (torch.tensor(dm[k]),))
+            assert rel(hvp.cpu().numpy()[k, j], mask_halfspace(hvt.numpy(), tab)) < TOL
+            assert rel(grad.cpu().numpy()[j], mask_halfspace(gt.numpy(), tab)) < TOL
+
+
 def test_vjp_hvp_general_cotangent():
     """Directional derivative of the backward (forward-over-reverse) for a
     cotangent with its own tangent: d/dt grad(m + t dm, G + t dG)."""
