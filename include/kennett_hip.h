@@ -105,44 +105,60 @@ int kh_forward_grid(const double* layers, int nl, const double* p, int np,
                     double* R, void* stream);
 
 /* kh_forward_grid with an explicit launch shape (as kh_forward_ex).  The
- * phase recurrence steps by block*dw, so results are bitwise identical for
- * equal `block` (any omega_per_item) and agree to rounding (about 1e-14
- * relative) across block sizes. */
+ * phase recurrence steps by block*dw from each lane's first frequency in a
+ * tile, so results are bitwise identical for equal (block, omega_per_item)
+ * and agree to rounding (about 1e-14 relative) across shapes. */
 int kh_forward_grid_ex(const double* layers, int nl, const double* p, int np,
                        double k0, double dw, double gamma, int nw, int free_surface,
                        double* R, int block, int omega_per_item, void* stream);
 
 /* ---- many models in one persistent launch ---- */
 
-/* One model of a kh_forward_multi batch (80 bytes, all offsets in elements). */
+/* One model of a kh_forward_multi batch (96 bytes, all offsets in elements). */
 typedef struct {
   long long layer_off; /* first row of this model in layers_all ((rows, 6) float64) */
   long long p_off;     /* first slowness of this model in p_all */
   long long r_off;     /* first complex element of this model's (np, nw) block in R_all */
-  long long item_off;  /* first work item of this model (prefix sum of np) */
+  long long item_off;  /* first work item of this model (prefix sum of the items per model) */
   double k0, dw, gamma; /* frequencies omega_j = (k0 + j)*dw + i*gamma, j < nw */
   int nl, np, nw;
-  int ntiles;          /* ceil(nw / (block * omega_per_item)) for the launch shape */
+  int ntiles;          /* kh_multi_tiles(nw, block, omega_per_item, half_tile) */
   int free_surface;
-  int pad;
+  int half_tile;       /* 0: omega_per_item frequencies per lane; nonzero (omega_per_item
+                          >= 2): omega_per_item / 2, for models whose nw fills at most
+                          about half a tile (the lanes would otherwise idle on the rest) */
+  int tiles_per_item;  /* <= 0: one work item per slowness (all ntiles tiles); else
+                          each slowness's tiles in runs of this many, one item each */
+  int reserved[3];     /* zero */
 } kh_model_desc;
 
-/* Work items of one model for a launch shape: one per slowness (the item
- * sweeps all ntiles frequency tiles), i.e. np; 0 if any argument is <= 0. */
-long long kh_multi_items(int np, int nw, int block, int omega_per_item);
+/* Frequency tiles of a model's row: ceil(nw / (block * f)) with f the
+ * frequencies per lane (omega_per_item, halved with half_tile); 0 if any
+ * argument is <= 0. */
+int kh_multi_tiles(int nw, int block, int omega_per_item, int half_tile);
+
+/* Work items of one model: np * ceil(ntiles / tiles_per_item) (np when
+ * tiles_per_item <= 0); 0 if np or ntiles is <= 0. */
+long long kh_multi_items(int np, int ntiles, int tiles_per_item);
+
+/* Workgroups a kh_forward_multi launch with workgroups <= 0 starts on the
+ * current device (CUs x resident workgroups, before the cap at total_items),
+ * written to *workgroups; for sizing tiles_per_item. */
+int kh_multi_workgroups(int block, int omega_per_item, int max_nl, int* workgroups);
 
 /* Reflectivity of nmodels models in one persistent launch: workgroups take
- * (model, slowness) items from a device counter until all total_items are
- * done, staging the model's tables at that slowness once per item.
- * desc (nmodels,) is a DEVICE array sorted by item_off (list the models
- * with the costliest items, ntiles * (nl - 1), first for the best drain
- * order); max_nl = the
+ * (model, slowness, run of tiles) items from a device counter until all
+ * total_items are done, staging the model's tables at that slowness once
+ * per item.  desc (nmodels,) is a DEVICE array sorted by item_off (list the
+ * models with the costliest items, tiles per item * (nl - 1), first for the
+ * best drain order); max_nl = the
  * largest nl; counter = 8 bytes of device memory (reset by the call);
  * nonfinite = NULL or a device int array (nmodels,) that receives, for each
  * desc entry k, the number of NaN/Inf outputs of that model (reset by the call);
  * workgroups <= 0 picks CUs x resident workgroups.  block in {64, 128, 256},
  * omega_per_item in {1, 2, 4, 8}.  Each model's output equals kh_forward_grid_ex
- * of that model alone with the same block and omega_per_item, bit for bit.
+ * of that model alone with the same block and its frequencies per lane
+ * (omega_per_item, or omega_per_item / 2 with half_tile), bit for bit.
  * Replaces a loop of kennett_reflectivity_batch calls, one per model
  * (Kennett_Reflectivity/kennett_reflectivity_gpu.py:88). */
 int kh_forward_multi(const kh_model_desc* desc, int nmodels, long long total_items, int max_nl,

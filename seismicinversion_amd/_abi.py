@@ -55,7 +55,9 @@ EXPORTS: dict[str, list] = {
     "kh_gmm_forward": [_vp, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _vp, _vp],
     "kh_gmm_forward_grid": [_vp, _c_int, _vp, _c_int, _c_double, _c_double, _c_double, _c_int,
                             _c_int, _vp, _vp],
-    "kh_multi_items": [_c_int, _c_int, _c_int, _c_int],
+    "kh_multi_tiles": [_c_int, _c_int, _c_int, _c_int],
+    "kh_multi_items": [_c_int, _c_int, _c_int],
+    "kh_multi_workgroups": [_c_int, _c_int, _c_int, _vp],
     "kh_forward_multi": [_vp, _c_int, ctypes.c_longlong, _c_int, _vp, _vp, _vp, _c_int, _c_int,
                          _c_int, _vp, _vp, _vp],
 }

@@ -2,14 +2,18 @@
 
 ``reflectivity_multi`` takes models of different depths with their own
 slownesses and uniform frequency grids and returns each model's (np, nw)
-complex128 reflectivity.  Models are queued by the cost of one work item
-(interfaces x frequency tiles of one slowness), longest first, so the
-persistent workgroups take the long items early; results are bitwise identical to one ``kh_forward_grid_ex`` launch
-per model with the same launch shape.
+complex128 reflectivity.  A work item is one slowness of one model (all of
+its frequency tiles, or a run of them for the few deep, wide models whose
+whole rows would leave the device idle at the end of the launch).  Models
+are queued by the cost of one item (interfaces x tiles), longest first, so
+the persistent workgroups take the long items early.  Results are bitwise
+identical to one ``kh_forward_grid_ex`` launch per model with the same block
+and frequencies per lane (``omega_per_lane``).
 """
 
 from __future__ import annotations
 
+import ctypes
 from dataclasses import dataclass
 
 import numpy as np
@@ -18,14 +22,16 @@ import torch
 from . import _abi, autotune, engine
 from .layer_model import LayerModel, pack_layers
 
-__all__ = ["ModelSpec", "MultiPlan", "plan_multi", "reflectivity_multi", "DESC_DTYPE"]
+__all__ = ["ModelSpec", "MultiPlan", "plan_multi", "reflectivity_multi", "omega_per_lane",
+           "DESC_DTYPE"]
 
 # numpy mirror of kh_model_desc (include/kennett_hip.h)
 DESC_DTYPE = np.dtype([("layer_off", "<i8"), ("p_off", "<i8"), ("r_off", "<i8"),
                        ("item_off", "<i8"), ("k0", "<f8"), ("dw", "<f8"), ("gamma", "<f8"),
                        ("nl", "<i4"), ("np", "<i4"), ("nw", "<i4"), ("ntiles", "<i4"),
-                       ("free_surface", "<i4"), ("pad", "<i4")])
-assert DESC_DTYPE.itemsize == 80
+                       ("free_surface", "<i4"), ("half_tile", "<i4"), ("tiles_per_item", "<i4"),
+                       ("reserved", "<i4", (3,))])
+assert DESC_DTYPE.itemsize == 96
 
 
 @dataclass
@@ -84,6 +90,50 @@ def _to_device(arr: np.ndarray, dev, non_blocking: bool) -> torch.Tensor:
     return t.to(dev, non_blocking=non_blocking)
 
 
+# Cost of a tile at half the frequencies per lane relative to a full tile:
+# the per-layer phase of a lane's first frequency and the loop overhead are
+# spread over half as many updates (64 x 4 runs about 10 % more VALU
+# instructions per update than 64 x 8).
+HALF_TILE_COST = 0.55
+
+# Largest item, as a fraction of the launch's work per resident workgroup:
+# rows of models whose whole-row items would be costlier are split into runs
+# of tiles.  With whole rows the drain at the end of a launch of a few deep,
+# wide models leaves most of the device idle (a batch of 8 such models of
+# the mixed stream: 25 % over the ideal time in a list-scheduling model of
+# the queue); finer items cost a restaging of the slowness's tables each.
+TAIL_FRACTION = 0.25
+
+_RESIDENT: dict = {}
+
+
+def resident_workgroups(block: int, wpt: int, max_nl: int, dev=None) -> int:
+    """Workgroups a persistent launch of this shape starts (cached)."""
+    dev = _abi.device(dev)
+    key = (dev.index, block, wpt, max_nl)
+    n = _RESIDENT.get(key)
+    if n is None:
+        out = ctypes.c_int(0)
+        with torch.cuda.device(dev):
+            _abi.check(_abi.lib().kh_multi_workgroups(block, wpt, max_nl, ctypes.byref(out)),
+                       "kh_multi_workgroups")
+        n = _RESIDENT[key] = max(int(out.value), 1)
+    return n
+
+
+def omega_per_lane(nw: int, block: int, wpt: int) -> int:
+    """Frequencies per lane a model of nw frequencies gets in a launch of
+    shape (block, wpt): wpt, or wpt // 2 when half-width tiles cost less
+    than full ones (a model that fills at most half a tile, e.g. 256
+    frequencies at 64 x 8, would otherwise leave half the lanes idle).
+    Either way the output is bitwise that of the same block size."""
+    if wpt < 2:
+        return wpt
+    full = -(-nw // (block * wpt))
+    half = -(-nw // (block * (wpt // 2)))
+    return wpt // 2 if half * HALF_TILE_COST < full else wpt
+
+
 def plan_multi(specs: list[ModelSpec], block: int, wpt: int, dev=None,
                non_blocking: bool = False) -> MultiPlan:
     """Pack specs into device buffers and descriptors for one launch shape.
@@ -94,13 +144,20 @@ def plan_multi(specs: list[ModelSpec], block: int, wpt: int, dev=None,
     for t in tabs:
         if t.shape[0] < 2:
             raise ValueError("every model needs at least 2 layers")
-    span = block * wpt
-    # Longest items first: an item sweeps all frequency tiles of one slowness,
-    # so its cost is ntiles * (nl - 1) tile steps; ordering the queue by that
-    # (then by the model's total) leaves the smallest items for the drain at
-    # the end of the launch.
-    item_cost = [-(-s.nw // span) * (t.shape[0] - 1) for s, t in zip(specs, tabs)]
-    total_cost = [len(s.p) * c for s, c in zip(specs, item_cost)]
+    lanes = [omega_per_lane(s.nw, block, wpt) for s in specs]
+    ntiles = [-(-s.nw // (block * f)) for s, f in zip(specs, lanes)]
+    # cost of one tile of a row in tile steps (a half-width tile costs
+    # HALF_TILE_COST of a full one)
+    tile_cost = [(HALF_TILE_COST if f < wpt else 1.0) * (t.shape[0] - 1)
+                 for f, t in zip(lanes, tabs)]
+    max_nl = max(t.shape[0] for t in tabs) if tabs else 2
+    slots = resident_workgroups(block, wpt, max_nl, dev) if specs else 1
+    cap = TAIL_FRACTION * sum(len(s.p) * n * c for s, n, c in zip(specs, ntiles, tile_cost)) / slots
+    tpi = [n if n * c <= cap else max(1, int(cap // c)) for n, c in zip(ntiles, tile_cost)]
+    # Longest items first (then the model's total): the smallest items are
+    # left for the drain at the end of the launch.
+    item_cost = [k * c for k, c in zip(tpi, tile_cost)]
+    total_cost = [len(s.p) * n * c for s, n, c in zip(specs, ntiles, tile_cost)]
     order = sorted(range(len(specs)), key=lambda i: (-item_cost[i], -total_cost[i]))
     desc = np.zeros(len(specs), DESC_DTYPE)
     layer_off = np.cumsum([0] + [t.shape[0] for t in tabs])
@@ -109,18 +166,19 @@ def plan_multi(specs: list[ModelSpec], block: int, wpt: int, dev=None,
     item = 0
     for slot, i in enumerate(order):
         s = specs[i]
-        ntiles = -(-s.nw // span)
+        whole = tpi[i] >= ntiles[i]
         desc[slot] = (layer_off[i], p_off[i], r_off[i], item, float(s.k0), float(s.dw),
-                      float(s.gamma), tabs[i].shape[0], len(s.p), s.nw, ntiles,
-                      int(bool(s.free_surface)), 0)
-        item += len(s.p)  # one work item per slowness
+                      float(s.gamma), tabs[i].shape[0], len(s.p), s.nw, ntiles[i],
+                      int(bool(s.free_surface)), int(lanes[i] < wpt), 0 if whole else tpi[i],
+                      (0, 0, 0))
+        item += len(s.p) * -(-ntiles[i] // tpi[i])
     layers = _to_device(np.concatenate(tabs, axis=0), dev, non_blocking)
     p = _to_device(np.concatenate([np.asarray(s.p, np.float64) for s in specs])
                    if specs else np.zeros(0), dev, non_blocking)
     R = torch.empty(int(r_off[-1]), dtype=torch.complex128, device=dev)
     return MultiPlan(desc=_to_device(desc.view(np.uint8), dev, non_blocking), layers=layers, p=p, R=R,
                      r_off=r_off, shapes=[(len(s.p), s.nw) for s in specs], total_items=item,
-                     max_nl=max(t.shape[0] for t in tabs) if tabs else 2, block=block, wpt=wpt,
+                     max_nl=max_nl, block=block, wpt=wpt,
                      # both are reset by the launch itself (no separate fill kernels)
                      counter=torch.empty(1, dtype=torch.int64, device=dev),
                      nonfinite=torch.empty(max(len(specs), 1), dtype=torch.int32, device=dev),

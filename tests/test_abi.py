@@ -60,3 +60,31 @@ def test_invalid_arguments_are_rejected_before_any_launch(lib):
     rc = lib.kh_forward_ex(vp(16), 3, vp(16), 4, vp(16), 4, 0, vp(16), 96, 1, vp(0))
     assert rc == 1 and b"block" in lib.kh_last_error()
     assert lib.kh_forward(vp(16), 3, vp(16), 0, vp(16), 4, 0, vp(16), vp(0)) == 0
+
+
+def test_multi_tile_and_item_counts(lib):
+    """Host-only helpers of the persistent launch: tiles per row (half-width
+    tiles for short rows) and items per model (runs of tiles)."""
+    from seismicinversion_amd import batch
+
+    lib.kh_multi_items.restype = ctypes.c_longlong
+    assert lib.kh_multi_tiles(1023, 64, 8, 0) == 2
+    assert lib.kh_multi_tiles(256, 64, 8, 1) == 1 and lib.kh_multi_tiles(256, 64, 8, 0) == 1
+    assert lib.kh_multi_tiles(700, 64, 8, 1) == 3
+    assert lib.kh_multi_tiles(130, 64, 1, 1) == 3  # no half width at 1 per lane
+    assert lib.kh_multi_tiles(0, 64, 8, 0) == 0
+    assert lib.kh_multi_items(5, 8, 0) == 5 and lib.kh_multi_items(5, 8, 8) == 5
+    assert lib.kh_multi_items(5, 8, 3) == 15 and lib.kh_multi_items(5, 8, 1) == 40
+    assert lib.kh_multi_items(0, 8, 1) == 0
+    for nw in (1, 100, 256, 257, 512, 513, 700, 1023, 1024, 3000, 4096):
+        for block, wpt in ((64, 8), (128, 4), (256, 2), (64, 1)):
+            f = batch.omega_per_lane(nw, block, wpt)
+            assert f in (wpt, wpt // 2)
+            assert -(-nw // (block * f)) == lib.kh_multi_tiles(nw, block, wpt, int(f < wpt))
+    # kh_multi_workgroups checks its arguments before it asks the device
+    lib.kh_last_error.restype = ctypes.c_char_p
+    out = ctypes.c_int(0)
+    assert lib.kh_multi_workgroups(96, 8, 20, ctypes.byref(out)) == 1
+    assert b"block" in lib.kh_last_error()
+    assert lib.kh_multi_workgroups(64, 3, 20, ctypes.byref(out)) == 1
+    assert lib.kh_multi_workgroups(64, 8, 20, None) == 1

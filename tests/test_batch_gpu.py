@@ -1,6 +1,6 @@
 """Many uneven models in one persistent launch (kh_forward_multi): every
-model's output equals the one-model-per-launch path with the same launch
-shape bit for bit, and the oracle to the parity bar."""
+model's output equals the one-model-per-launch path with the same block and
+frequencies per lane bit for bit, and the oracle to the parity bar."""
 
 from __future__ import annotations
 
@@ -30,7 +30,8 @@ def test_multi_equals_plain_bitwise(block, wpt):
     for s, R in zip(specs, outs):
         L = engine.layers_tensor(s.model)
         Rp = engine.forward_grid(L, torch.from_numpy(s.p), s.k0, s.dw, s.gamma, s.nw,
-                                 s.free_surface, block=block, omega_per_item=wpt)
+                                 s.free_surface, block=block,
+                                 omega_per_item=batch.omega_per_lane(s.nw, block, wpt))
         assert torch.equal(R, Rp), "persistent launch differs from the plain launch"
 
 
@@ -58,24 +59,28 @@ def test_full_uneven_batch_and_autotuner():
     for i in (0, 100, 255):
         s = specs[i]
         Rp = engine.forward_grid(engine.layers_tensor(s.model), torch.from_numpy(s.p), s.k0, s.dw,
-                                 s.gamma, s.nw, False, block=shape[0], omega_per_item=shape[1])
+                                 s.gamma, s.nw, False, block=shape[0],
+                                 omega_per_item=batch.omega_per_lane(s.nw, *shape))
         assert torch.equal(outs[i], Rp)
     assert all(torch.isfinite(torch.view_as_real(R)).all() for R in outs)
 
 
 def test_multi_mixed_grids_bitwise():
     """Models with their own frequency counts (one partial tile up to eight
-    full ones per slowness at 64 x 8) and start frequencies in one launch:
-    each equals its own plain launch bit for bit."""
-    specs = specs_c3(9)
-    for s, nw, k0 in zip(specs, (100, 700, 4096, 512, 513, 1, 2048, 3000, 64),
-                         (1.0, 3.0, 1.0, 7.0, 1.0, 2.0, 1.0, 5.0, 1.0)):
+    full ones per slowness at 64 x 8, half-width tiles of 4 frequencies per
+    lane for the counts that waste less that way) and start frequencies in
+    one launch: each equals its own plain launch bit for bit."""
+    specs = specs_c3(10)
+    for s, nw, k0 in zip(specs, (100, 700, 4096, 512, 513, 1, 2048, 3000, 64, 256),
+                         (1.0, 3.0, 1.0, 7.0, 1.0, 2.0, 1.0, 5.0, 1.0, 1.0)):
         s.nw, s.k0 = nw, k0
+    lanes = [batch.omega_per_lane(s.nw, 64, 8) for s in specs]
+    assert lanes == [4, 4, 8, 8, 4, 4, 8, 8, 4, 4]
     outs = batch.reflectivity_multi(specs, block=64, omega_per_item=8)
-    for s, R in zip(specs, outs):
+    for s, R, f in zip(specs, outs, lanes):
         L = engine.layers_tensor(s.model)
         Rp = engine.forward_grid(L, torch.from_numpy(s.p), s.k0, s.dw, s.gamma, s.nw,
-                                 s.free_surface, block=64, omega_per_item=8)
+                                 s.free_surface, block=64, omega_per_item=f)
         assert R.shape == (len(s.p), s.nw)
         assert torch.equal(R, Rp), f"nw {s.nw}: persistent launch differs from the plain launch"
 
@@ -97,3 +102,30 @@ def test_nonfinite_counts_per_model(block, wpt):
     # a second launch of the same plan resets the counts before counting again
     batch.launch(plan)
     assert batch.nonfinite_counts(plan, plan.nonfinite.cpu().numpy())[2] == 300
+
+
+@pytest.mark.parametrize("tail", [1e-9, None])
+def test_tile_runs_for_a_thin_batch(monkeypatch, tail):
+    """Three deep models with many frequency tiles and few slownesses: whole
+    rows would leave most workgroups idle, so their rows are split into runs
+    of tiles (one tile per item with a tiny tail fraction).  Outputs equal
+    the plain launches bit for bit, and NaN counts add up over the runs."""
+    if tail is not None:
+        monkeypatch.setattr(batch, "TAIL_FRACTION", tail)
+    specs = [batch.ModelSpec(model=seeded_model(nl, 40 + nl), p=np.linspace(0.01, 0.6, npp),
+                             nw=nw, dw=2 * np.pi / 64.0, k0=1.0, free_surface=(nl == 60))
+             for nl, npp, nw in ((90, 300, 4096), (60, 200, 3000), (20, 50, 700))]
+    specs[1].p[7] = np.nan
+    plan = batch.plan_multi(specs, 64, 8)
+    desc = plan.desc.cpu().numpy().view(batch.DESC_DTYPE)
+    tpi = {int(d["nl"]): int(d["tiles_per_item"]) for d in desc}
+    assert tpi[90] > 0 and tpi[60] > 0, tpi  # rows split
+    if tail is not None:
+        assert tpi == {90: 1, 60: 1, 20: 1}
+    batch.launch(plan)
+    assert batch.nonfinite_counts(plan, plan.nonfinite.cpu().numpy()) == [0, 3000, 0]
+    for s, R in zip(specs, batch.split(plan)):
+        Rp = engine.forward_grid(engine.layers_tensor(s.model), torch.from_numpy(s.p), s.k0,
+                                 s.dw, s.gamma, s.nw, s.free_surface, block=64,
+                                 omega_per_item=batch.omega_per_lane(s.nw, 64, 8))
+        assert torch.equal(torch.nan_to_num(R), torch.nan_to_num(Rp)), f"nl={s.n_layers}"

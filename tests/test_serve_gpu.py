@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 import torch
 
-from seismicinversion_amd import engine
+from seismicinversion_amd import batch, engine
 from seismicinversion_amd.batch import ModelSpec
 from seismicinversion_amd.serve import DeviceBackend, InvertSpec, KennettServer
 from seismicinversion_amd.taup_inversion import invert_taup
@@ -59,7 +59,8 @@ def test_results_identical_alone_and_in_the_mix():
     for s, R in zip(specs, outs):
         alone = engine.forward_grid(engine.layers_tensor(s.model), torch.from_numpy(s.p), s.k0,
                                     s.dw, s.gamma, s.nw, s.free_surface, block=64,
-                                    omega_per_item=8).cpu().numpy()
+                                    omega_per_item=batch.omega_per_lane(s.nw, 64, 8)
+                                    ).cpu().numpy()
         if isinstance(R, torch.Tensor):  # device-resident result
             assert R.is_cuda
             R = R.cpu().numpy()
