@@ -95,7 +95,7 @@ __global__ void interface_direct_kernel(int n, int ocean, const double* __restri
 // frequencies tile * blockDim * W + threadIdx.x + k * blockDim, k < W.
 // GRID: omega_j = (k0 + j) * dw + i * gamma (omega unused); otherwise omega[j].
 template <int W, bool GRID>
-__global__ void __launch_bounds__(256) forward_kernel(
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GRID ? 2 : 1))) forward_kernel(
     const double* __restrict__ layers, int nl, const double* __restrict__ p,
     const cd* __restrict__ omega, double k0, double dw, double gamma, int nw, int free_surface,
     cd* __restrict__ R, int ntiles) {

@@ -80,7 +80,7 @@ __device__ __forceinline__ int sweep_item(const kh_model_desc& md, int t0, int t
 }
 
 template <int W>
-__global__ void __launch_bounds__(256) multi_kernel(const kh_model_desc* __restrict__ desc,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) multi_kernel(const kh_model_desc* __restrict__ desc,
                                                     int nmodels, long long total,
                                                     const double* __restrict__ layers_all,
                                                     const double* __restrict__ p_all,

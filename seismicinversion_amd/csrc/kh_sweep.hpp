@@ -86,13 +86,40 @@ __host__ __device__ __forceinline__ void kennett_step(cd& r00, cd& r01, cd& r10,
   kennett_step_e2(r00, r01, r10, r11, ea * ea, eb * eb, ea * eb, f);
 }
 
+// A value that every lane of the wave holds, moved to scalar registers:
+// the step's FMAs then take it as their one scalar operand instead of
+// holding 20 doubles per layer in vector registers (which is what lets the
+// 8-frequency sweep run two waves per SIMD).
+__host__ __device__ __forceinline__ double wave_uniform(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const long long b = __double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+#else
+  return x;
+#endif
+}
+
+__host__ __device__ __forceinline__ cd wave_uniform(cd z) {
+  return cd{wave_uniform(z.r), wave_uniform(z.i)};
+}
+
+__host__ __device__ __forceinline__ Iface wave_uniform(const Iface& f) {
+  return Iface{wave_uniform(f.rd00), wave_uniform(f.rd01), wave_uniform(f.rd11),
+               wave_uniform(f.ru00), wave_uniform(f.ru01), wave_uniform(f.ru11),
+               wave_uniform(f.td00), wave_uniform(f.td01), wave_uniform(f.td10),
+               wave_uniform(f.td11)};
+}
+
 // RRd00 just below the water layer for W frequencies of one lane.  With
 // GRID, om[k] = om[0] + k * shift (real shift) and `steps` holds the
 // per-layer factors for that shift: the phase products of frequency k are
 // those of frequency k-1 times the products of the factors (ea^2 steps by
 // a^2, eb^2 by b^2, ea eb by a b), evaluated once per layer and lane.
-// Otherwise every phase is evaluated.
-template <int W, bool GRID>
+// Otherwise every phase is evaluated.  With UNIFORM the interface tables
+// must be the same for every lane of the wave (one slowness per workgroup).
+template <int W, bool GRID, bool UNIFORM = true>
 __host__ __device__ __forceinline__ void sweep_rrd00(const LayerPt* __restrict__ L,
                                                      const Iface* __restrict__ F,
                                                      const LayerStep* __restrict__ steps, int nl,
@@ -111,7 +138,7 @@ __host__ __device__ __forceinline__ void sweep_rrd00(const LayerPt* __restrict__
   const cd one{1.0, 0.0};
   for (int i = nl - 3; i >= 0; --i) {
     const LayerPt& lb = L[i + 1];
-    const Iface f = F[i];
+    const Iface f = UNIFORM ? wave_uniform(F[i]) : F[i];
     const bool hs = lb.halfspace;  // unit phase factors
     cd aa = one, bb = one, ab = one, sa2 = one, sb2 = one, sab = one;
     if (GRID && !hs) {
@@ -173,25 +200,26 @@ __host__ __device__ __forceinline__ void surface_pp(const LayerPt& water, const 
   }
 }
 
-// Full lane: PP reflectivity at W frequencies.
-template <int W, bool GRID>
+// Full lane: PP reflectivity at W frequencies (UNIFORM as sweep_rrd00).
+template <int W, bool GRID, bool UNIFORM = true>
 __host__ __device__ __forceinline__ void sweep_lanes(const LayerPt* __restrict__ L,
                                                      const Iface* __restrict__ F,
                                                      const LayerStep* __restrict__ steps, int nl,
                                                      const cd (&om)[W], int free_surface,
                                                      cd (&out)[W]) {
-  sweep_rrd00<W, GRID>(L, F, steps, nl, om, out);
+  sweep_rrd00<W, GRID, UNIFORM>(L, F, steps, nl, om, out);
   const LayerStep s0 = GRID ? steps[0] : LayerStep{{1.0, 0.0}, {1.0, 0.0}};
   surface_pp<W, GRID>(L[0], s0, om, out, free_surface);
 }
 
-// Single frequency, generic omega (host tests and simple callers).
+// Single frequency, generic omega (host tests and simple callers; the
+// tables may differ from lane to lane).
 __host__ __device__ __forceinline__ cd sweep_lane(const LayerPt* __restrict__ L,
                                                   const Iface* __restrict__ F, int nl, cd om,
                                                   int free_surface) {
   const cd o[1] = {om};
   cd r[1];
-  sweep_lanes<1, false>(L, F, nullptr, nl, o, free_surface, r);
+  sweep_lanes<1, false, false>(L, F, nullptr, nl, o, free_surface, r);
   return r[0];
 }
 
