@@ -248,24 +248,24 @@ __host__ __device__ inline Z chain_column(const double* layers, const double* dl
                                           int col) {
   using W = typename Up<Z>::W;
   if (col == COL_H) return lc.h;
-  LayerZ<W> Ls[3];
-  for (int q = 0; q < 3; ++q) {
-    const int ll = l - 1 + q;
-    if (ll < 0 || ll >= nl) continue;
+  // Layer ll with the outer seed on (l, col); one neighbour at a time, so
+  // that two dual-of-dual layers and one interface are live at once.
+  auto layer = [&](int ll) {
     W x[NCOL];
     for (int c = 0; c < NCOL; ++c) {
       const double dm = dlayers ? dlayers[(size_t)ll * NCOL + c] : 0.0;
       x[c] = Up<Z>::seed(layers[(size_t)ll * NCOL + c], dm, (ll == l && c == col) ? 1.0 : 0.0);
     }
-    Ls[q] = layer_t<W>(x, ll, p);
-  }
-  Z acc = re_dot(Up<Z>::outer(Ls[1].eta), lc.eta) + re_dot(Up<Z>::outer(Ls[1].neta), lc.neta);
+    return layer_t<W>(x, ll, p);
+  };
+  const LayerZ<W> mid = layer(l);
+  Z acc = re_dot(Up<Z>::outer(mid.eta), lc.eta) + re_dot(Up<Z>::outer(mid.neta), lc.neta);
   if (l >= 1) {
-    const IfaceZ<W> f = interface_t<W>(l - 1, p, Ls[0], Ls[1]);
+    const IfaceZ<W> f = interface_t<W>(l - 1, p, layer(l - 1), mid);
     acc = acc + iface_dot(outer_iface<W, Z>(f), cb_up);
   }
   if (l <= nl - 2) {
-    const IfaceZ<W> f = interface_t<W>(l, p, Ls[1], Ls[2]);
+    const IfaceZ<W> f = interface_t<W>(l, p, mid, layer(l + 1));
     acc = acc + iface_dot(outer_iface<W, Z>(f), cb_dn);
   }
   return acc;
