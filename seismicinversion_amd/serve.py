@@ -89,7 +89,7 @@ class Request:
     arrival: float
     deadline: float | None = None
     on_device: bool = False
-    mem: int = 0
+    mem: int | None = None  # device bytes, set when the request is validated
     record: dict = field(default_factory=dict)
 
 
@@ -523,43 +523,54 @@ class KennettServer:
 
             pack_layers(spec.true_model)
 
-    def _admit(self) -> list[Request]:
-        """Validated requests leave the arrival queue for one of two FIFOs.
+    def _admit(self, fwd_room: float = float("inf")) -> list[Request]:
+        """Arrivals join one of two FIFOs; requests leave them admitted.
         Forward requests are admitted in arrival order as device memory allows
-        (once one waits, later ones wait too).  An inversion is admitted only
-        when an inversion slot is free and its workspace fits, so inversions
-        waiting for a slot hold no memory and never hold up forward requests."""
+        (once one waits, later ones wait too), and only up to `fwd_room`
+        updates per call: the scheduler asks for about one batch at a time,
+        so a burst of arrivals is validated and reserved while the previous
+        batch runs instead of all before the first launch.  An inversion is
+        admitted only when an inversion slot is free and its workspace fits,
+        so inversions waiting for a slot hold no memory and never hold up
+        forward requests.  A request is validated when it reaches the head
+        of its FIFO; an unusable one fails alone."""
         while True:
             try:
                 item = self.incoming.get_nowait()
             except queue.Empty:
                 break
             for req in (item if isinstance(item, list) else (item,)):
-                try:
-                    self._validate(req)
-                    req.mem = self._mem_of(req)
-                    if req.mem > self.budget:
-                        raise MemoryError(f"request needs {req.mem} bytes, "
-                                          f"budget is {self.budget}")
-                except (ValueError, MemoryError) as e:
-                    self._finish(req, error=e)
-                    continue
                 (self.waiting if req.kind == "forward" else self.waiting_inv).append(req)
         admitted = []
         now = time.perf_counter()
         for fifo, is_inv in ((self.waiting, False), (self.waiting_inv, True)):
             keep = []
-            for req in fifo:
+            room = fwd_room
+            for k, req in enumerate(fifo):
+                if keep or (not is_inv and room <= 0):
+                    keep.extend(fifo[k:])  # FIFO: the rest waits behind
+                    break
                 if req.deadline is not None and now > req.deadline:
                     self._finish(req, error=TimeoutError("deadline passed before admission"))
                     continue
-                if not keep and (not is_inv or self.lm_running < self.max_lm) and \
-                        self._reserve(req.mem):
+                if req.mem is None:
+                    try:
+                        self._validate(req)
+                        req.mem = self._mem_of(req)
+                        if req.mem > self.budget:
+                            raise MemoryError(f"request needs {req.mem} bytes, "
+                                              f"budget is {self.budget}")
+                    except (ValueError, MemoryError) as e:
+                        self._finish(req, error=e)
+                        continue
+                if (not is_inv or self.lm_running < self.max_lm) and self._reserve(req.mem):
                     req.record["admit"] = now
                     admitted.append(req)
                     if is_inv:
                         with self.lock:
                             self.lm_running += 1
+                    else:
+                        room -= req.record["updates"]
                 else:
                     keep.append(req)
             fifo[:] = keep
@@ -580,7 +591,8 @@ class KennettServer:
         pending_fwd: list[Request] = []
         while not self._stop.is_set():
             progressed = False
-            for req in self._admit():
+            queued = sum(r.record["updates"] for r in pending_fwd)
+            for req in self._admit(self.max_batch_updates - queued):
                 progressed = True
                 if req.kind == "forward":
                     pending_fwd.append(req)
