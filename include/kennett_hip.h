@@ -129,7 +129,8 @@ typedef struct {
                           about half a tile (the lanes would otherwise idle on the rest) */
   int tiles_per_item;  /* <= 0: one work item per slowness (all ntiles tiles); else
                           each slowness's tiles in runs of this many, one item each */
-  int reserved[3];     /* zero */
+  int reserved;        /* zero */
+  long long table_off; /* first table record of this model (prefix sum of np * nl) */
 } kh_model_desc;
 
 /* Frequency tiles of a model's row: ceil(nw / (block * f)) with f the
@@ -141,20 +142,28 @@ int kh_multi_tiles(int nw, int block, int omega_per_item, int half_tile);
  * tiles_per_item <= 0); 0 if np or ntiles is <= 0. */
 long long kh_multi_items(int np, int ntiles, int tiles_per_item);
 
+/* Bytes of the table workspace kh_forward_multi needs for `records` table
+ * records (the sum of np * nl over the batch's models). */
+size_t kh_multi_table_bytes(long long records);
+
 /* Workgroups a kh_forward_multi launch with workgroups <= 0 starts on the
  * current device (CUs x resident workgroups, before the cap at total_items),
  * written to *workgroups; for sizing tiles_per_item. */
 int kh_multi_workgroups(int block, int omega_per_item, int max_nl, int* workgroups);
 
-/* Reflectivity of nmodels models in one persistent launch: workgroups take
- * (model, slowness, run of tiles) items from a device counter until all
- * total_items are done, staging the model's tables at that slowness once
- * per item.  desc (nmodels,) is a DEVICE array sorted by item_off (list the
- * models with the costliest items, tiles per item * (nl - 1), first for the
- * best drain order); max_nl = the
- * largest nl; counter = 8 bytes of device memory (reset by the call);
- * nonfinite = NULL or a device int array (nmodels,) that receives, for each
- * desc entry k, the number of NaN/Inf outputs of that model (reset by the call);
+/* Reflectivity of nmodels models in one persistent launch.  First every
+ * (model, slowness, layer) record of the layer, its phase step and the
+ * interface below it is computed into `tables` (caller-owned device memory
+ * of at least kh_multi_table_bytes(records) bytes; records = the sum of
+ * np * nl, each model's records starting at its table_off).  Then
+ * workgroups take (model, slowness, run of tiles) items from a device
+ * counter until all total_items are done, reading the tables with scalar
+ * loads.  desc (nmodels,) is a DEVICE array sorted by item_off and by
+ * table_off (list the models with the costliest items, tiles per item *
+ * (nl - 1), first for the best drain order); max_nl = the largest nl;
+ * counter = 8 bytes of device memory (reset by the call); nonfinite = NULL
+ * or a device int array (nmodels,) that receives, for each desc entry k,
+ * the number of NaN/Inf outputs of that model (reset by the call);
  * workgroups <= 0 picks CUs x resident workgroups.  block in {64, 128, 256},
  * omega_per_item in {1, 2, 4, 8}.  Each model's output equals kh_forward_grid_ex
  * of that model alone with the same block and its frequencies per lane
@@ -163,8 +172,8 @@ int kh_multi_workgroups(int block, int omega_per_item, int max_nl, int* workgrou
  * (Kennett_Reflectivity/kennett_reflectivity_gpu.py:88). */
 int kh_forward_multi(const kh_model_desc* desc, int nmodels, long long total_items, int max_nl,
                      const double* layers_all, const double* p_all, double* R_all, int block,
-                     int omega_per_item, int workgroups, void* counter, int* nonfinite,
-                     void* stream);
+                     int omega_per_item, int workgroups, long long records, void* tables,
+                     size_t table_bytes, void* counter, int* nonfinite, void* stream);
 
 /* ---- gather: discrete-wavenumber sum and inverse FFT to traces ---- */
 

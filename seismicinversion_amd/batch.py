@@ -30,8 +30,8 @@ DESC_DTYPE = np.dtype([("layer_off", "<i8"), ("p_off", "<i8"), ("r_off", "<i8"),
                        ("item_off", "<i8"), ("k0", "<f8"), ("dw", "<f8"), ("gamma", "<f8"),
                        ("nl", "<i4"), ("np", "<i4"), ("nw", "<i4"), ("ntiles", "<i4"),
                        ("free_surface", "<i4"), ("half_tile", "<i4"), ("tiles_per_item", "<i4"),
-                       ("reserved", "<i4", (3,))])
-assert DESC_DTYPE.itemsize == 96
+                       ("reserved", "<i4"), ("table_off", "<i8")])
+assert DESC_DTYPE.itemsize == 96 and DESC_DTYPE.fields["table_off"][1] == 88
 
 
 @dataclass
@@ -81,6 +81,8 @@ class MultiPlan:
     counter: torch.Tensor
     nonfinite: torch.Tensor     # int32 (nmodels,) NaN/Inf output counts, queue order
     order: list[int]            # queue slot -> input index
+    records: int                # table records: sum of np * nl
+    tables: torch.Tensor        # uint8 table workspace (kh_multi_table_bytes(records))
 
 
 def _to_device(arr: np.ndarray, dev, non_blocking: bool) -> torch.Tensor:
@@ -101,12 +103,19 @@ HALF_TILE_COST = 0.55
 # of tiles.  With whole rows the drain at the end of a launch of a few deep,
 # wide models leaves most of the device idle (a batch of 8 such models of
 # the mixed stream: 25 % over the ideal time in a list-scheduling model of
-# the queue); finer items cost a restaging of the slowness's tables each.
-# The ten batches of the 400-request stream, launched one by one on MI355X:
-# 56.1 ms in all at 0.25, 54.3 at 0.15, 54.6 at 0.1, 55.1 at 0.03.
-TAIL_FRACTION = 0.15
+# the queue).  The tables are staged once per slowness before the sweep, so
+# a finer split costs no restaging; over the ten batches of the 400-request
+# stream launched one by one on MI355X the total is flat between 0.02 and
+# 0.15 (53.1-53.4 ms) and 1 % higher at 0.25.
+TAIL_FRACTION = 0.1
 
 _RESIDENT: dict = {}
+
+
+def table_bytes(records: int) -> int:
+    """Bytes of the table workspace a launch of `records` (sum of np * nl)
+    table records needs."""
+    return int(_abi.lib().kh_multi_table_bytes(int(records))) if records > 0 else 0
 
 
 def resident_workgroups(block: int, wpt: int, max_nl: int, dev=None) -> int:
@@ -166,14 +175,16 @@ def plan_multi(specs: list[ModelSpec], block: int, wpt: int, dev=None,
     p_off = np.cumsum([0] + [len(s.p) for s in specs])
     r_off = np.cumsum([0] + [len(s.p) * s.nw for s in specs])
     item = 0
+    records = 0
     for slot, i in enumerate(order):
         s = specs[i]
         whole = tpi[i] >= ntiles[i]
         desc[slot] = (layer_off[i], p_off[i], r_off[i], item, float(s.k0), float(s.dw),
                       float(s.gamma), tabs[i].shape[0], len(s.p), s.nw, ntiles[i],
                       int(bool(s.free_surface)), int(lanes[i] < wpt), 0 if whole else tpi[i],
-                      (0, 0, 0))
+                      0, records)
         item += len(s.p) * -(-ntiles[i] // tpi[i])
+        records += len(s.p) * tabs[i].shape[0]
     layers = _to_device(np.concatenate(tabs, axis=0), dev, non_blocking)
     p = _to_device(np.concatenate([np.asarray(s.p, np.float64) for s in specs])
                    if specs else np.zeros(0), dev, non_blocking)
@@ -184,7 +195,9 @@ def plan_multi(specs: list[ModelSpec], block: int, wpt: int, dev=None,
                      # both are reset by the launch itself (no separate fill kernels)
                      counter=torch.empty(1, dtype=torch.int64, device=dev),
                      nonfinite=torch.empty(max(len(specs), 1), dtype=torch.int32, device=dev),
-                     order=order)
+                     order=order, records=records,
+                     tables=torch.empty(max(int(table_bytes(records)), 1), dtype=torch.uint8,
+                                        device=dev))
 
 
 def launch(plan: MultiPlan, workgroups: int = 0) -> torch.Tensor:
@@ -193,6 +206,7 @@ def launch(plan: MultiPlan, workgroups: int = 0) -> torch.Tensor:
     rc = _abi.lib().kh_forward_multi(_abi.ptr(plan.desc), len(plan.shapes), plan.total_items,
                                      plan.max_nl, _abi.ptr(plan.layers), _abi.ptr(plan.p),
                                      _abi.ptr(plan.R), plan.block, plan.wpt, int(workgroups),
+                                     plan.records, _abi.ptr(plan.tables), plan.tables.numel(),
                                      _abi.ptr(plan.counter), _abi.ptr(plan.nonfinite),
                                      _abi.stream_handle(dev))
     _abi.check(rc, "kh_forward_multi")

@@ -1,23 +1,26 @@
 // Many earth models of different depths, slowness counts and frequency
 // grids in one persistent launch.
 //
-// The work is a list of items (model, slowness, run of frequency tiles).
-// Items are numbered model by model (callers list the models with the
-// costliest items first, so the queue drains longest-first); each workgroup
-// repeatedly takes the next item number from a device-wide counter (one
-// atomic per item), stages that model's layers and interfaces at that
-// slowness in LDS once and sweeps the item's frequency tiles with them,
-// until the counter passes the item count.  By default an item holds all of
-// a slowness's tiles (one item per tile restaged the same tables ntiles
-// times: the served mixed-size stream ran 5.6 % slower that way); a caller
-// splits the rows of a few deep, wide models into runs of tiles_per_item
-// tiles when whole rows would leave the device idle at the end of the launch.  A model whose
-// frequencies fill at most about half a tile (descriptor flag half_tile,
-// set by the caller) is swept with W/2 frequencies per lane instead of
-// leaving half of the lanes idle.  A lane's arithmetic is that of
-// forward_kernel with the same (block, frequencies per lane), so every
-// model's output is bitwise identical to a kh_forward_grid_ex launch of
-// that model alone with that shape.
+// Two kernels.  stage_tables_kernel computes, for every (model, slowness,
+// layer) of the batch, the layer's vertical slownesses, its phase step for
+// the frequency recurrence and the interface below it, into a caller-owned
+// table in global memory (one thread per record, all of them at once).
+// multi_kernel then works through a list of items (model, slowness, run of
+// frequency tiles).  Items are numbered model by model (callers list the
+// models with the costliest items first, so the queue drains longest-first);
+// each workgroup repeatedly takes the next item number from a device-wide
+// counter (one atomic per item) and sweeps the item's tiles, reading that
+// slowness's tables with scalar loads (the addresses are the same for every
+// lane of the wave), until the counter passes the item count.  The sweep
+// uses no LDS, so occupancy is set by its registers alone (two waves per
+// SIMD at 8 frequencies per lane, whatever the depth), and an item costs no
+// restaging: rows are split into runs of tiles wherever that balances the
+// drain at the end of the launch.  A model whose frequencies fill at most
+// about half a tile (descriptor flag half_tile, set by the caller) is swept
+// with W/2 frequencies per lane instead of leaving half of the lanes idle.
+// A lane's arithmetic is that of forward_kernel with the same (block,
+// frequencies per lane), so every model's output is bitwise identical to a
+// kh_forward_grid_ex launch of that model alone with that shape.
 #include "kh_physics.hpp"
 #include "kh_runtime.hpp"
 #include "kh_sweep.hpp"
@@ -25,6 +28,38 @@
 namespace kh {
 
 static_assert(sizeof(kh_model_desc) == 96, "kh_model_desc layout");
+
+typedef const KH_CONST kh_model_desc* CDesc;
+typedef const KH_CONST LayerPt* CLayer;
+typedef const KH_CONST Iface* CIface;
+typedef const KH_CONST LayerStep* CStep;
+
+// One table record per (model, slowness, layer): the layer at that
+// slowness, its phase step and the interface below it (unused for the last
+// layer).  The three arrays are carved from one workspace.
+struct Tables {
+  LayerPt* layer;
+  Iface* iface;
+  LayerStep* step;
+};
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static Tables carve(void* ws, long long records) {
+  char* b = static_cast<char*>(ws);
+  Tables t;
+  t.layer = reinterpret_cast<LayerPt*>(b);
+  b += align256((size_t)records * sizeof(LayerPt));
+  t.iface = reinterpret_cast<Iface*>(b);
+  b += align256((size_t)records * sizeof(Iface));
+  t.step = reinterpret_cast<LayerStep*>(b);
+  return t;
+}
+
+static size_t table_bytes(long long records) {
+  return align256((size_t)records * sizeof(LayerPt)) + align256((size_t)records * sizeof(Iface)) +
+         (size_t)records * sizeof(LayerStep);
+}
 
 // Resets the work counter and, when requested, the per-model counts of
 // non-finite outputs.
@@ -38,23 +73,53 @@ __device__ __forceinline__ bool finite_cd(const cd& z) {
   return z.r == z.r && z.i == z.i && fabs(z.r) != INFINITY && fabs(z.i) != INFINITY;
 }
 
-__device__ __forceinline__ int find_model(const kh_model_desc* __restrict__ d, int n, long long item) {
+// Last model whose `field` offset is <= x (descriptors are in queue order,
+// both offsets increase along it).
+template <class D, class F>
+__device__ __forceinline__ int find_model(D d, int n, long long x, F field) {
   int lo = 0, hi = n - 1;
-  while (lo < hi) {  // last model with item_off <= item
+  while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (d[mid].item_off <= item) lo = mid;
+    if (field(d[mid]) <= x) lo = mid;
     else hi = mid - 1;
   }
   return lo;
 }
 
-// Frequency tiles [t0, t1) of one staged (model, slowness): each lane takes WL
+__device__ __forceinline__ long long wave_uniform_i64(long long x) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)x);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(x >> 32));
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// One thread per table record r: layer l of slowness ip of the model whose
+// records start at table_off <= r.  The arithmetic is stage_slowness's:
+// layer_at for layers l and l + 1, interface_at between them, layer_step.
+__global__ void __launch_bounds__(256) stage_tables_kernel(
+    const kh_model_desc* __restrict__ desc, int nmodels, long long records,
+    const double* __restrict__ layers_all, const double* __restrict__ p_all, int block,
+    LayerPt* __restrict__ tl, Iface* __restrict__ tf, LayerStep* __restrict__ ts) {
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= records) return;
+  const int m = find_model(desc, nmodels, r, [](const kh_model_desc& d) { return d.table_off; });
+  const kh_model_desc& md = desc[m];
+  const long long local = r - md.table_off;
+  const int ip = (int)(local / md.nl);
+  const int l = (int)(local - (long long)ip * md.nl);
+  const double pv = p_all[md.p_off + ip];
+  const double* lay = layers_all + md.layer_off * NCOL;
+  const LayerPt a = layer_at(lay + (size_t)l * NCOL, l, pv);
+  tl[r] = a;
+  ts[r] = layer_step(a, (double)block * md.dw);
+  if (l < md.nl - 1) tf[r] = interface_at(l, pv, a, layer_at(lay + (size_t)(l + 1) * NCOL, l + 1, pv));
+}
+
+// Frequency tiles [t0, t1) of one (model, slowness): each lane takes WL
 // frequencies strided by the block size, as forward_kernel<WL> does.  Returns
 // the lane's count of non-finite outputs.
 template <int WL>
-__device__ __forceinline__ int sweep_item(const kh_model_desc& md, int t0, int t1,
-                                          const LayerPt* s_layer, const Iface* s_if,
-                                          const LayerStep* s_step, cd* __restrict__ Rrow) {
+__device__ __forceinline__ int sweep_item(const kh_model_desc& md, int t0, int t1, CLayer L,
+                                          CIface F, CStep S, cd* __restrict__ Rrow) {
   const int nw = md.nw;
   int bad = 0;
   for (int tile = t0; tile < t1; ++tile) {
@@ -66,7 +131,7 @@ __device__ __forceinline__ int sweep_item(const kh_model_desc& md, int t0, int t
       om[k] = cd{(md.k0 + (double)w) * md.dw, md.gamma};
     }
     cd out[WL];
-    sweep_lanes<WL, true>(s_layer, s_if, s_step, md.nl, om, md.free_surface, out);
+    sweep_lanes<WL, true, false>(L, F, S, md.nl, om, md.free_surface, out);
 #pragma unroll
     for (int k = 0; k < WL; ++k) {
       const int w = base + k * blockDim.x;
@@ -80,61 +145,63 @@ __device__ __forceinline__ int sweep_item(const kh_model_desc& md, int t0, int t
 }
 
 template <int W>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) multi_kernel(const kh_model_desc* __restrict__ desc,
-                                                    int nmodels, long long total,
-                                                    const double* __restrict__ layers_all,
-                                                    const double* __restrict__ p_all,
-                                                    cd* __restrict__ R_all, int max_nl,
-                                                    unsigned long long* __restrict__ counter,
-                                                    int* __restrict__ nonfinite) {
-  extern __shared__ double smem_m[];
-  LayerPt* s_layer = reinterpret_cast<LayerPt*>(smem_m);
-  Iface* s_if = reinterpret_cast<Iface*>(s_layer + max_nl);
-  LayerStep* s_step = reinterpret_cast<LayerStep*>(s_if + (max_nl - 1));
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) multi_kernel(
+    const kh_model_desc* __restrict__ desc_g, int nmodels, long long total,
+    const LayerPt* __restrict__ tl, const Iface* __restrict__ tf, const LayerStep* __restrict__ ts,
+    cd* __restrict__ R_all, unsigned long long* __restrict__ counter,
+    int* __restrict__ nonfinite) {
+  const CDesc desc = (CDesc)desc_g;
   __shared__ long long s_item;
   for (;;) {
     if (threadIdx.x == 0) s_item = (long long)atomicAdd(counter, 1ULL);
     __syncthreads();
-    const long long item = s_item;
+    const long long item = wave_uniform_i64(s_item);  // same for the whole workgroup
     __syncthreads();
     if (item >= total) break;  // every wave of every workgroup reaches this
-    const int mi = find_model(desc, nmodels, item);
-    const kh_model_desc& md = desc[mi];
+    const int mi = find_model(desc, nmodels, item,
+                              [](const KH_CONST kh_model_desc& d) { return d.item_off; });
+    const kh_model_desc md = desc[mi];
     const long long local = item - md.item_off;
     const int tpi = md.tiles_per_item > 0 ? md.tiles_per_item : md.ntiles;
     const int nch = (md.ntiles + tpi - 1) / tpi;  // items per slowness
     const int ip = (int)(local / nch);
     const int t0 = (int)(local - (long long)ip * nch) * tpi;
     const int t1 = min(md.ntiles, t0 + tpi);
-    const int nl = md.nl, nw = md.nw;
-    stage_slowness(layers_all + md.layer_off * NCOL, nl, p_all[md.p_off + ip], s_layer, s_if);
-    if (W > 1) {
-      const double shift = (double)blockDim.x * md.dw;
-      for (int l = threadIdx.x; l < nl; l += blockDim.x) s_step[l] = layer_step(s_layer[l], shift);
-      __syncthreads();
-    }
-    cd* Rrow = R_all + md.r_off + (long long)ip * nw;
+    const long long rec = md.table_off + (long long)ip * md.nl;
+    const CLayer L = (CLayer)tl + rec;
+    const CIface F = (CIface)tf + rec;
+    const CStep S = (CStep)ts + rec;
+    cd* Rrow = R_all + md.r_off + (long long)ip * md.nw;
     const int bad = (W > 1 && md.half_tile != 0)
-                        ? sweep_item<(W > 1 ? W / 2 : 1)>(md, t0, t1, s_layer, s_if, s_step, Rrow)
-                        : sweep_item<W>(md, t0, t1, s_layer, s_if, s_step, Rrow);
+                        ? sweep_item<(W > 1 ? W / 2 : 1)>(md, t0, t1, L, F, S, Rrow)
+                        : sweep_item<W>(md, t0, t1, L, F, S, Rrow);
     if (nonfinite != nullptr && bad != 0) atomicAdd(nonfinite + mi, bad);
-    __syncthreads();  // LDS is restaged by the next item
   }
 }
 
-static size_t multi_lds_bytes(int max_nl) {
-  return (size_t)max_nl * (sizeof(LayerPt) + sizeof(LayerStep)) +
-         (size_t)(max_nl - 1) * sizeof(Iface);
+template <int W>
+static int launch_multi(const kh_model_desc* desc, int nmodels, long long total, const Tables& t,
+                        cd* R_all, int block, unsigned long long* counter, int* nonfinite, int wgs,
+                        hipStream_t st) {
+  hipLaunchKernelGGL((multi_kernel<W>), dim3(wgs), dim3(block), 0, st, desc, nmodels, total,
+                     t.layer, t.iface, t.step, R_all, counter, nonfinite);
+  return check_launch("kh_forward_multi");
 }
 
-template <int W>
-static int launch_multi(const kh_model_desc* desc, int nmodels, long long total,
-                        const double* layers_all, const double* p_all, cd* R_all, int block,
-                        int max_nl, unsigned long long* counter, int* nonfinite, int wgs,
-                        hipStream_t st) {
-  hipLaunchKernelGGL((multi_kernel<W>), dim3(wgs), dim3(block), multi_lds_bytes(max_nl), st, desc,
-                     nmodels, total, layers_all, p_all, R_all, max_nl, counter, nonfinite);
-  return check_launch("kh_forward_multi");
+static int multi_resident(int block, int omega_per_item) {
+  int dev = 0, cus = 256, per_cu = 1;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  hipError_t e = hipSuccess;
+  switch (omega_per_item) {
+    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, multi_kernel<1>, block, 0); break;
+    case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, multi_kernel<2>, block, 0); break;
+    case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, multi_kernel<4>, block, 0); break;
+    case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, multi_kernel<8>, block, 0); break;
+    default: break;
+  }
+  if (e != hipSuccess || per_cu < 1) per_cu = 1;
+  return cus * per_cu;
 }
 
 }  // namespace kh
@@ -154,21 +221,8 @@ extern "C" long long kh_multi_items(int np, int ntiles, int tiles_per_item) {
   return (long long)np * ((ntiles + tpi - 1) / tpi);
 }
 
-static int multi_resident(int block, int omega_per_item, int max_nl) {
-  int dev = 0, cus = 256, per_cu = 1;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  const size_t lds = multi_lds_bytes(max_nl);
-  hipError_t e = hipSuccess;
-  switch (omega_per_item) {
-    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, multi_kernel<1>, block, lds); break;
-    case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, multi_kernel<2>, block, lds); break;
-    case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, multi_kernel<4>, block, lds); break;
-    case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, multi_kernel<8>, block, lds); break;
-    default: break;
-  }
-  if (e != hipSuccess || per_cu < 1) per_cu = 1;
-  return cus * per_cu;
+extern "C" size_t kh_multi_table_bytes(long long records) {
+  return records > 0 ? table_bytes(records) : 0;
 }
 
 extern "C" int kh_multi_workgroups(int block, int omega_per_item, int max_nl, int* workgroups) {
@@ -182,40 +236,50 @@ extern "C" int kh_multi_workgroups(int block, int omega_per_item, int max_nl, in
              omega_per_item);
   KH_REQUIRE(max_nl >= 2 && max_nl <= KH_MAX_LAYERS, KH_ERR_UNSUPPORTED,
              "kh_multi_workgroups: max_nl %d outside [2, %d]", max_nl, KH_MAX_LAYERS);
-  *workgroups = multi_resident(block, omega_per_item, max_nl);
+  *workgroups = multi_resident(block, omega_per_item);
   return KH_OK;
 }
 
 extern "C" int kh_forward_multi(const kh_model_desc* desc, int nmodels, long long total_items,
                                 int max_nl, const double* layers_all, const double* p_all,
                                 double* R_all, int block, int omega_per_item, int workgroups,
+                                long long records, void* tables, size_t table_bytes_given,
                                 void* counter, int* nonfinite, void* stream) {
   clear_error();
-  KH_REQUIRE(nmodels >= 0 && total_items >= 0, KH_ERR_INVALID_ARGUMENT,
+  KH_REQUIRE(nmodels >= 0 && total_items >= 0 && records >= 0, KH_ERR_INVALID_ARGUMENT,
              "kh_forward_multi: negative size");
   if (nmodels == 0 || total_items == 0) return KH_OK;
-  KH_REQUIRE(desc && layers_all && p_all && R_all && counter, KH_ERR_INVALID_ARGUMENT,
+  KH_REQUIRE(desc && layers_all && p_all && R_all && counter && tables, KH_ERR_INVALID_ARGUMENT,
              "kh_forward_multi: NULL pointer");
+  KH_REQUIRE(records > 0, KH_ERR_INVALID_ARGUMENT, "kh_forward_multi: no table records");
+  KH_REQUIRE(table_bytes_given >= table_bytes(records), KH_ERR_OUT_OF_MEMORY,
+             "kh_forward_multi: table workspace of %zu bytes, %zu needed", table_bytes_given,
+             table_bytes(records));
   KH_REQUIRE(max_nl >= 2 && max_nl <= KH_MAX_LAYERS, KH_ERR_UNSUPPORTED,
              "kh_forward_multi: max_nl %d outside [2, %d]", max_nl, KH_MAX_LAYERS);
   KH_REQUIRE(block == 64 || block == 128 || block == 256, KH_ERR_INVALID_ARGUMENT,
              "kh_forward_multi: block %d not in {64, 128, 256}", block);
+  KH_REQUIRE(omega_per_item == 1 || omega_per_item == 2 || omega_per_item == 4 ||
+                 omega_per_item == 8,
+             KH_ERR_INVALID_ARGUMENT, "kh_forward_multi: omega_per_item %d not in {1, 2, 4, 8}",
+             omega_per_item);
   hipStream_t st = as_stream(stream);
-  int wgs = workgroups;
-  if (wgs <= 0) wgs = multi_resident(block, omega_per_item, max_nl);  // fill the device
+  int wgs = workgroups > 0 ? workgroups : multi_resident(block, omega_per_item);
   if ((long long)wgs > total_items) wgs = (int)total_items;
   auto* c = reinterpret_cast<unsigned long long*>(counter);
-  const int zero_blocks = nonfinite != nullptr ? (nmodels + 255) / 256 : 1;
-  hipLaunchKernelGGL(zero_counter_kernel, dim3(zero_blocks), dim3(256), 0, st, c, nonfinite, nmodels);
-  KH_TRY(check_launch("kh_forward_multi (counter)"));
+  hipLaunchKernelGGL(zero_counter_kernel, dim3((nmodels + 255) / 256), dim3(256), 0, st, c,
+                     nonfinite, nmodels);
+  KH_TRY(check_launch("kh_forward_multi"));
+  const Tables t = carve(tables, records);
+  hipLaunchKernelGGL(stage_tables_kernel, dim3((unsigned)((records + 255) / 256)), dim3(256), 0,
+                     st, desc, nmodels, records, layers_all, p_all, block, t.layer, t.iface,
+                     t.step);
+  KH_TRY(check_launch("kh_forward_multi"));
   cd* R = reinterpret_cast<cd*>(R_all);
   switch (omega_per_item) {
-    case 1: return launch_multi<1>(desc, nmodels, total_items, layers_all, p_all, R, block, max_nl, c, nonfinite, wgs, st);
-    case 2: return launch_multi<2>(desc, nmodels, total_items, layers_all, p_all, R, block, max_nl, c, nonfinite, wgs, st);
-    case 4: return launch_multi<4>(desc, nmodels, total_items, layers_all, p_all, R, block, max_nl, c, nonfinite, wgs, st);
-    case 8: return launch_multi<8>(desc, nmodels, total_items, layers_all, p_all, R, block, max_nl, c, nonfinite, wgs, st);
-    default:
-      set_error("kh_forward_multi: omega_per_item %d not in {1, 2, 4, 8}", omega_per_item);
-      return KH_ERR_INVALID_ARGUMENT;
+    case 1: return launch_multi<1>(desc, nmodels, total_items, t, R, block, c, nonfinite, wgs, st);
+    case 2: return launch_multi<2>(desc, nmodels, total_items, t, R, block, c, nonfinite, wgs, st);
+    case 4: return launch_multi<4>(desc, nmodels, total_items, t, R, block, c, nonfinite, wgs, st);
+    default: return launch_multi<8>(desc, nmodels, total_items, t, R, block, c, nonfinite, wgs, st);
   }
 }

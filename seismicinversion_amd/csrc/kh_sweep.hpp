@@ -21,6 +21,14 @@
 
 #include "kh_physics.hpp"
 
+// Read-only global memory: loads at wave-uniform addresses through such a
+// pointer become scalar loads into scalar registers.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define KH_CONST __attribute__((address_space(4)))
+#else
+#define KH_CONST
+#endif
+
 namespace kh {
 
 struct LayerStep {  // per-layer phase step for a frequency shift of B*dw
@@ -118,15 +126,16 @@ __host__ __device__ __forceinline__ Iface wave_uniform(const Iface& f) {
 // those of frequency k-1 times the products of the factors (ea^2 steps by
 // a^2, eb^2 by b^2, ea eb by a b), evaluated once per layer and lane.
 // Otherwise every phase is evaluated.  With UNIFORM the interface tables
-// must be the same for every lane of the wave (one slowness per workgroup).
-template <int W, bool GRID, bool UNIFORM = true>
-__host__ __device__ __forceinline__ void sweep_rrd00(const LayerPt* __restrict__ L,
-                                                     const Iface* __restrict__ F,
-                                                     const LayerStep* __restrict__ steps, int nl,
+// must be the same for every lane of the wave (one slowness per workgroup)
+// and are moved to scalar registers.  LP, FP and SP are pointers to the
+// tables, in LDS or in read-only global memory (KH_CONST, read with scalar
+// loads at wave-uniform addresses).
+template <int W, bool GRID, bool UNIFORM = true, class LP, class FP, class SP>
+__host__ __device__ __forceinline__ void sweep_rrd00(LP L, FP F, SP steps, int nl,
                                                      const cd (&om)[W], cd (&out)[W]) {
   cd q00[W], q01[W], q10[W], q11[W];
   {
-    const Iface& fb = F[nl - 2];  // below it RRd = 0, so RRd = Rd
+    const Iface fb = F[nl - 2];  // below it RRd = 0, so RRd = Rd
 #pragma unroll
     for (int k = 0; k < W; ++k) {
       q00[k] = fb.rd00;
@@ -137,8 +146,10 @@ __host__ __device__ __forceinline__ void sweep_rrd00(const LayerPt* __restrict__
   }
   const cd one{1.0, 0.0};
   for (int i = nl - 3; i >= 0; --i) {
-    const LayerPt& lb = L[i + 1];
-    const Iface f = UNIFORM ? wave_uniform(F[i]) : F[i];
+    auto&& lb = L[i + 1];
+    Iface f;
+    if constexpr (UNIFORM) f = wave_uniform(F[i]);
+    else f = F[i];
     const bool hs = lb.halfspace;  // unit phase factors
     cd aa = one, bb = one, ab = one, sa2 = one, sb2 = one, sab = one;
     if (GRID && !hs) {
@@ -200,16 +211,17 @@ __host__ __device__ __forceinline__ void surface_pp(const LayerPt& water, const 
   }
 }
 
-// Full lane: PP reflectivity at W frequencies (UNIFORM as sweep_rrd00).
-template <int W, bool GRID, bool UNIFORM = true>
-__host__ __device__ __forceinline__ void sweep_lanes(const LayerPt* __restrict__ L,
-                                                     const Iface* __restrict__ F,
-                                                     const LayerStep* __restrict__ steps, int nl,
+// Full lane: PP reflectivity at W frequencies (UNIFORM and the table
+// pointers as sweep_rrd00).
+template <int W, bool GRID, bool UNIFORM = true, class LP, class FP, class SP>
+__host__ __device__ __forceinline__ void sweep_lanes(LP L, FP F, SP steps, int nl,
                                                      const cd (&om)[W], int free_surface,
                                                      cd (&out)[W]) {
   sweep_rrd00<W, GRID, UNIFORM>(L, F, steps, nl, om, out);
-  const LayerStep s0 = GRID ? steps[0] : LayerStep{{1.0, 0.0}, {1.0, 0.0}};
-  surface_pp<W, GRID>(L[0], s0, om, out, free_surface);
+  LayerStep s0{{1.0, 0.0}, {1.0, 0.0}};
+  if (GRID) s0 = steps[0];
+  const LayerPt water = L[0];
+  surface_pp<W, GRID>(water, s0, om, out, free_surface);
 }
 
 // Single frequency, generic omega (host tests and simple callers; the
@@ -219,7 +231,8 @@ __host__ __device__ __forceinline__ cd sweep_lane(const LayerPt* __restrict__ L,
                                                   int free_surface) {
   const cd o[1] = {om};
   cd r[1];
-  sweep_lanes<1, false, false>(L, F, nullptr, nl, o, free_surface, r);
+  sweep_lanes<1, false, false>(L, F, static_cast<const LayerStep*>(nullptr), nl, o,
+                               free_surface, r);
   return r[0];
 }
 
