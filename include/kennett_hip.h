@@ -147,20 +147,23 @@ long long kh_multi_items(int np, int ntiles, int tiles_per_item);
 size_t kh_multi_table_bytes(long long records);
 
 /* Workgroups a kh_forward_multi launch with workgroups <= 0 starts on the
- * current device (CUs x resident workgroups, before the cap at total_items),
- * written to *workgroups; for sizing tiles_per_item. */
-int kh_multi_workgroups(int block, int omega_per_item, int max_nl, int* workgroups);
+ * current device (CUs x resident workgroups, before the cap at total_items)
+ * on the staged-table path (tables != 0) or the in-LDS path (whose LDS grows
+ * with max_nl), written to *workgroups; for sizing tiles_per_item. */
+int kh_multi_workgroups(int block, int omega_per_item, int max_nl, int tables, int* workgroups);
 
-/* Reflectivity of nmodels models in one persistent launch.  First every
- * (model, slowness, layer) record of the layer, its phase step and the
- * interface below it is computed into `tables` (caller-owned device memory
- * of at least kh_multi_table_bytes(records) bytes; records = the sum of
- * np * nl, each model's records starting at its table_off).  Then
- * workgroups take (model, slowness, run of tiles) items from a device
- * counter until all total_items are done, reading the tables with scalar
- * loads.  desc (nmodels,) is a DEVICE array sorted by item_off and by
- * table_off (list the models with the costliest items, tiles per item *
- * (nl - 1), first for the best drain order); max_nl = the largest nl;
+/* Reflectivity of nmodels models in one persistent launch: workgroups take
+ * (model, slowness, run of tiles) items from a device counter until all
+ * total_items are done.  With `tables` (caller-owned device memory of at
+ * least kh_multi_table_bytes(records) bytes; records = the sum of np * nl,
+ * each model's records starting at its table_off), every (model, slowness,
+ * layer) record of the layer, its phase step and the interface below it is
+ * computed there first and the sweep reads it with scalar loads; with
+ * tables = NULL (records and table_bytes ignored) each item's workgroup
+ * stages its slowness's tables in LDS, and the batch is a single kernel.
+ * desc (nmodels,) is a DEVICE array sorted by item_off and by table_off
+ * (list the models with the costliest items, tiles per item * (nl - 1),
+ * first for the best drain order); max_nl = the largest nl;
  * counter = 8 bytes of device memory (reset by the call); nonfinite = NULL
  * or a device int array (nmodels,) that receives, for each desc entry k,
  * the number of NaN/Inf outputs of that model (reset by the call);

@@ -57,7 +57,7 @@ EXPORTS: dict[str, list] = {
                             _c_int, _vp, _vp],
     "kh_multi_tiles": [_c_int, _c_int, _c_int, _c_int],
     "kh_multi_items": [_c_int, _c_int, _c_int],
-    "kh_multi_workgroups": [_c_int, _c_int, _c_int, _vp],
+    "kh_multi_workgroups": [_c_int, _c_int, _c_int, _c_int, _vp],
     "kh_multi_table_bytes": [ctypes.c_longlong],
     "kh_forward_multi": [_vp, _c_int, ctypes.c_longlong, _c_int, _vp, _vp, _vp, _c_int, _c_int,
                          _c_int, ctypes.c_longlong, _vp, ctypes.c_size_t, _vp, _vp, _vp],

@@ -82,7 +82,7 @@ class MultiPlan:
     nonfinite: torch.Tensor     # int32 (nmodels,) NaN/Inf output counts, queue order
     order: list[int]            # queue slot -> input index
     records: int                # table records: sum of np * nl
-    tables: torch.Tensor        # uint8 table workspace (kh_multi_table_bytes(records))
+    tables: torch.Tensor | None  # uint8 table workspace, or None: tables staged in LDS
 
 
 def _to_device(arr: np.ndarray, dev, non_blocking: bool) -> torch.Tensor:
@@ -118,15 +118,18 @@ def table_bytes(records: int) -> int:
     return int(_abi.lib().kh_multi_table_bytes(int(records))) if records > 0 else 0
 
 
-def resident_workgroups(block: int, wpt: int, max_nl: int, dev=None) -> int:
-    """Workgroups a persistent launch of this shape starts (cached)."""
+def resident_workgroups(block: int, wpt: int, max_nl: int, tables: bool = True,
+                        dev=None) -> int:
+    """Workgroups a persistent launch of this shape starts on the
+    staged-table or in-LDS path (cached)."""
     dev = _abi.device(dev)
-    key = (dev.index, block, wpt, max_nl)
+    key = (dev.index, block, wpt, max_nl if not tables else 0, bool(tables))
     n = _RESIDENT.get(key)
     if n is None:
         out = ctypes.c_int(0)
         with torch.cuda.device(dev):
-            _abi.check(_abi.lib().kh_multi_workgroups(block, wpt, max_nl, ctypes.byref(out)),
+            _abi.check(_abi.lib().kh_multi_workgroups(block, wpt, max_nl, int(bool(tables)),
+                                                      ctypes.byref(out)),
                        "kh_multi_workgroups")
         n = _RESIDENT[key] = max(int(out.value), 1)
     return n
@@ -145,11 +148,24 @@ def omega_per_lane(nw: int, block: int, wpt: int) -> int:
     return wpt // 2 if half * HALF_TILE_COST < full else wpt
 
 
+# Batches of at least this many updates stage their tables with a kernel of
+# their own (a workspace of 264 B per (model, slowness, layer)) and sweep
+# without LDS; smaller ones stage them in LDS within the one sweep kernel.
+# The served 400-request stream (ten batches of about 9e8 updates) runs
+# 3.7 % faster on staged tables; a small batch launched while other work
+# occupies the device waits for free compute units once instead of twice
+# (the open-loop stream's forward p50 went from 2.2 to 4.4 ms with staged
+# tables for every batch).
+TABLES_MIN_UPDATES = 2e8
+
+
 def plan_multi(specs: list[ModelSpec], block: int, wpt: int, dev=None,
-               non_blocking: bool = False) -> MultiPlan:
+               non_blocking: bool = False, tables: bool | None = None) -> MultiPlan:
     """Pack specs into device buffers and descriptors for one launch shape.
     With ``non_blocking`` the host->device copies go through pinned memory and
-    are only queued on the current stream (the host does not wait)."""
+    are only queued on the current stream (the host does not wait).
+    ``tables`` picks the staged-table path (default: batches of at least
+    TABLES_MIN_UPDATES updates)."""
     dev = _abi.device(dev)
     tabs = [s.table() for s in specs]
     for t in tabs:
@@ -162,7 +178,9 @@ def plan_multi(specs: list[ModelSpec], block: int, wpt: int, dev=None,
     tile_cost = [(HALF_TILE_COST if f < wpt else 1.0) * (t.shape[0] - 1)
                  for f, t in zip(lanes, tabs)]
     max_nl = max(t.shape[0] for t in tabs) if tabs else 2
-    slots = resident_workgroups(block, wpt, max_nl, dev) if specs else 1
+    if tables is None:
+        tables = sum(s.updates() for s in specs) >= TABLES_MIN_UPDATES
+    slots = resident_workgroups(block, wpt, max_nl, tables, dev) if specs else 1
     cap = TAIL_FRACTION * sum(len(s.p) * n * c for s, n, c in zip(specs, ntiles, tile_cost)) / slots
     tpi = [n if n * c <= cap else max(1, int(cap // c)) for n, c in zip(ntiles, tile_cost)]
     # Longest items first (then the model's total): the smallest items are
@@ -196,8 +214,8 @@ def plan_multi(specs: list[ModelSpec], block: int, wpt: int, dev=None,
                      counter=torch.empty(1, dtype=torch.int64, device=dev),
                      nonfinite=torch.empty(max(len(specs), 1), dtype=torch.int32, device=dev),
                      order=order, records=records,
-                     tables=torch.empty(max(int(table_bytes(records)), 1), dtype=torch.uint8,
-                                        device=dev))
+                     tables=(torch.empty(max(int(table_bytes(records)), 1), dtype=torch.uint8,
+                                         device=dev) if tables else None))
 
 
 def launch(plan: MultiPlan, workgroups: int = 0) -> torch.Tensor:
@@ -206,7 +224,9 @@ def launch(plan: MultiPlan, workgroups: int = 0) -> torch.Tensor:
     rc = _abi.lib().kh_forward_multi(_abi.ptr(plan.desc), len(plan.shapes), plan.total_items,
                                      plan.max_nl, _abi.ptr(plan.layers), _abi.ptr(plan.p),
                                      _abi.ptr(plan.R), plan.block, plan.wpt, int(workgroups),
-                                     plan.records, _abi.ptr(plan.tables), plan.tables.numel(),
+                                     plan.records,
+                                     _abi.ptr(plan.tables) if plan.tables is not None else None,
+                                     plan.tables.numel() if plan.tables is not None else 0,
                                      _abi.ptr(plan.counter), _abi.ptr(plan.nonfinite),
                                      _abi.stream_handle(dev))
     _abi.check(rc, "kh_forward_multi")
@@ -228,13 +248,15 @@ def split(plan: MultiPlan) -> list[torch.Tensor]:
 
 
 def reflectivity_multi(specs: list[ModelSpec], block: int | None = None,
-                       omega_per_item: int | None = None, dev=None) -> list[torch.Tensor]:
+                       omega_per_item: int | None = None, dev=None,
+                       tables: bool | None = None) -> list[torch.Tensor]:
     """Each spec's R (np, nw) on the device, all from one persistent launch.
-    The launch shape comes from the autotuner unless given."""
+    The launch shape comes from the autotuner unless given; ``tables`` as in
+    plan_multi."""
     if not specs:
         return []
     if block is None or omega_per_item is None:
         block, omega_per_item = autotune.multi_shape(specs, dev=dev)
-    plan = plan_multi(specs, block, omega_per_item, dev)
+    plan = plan_multi(specs, block, omega_per_item, dev, tables=tables)
     launch(plan)
     return split(plan)

@@ -1,26 +1,33 @@
 // Many earth models of different depths, slowness counts and frequency
 // grids in one persistent launch.
 //
-// Two kernels.  stage_tables_kernel computes, for every (model, slowness,
-// layer) of the batch, the layer's vertical slownesses, its phase step for
-// the frequency recurrence and the interface below it, into a caller-owned
-// table in global memory (one thread per record, all of them at once).
-// multi_kernel then works through a list of items (model, slowness, run of
-// frequency tiles).  Items are numbered model by model (callers list the
-// models with the costliest items first, so the queue drains longest-first);
-// each workgroup repeatedly takes the next item number from a device-wide
-// counter (one atomic per item) and sweeps the item's tiles, reading that
-// slowness's tables with scalar loads (the addresses are the same for every
-// lane of the wave), until the counter passes the item count.  The sweep
-// uses no LDS, so occupancy is set by its registers alone (two waves per
-// SIMD at 8 frequencies per lane, whatever the depth), and an item costs no
-// restaging: rows are split into runs of tiles wherever that balances the
-// drain at the end of the launch.  A model whose frequencies fill at most
-// about half a tile (descriptor flag half_tile, set by the caller) is swept
-// with W/2 frequencies per lane instead of leaving half of the lanes idle.
-// A lane's arithmetic is that of forward_kernel with the same (block,
-// frequencies per lane), so every model's output is bitwise identical to a
-// kh_forward_grid_ex launch of that model alone with that shape.
+// The work is a list of items (model, slowness, run of frequency tiles).
+// Items are numbered model by model (callers list the models with the
+// costliest items first, so the queue drains longest-first); each workgroup
+// repeatedly takes the next item number from a device-wide counter (one
+// atomic per item) and sweeps the item's tiles with that slowness's layer,
+// interface and phase-step tables, until the counter passes the item count.
+// The tables come one of two ways:
+//   - staged tables (a caller workspace is given): stage_tables_kernel first
+//     computes every (model, slowness, layer) record of the batch, one thread
+//     per record; the sweep (multi_kernel) reads them with scalar loads, as
+//     the addresses are the same for every lane of a wave.  The sweep uses
+//     no LDS, so occupancy is set by its registers alone (two waves per SIMD
+//     at 8 frequencies per lane, whatever the depth), and an item costs no
+//     restaging, so rows are split into runs of tiles wherever that balances
+//     the drain at the end of the launch.  Each tile re-reads its tables
+//     from L2 or memory.
+//   - in-LDS tables (no workspace): each item's workgroup stages its
+//     slowness's tables in LDS (multi_lds_kernel), one kernel in all.  LDS
+//     bounds occupancy for deep models, and every item restages; a batch is
+//     one kernel, which waits for free compute units once rather than twice
+//     when other work shares the device.
+// A model whose frequencies fill at most about half a tile (descriptor flag
+// half_tile, set by the caller) is swept with W/2 frequencies per lane
+// instead of leaving half of the lanes idle.  A lane's arithmetic is that
+// of forward_kernel with the same (block, frequencies per lane), so every
+// model's output is bitwise identical to a kh_forward_grid_ex launch of that
+// model alone with that shape, on either path.
 #include "kh_physics.hpp"
 #include "kh_runtime.hpp"
 #include "kh_sweep.hpp"
@@ -117,9 +124,9 @@ __global__ void __launch_bounds__(256) stage_tables_kernel(
 // Frequency tiles [t0, t1) of one (model, slowness): each lane takes WL
 // frequencies strided by the block size, as forward_kernel<WL> does.  Returns
 // the lane's count of non-finite outputs.
-template <int WL>
-__device__ __forceinline__ int sweep_item(const kh_model_desc& md, int t0, int t1, CLayer L,
-                                          CIface F, CStep S, cd* __restrict__ Rrow) {
+template <int WL, bool UNIFORM, class LP, class FP, class SP>
+__device__ __forceinline__ int sweep_item(const kh_model_desc& md, int t0, int t1, LP L, FP F,
+                                          SP S, cd* __restrict__ Rrow) {
   const int nw = md.nw;
   int bad = 0;
   for (int tile = t0; tile < t1; ++tile) {
@@ -131,7 +138,7 @@ __device__ __forceinline__ int sweep_item(const kh_model_desc& md, int t0, int t
       om[k] = cd{(md.k0 + (double)w) * md.dw, md.gamma};
     }
     cd out[WL];
-    sweep_lanes<WL, true, false>(L, F, S, md.nl, om, md.free_surface, out);
+    sweep_lanes<WL, true, UNIFORM>(L, F, S, md.nl, om, md.free_surface, out);
 #pragma unroll
     for (int k = 0; k < WL; ++k) {
       const int w = base + k * blockDim.x;
@@ -173,35 +180,115 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) m
     const CStep S = (CStep)ts + rec;
     cd* Rrow = R_all + md.r_off + (long long)ip * md.nw;
     const int bad = (W > 1 && md.half_tile != 0)
-                        ? sweep_item<(W > 1 ? W / 2 : 1)>(md, t0, t1, L, F, S, Rrow)
-                        : sweep_item<W>(md, t0, t1, L, F, S, Rrow);
+                        ? sweep_item<(W > 1 ? W / 2 : 1), false>(md, t0, t1, L, F, S, Rrow)
+                        : sweep_item<W, false>(md, t0, t1, L, F, S, Rrow);
     if (nonfinite != nullptr && bad != 0) atomicAdd(nonfinite + mi, bad);
   }
 }
 
+// The same items with the tables staged per item in LDS (interface values
+// moved to scalar registers by the sweep).
 template <int W>
-static int launch_multi(const kh_model_desc* desc, int nmodels, long long total, const Tables& t,
-                        cd* R_all, int block, unsigned long long* counter, int* nonfinite, int wgs,
-                        hipStream_t st) {
-  hipLaunchKernelGGL((multi_kernel<W>), dim3(wgs), dim3(block), 0, st, desc, nmodels, total,
-                     t.layer, t.iface, t.step, R_all, counter, nonfinite);
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) multi_lds_kernel(
+    const kh_model_desc* __restrict__ desc, int nmodels, long long total,
+    const double* __restrict__ layers_all, const double* __restrict__ p_all,
+    cd* __restrict__ R_all, int max_nl, unsigned long long* __restrict__ counter,
+    int* __restrict__ nonfinite) {
+  extern __shared__ double smem_m[];
+  LayerPt* s_layer = reinterpret_cast<LayerPt*>(smem_m);
+  Iface* s_if = reinterpret_cast<Iface*>(s_layer + max_nl);
+  LayerStep* s_step = reinterpret_cast<LayerStep*>(s_if + (max_nl - 1));
+  __shared__ long long s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = (long long)atomicAdd(counter, 1ULL);
+    __syncthreads();
+    const long long item = s_item;
+    __syncthreads();
+    if (item >= total) break;  // every wave of every workgroup reaches this
+    const int mi = find_model(desc, nmodels, item,
+                              [](const kh_model_desc& d) { return d.item_off; });
+    const kh_model_desc& md = desc[mi];
+    const long long local = item - md.item_off;
+    const int tpi = md.tiles_per_item > 0 ? md.tiles_per_item : md.ntiles;
+    const int nch = (md.ntiles + tpi - 1) / tpi;  // items per slowness
+    const int ip = (int)(local / nch);
+    const int t0 = (int)(local - (long long)ip * nch) * tpi;
+    const int t1 = min(md.ntiles, t0 + tpi);
+    const int nl = md.nl;
+    stage_slowness(layers_all + md.layer_off * NCOL, nl, p_all[md.p_off + ip], s_layer, s_if);
+    if (W > 1) {
+      const double shift = (double)blockDim.x * md.dw;
+      for (int l = threadIdx.x; l < nl; l += blockDim.x) s_step[l] = layer_step(s_layer[l], shift);
+      __syncthreads();
+    }
+    cd* Rrow = R_all + md.r_off + (long long)ip * md.nw;
+    const LayerPt* L = s_layer;
+    const Iface* F = s_if;
+    const LayerStep* S = s_step;
+    const int bad = (W > 1 && md.half_tile != 0)
+                        ? sweep_item<(W > 1 ? W / 2 : 1), true>(md, t0, t1, L, F, S, Rrow)
+                        : sweep_item<W, true>(md, t0, t1, L, F, S, Rrow);
+    if (nonfinite != nullptr && bad != 0) atomicAdd(nonfinite + mi, bad);
+    __syncthreads();  // LDS is restaged by the next item
+  }
+}
+
+static size_t multi_lds_bytes(int max_nl) {
+  return (size_t)max_nl * (sizeof(LayerPt) + sizeof(LayerStep)) +
+         (size_t)(max_nl - 1) * sizeof(Iface);
+}
+
+// Launch arguments shared by both paths.
+struct MultiArgs {
+  const kh_model_desc* desc;
+  int nmodels;
+  long long total;
+  const double* layers_all;
+  const double* p_all;
+  cd* R;
+  int block, max_nl;
+  unsigned long long* counter;
+  int* nonfinite;
+  int wgs;
+  hipStream_t st;
+};
+
+template <int W>
+static int launch_multi(const MultiArgs& a, const Tables* t) {
+  if (t != nullptr) {
+    hipLaunchKernelGGL((multi_kernel<W>), dim3(a.wgs), dim3(a.block), 0, a.st, a.desc, a.nmodels,
+                       a.total, t->layer, t->iface, t->step, a.R, a.counter, a.nonfinite);
+  } else {
+    hipLaunchKernelGGL((multi_lds_kernel<W>), dim3(a.wgs), dim3(a.block),
+                       multi_lds_bytes(a.max_nl), a.st, a.desc, a.nmodels, a.total, a.layers_all,
+                       a.p_all, a.R, a.max_nl, a.counter, a.nonfinite);
+  }
   return check_launch("kh_forward_multi");
 }
 
-static int multi_resident(int block, int omega_per_item) {
-  int dev = 0, cus = 256, per_cu = 1;
+template <class K>
+static int per_cu(K kernel, int block, size_t lds) {
+  int n = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, block, lds) != hipSuccess || n < 1)
+    n = 1;
+  return n;
+}
+
+// CUs x resident workgroups of the path's kernel (max_nl sizes the LDS of
+// the in-LDS path).
+static int multi_resident(int block, int omega_per_item, int max_nl, bool tables) {
+  int dev = 0, cus = 256;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  hipError_t e = hipSuccess;
+  const size_t lds = multi_lds_bytes(max_nl);
+  int n = 1;
   switch (omega_per_item) {
-    case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, multi_kernel<1>, block, 0); break;
-    case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, multi_kernel<2>, block, 0); break;
-    case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, multi_kernel<4>, block, 0); break;
-    case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, multi_kernel<8>, block, 0); break;
-    default: break;
+    case 1: n = tables ? per_cu(multi_kernel<1>, block, 0) : per_cu(multi_lds_kernel<1>, block, lds); break;
+    case 2: n = tables ? per_cu(multi_kernel<2>, block, 0) : per_cu(multi_lds_kernel<2>, block, lds); break;
+    case 4: n = tables ? per_cu(multi_kernel<4>, block, 0) : per_cu(multi_lds_kernel<4>, block, lds); break;
+    default: n = tables ? per_cu(multi_kernel<8>, block, 0) : per_cu(multi_lds_kernel<8>, block, lds); break;
   }
-  if (e != hipSuccess || per_cu < 1) per_cu = 1;
-  return cus * per_cu;
+  return cus * n;
 }
 
 }  // namespace kh
@@ -225,7 +312,8 @@ extern "C" size_t kh_multi_table_bytes(long long records) {
   return records > 0 ? table_bytes(records) : 0;
 }
 
-extern "C" int kh_multi_workgroups(int block, int omega_per_item, int max_nl, int* workgroups) {
+extern "C" int kh_multi_workgroups(int block, int omega_per_item, int max_nl, int tables,
+                                   int* workgroups) {
   clear_error();
   KH_REQUIRE(workgroups != nullptr, KH_ERR_INVALID_ARGUMENT, "kh_multi_workgroups: NULL pointer");
   KH_REQUIRE(block == 64 || block == 128 || block == 256, KH_ERR_INVALID_ARGUMENT,
@@ -236,7 +324,7 @@ extern "C" int kh_multi_workgroups(int block, int omega_per_item, int max_nl, in
              omega_per_item);
   KH_REQUIRE(max_nl >= 2 && max_nl <= KH_MAX_LAYERS, KH_ERR_UNSUPPORTED,
              "kh_multi_workgroups: max_nl %d outside [2, %d]", max_nl, KH_MAX_LAYERS);
-  *workgroups = multi_resident(block, omega_per_item);
+  *workgroups = multi_resident(block, omega_per_item, max_nl, tables != 0);
   return KH_OK;
 }
 
@@ -249,12 +337,14 @@ extern "C" int kh_forward_multi(const kh_model_desc* desc, int nmodels, long lon
   KH_REQUIRE(nmodels >= 0 && total_items >= 0 && records >= 0, KH_ERR_INVALID_ARGUMENT,
              "kh_forward_multi: negative size");
   if (nmodels == 0 || total_items == 0) return KH_OK;
-  KH_REQUIRE(desc && layers_all && p_all && R_all && counter && tables, KH_ERR_INVALID_ARGUMENT,
+  KH_REQUIRE(desc && layers_all && p_all && R_all && counter, KH_ERR_INVALID_ARGUMENT,
              "kh_forward_multi: NULL pointer");
-  KH_REQUIRE(records > 0, KH_ERR_INVALID_ARGUMENT, "kh_forward_multi: no table records");
-  KH_REQUIRE(table_bytes_given >= table_bytes(records), KH_ERR_OUT_OF_MEMORY,
-             "kh_forward_multi: table workspace of %zu bytes, %zu needed", table_bytes_given,
-             table_bytes(records));
+  if (tables != nullptr) {
+    KH_REQUIRE(records > 0, KH_ERR_INVALID_ARGUMENT, "kh_forward_multi: no table records");
+    KH_REQUIRE(table_bytes_given >= table_bytes(records), KH_ERR_OUT_OF_MEMORY,
+               "kh_forward_multi: table workspace of %zu bytes, %zu needed", table_bytes_given,
+               table_bytes(records));
+  }
   KH_REQUIRE(max_nl >= 2 && max_nl <= KH_MAX_LAYERS, KH_ERR_UNSUPPORTED,
              "kh_forward_multi: max_nl %d outside [2, %d]", max_nl, KH_MAX_LAYERS);
   KH_REQUIRE(block == 64 || block == 128 || block == 256, KH_ERR_INVALID_ARGUMENT,
@@ -264,22 +354,28 @@ extern "C" int kh_forward_multi(const kh_model_desc* desc, int nmodels, long lon
              KH_ERR_INVALID_ARGUMENT, "kh_forward_multi: omega_per_item %d not in {1, 2, 4, 8}",
              omega_per_item);
   hipStream_t st = as_stream(stream);
-  int wgs = workgroups > 0 ? workgroups : multi_resident(block, omega_per_item);
+  const bool use_tables = tables != nullptr;
+  int wgs = workgroups > 0 ? workgroups : multi_resident(block, omega_per_item, max_nl, use_tables);
   if ((long long)wgs > total_items) wgs = (int)total_items;
   auto* c = reinterpret_cast<unsigned long long*>(counter);
   hipLaunchKernelGGL(zero_counter_kernel, dim3((nmodels + 255) / 256), dim3(256), 0, st, c,
                      nonfinite, nmodels);
   KH_TRY(check_launch("kh_forward_multi"));
-  const Tables t = carve(tables, records);
-  hipLaunchKernelGGL(stage_tables_kernel, dim3((unsigned)((records + 255) / 256)), dim3(256), 0,
-                     st, desc, nmodels, records, layers_all, p_all, block, t.layer, t.iface,
-                     t.step);
-  KH_TRY(check_launch("kh_forward_multi"));
-  cd* R = reinterpret_cast<cd*>(R_all);
+  Tables t{};
+  if (use_tables) {
+    t = carve(tables, records);
+    hipLaunchKernelGGL(stage_tables_kernel, dim3((unsigned)((records + 255) / 256)), dim3(256), 0,
+                       st, desc, nmodels, records, layers_all, p_all, block, t.layer, t.iface,
+                       t.step);
+    KH_TRY(check_launch("kh_forward_multi"));
+  }
+  const MultiArgs a{desc, nmodels, total_items, layers_all, p_all, reinterpret_cast<cd*>(R_all),
+                    block, max_nl, c, nonfinite, wgs, st};
+  const Tables* tp = use_tables ? &t : nullptr;
   switch (omega_per_item) {
-    case 1: return launch_multi<1>(desc, nmodels, total_items, t, R, block, c, nonfinite, wgs, st);
-    case 2: return launch_multi<2>(desc, nmodels, total_items, t, R, block, c, nonfinite, wgs, st);
-    case 4: return launch_multi<4>(desc, nmodels, total_items, t, R, block, c, nonfinite, wgs, st);
-    default: return launch_multi<8>(desc, nmodels, total_items, t, R, block, c, nonfinite, wgs, st);
+    case 1: return launch_multi<1>(a, tp);
+    case 2: return launch_multi<2>(a, tp);
+    case 4: return launch_multi<4>(a, tp);
+    default: return launch_multi<8>(a, tp);
   }
 }

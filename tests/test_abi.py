@@ -84,15 +84,15 @@ def test_multi_tile_and_item_counts(lib):
     # kh_multi_workgroups checks its arguments before it asks the device
     lib.kh_last_error.restype = ctypes.c_char_p
     out = ctypes.c_int(0)
-    assert lib.kh_multi_workgroups(96, 8, 20, ctypes.byref(out)) == 1
+    assert lib.kh_multi_workgroups(96, 8, 20, 1, ctypes.byref(out)) == 1
     assert b"block" in lib.kh_last_error()
-    assert lib.kh_multi_workgroups(64, 3, 20, ctypes.byref(out)) == 1
-    assert lib.kh_multi_workgroups(64, 8, 20, None) == 1
+    assert lib.kh_multi_workgroups(64, 3, 20, 0, ctypes.byref(out)) == 1
+    assert lib.kh_multi_workgroups(64, 8, 20, 1, None) == 1
 
 
 def test_multi_table_workspace_is_checked_before_any_launch(lib):
-    """kh_forward_multi stages its per-(slowness, layer) tables in a caller
-    workspace of kh_multi_table_bytes(records) bytes; a smaller one is
+    """With a table workspace, kh_forward_multi stages its per-(slowness,
+    layer) tables there; one smaller than kh_multi_table_bytes(records) is
     refused with KH_ERR_OUT_OF_MEMORY before anything is launched."""
     lib.kh_multi_table_bytes.restype = ctypes.c_size_t
     lib.kh_multi_table_bytes.argtypes = [ctypes.c_longlong]
@@ -107,7 +107,7 @@ def test_multi_table_workspace_is_checked_before_any_launch(lib):
     args = [vp(16), 2, 10, 20, vp(16), vp(16), vp(16), 64, 8, 0]
     rc = f(*args, 1000, vp(16), need - 1, vp(16), None, None)
     assert rc == 2 and b"table workspace" in lib.kh_last_error()
-    rc = f(*args, 1000, None, need, vp(16), None, None)
+    rc = f(*args, 1000, vp(16), need, None, None, None)  # no counter
     assert rc == 1 and b"NULL" in lib.kh_last_error()
     rc = f(*args[:7], 96, 8, 0, 1000, vp(16), need, vp(16), None, None)
     assert rc == 1 and b"block" in lib.kh_last_error()

@@ -21,12 +21,15 @@ def specs_c3(n_models, nw=1024, seed=42):
             for m, p in uneven_batch_models(n_models, seed)]
 
 
+@pytest.mark.parametrize("tables", [False, True])
 @pytest.mark.parametrize("block,wpt", [(256, 2), (128, 1), (256, 4), (64, 2), (64, 8), (128, 8)])
-def test_multi_equals_plain_bitwise(block, wpt):
+def test_multi_equals_plain_bitwise(block, wpt, tables):
+    """Both ways of staging the tables (LDS per item, or a staging kernel
+    and scalar loads) give the plain launch's bits."""
     specs = specs_c3(20)
     specs[3].free_surface = True
     specs[5].gamma = np.pi / 64.0
-    outs = batch.reflectivity_multi(specs, block=block, omega_per_item=wpt)
+    outs = batch.reflectivity_multi(specs, block=block, omega_per_item=wpt, tables=tables)
     for s, R in zip(specs, outs):
         L = engine.layers_tensor(s.model)
         Rp = engine.forward_grid(L, torch.from_numpy(s.p), s.k0, s.dw, s.gamma, s.nw,
@@ -65,7 +68,8 @@ def test_full_uneven_batch_and_autotuner():
     assert all(torch.isfinite(torch.view_as_real(R)).all() for R in outs)
 
 
-def test_multi_mixed_grids_bitwise():
+@pytest.mark.parametrize("tables", [False, True])
+def test_multi_mixed_grids_bitwise(tables):
     """Models with their own frequency counts (one partial tile up to eight
     full ones per slowness at 64 x 8, half-width tiles of 4 frequencies per
     lane for the counts that waste less that way) and start frequencies in
@@ -76,7 +80,7 @@ def test_multi_mixed_grids_bitwise():
         s.nw, s.k0 = nw, k0
     lanes = [batch.omega_per_lane(s.nw, 64, 8) for s in specs]
     assert lanes == [4, 4, 8, 8, 4, 4, 8, 8, 4, 4]
-    outs = batch.reflectivity_multi(specs, block=64, omega_per_item=8)
+    outs = batch.reflectivity_multi(specs, block=64, omega_per_item=8, tables=tables)
     for s, R, f in zip(specs, outs, lanes):
         L = engine.layers_tensor(s.model)
         Rp = engine.forward_grid(L, torch.from_numpy(s.p), s.k0, s.dw, s.gamma, s.nw,
@@ -85,15 +89,16 @@ def test_multi_mixed_grids_bitwise():
         assert torch.equal(R, Rp), f"nw {s.nw}: persistent launch differs from the plain launch"
 
 
+@pytest.mark.parametrize("tables", [False, True])
 @pytest.mark.parametrize("block,wpt", [(256, 2), (64, 2)])
-def test_nonfinite_counts_per_model(block, wpt):
+def test_nonfinite_counts_per_model(block, wpt, tables):
     """A NaN slowness makes its whole row NaN; the launch counts the non-finite
     outputs of each model (over all frequency tiles of the row: three at
     64 x 2) and the others stay at zero."""
     specs = specs_c3(6, nw=300)
     specs[2].p = specs[2].p.copy()
     specs[2].p[1] = np.nan
-    plan = batch.plan_multi(specs, block, wpt)
+    plan = batch.plan_multi(specs, block, wpt, tables=tables)
     batch.launch(plan)
     counts = batch.nonfinite_counts(plan, plan.nonfinite.cpu().numpy())
     assert counts == [300 if i == 2 else 0 for i in range(6)]
@@ -104,8 +109,9 @@ def test_nonfinite_counts_per_model(block, wpt):
     assert batch.nonfinite_counts(plan, plan.nonfinite.cpu().numpy())[2] == 300
 
 
+@pytest.mark.parametrize("tables", [False, True])
 @pytest.mark.parametrize("tail", [1e-9, None])
-def test_tile_runs_for_a_thin_batch(monkeypatch, tail):
+def test_tile_runs_for_a_thin_batch(monkeypatch, tail, tables):
     """Three deep models with many frequency tiles and few slownesses: whole
     rows would leave most workgroups idle, so their rows are split into runs
     of tiles (one tile per item with a tiny tail fraction).  Outputs equal
@@ -116,7 +122,7 @@ def test_tile_runs_for_a_thin_batch(monkeypatch, tail):
                              nw=nw, dw=2 * np.pi / 64.0, k0=1.0, free_surface=(nl == 60))
              for nl, npp, nw in ((90, 300, 4096), (60, 200, 3000), (20, 50, 700))]
     specs[1].p[7] = np.nan
-    plan = batch.plan_multi(specs, 64, 8)
+    plan = batch.plan_multi(specs, 64, 8, tables=tables)
     desc = plan.desc.cpu().numpy().view(batch.DESC_DTYPE)
     tpi = {int(d["nl"]): int(d["tiles_per_item"]) for d in desc}
     assert tpi[90] > 0 and tpi[60] > 0, tpi  # rows split
