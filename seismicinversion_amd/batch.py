@@ -155,7 +155,9 @@ def omega_per_lane(nw: int, block: int, wpt: int) -> int:
 # 3.7 % faster on staged tables; a small batch launched while other work
 # occupies the device waits for free compute units once instead of twice
 # (the open-loop stream's forward p50 went from 2.2 to 4.4 ms with staged
-# tables for every batch).
+# tables for every batch).  With the threshold at 5e8 one open-loop run gave
+# forward p50 2.8 ms and inversion p50 1.21 s, against 2.3 ms and 0.83-0.90 s
+# in two runs at 2e8.
 TABLES_MIN_UPDATES = 2e8
 
 
