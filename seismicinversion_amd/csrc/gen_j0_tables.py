@@ -1,7 +1,7 @@
 """Writes kh_j0_tables.hpp: the coefficients of the device J0 (kh_j0.hpp).
 
 Run from any directory:  python gen_j0_tables.py   (needs mpmath; 50-digit
-arithmetic, a minute or so).  The output is committed, so building the
+arithmetic, a few seconds).  The output is committed, so building the
 library does not need mpmath.
 
 Two regimes, split at x = J0_XA:
