@@ -172,4 +172,21 @@ __host__ __device__ __forceinline__ cd cexp(cd z) {
 
 __host__ __device__ __forceinline__ double cabs2(cd a) { return a.r * a.r + a.i * a.i; }
 
+// A value that every lane of the wave holds, moved to scalar registers
+// (v_readfirstlane); the caller guarantees it is the same in every lane.
+__host__ __device__ __forceinline__ double wave_uniform(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const long long b = __double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+#else
+  return x;
+#endif
+}
+
+__host__ __device__ __forceinline__ cd wave_uniform(cd z) {
+  return cd{wave_uniform(z.r), wave_uniform(z.i)};
+}
+
 }  // namespace kh

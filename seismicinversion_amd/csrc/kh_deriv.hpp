@@ -76,6 +76,21 @@ __host__ __device__ __forceinline__ void phases_of(const LZ& lb, cd om, Z& ea, Z
   }
 }
 
+// The primal parts of an interface's coefficients, which are the same for
+// every lane of the wave (one (direction, slowness) per workgroup), moved
+// to scalar registers; tangent parts stay per lane.
+__host__ __device__ __forceinline__ cd primal_uniform(const cd& z) { return wave_uniform(z); }
+__host__ __device__ __forceinline__ dcd primal_uniform(const dcd& z) {
+  return dcd{wave_uniform(z.v), z.d};
+}
+template <class Z>
+__host__ __device__ __forceinline__ IfaceZ<Z> primal_uniform(const IfaceZ<Z>& f) {
+  return IfaceZ<Z>{primal_uniform(f.rd00), primal_uniform(f.rd01), primal_uniform(f.rd11),
+                   primal_uniform(f.ru00), primal_uniform(f.ru01), primal_uniform(f.ru11),
+                   primal_uniform(f.td00), primal_uniform(f.td01), primal_uniform(f.td10),
+                   primal_uniform(f.td11)};
+}
+
 // Storage: st(e) returns a reference to Z-slot e of this lane.
 // Sink: sink.iface(i, IfaceZ<Z>), sink.layer(l, LayerCot<Z>).
 // G_of(R) -> cotangent of R.  Returns R.
@@ -170,7 +185,7 @@ __host__ __device__ inline Z lane_adjoint(const LZ* __restrict__ L,
       const Z ea = buf(k, 4), eb = buf(k, 5);
       M2<Z> rb;
       Z eab, ebb;
-      step_rev(Sin, ea, eb, F[i], sb, rb, fbar, eab, ebb);
+      step_rev(Sin, ea, eb, primal_uniform(F[i]), sb, rb, fbar, eab, ebb);
       sink.iface(i, fbar);
       LayerCot<Z> c{zero_z<Z>(), zero_z<Z>(), zero_z<Z>()};
       const LZ& lb = L[i + 1];

@@ -94,25 +94,10 @@ __host__ __device__ __forceinline__ void kennett_step(cd& r00, cd& r01, cd& r10,
   kennett_step_e2(r00, r01, r10, r11, ea * ea, eb * eb, ea * eb, f);
 }
 
-// A value that every lane of the wave holds, moved to scalar registers:
-// the step's FMAs then take it as their one scalar operand instead of
+// The interface coefficients of a layer in scalar registers (wave_uniform):
+// the step's FMAs then take them as their one scalar operand instead of
 // holding 20 doubles per layer in vector registers (which is what lets the
 // 8-frequency sweep run two waves per SIMD).
-__host__ __device__ __forceinline__ double wave_uniform(double x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  const long long b = __double_as_longlong(x);
-  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(b >> 32));
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-#else
-  return x;
-#endif
-}
-
-__host__ __device__ __forceinline__ cd wave_uniform(cd z) {
-  return cd{wave_uniform(z.r), wave_uniform(z.i)};
-}
-
 __host__ __device__ __forceinline__ Iface wave_uniform(const Iface& f) {
   return Iface{wave_uniform(f.rd00), wave_uniform(f.rd01), wave_uniform(f.rd11),
                wave_uniform(f.ru00), wave_uniform(f.ru01), wave_uniform(f.ru11),
