@@ -135,3 +135,24 @@ def test_tile_runs_for_a_thin_batch(monkeypatch, tail, tables):
                                  s.dw, s.gamma, s.nw, s.free_surface, block=64,
                                  omega_per_item=batch.omega_per_lane(s.nw, 64, 8))
         assert torch.equal(torch.nan_to_num(R), torch.nan_to_num(Rp)), f"nl={s.n_layers}"
+
+
+@pytest.mark.parametrize("tables", [False, True])
+def test_deepest_and_shallowest_models_together(tables):
+    """A model at the layer limit next to a two-interface one, both ways of
+    staging the tables: each equals its plain launch bit for bit."""
+    nl_max = 240  # KH_MAX_LAYERS
+    specs = [batch.ModelSpec(model=seeded_model(nl, 7 + nl), p=np.linspace(0.02, 0.7, npp),
+                             nw=nw, dw=2 * np.pi / 64.0, k0=1.0)
+             for nl, npp, nw in ((nl_max, 40, 1024), (3, 300, 256))]
+    outs = batch.reflectivity_multi(specs, block=64, omega_per_item=8, tables=tables)
+    for s, R in zip(specs, outs):
+        Rp = engine.forward_grid(engine.layers_tensor(s.model), torch.from_numpy(s.p), s.k0, s.dw,
+                                 s.gamma, s.nw, False, block=64,
+                                 omega_per_item=batch.omega_per_lane(s.nw, 64, 8))
+        assert torch.equal(R, Rp), f"nl={s.n_layers}"
+        rows = np.array([0, len(s.p) - 1])
+        w = (s.k0 + np.arange(s.nw)) * s.dw
+        O = K.reflectivity_batch(s.model, s.p[rows], w, False)
+        assert_rows_close(R.cpu().numpy()[rows], O, row_tolerance(s.model, s.p[rows], w, False),
+                          f"nl={s.n_layers}")
