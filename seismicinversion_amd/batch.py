@@ -22,8 +22,8 @@ import torch
 from . import _abi, autotune, engine
 from .layer_model import LayerModel, pack_layers
 
-__all__ = ["ModelSpec", "MultiPlan", "plan_multi", "reflectivity_multi", "omega_per_lane",
-           "DESC_DTYPE"]
+__all__ = ["ModelSpec", "MultiPlan", "plan_multi", "describe", "reflectivity_multi",
+           "omega_per_lane", "DESC_DTYPE"]
 
 # numpy mirror of kh_model_desc (include/kennett_hip.h)
 DESC_DTYPE = np.dtype([("layer_off", "<i8"), ("p_off", "<i8"), ("r_off", "<i8"),
@@ -161,14 +161,17 @@ def omega_per_lane(nw: int, block: int, wpt: int) -> int:
 TABLES_MIN_UPDATES = 2e8
 
 
-def plan_multi(specs: list[ModelSpec], block: int, wpt: int, dev=None,
-               non_blocking: bool = False, tables: bool | None = None) -> MultiPlan:
-    """Pack specs into device buffers and descriptors for one launch shape.
-    With ``non_blocking`` the host->device copies go through pinned memory and
-    are only queued on the current stream (the host does not wait).
-    ``tables`` picks the staged-table path (default: batches of at least
-    TABLES_MIN_UPDATES updates)."""
-    dev = _abi.device(dev)
+def describe(specs: list[ModelSpec], block: int, wpt: int, slots: int):
+    """Host side of a persistent launch: the descriptors in queue order, the
+    queue order (slot -> input index), the work item count and the table
+    record count, for `slots` resident workgroups.
+
+    Each model gets its frequencies per lane (omega_per_lane), its tile
+    count and, when a whole row would cost more than TAIL_FRACTION of the
+    work per workgroup, runs of tiles per item.  Models are queued longest
+    item first (then by the model's total), so the smallest items are left
+    for the drain at the end of the launch; item_off and table_off are the
+    prefix sums of the items and of np * nl in that order."""
     tabs = [s.table() for s in specs]
     for t in tabs:
         if t.shape[0] < 2:
@@ -179,14 +182,8 @@ def plan_multi(specs: list[ModelSpec], block: int, wpt: int, dev=None,
     # HALF_TILE_COST of a full one)
     tile_cost = [(HALF_TILE_COST if f < wpt else 1.0) * (t.shape[0] - 1)
                  for f, t in zip(lanes, tabs)]
-    max_nl = max(t.shape[0] for t in tabs) if tabs else 2
-    if tables is None:
-        tables = sum(s.updates() for s in specs) >= TABLES_MIN_UPDATES
-    slots = resident_workgroups(block, wpt, max_nl, tables, dev) if specs else 1
     cap = TAIL_FRACTION * sum(len(s.p) * n * c for s, n, c in zip(specs, ntiles, tile_cost)) / slots
     tpi = [n if n * c <= cap else max(1, int(cap // c)) for n, c in zip(ntiles, tile_cost)]
-    # Longest items first (then the model's total): the smallest items are
-    # left for the drain at the end of the launch.
     item_cost = [k * c for k, c in zip(tpi, tile_cost)]
     total_cost = [len(s.p) * n * c for s, n, c in zip(specs, ntiles, tile_cost)]
     order = sorted(range(len(specs)), key=lambda i: (-item_cost[i], -total_cost[i]))
@@ -205,12 +202,30 @@ def plan_multi(specs: list[ModelSpec], block: int, wpt: int, dev=None,
                       0, records)
         item += len(s.p) * -(-ntiles[i] // tpi[i])
         records += len(s.p) * tabs[i].shape[0]
+    return desc, order, item, records
+
+
+def plan_multi(specs: list[ModelSpec], block: int, wpt: int, dev=None,
+               non_blocking: bool = False, tables: bool | None = None) -> MultiPlan:
+    """Pack specs into device buffers and descriptors for one launch shape.
+    With ``non_blocking`` the host->device copies go through pinned memory and
+    are only queued on the current stream (the host does not wait).
+    ``tables`` picks the staged-table path (default: batches of at least
+    TABLES_MIN_UPDATES updates)."""
+    dev = _abi.device(dev)
+    tabs = [s.table() for s in specs]
+    max_nl = max(t.shape[0] for t in tabs) if tabs else 2
+    if tables is None:
+        tables = sum(s.updates() for s in specs) >= TABLES_MIN_UPDATES
+    slots = resident_workgroups(block, wpt, max_nl, tables, dev) if specs else 1
+    desc, order, items, records = describe(specs, block, wpt, slots)
+    r_off = np.cumsum([0] + [len(s.p) * s.nw for s in specs])
     layers = _to_device(np.concatenate(tabs, axis=0), dev, non_blocking)
     p = _to_device(np.concatenate([np.asarray(s.p, np.float64) for s in specs])
                    if specs else np.zeros(0), dev, non_blocking)
     R = torch.empty(int(r_off[-1]), dtype=torch.complex128, device=dev)
     return MultiPlan(desc=_to_device(desc.view(np.uint8), dev, non_blocking), layers=layers, p=p, R=R,
-                     r_off=r_off, shapes=[(len(s.p), s.nw) for s in specs], total_items=item,
+                     r_off=r_off, shapes=[(len(s.p), s.nw) for s in specs], total_items=items,
                      max_nl=max_nl, block=block, wpt=wpt,
                      # both are reset by the launch itself (no separate fill kernels)
                      counter=torch.empty(1, dtype=torch.int64, device=dev),
