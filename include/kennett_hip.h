@@ -155,7 +155,8 @@ int kh_multi_workgroups(int block, int omega_per_item, int max_nl, int tables, i
 /* Reflectivity of nmodels models in one persistent launch: workgroups take
  * (model, slowness, run of tiles) items from a device counter until all
  * total_items are done.  With `tables` (caller-owned device memory of at
- * least kh_multi_table_bytes(records) bytes; records = the sum of np * nl,
+ * least kh_multi_table_bytes(records) bytes, aligned to 256 bytes, one
+ * 256-byte record per (model, slowness, layer); records = the sum of np * nl,
  * each model's records starting at its table_off), every (model, slowness,
  * layer) record of the layer, its phase step and the interface below it is
  * computed there first and the sweep reads it with scalar loads; with

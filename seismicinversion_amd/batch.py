@@ -149,7 +149,7 @@ def omega_per_lane(nw: int, block: int, wpt: int) -> int:
 
 
 # Batches of at least this many updates stage their tables with a kernel of
-# their own (a workspace of 264 B per (model, slowness, layer)) and sweep
+# their own (a workspace of 256 B per (model, slowness, layer)) and sweep
 # without LDS; smaller ones stage them in LDS within the one sweep kernel.
 # The served 400-request stream (ten batches of about 9e8 updates) runs
 # 3.7 % faster on staged tables; a small batch launched while other work

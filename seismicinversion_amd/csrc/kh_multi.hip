@@ -37,36 +37,37 @@ namespace kh {
 static_assert(sizeof(kh_model_desc) == 96, "kh_model_desc layout");
 
 typedef const KH_CONST kh_model_desc* CDesc;
-typedef const KH_CONST LayerPt* CLayer;
-typedef const KH_CONST Iface* CIface;
-typedef const KH_CONST LayerStep* CStep;
 
-// One table record per (model, slowness, layer): the layer at that
-// slowness, its phase step and the interface below it (unused for the last
-// layer).  The three arrays are carved from one workspace.
-struct Tables {
-  LayerPt* layer;
-  Iface* iface;
-  LayerStep* step;
+// One staged record per (model, slowness, layer), 256 bytes (four 64-byte
+// lines that the sweep reads together): the interface below the layer
+// (unused for the last layer), what the sweep reads of the layer (vertical
+// slownesses, thickness, half-space flag) and its phase step.
+struct TableRec {
+  Iface f;
+  cd eta, neta;
+  double h;
+  int halfspace, pad_;
+  LayerStep s;
+  double pad2_[2];
+};
+static_assert(sizeof(TableRec) == 256, "TableRec layout");
+typedef const KH_CONST TableRec* CRec;
+
+// The record array seen as the sweep's three tables.
+struct RecLayers {
+  CRec r;
+  __device__ __forceinline__ const KH_CONST TableRec& operator[](int i) const { return r[i]; }
+};
+struct RecIfaces {
+  CRec r;
+  __device__ __forceinline__ const KH_CONST Iface& operator[](int i) const { return r[i].f; }
+};
+struct RecSteps {
+  CRec r;
+  __device__ __forceinline__ const KH_CONST LayerStep& operator[](int i) const { return r[i].s; }
 };
 
-static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-
-static Tables carve(void* ws, long long records) {
-  char* b = static_cast<char*>(ws);
-  Tables t;
-  t.layer = reinterpret_cast<LayerPt*>(b);
-  b += align256((size_t)records * sizeof(LayerPt));
-  t.iface = reinterpret_cast<Iface*>(b);
-  b += align256((size_t)records * sizeof(Iface));
-  t.step = reinterpret_cast<LayerStep*>(b);
-  return t;
-}
-
-static size_t table_bytes(long long records) {
-  return align256((size_t)records * sizeof(LayerPt)) + align256((size_t)records * sizeof(Iface)) +
-         (size_t)records * sizeof(LayerStep);
-}
+static size_t table_bytes(long long records) { return (size_t)records * sizeof(TableRec); }
 
 // Resets the work counter and, when requested, the per-model counts of
 // non-finite outputs.
@@ -105,7 +106,7 @@ __device__ __forceinline__ long long wave_uniform_i64(long long x) {
 __global__ void __launch_bounds__(256) stage_tables_kernel(
     const kh_model_desc* __restrict__ desc, int nmodels, long long records,
     const double* __restrict__ layers_all, const double* __restrict__ p_all, int block,
-    LayerPt* __restrict__ tl, Iface* __restrict__ tf, LayerStep* __restrict__ ts) {
+    TableRec* __restrict__ T) {
   const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= records) return;
   const int m = find_model(desc, nmodels, r, [](const kh_model_desc& d) { return d.table_off; });
@@ -116,9 +117,14 @@ __global__ void __launch_bounds__(256) stage_tables_kernel(
   const double pv = p_all[md.p_off + ip];
   const double* lay = layers_all + md.layer_off * NCOL;
   const LayerPt a = layer_at(lay + (size_t)l * NCOL, l, pv);
-  tl[r] = a;
-  ts[r] = layer_step(a, (double)block * md.dw);
-  if (l < md.nl - 1) tf[r] = interface_at(l, pv, a, layer_at(lay + (size_t)(l + 1) * NCOL, l + 1, pv));
+  TableRec& t = T[r];
+  t.eta = a.eta;
+  t.neta = a.neta;
+  t.h = a.h;
+  t.halfspace = a.halfspace;
+  t.pad_ = 0;
+  t.s = layer_step(a, (double)block * md.dw);
+  if (l < md.nl - 1) t.f = interface_at(l, pv, a, layer_at(lay + (size_t)(l + 1) * NCOL, l + 1, pv));
 }
 
 // Frequency tiles [t0, t1) of one (model, slowness): each lane takes WL
@@ -154,9 +160,8 @@ __device__ __forceinline__ int sweep_item(const kh_model_desc& md, int t0, int t
 template <int W>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) multi_kernel(
     const kh_model_desc* __restrict__ desc_g, int nmodels, long long total,
-    const LayerPt* __restrict__ tl, const Iface* __restrict__ tf, const LayerStep* __restrict__ ts,
-    cd* __restrict__ R_all, unsigned long long* __restrict__ counter,
-    int* __restrict__ nonfinite) {
+    const TableRec* __restrict__ T, cd* __restrict__ R_all,
+    unsigned long long* __restrict__ counter, int* __restrict__ nonfinite) {
   const CDesc desc = (CDesc)desc_g;
   __shared__ long long s_item;
   for (;;) {
@@ -174,10 +179,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) m
     const int ip = (int)(local / nch);
     const int t0 = (int)(local - (long long)ip * nch) * tpi;
     const int t1 = min(md.ntiles, t0 + tpi);
-    const long long rec = md.table_off + (long long)ip * md.nl;
-    const CLayer L = (CLayer)tl + rec;
-    const CIface F = (CIface)tf + rec;
-    const CStep S = (CStep)ts + rec;
+    const CRec rec = (CRec)T + md.table_off + (long long)ip * md.nl;
+    const RecLayers L{rec};
+    const RecIfaces F{rec};
+    const RecSteps S{rec};
     cd* Rrow = R_all + md.r_off + (long long)ip * md.nw;
     const int bad = (W > 1 && md.half_tile != 0)
                         ? sweep_item<(W > 1 ? W / 2 : 1), false>(md, t0, t1, L, F, S, Rrow)
@@ -254,10 +259,10 @@ struct MultiArgs {
 };
 
 template <int W>
-static int launch_multi(const MultiArgs& a, const Tables* t) {
+static int launch_multi(const MultiArgs& a, const TableRec* t) {
   if (t != nullptr) {
     hipLaunchKernelGGL((multi_kernel<W>), dim3(a.wgs), dim3(a.block), 0, a.st, a.desc, a.nmodels,
-                       a.total, t->layer, t->iface, t->step, a.R, a.counter, a.nonfinite);
+                       a.total, t, a.R, a.counter, a.nonfinite);
   } else {
     hipLaunchKernelGGL((multi_lds_kernel<W>), dim3(a.wgs), dim3(a.block),
                        multi_lds_bytes(a.max_nl), a.st, a.desc, a.nmodels, a.total, a.layers_all,
@@ -341,6 +346,8 @@ extern "C" int kh_forward_multi(const kh_model_desc* desc, int nmodels, long lon
              "kh_forward_multi: NULL pointer");
   if (tables != nullptr) {
     KH_REQUIRE(records > 0, KH_ERR_INVALID_ARGUMENT, "kh_forward_multi: no table records");
+    KH_REQUIRE((reinterpret_cast<uintptr_t>(tables) & 255) == 0, KH_ERR_INVALID_ARGUMENT,
+               "kh_forward_multi: table workspace not aligned to 256 bytes");
     KH_REQUIRE(table_bytes_given >= table_bytes(records), KH_ERR_OUT_OF_MEMORY,
                "kh_forward_multi: table workspace of %zu bytes, %zu needed", table_bytes_given,
                table_bytes(records));
@@ -361,21 +368,18 @@ extern "C" int kh_forward_multi(const kh_model_desc* desc, int nmodels, long lon
   hipLaunchKernelGGL(zero_counter_kernel, dim3((nmodels + 255) / 256), dim3(256), 0, st, c,
                      nonfinite, nmodels);
   KH_TRY(check_launch("kh_forward_multi"));
-  Tables t{};
+  TableRec* t = use_tables ? static_cast<TableRec*>(tables) : nullptr;
   if (use_tables) {
-    t = carve(tables, records);
     hipLaunchKernelGGL(stage_tables_kernel, dim3((unsigned)((records + 255) / 256)), dim3(256), 0,
-                       st, desc, nmodels, records, layers_all, p_all, block, t.layer, t.iface,
-                       t.step);
+                       st, desc, nmodels, records, layers_all, p_all, block, t);
     KH_TRY(check_launch("kh_forward_multi"));
   }
   const MultiArgs a{desc, nmodels, total_items, layers_all, p_all, reinterpret_cast<cd*>(R_all),
                     block, max_nl, c, nonfinite, wgs, st};
-  const Tables* tp = use_tables ? &t : nullptr;
   switch (omega_per_item) {
-    case 1: return launch_multi<1>(a, tp);
-    case 2: return launch_multi<2>(a, tp);
-    case 4: return launch_multi<4>(a, tp);
-    default: return launch_multi<8>(a, tp);
+    case 1: return launch_multi<1>(a, t);
+    case 2: return launch_multi<2>(a, t);
+    case 4: return launch_multi<4>(a, t);
+    default: return launch_multi<8>(a, t);
   }
 }

@@ -173,8 +173,8 @@ __host__ __device__ __forceinline__ void sweep_rrd00(LP L, FP F, SP steps, int n
 }
 
 // Water-layer phase and optional free surface (kennett_reflectivity_gpu.py:259-272).
-template <int W, bool GRID>
-__host__ __device__ __forceinline__ void surface_pp(const LayerPt& water, const LayerStep& step,
+template <int W, bool GRID, class LT>
+__host__ __device__ __forceinline__ void surface_pp(const LT& water, const LayerStep& step,
                                                     const cd (&om)[W], cd (&r)[W],
                                                     int free_surface) {
   cd ea0[W];
@@ -205,8 +205,7 @@ __host__ __device__ __forceinline__ void sweep_lanes(LP L, FP F, SP steps, int n
   sweep_rrd00<W, GRID, UNIFORM>(L, F, steps, nl, om, out);
   LayerStep s0{{1.0, 0.0}, {1.0, 0.0}};
   if (GRID) s0 = steps[0];
-  const LayerPt water = L[0];
-  surface_pp<W, GRID>(water, s0, om, out, free_surface);
+  surface_pp<W, GRID>(L[0], s0, om, out, free_surface);
 }
 
 // Single frequency, generic omega (host tests and simple callers; the

@@ -194,9 +194,9 @@ class DeviceBackend:
     @staticmethod
     def forward_bytes(spec: ModelSpec) -> int:
         # output, slownesses, layer table, the persistent launch's per-(slowness,
-        # layer) records (264 B each), descriptor and NaN/Inf count
+        # layer) records (256 B each), descriptor and NaN/Inf count
         n_p, nl = len(spec.p), spec.n_layers
-        return 16 * n_p * spec.nw + 8 * n_p + 48 * nl + 264 * n_p * nl + 100
+        return 16 * n_p * spec.nw + 8 * n_p + 48 * nl + 256 * n_p * nl + 100
 
     @staticmethod
     def invert_bytes(spec: InvertSpec) -> int:

@@ -93,21 +93,24 @@ def test_multi_tile_and_item_counts(lib):
 def test_multi_table_workspace_is_checked_before_any_launch(lib):
     """With a table workspace, kh_forward_multi stages its per-(slowness,
     layer) tables there; one smaller than kh_multi_table_bytes(records) is
-    refused with KH_ERR_OUT_OF_MEMORY before anything is launched."""
+    refused with KH_ERR_OUT_OF_MEMORY, and one not aligned to 256 bytes
+    with KH_ERR_INVALID_ARGUMENT, before anything is launched."""
     lib.kh_multi_table_bytes.restype = ctypes.c_size_t
     lib.kh_multi_table_bytes.argtypes = [ctypes.c_longlong]
     lib.kh_last_error.restype = ctypes.c_char_p
     assert lib.kh_multi_table_bytes(0) == 0
     need = lib.kh_multi_table_bytes(1000)
-    assert need >= 1000 * (72 + 160 + 32)
+    assert need == 1000 * 256  # one 256-byte record per (model, slowness, layer)
     vp = ctypes.c_void_p
     f = lib.kh_forward_multi
     f.argtypes = [vp, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, vp, vp, vp, ctypes.c_int,
                   ctypes.c_int, ctypes.c_int, ctypes.c_longlong, vp, ctypes.c_size_t, vp, vp, vp]
     args = [vp(16), 2, 10, 20, vp(16), vp(16), vp(16), 64, 8, 0]
-    rc = f(*args, 1000, vp(16), need - 1, vp(16), None, None)
+    rc = f(*args, 1000, vp(256), need - 1, vp(16), None, None)
     assert rc == 2 and b"table workspace" in lib.kh_last_error()
-    rc = f(*args, 1000, vp(16), need, None, None, None)  # no counter
+    rc = f(*args, 1000, vp(16), need, vp(16), None, None)
+    assert rc == 1 and b"aligned" in lib.kh_last_error()
+    rc = f(*args, 1000, vp(256), need, None, None, None)  # no counter
     assert rc == 1 and b"NULL" in lib.kh_last_error()
-    rc = f(*args[:7], 96, 8, 0, 1000, vp(16), need, vp(16), None, None)
+    rc = f(*args[:7], 96, 8, 0, 1000, vp(256), need, vp(16), None, None)
     assert rc == 1 and b"block" in lib.kh_last_error()
