@@ -65,6 +65,8 @@ def parse():
                     help="processes of the all-core CPU baseline (the box's share is 16 cores)")
     ap.add_argument("--warmup-seconds", type=float, default=2.0,
                     help="untimed forward launches before anything is timed (clock ramp)")
+    ap.add_argument("--section", choices=("all", "lm"), default="all",
+                    help="'lm': only the 20-layer Newton-LM figures (for profiling its kernels)")
     ap.add_argument("--traffic-json", default=os.path.join(HERE, "profiles", "traffic_multi_served.json"),
                     help="HBM bytes per launch of the multi-model kernel from rocprofv3 --pmc")
     return ap.parse_args()
@@ -374,6 +376,13 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.section == "lm":
+        dev = torch.device("cuda", 0)
+        torch.cuda.set_device(dev)
+        warm_clocks(torch, dev, args.warmup_seconds)
+        print(json.dumps({"section": "lm", "lm_20_layers": lm_figures(torch, dev)}))
+        return
 
     from seismicinversion_amd import engine
     from seismicinversion_amd.kennett_reflectivity_gpu import kennett_reflectivity_batch

@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from . import _abi, autotune, engine
-from .layer_model import LayerModel, pack_layers
+from .layer_model import LayerModel, check_table, pack_layers
 
 __all__ = ["ModelSpec", "MultiPlan", "plan_multi", "describe", "reflectivity_multi",
            "omega_per_lane", "DESC_DTYPE"]
@@ -48,15 +48,25 @@ class ModelSpec:
     free_surface: bool = False
 
     def table(self) -> np.ndarray:
-        """The validated (nl, 6) layer table (computed once)."""
+        """The validated (nl, 6) layer table (computed once; ``ValueError``
+        for a model the kernels cannot take, see ``layer_model.check_table``)."""
         tab = self.__dict__.get("_table")
         if tab is None:
             if isinstance(self.model, LayerModel):
                 tab = pack_layers(self.model)
             else:
-                tab = np.ascontiguousarray(self.model, dtype=np.float64)
+                tab = check_table(self.model)
             self.__dict__["_table"] = tab
         return tab
+
+    def check(self) -> None:
+        """Raises ValueError unless the spec can be launched: a valid layer
+        table, at least one slowness (a 1-D array) and one frequency."""
+        self.table()
+        if np.ndim(self.p) != 1 or len(self.p) == 0:
+            raise ValueError("a forward request needs a 1-D array of at least one slowness")
+        if int(self.nw) <= 0:
+            raise ValueError(f"a forward request needs at least one frequency, got nw={self.nw}")
 
     @property
     def n_layers(self) -> int:
@@ -172,10 +182,9 @@ def describe(specs: list[ModelSpec], block: int, wpt: int, slots: int):
     item first (then by the model's total), so the smallest items are left
     for the drain at the end of the launch; item_off and table_off are the
     prefix sums of the items and of np * nl in that order."""
+    for s in specs:
+        s.check()
     tabs = [s.table() for s in specs]
-    for t in tabs:
-        if t.shape[0] < 2:
-            raise ValueError("every model needs at least 2 layers")
     lanes = [omega_per_lane(s.nw, block, wpt) for s in specs]
     ntiles = [-(-s.nw // (block * f)) for s, f in zip(specs, lanes)]
     # cost of one tile of a row in tile steps (a half-width tile costs

@@ -16,11 +16,17 @@ from dataclasses import dataclass
 
 import numpy as np
 
-__all__ = ["LayerModel", "complex_slowness", "vertical_slowness", "pack_layers"]
+__all__ = ["LayerModel", "complex_slowness", "vertical_slowness", "pack_layers", "check_table",
+           "MAX_LAYERS", "MAX_LAYERS_DERIV"]
 
 # Column order of the packed (n_layers, 6) float64 layer table that the
 # C-ABI takes (include/kennett_hip.h).
 LAYER_COLUMNS = ("alpha", "beta", "rho", "thickness", "Q_alpha", "Q_beta")
+
+# Layer caps of the kernels (KH_MAX_LAYERS and KH_MAX_LAYERS_DERIV in
+# include/kennett_hip.h): forward launches, derivative launches.
+MAX_LAYERS = 240
+MAX_LAYERS_DERIV = 120
 
 
 def complex_slowness(velocity: float, Q: float) -> complex:
@@ -98,18 +104,40 @@ class LayerModel:
         return out
 
 
-def pack_layers(model: LayerModel) -> np.ndarray:
-    """(n_layers, 6) float64 table [alpha, beta, rho, h, Q_alpha, Q_beta] per row.
-
-    Raises ``ValueError`` for inputs the kernels cannot take: fewer than two
-    layers, a non-positive or non-finite Q, or a NaN anywhere."""
-    tab = np.stack([getattr(model, c) for c in LAYER_COLUMNS], axis=1).astype(np.float64)
-    if tab.shape[0] < 2:
+def check_table(table, max_layers: int = MAX_LAYERS) -> np.ndarray:
+    """The contiguous (n_layers, 6) float64 layer table of ``table`` (rows
+    [alpha, beta, rho, h, Q_alpha, Q_beta]), after the checks every kernel
+    relies on.  Raises ``ValueError`` for a table of another shape, fewer
+    than two or more than ``max_layers`` layers, a NaN anywhere, or a value
+    LayerModel or the kernels reject (alpha <= 0, beta < 0, rho <= 0,
+    Q_alpha <= 0, Q_beta <= 0 for a solid layer)."""
+    tab = np.ascontiguousarray(np.asarray(table, dtype=np.float64))
+    if tab.ndim != 2 or tab.shape[1] != len(LAYER_COLUMNS):
+        raise ValueError(f"layer table must be (n_layers, 6), got {tab.shape}")
+    nl = tab.shape[0]
+    if nl < 2:
         raise ValueError("model needs at least 2 layers (water + half-space)")
+    if nl > max_layers:
+        raise ValueError(f"model has {nl} layers; the kernels take at most {max_layers}")
     if np.isnan(tab).any():
         raise ValueError("model contains NaN")
-    if np.any(model.Q_alpha <= 0):
+    alpha, beta, rho, _, q_a, q_b = tab.T
+    if np.any(alpha <= 0):
+        raise ValueError("All alpha must be positive")
+    if np.any(beta < 0):
+        raise ValueError("All beta must be non-negative")
+    if np.any(rho <= 0):
+        raise ValueError("All rho must be positive")
+    if np.any(q_a <= 0) or np.any((beta > 0) & (q_b <= 0)):
         raise ValueError("Q must be positive")
-    if np.any((model.beta > 0) & (model.Q_beta <= 0)):
-        raise ValueError("Q must be positive")
-    return np.ascontiguousarray(tab)
+    return tab
+
+
+def pack_layers(model: LayerModel, max_layers: int = MAX_LAYERS) -> np.ndarray:
+    """(n_layers, 6) float64 table [alpha, beta, rho, h, Q_alpha, Q_beta] per row.
+
+    Raises ``ValueError`` for inputs the kernels cannot take (``check_table``):
+    fewer than two or more than ``max_layers`` layers, a non-positive Q, or a
+    NaN anywhere."""
+    return check_table(np.stack([getattr(model, c) for c in LAYER_COLUMNS], axis=1),
+                       max_layers)

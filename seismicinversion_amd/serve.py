@@ -233,6 +233,13 @@ class DeviceBackend:
                             torch.cuda.Event(enable_timing=True))
                     t_ev[0].record(self.compute)
                 batch.launch(plan)
+                # the plan's buffers were allocated on the copy stream and are
+                # written here: if anything below raises, the allocator must
+                # not hand them out again before this kernel has finished
+                for t in (plan.R, plan.desc, plan.layers, plan.p, plan.counter, plan.nonfinite,
+                          plan.tables):
+                    if t is not None:
+                        t.record_stream(self.compute)
                 if t_ev is not None:
                     t_ev[1].record(self.compute)
                 kernel_done = torch.cuda.Event()
@@ -515,16 +522,19 @@ class KennettServer:
         """Raises ValueError for an unusable request.  A forward spec's layer
         table is validated once and cached on the spec (ModelSpec.table), so
         resubmitted specs cost nothing here."""
+        from .layer_model import MAX_LAYERS_DERIV, check_table, pack_layers
+
         spec = req.spec
         if req.kind == "forward":
-            if not isinstance(spec.model, np.ndarray):
-                spec.table()
-            if spec.nw <= 0 or len(spec.p) == 0:
-                raise ValueError("forward request needs at least one slowness and one frequency")
-        elif not isinstance(spec.true_model, np.ndarray):
-            from .layer_model import pack_layers
-
-            pack_layers(spec.true_model)
+            spec.check()  # table (shape, values, layer cap), slownesses, frequencies
+        else:
+            m = spec.true_model
+            if isinstance(m, np.ndarray):
+                check_table(m, MAX_LAYERS_DERIV)
+            else:
+                pack_layers(m, MAX_LAYERS_DERIV)
+            if len(spec.p) == 0 or spec.nfreq <= 0 or spec.iterations < 0:
+                raise ValueError("an inversion needs at least one slowness and one frequency")
 
     def _admit(self, fwd_room: float = float("inf")) -> list[Request]:
         """Arrivals join one of two FIFOs; requests leave them admitted.

@@ -169,3 +169,39 @@ def test_submit_many_batches_and_fails_bad_requests_alone():
         w = (s.k0 + np.arange(s.nw)) * s.dw
         np.testing.assert_array_equal(R, K.reflectivity_batch(s.model, s.p, w, s.free_surface))
     assert srv.outstanding == 0
+
+
+def test_oversized_and_malformed_models_fail_alone():
+    """A 241-layer model (above the kernels' 240-layer cap), an (nl, 5)
+    table and an empty frequency grid each fail with ValueError at
+    admission; the good requests submitted around them are batched and
+    served."""
+    deep = seeded_model(241, 4)
+    bad_specs = [ModelSpec(model=deep, p=np.array([0.1]), nw=8),
+                 ModelSpec(model=np.ones((4, 5)), p=np.array([0.1]), nw=8),
+                 ModelSpec(model=seeded_model(5, 2), p=np.array([0.1]), nw=0)]
+    good = [fwd(4 + i, 3, 8, seed=i) for i in range(4)]
+    be = HostBackend(delay=0.02)
+    with KennettServer(backend=be, hbm_budget=1 << 30) as srv:
+        futs = srv.submit_forward_many([good[0], bad_specs[0], good[1], bad_specs[1],
+                                        good[2], bad_specs[2], good[3]])
+        bad_futs = [futs[1], futs[3], futs[5]]
+        for f in bad_futs:
+            with pytest.raises(ValueError):
+                f.result(timeout=30)
+        for f, s in zip([futs[0], futs[2], futs[4], futs[6]], good):
+            R = f.result(timeout=30)
+            w = (s.k0 + np.arange(s.nw)) * s.dw
+            np.testing.assert_array_equal(R, K.reflectivity_batch(s.model, s.p, w, False))
+    assert srv.stats()["failed"] == 3
+    assert sum(be.launches) == 4
+
+
+def test_inversion_above_the_derivative_cap_fails_alone():
+    be = HostBackend()
+    with KennettServer(backend=be, hbm_budget=1 << 30) as srv:
+        f_bad = srv.submit_invert(InvertSpec(true_model=seeded_model(121, 1), p=np.array([0.1])))
+        f_ok = srv.submit_invert(InvertSpec(true_model=seeded_model(5, 1), p=np.array([0.1])))
+        with pytest.raises(ValueError):
+            f_bad.result(timeout=30)
+        assert f_ok.result(timeout=30) == {"iterations": 5}
