@@ -11,7 +11,7 @@ import numpy as np
 import torch
 
 from . import _abi
-from .layer_model import LayerModel, pack_layers
+from .layer_model import LayerModel, check_table, pack_layers
 
 __all__ = [
     "layers_tensor",
@@ -36,7 +36,7 @@ def layers_tensor(model: LayerModel | np.ndarray | torch.Tensor, dev=None) -> to
     if isinstance(model, LayerModel):
         tab = torch.from_numpy(pack_layers(model))
     elif isinstance(model, np.ndarray):
-        tab = torch.from_numpy(np.ascontiguousarray(model, dtype=np.float64))
+        tab = torch.from_numpy(check_table(model, values=False))
     else:
         check_precision(model, "layer table")
         tab = model.detach().to(torch.float64)

@@ -104,13 +104,15 @@ class LayerModel:
         return out
 
 
-def check_table(table, max_layers: int = MAX_LAYERS) -> np.ndarray:
+def check_table(table, max_layers: int = MAX_LAYERS, values: bool = True) -> np.ndarray:
     """The contiguous (n_layers, 6) float64 layer table of ``table`` (rows
     [alpha, beta, rho, h, Q_alpha, Q_beta]), after the checks every kernel
-    relies on.  Raises ``ValueError`` for a table of another shape, fewer
-    than two or more than ``max_layers`` layers, a NaN anywhere, or a value
-    LayerModel or the kernels reject (alpha <= 0, beta < 0, rho <= 0,
-    Q_alpha <= 0, Q_beta <= 0 for a solid layer)."""
+    relies on.  Raises ``ValueError`` for a table of another shape or with
+    fewer than two or more than ``max_layers`` layers (the kernels index
+    every row as six doubles and stage at most that many layers) and, with
+    ``values``, for a NaN anywhere or a value LayerModel or the kernels
+    reject (alpha <= 0, beta < 0, rho <= 0, Q_alpha <= 0, Q_beta <= 0 for a
+    solid layer)."""
     tab = np.ascontiguousarray(np.asarray(table, dtype=np.float64))
     if tab.ndim != 2 or tab.shape[1] != len(LAYER_COLUMNS):
         raise ValueError(f"layer table must be (n_layers, 6), got {tab.shape}")
@@ -118,7 +120,10 @@ def check_table(table, max_layers: int = MAX_LAYERS) -> np.ndarray:
     if nl < 2:
         raise ValueError("model needs at least 2 layers (water + half-space)")
     if nl > max_layers:
-        raise ValueError(f"model has {nl} layers; the kernels take at most {max_layers}")
+        cap = "KH_MAX_LAYERS_DERIV" if max_layers == MAX_LAYERS_DERIV else "KH_MAX_LAYERS"
+        raise ValueError(f"model has {nl} layers; the kernels take at most {max_layers} ({cap})")
+    if not values:
+        return tab
     if np.isnan(tab).any():
         raise ValueError("model contains NaN")
     alpha, beta, rho, _, q_a, q_b = tab.T

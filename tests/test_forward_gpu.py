@@ -158,7 +158,8 @@ def test_max_layers_and_errors():
     O = K.reflectivity_batch(m, np.array([0.1, 0.3]), omega_grid(16))
     assert_rows_close(R, O, row_tolerance(m, np.array([0.1, 0.3]), omega_grid(16), False),
                       "240 layers")
-    with pytest.raises(RuntimeError, match="KH_MAX_LAYERS"):
+    # refused on the host, before any launch
+    with pytest.raises(ValueError, match="KH_MAX_LAYERS"):
         kennett_reflectivity_batch(seeded_model(241, 1), np.array([0.1]), omega_grid(4))
     with pytest.raises(ValueError):
         LayerModel.from_arrays(alpha=[1.5, -2], beta=[0, 1], rho=[1, 2], thickness=[1, np.inf],
