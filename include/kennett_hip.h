@@ -179,6 +179,33 @@ int kh_forward_multi(const kh_model_desc* desc, int nmodels, long long total_ite
                      int omega_per_item, int workgroups, long long records, void* tables,
                      size_t table_bytes, void* counter, int* nonfinite, void* stream);
 
+/* Scratch that kh_forward_multi_static keeps in the library, one copy per
+ * device, for the life of the process: a 256-byte work counter and up to
+ * KH_STATIC_MAX_RECORDS 256-byte table records (32 MiB in all). */
+#define KH_STATIC_SCRATCH_BYTES (32u << 20)
+#define KH_STATIC_COUNTER_BYTES 256u
+#define KH_STATIC_MAX_RECORDS ((KH_STATIC_SCRATCH_BYTES - KH_STATIC_COUNTER_BYTES) / 256u)
+
+/* kh_forward_multi that allocates nothing and takes no workspace: the work
+ * counter and, with use_tables != 0, the table records live in the
+ * library's own device scratch (KH_STATIC_SCRATCH_BYTES per device).  With
+ * use_tables != 0 the staged-table path runs and records (the sum of
+ * np * nl) must not exceed KH_STATIC_MAX_RECORDS (KH_ERR_INVALID_ARGUMENT
+ * otherwise, before any launch); with use_tables = 0 the in-LDS path runs,
+ * for any batch.  Other arguments and the output are kh_forward_multi's
+ * (same kernels, bit for bit).  The first call on a device looks up the
+ * scratch address (make it outside stream capture; a call with nmodels = 0
+ * does only that).  Every call on a device shares that device's scratch:
+ * calls that overlap on one device (concurrent streams or threads) race.
+ * The counter is reset by the call with a kernel, not a memset.
+ * No counterpart in the reference: this entry point is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). */
+int kh_forward_multi_static(const kh_model_desc* desc, int nmodels, long long total_items,
+                            int max_nl, const double* layers_all, const double* p_all,
+                            double* R_all, int block, int omega_per_item, int workgroups,
+                            long long records, int use_tables, int* nonfinite, void* stream);
+
 /* ---- gather: discrete-wavenumber sum and inverse FFT to traces ---- */
 
 /* Workspace bytes kh_bessel_sum needs for np slownesses, nw - 1 frequencies

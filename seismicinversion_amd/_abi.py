@@ -61,6 +61,8 @@ EXPORTS: dict[str, list] = {
     "kh_multi_table_bytes": [ctypes.c_longlong],
     "kh_forward_multi": [_vp, _c_int, ctypes.c_longlong, _c_int, _vp, _vp, _vp, _c_int, _c_int,
                          _c_int, ctypes.c_longlong, _vp, ctypes.c_size_t, _vp, _vp, _vp],
+    "kh_forward_multi_static": [_vp, _c_int, ctypes.c_longlong, _c_int, _vp, _vp, _vp, _c_int,
+                                _c_int, _c_int, ctypes.c_longlong, _c_int, _vp, _vp],
 }
 _RESTYPES = {"kh_multi_items": ctypes.c_longlong, "kh_multi_table_bytes": ctypes.c_size_t, "kh_deriv_workspace_bytes": ctypes.c_size_t, "kh_gram_workspace_bytes": ctypes.c_size_t,
              "kh_version": ctypes.c_char_p, "kh_last_error": ctypes.c_char_p,

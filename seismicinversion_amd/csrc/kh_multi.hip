@@ -28,6 +28,8 @@
 // of forward_kernel with the same (block, frequencies per lane), so every
 // model's output is bitwise identical to a kh_forward_grid_ex launch of that
 // model alone with that shape, on either path.
+#include <mutex>
+
 #include "kh_physics.hpp"
 #include "kh_runtime.hpp"
 #include "kh_sweep.hpp"
@@ -296,6 +298,90 @@ static int multi_resident(int block, int omega_per_item, int max_nl, bool tables
   return cus * n;
 }
 
+// Argument checks and launches shared by kh_forward_multi and
+// kh_forward_multi_static (tables = NULL: the in-LDS path).
+static int forward_multi_impl(const char* what, const kh_model_desc* desc, int nmodels,
+                              long long total_items, int max_nl, const double* layers_all,
+                              const double* p_all, double* R_all, int block, int omega_per_item,
+                              int workgroups, long long records, void* tables,
+                              size_t table_bytes_given, void* counter, int* nonfinite,
+                              hipStream_t st) {
+  KH_REQUIRE(nmodels >= 0 && total_items >= 0 && records >= 0, KH_ERR_INVALID_ARGUMENT,
+             "%s: negative size", what);
+  if (nmodels == 0 || total_items == 0) return KH_OK;
+  KH_REQUIRE(desc && layers_all && p_all && R_all && counter, KH_ERR_INVALID_ARGUMENT,
+             "%s: NULL pointer", what);
+  if (tables != nullptr) {
+    KH_REQUIRE(records > 0, KH_ERR_INVALID_ARGUMENT, "%s: no table records", what);
+    KH_REQUIRE((reinterpret_cast<uintptr_t>(tables) & 255) == 0, KH_ERR_INVALID_ARGUMENT,
+               "%s: table workspace not aligned to 256 bytes", what);
+    KH_REQUIRE(table_bytes_given >= table_bytes(records), KH_ERR_OUT_OF_MEMORY,
+               "%s: table workspace of %zu bytes, %zu needed", what, table_bytes_given,
+               table_bytes(records));
+  }
+  KH_REQUIRE(max_nl >= 2 && max_nl <= KH_MAX_LAYERS, KH_ERR_UNSUPPORTED,
+             "%s: max_nl %d outside [2, %d]", what, max_nl, KH_MAX_LAYERS);
+  KH_REQUIRE(block == 64 || block == 128 || block == 256, KH_ERR_INVALID_ARGUMENT,
+             "%s: block %d not in {64, 128, 256}", what, block);
+  KH_REQUIRE(omega_per_item == 1 || omega_per_item == 2 || omega_per_item == 4 ||
+                 omega_per_item == 8,
+             KH_ERR_INVALID_ARGUMENT, "%s: omega_per_item %d not in {1, 2, 4, 8}", what,
+             omega_per_item);
+  const bool use_tables = tables != nullptr;
+  int wgs = workgroups > 0 ? workgroups : multi_resident(block, omega_per_item, max_nl, use_tables);
+  if ((long long)wgs > total_items) wgs = (int)total_items;
+  auto* c = reinterpret_cast<unsigned long long*>(counter);
+  hipLaunchKernelGGL(zero_counter_kernel, dim3((nmodels + 255) / 256), dim3(256), 0, st, c,
+                     nonfinite, nmodels);
+  KH_TRY(check_launch(what));
+  TableRec* t = use_tables ? static_cast<TableRec*>(tables) : nullptr;
+  if (use_tables) {
+    hipLaunchKernelGGL(stage_tables_kernel, dim3((unsigned)((records + 255) / 256)), dim3(256), 0,
+                       st, desc, nmodels, records, layers_all, p_all, block, t);
+    KH_TRY(check_launch(what));
+  }
+  const MultiArgs a{desc, nmodels, total_items, layers_all, p_all, reinterpret_cast<cd*>(R_all),
+                    block, max_nl, c, nonfinite, wgs, st};
+  switch (omega_per_item) {
+    case 1: return launch_multi<1>(a, t);
+    case 2: return launch_multi<2>(a, t);
+    case 4: return launch_multi<4>(a, t);
+    default: return launch_multi<8>(a, t);
+  }
+}
+
+// Library-owned scratch of kh_forward_multi_static, one copy per device:
+// the work counter in the first 256 bytes, then up to KH_STATIC_MAX_RECORDS
+// table records.  Host code never passes this name to a kernel (on the host
+// it is a stand-in, not a device address); it uses the device address that
+// hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[KH_STATIC_SCRATCH_BYTES];
+
+constexpr int KH_STATIC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[KH_STATIC_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (the first call on a device makes the lookup; later calls read the
+// cached address).
+static int static_scratch(const char* what, unsigned char** out) {
+  int dev = -1;
+  KH_REQUIRE(hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < KH_STATIC_MAX_DEVICES,
+             KH_ERR_LAUNCH, "%s: no current HIP device (or device index >= %d)", what,
+             KH_STATIC_MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    KH_REQUIRE(e == hipSuccess && ptr != nullptr, KH_ERR_LAUNCH,
+               "%s: hipGetSymbolAddress(scratch) failed on device %d: %s", what, dev,
+               hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = static_cast<unsigned char*>(g_scratch_ptr[dev]);
+  return KH_OK;
+}
+
 }  // namespace kh
 
 using namespace kh;
@@ -339,47 +425,31 @@ extern "C" int kh_forward_multi(const kh_model_desc* desc, int nmodels, long lon
                                 long long records, void* tables, size_t table_bytes_given,
                                 void* counter, int* nonfinite, void* stream) {
   clear_error();
-  KH_REQUIRE(nmodels >= 0 && total_items >= 0 && records >= 0, KH_ERR_INVALID_ARGUMENT,
-             "kh_forward_multi: negative size");
-  if (nmodels == 0 || total_items == 0) return KH_OK;
-  KH_REQUIRE(desc && layers_all && p_all && R_all && counter, KH_ERR_INVALID_ARGUMENT,
-             "kh_forward_multi: NULL pointer");
-  if (tables != nullptr) {
-    KH_REQUIRE(records > 0, KH_ERR_INVALID_ARGUMENT, "kh_forward_multi: no table records");
-    KH_REQUIRE((reinterpret_cast<uintptr_t>(tables) & 255) == 0, KH_ERR_INVALID_ARGUMENT,
-               "kh_forward_multi: table workspace not aligned to 256 bytes");
-    KH_REQUIRE(table_bytes_given >= table_bytes(records), KH_ERR_OUT_OF_MEMORY,
-               "kh_forward_multi: table workspace of %zu bytes, %zu needed", table_bytes_given,
-               table_bytes(records));
-  }
-  KH_REQUIRE(max_nl >= 2 && max_nl <= KH_MAX_LAYERS, KH_ERR_UNSUPPORTED,
-             "kh_forward_multi: max_nl %d outside [2, %d]", max_nl, KH_MAX_LAYERS);
-  KH_REQUIRE(block == 64 || block == 128 || block == 256, KH_ERR_INVALID_ARGUMENT,
-             "kh_forward_multi: block %d not in {64, 128, 256}", block);
-  KH_REQUIRE(omega_per_item == 1 || omega_per_item == 2 || omega_per_item == 4 ||
-                 omega_per_item == 8,
-             KH_ERR_INVALID_ARGUMENT, "kh_forward_multi: omega_per_item %d not in {1, 2, 4, 8}",
-             omega_per_item);
-  hipStream_t st = as_stream(stream);
-  const bool use_tables = tables != nullptr;
-  int wgs = workgroups > 0 ? workgroups : multi_resident(block, omega_per_item, max_nl, use_tables);
-  if ((long long)wgs > total_items) wgs = (int)total_items;
-  auto* c = reinterpret_cast<unsigned long long*>(counter);
-  hipLaunchKernelGGL(zero_counter_kernel, dim3((nmodels + 255) / 256), dim3(256), 0, st, c,
-                     nonfinite, nmodels);
-  KH_TRY(check_launch("kh_forward_multi"));
-  TableRec* t = use_tables ? static_cast<TableRec*>(tables) : nullptr;
-  if (use_tables) {
-    hipLaunchKernelGGL(stage_tables_kernel, dim3((unsigned)((records + 255) / 256)), dim3(256), 0,
-                       st, desc, nmodels, records, layers_all, p_all, block, t);
-    KH_TRY(check_launch("kh_forward_multi"));
-  }
-  const MultiArgs a{desc, nmodels, total_items, layers_all, p_all, reinterpret_cast<cd*>(R_all),
-                    block, max_nl, c, nonfinite, wgs, st};
-  switch (omega_per_item) {
-    case 1: return launch_multi<1>(a, t);
-    case 2: return launch_multi<2>(a, t);
-    case 4: return launch_multi<4>(a, t);
-    default: return launch_multi<8>(a, t);
-  }
+  return forward_multi_impl("kh_forward_multi", desc, nmodels, total_items, max_nl, layers_all,
+                            p_all, R_all, block, omega_per_item, workgroups, records, tables,
+                            table_bytes_given, counter, nonfinite, as_stream(stream));
+}
+
+extern "C" int kh_forward_multi_static(const kh_model_desc* desc, int nmodels,
+                                       long long total_items, int max_nl,
+                                       const double* layers_all, const double* p_all,
+                                       double* R_all, int block, int omega_per_item,
+                                       int workgroups, long long records, int use_tables,
+                                       int* nonfinite, void* stream) {
+  const char* what = "kh_forward_multi_static";
+  clear_error();
+  unsigned char* base = nullptr;
+  KH_TRY(static_scratch(what, &base));  // resolved even when there is nothing to launch
+  KH_REQUIRE(records >= 0, KH_ERR_INVALID_ARGUMENT, "%s: negative size", what);
+  KH_REQUIRE(use_tables == 0 || records <= (long long)KH_STATIC_MAX_RECORDS,
+             KH_ERR_INVALID_ARGUMENT,
+             "%s: %lld table records exceed the static scratch maximum of %lld "
+             "(KH_STATIC_MAX_RECORDS; use_tables = 0 takes any batch)",
+             what, records, (long long)KH_STATIC_MAX_RECORDS);
+  void* tables = use_tables != 0 ? static_cast<void*>(base + KH_STATIC_COUNTER_BYTES) : nullptr;
+  const size_t tbytes = use_tables != 0 ? (size_t)KH_STATIC_SCRATCH_BYTES - KH_STATIC_COUNTER_BYTES
+                                        : 0;
+  return forward_multi_impl(what, desc, nmodels, total_items, max_nl, layers_all, p_all, R_all,
+                            block, omega_per_item, workgroups, records, tables, tbytes,
+                            static_cast<void*>(base), nonfinite, as_stream(stream));
 }
