@@ -7,31 +7,37 @@ mixed-size serving distribution (seed 42: layer counts round(exp(U[ln 3,
 ln 100])), slownesses round(exp(U[ln 16, ln 2048])) uniform in (0, 0.8] s/km,
 frequency counts uniform over {256, 1024, 4096}, omega_k = k*2*pi/64),
 submitted at once (``submit_forward_many``) to the continuous-batching
-server and served with
-device-resident results.  ``value`` = p*omega*layer updates per second
-(one update = one interface step of one (p, omega) lane).
+server and served with device-resident results.  Each step submits new
+request objects, so validation and packing of every layer table are timed.
+``value`` = p*omega*layer updates per second (one update = one interface
+step of one (p, omega) lane).
 
 Beside it, on the same requests:
 * plain: the one-model-per-launch synchronous GPU path (one kh_forward_grid
   launch per request, synchronised), with device results and, through the
   kennett_reflectivity_batch drop-in, with host results;
 * served with host results (pinned staging + copy-out);
-* an open-loop run: Poisson arrivals at --rate req/s for --open-seconds,
-  20 % of them 5-iteration LM inversions (slownesses <= 32, frequencies <= 128),
-  results to the host; p50/p99 latency from arrival to result for all forward
-  requests, for 20-layer forward requests and for inversions;
+* served / plain per size class (small: nl <= 20 and np*nw <= 2.6e5, where
+  one request leaves most of the device idle; medium; large);
+* an open-loop sweep over arrival rates (default 100, 200, 500, 1000, 2000,
+  5000 req/s): Poisson arrivals, 20 % of them 5-iteration LM inversions
+  (slownesses <= 32, frequencies <= 128), results to the host, admission
+  deadlines; per rate p50/p99 latency from arrival to result for all
+  forward requests, for 20-layer forward requests and for inversions,
+  forward updates/s, completed and rejected counts and queue lengths;
 * kernels: the gather workload's forward kernel (20 layers, 8192 p x 1023
   damped omega, free surface), the full gather (forward + J0 wavenumber sum +
-  inverse FFT, 100 offsets), the 256-model uneven batch (persistent
-  launch vs one launch per model) and the 20-layer Newton-LM inversion
-  (75 parameters, 128 p x 512 omega, 10 iterations, eager and from HIP
-  graphs).
+  inverse FFT, 100 offsets, nt 2048, and the reference CLI's -n 2048 shape,
+  nt 4096), the 256-model uneven batch (persistent launch vs one launch per
+  model) and the 20-layer Newton-LM inversion (75 parameters, 128 p x 512
+  omega, 10 iterations, eager and from HIP graphs).
 
 ``roofline``: the persistent multi-model kernel of the served steps, FP64
 algorithmic FLOPs (340 per update) per launch over the average launch
 duration (HIP events on the server's compute stream, where the batch
-kernels run back to back).  ``cpu_baseline``: the NumPy
-oracle, one process, on the middle slowness row of every request.
+kernels run back to back).  ``cpu_baseline``: the NumPy oracle
+(vectorised over slownesses) on the host cores, on evenly spaced slowness
+rows of every request; ``single_core`` is one process on the middle row.
 """
 
 from __future__ import annotations
@@ -40,6 +46,7 @@ import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -59,14 +66,23 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--requests", type=int, default=400)
-    ap.add_argument("--rate", type=float, default=200.0, help="open-loop arrivals per second")
-    ap.add_argument("--open-seconds", type=float, default=4.0)
+    ap.add_argument("--rates", default="100,200,500,1000,2000,5000",
+                    help="open-loop arrival rates (req/s), comma separated")
+    ap.add_argument("--open-seconds", type=float, default=2.5,
+                    help="arrival window of each open-loop rate")
+    ap.add_argument("--forward-deadline", type=float, default=0.5,
+                    help="open loop: a forward request not admitted within this many seconds "
+                         "is rejected")
+    ap.add_argument("--invert-deadline", type=float, default=2.0,
+                    help="open loop: an inversion not started within this many seconds is "
+                         "rejected")
     ap.add_argument("--cpu-workers", type=int, default=min(16, os.cpu_count() or 1),
                     help="processes of the all-core CPU baseline (the box's share is 16 cores)")
     ap.add_argument("--warmup-seconds", type=float, default=2.0,
                     help="untimed forward launches before anything is timed (clock ramp)")
-    ap.add_argument("--section", choices=("all", "lm"), default="all",
-                    help="'lm': only the 20-layer Newton-LM figures (for profiling its kernels)")
+    ap.add_argument("--section", choices=("all", "lm", "open"), default="all",
+                    help="'lm': only the 20-layer Newton-LM figures (for profiling its kernels); "
+                         "'open': only the open-loop rate sweep")
     ap.add_argument("--traffic-json", default=os.path.join(HERE, "profiles", "traffic_multi_served.json"),
                     help="HBM bytes per launch of the multi-model kernel from rocprofv3 --pmc")
     return ap.parse_args()
@@ -178,7 +194,8 @@ def cpu_baselines(specs, workers):
                           offsets=cfg.offsets, dp=cfg.p_max / cfg.np_slow)
     chunks = [(i, min(len(specs), i + 5), workers) for i in range(0, len(specs), 5)]
     ranges = [(j, min(cfg.np_slow, j + 32)) for j in range(0, cfg.np_slow, 32)]
-    with mp.get_context("fork").Pool(workers) as pool:
+    pool = mp.get_context("fork").Pool(workers)
+    try:
         pool.map(_cpu_rows, [(0, 1, 1)] * workers)  # start-up
         t0 = time.perf_counter()
         updn = sum(pool.map(_cpu_rows, chunks, chunksize=1))
@@ -192,10 +209,19 @@ def cpu_baselines(specs, workers):
         S = K.ricker_spectrum(w, cfg.ricker_omega_max)
         _, traces = K.hermitian_ifft(U * S[None, :], cfg.nw, np.pi / cfg.T, cfg.T)
         dtg = time.perf_counter() - t0
+    finally:
+        # close and join (not terminate): every worker exits before the GPU
+        # part starts and none outlives the run
+        pool.close()
+        pool.join()
     cpu = {"value": updn / dtn, "unit": UNIT, "cores": workers, "kind": "port",
            "sample": f"{workers} evenly spaced slowness rows of each of the {len(specs)} requests "
                      f"({updn:.3e} updates), NumPy oracle in {workers} forked processes "
-                     f"(one thread each), {dtn:.1f} s",
+                     f"(one thread each), {dtn:.1f} s; the oracle is vectorised over "
+                     "slownesses in the reference's (np, nw, 2, 2) layout, while the "
+                     "reference loops in Python over slownesses and layers "
+                     "(kennett_reflectivity_gpu.py:142-147, 157-191), so the reference "
+                     "itself would be slower and GPU/CPU ratios are conservative",
            "single_core": {"value": upd1 / dt1, "cores": 1,
                            "sample": f"middle slowness row of each request ({upd1:.3e} updates), "
                                      f"{dt1:.1f} s"}}
@@ -296,6 +322,19 @@ def kernel_figures(torch, dev):
     out["gather"] = {"workload": "20 layers, 8192 p x 1023 omega, 100 offsets, nt 2048",
                      "ms": event_ms(gat, 5, torch)}
     out["_gather_traces"] = gat().cpu().numpy()
+    # the reference CLI's meaning of -n 2048: nw = 2048 (2047 evaluated
+    # frequencies), nt = 4096, same model, slownesses and offsets
+    cfg2 = gather_config_c2(2048)
+    w2 = np.arange(1, cfg2.nw) * dw
+    S2 = torch.from_numpy(ricker_spectrum(w2, cfg2.ricker_omega_max)).to(dev)
+    gat2 = lambda: gather_device(L, off, cfg2.T, cfg2.nw, cfg2.np_slow, cfg2.p_max, gamma, S2, True)  # noqa: E731
+    gat2()
+    ms2 = event_ms(gat2, 5, torch)
+    upd2 = cfg2.np_slow * (cfg2.nw - 1) * (cfg2.model.n_layers - 1)
+    out["gather_nw2048"] = {"workload": "20 layers, 8192 p x 2047 omega, 100 offsets, nt 4096 "
+                                        "(kennett_gather_gpu CLI -n 2048)",
+                            "ms": ms2, "forward_updates": upd2,
+                            "bessel_terms": len(cfg2.offsets) * cfg2.np_slow * (cfg2.nw - 1)}
     # 256 uneven models: persistent launch vs one launch per model
     specs = [batch.ModelSpec(model=m, p=pp, nw=1024) for m, pp in uneven_batch_models(256, 42)]
     upd = sum(s.updates() for s in specs)
@@ -367,6 +406,83 @@ def lm_figures(torch, dev):
             "tangent_over_adjoint_lane_updates": prob.P * 128 * 512 * 19}
 
 
+def open_loop_sweep(torch, rates, seconds, fwd_deadline, inv_deadline):
+    """The serving stream in open loop, one arrival rate after another on one
+    server: Poisson arrivals for `seconds` at each rate, 80 % forward
+    requests and 20 % 5-iteration Newton-LM inversions (drawn sizes; the
+    inversions keep nl and cap slownesses at 32 and frequencies at 128),
+    results to the host.  20-layer forward requests (slowness and frequency
+    counts from the same distributions) arrive beside the mix at
+    max(20, 5 % of the rate) per second, so that the 20-layer percentiles rest
+    on more than the mix's own few 20-layer draws.  A forward request not
+    admitted within `fwd_deadline` s, or an inversion not started within
+    `inv_deadline` s, is rejected (TimeoutError) and counted.  Results are
+    dropped as they complete (only their records are kept), so host memory
+    stays bounded at any rate."""
+    from seismicinversion_amd.serve import DeviceBackend, KennettServer, stream_requests
+    from seismicinversion_amd.workloads import serving_stream
+
+    def run(osrv, stream, reqs):
+        lock = threading.Lock()
+        left = [len(reqs)]
+        all_done = threading.Event()
+
+        def on_done(_f):
+            with lock:
+                left[0] -= 1
+                if left[0] == 0:
+                    all_done.set()
+
+        t0 = time.perf_counter()
+        for it, (kind, spec) in zip(stream, reqs):
+            d = it.t - (time.perf_counter() - t0)
+            if d > 0:
+                time.sleep(d)
+            f = (osrv.submit_forward(spec, deadline_s=fwd_deadline) if kind == "forward"
+                 else osrv.submit_invert(spec, deadline_s=inv_deadline))
+            f.add_done_callback(on_done)  # the future itself is not kept
+        t_submit = time.perf_counter() - t0
+        all_done.wait()
+        return t_submit, time.perf_counter() - t0
+
+    out = []
+    with KennettServer(backend=DeviceBackend(n_streams=3)) as osrv:
+        # untimed warm-up at the first rate: populates the allocator's cached
+        # blocks of the server's streams (a cold server's first second runs
+        # forward p99 several times higher)
+        warm = serving_stream(rates[0], max(1, int(rates[0])), seed=41, lm_fraction=0.2)
+        run(osrv, warm, stream_requests(warm))
+        for k, rate in enumerate(rates):
+            n = max(1, int(rate * seconds))
+            mix = serving_stream(rate, n, seed=42 + k, lm_fraction=0.2)
+            probe_rate = max(20.0, 0.05 * rate)
+            probe = serving_stream(probe_rate, max(1, int(probe_rate * seconds)), seed=700 + k,
+                                   lm_fraction=0.0, nl_range=(20, 20))
+            stream = sorted(mix + probe, key=lambda it: it.t)
+            reqs = stream_requests(stream)
+            osrv.reset_stats()
+            t_submit, t_all = run(osrv, stream, reqs)
+            st = osrv.stats()
+            print(f"open loop {rate:g} req/s: {len(stream)} requests in {t_submit:.2f} s, "
+                  f"drained at {t_all:.2f} s", file=sys.stderr, flush=True)
+            out.append({
+                "rate_req_per_s": rate, "window_s": seconds, "requests": n,
+                "inversions": sum(1 for it in mix if it.kind == "invert"),
+                "probe_20_layer_rate_req_per_s": probe_rate, "probe_requests": len(probe),
+                "achieved_arrival_rate_req_per_s": len(stream) / t_submit,
+                "drain_s": t_all,
+                "forward_updates_per_s": st["forward_updates_per_s"],
+                "completed": st["completed_by_kind"], "rejected": st["rejected_by_kind"],
+                "failed_other": st["failed"] - sum(st["rejected_by_kind"].values()),
+                "admission_wait_ms": st["admission_wait"], "queued_max": st["waiting_max"],
+                "forward_phases_ms": st["forward_phases"],
+                "batches": st["batches"],
+                "latency_forward_ms": st["latency_forward"],
+                "latency_forward_20_layers_ms": st["latency_forward_20_layers"],
+                "latency_invert_ms": st["latency_invert"]})
+    return {"deadlines_s": {"forward": fwd_deadline, "invert": inv_deadline}, "rates": out}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -383,9 +499,18 @@ def main():
         warm_clocks(torch, dev, args.warmup_seconds)
         print(json.dumps({"section": "lm", "lm_20_layers": lm_figures(torch, dev)}))
         return
+    if args.section == "open":
+        dev = torch.device("cuda", 0)
+        torch.cuda.set_device(dev)
+        warm_clocks(torch, dev, args.warmup_seconds)
+        print(json.dumps({"section": "open", "open_loop": open_loop_sweep(
+            torch, [float(r) for r in args.rates.split(",") if r], args.open_seconds,
+            args.forward_deadline, args.invert_deadline)}))
+        return
 
     from seismicinversion_amd import engine
     from seismicinversion_amd.kennett_reflectivity_gpu import kennett_reflectivity_batch
+    from seismicinversion_amd.batch import ModelSpec
     from seismicinversion_amd.serve import DeviceBackend, KennettServer, stream_requests
     from seismicinversion_amd.workloads import serving_stream
 
@@ -402,8 +527,15 @@ def main():
     be = DeviceBackend(n_streams=3, kernel_timing=True)
     srv = KennettServer(backend=be).start()
 
-    def served(on_device=True):
-        for f in srv.submit_forward_many(specs, on_device=on_device):
+    def fresh(batch_specs):
+        """New request objects for the same models and grids: the server
+        validates and packs every layer table again (ModelSpec caches its
+        table on the object), as it would for new requests."""
+        return [ModelSpec(model=x.model, p=x.p, nw=x.nw, dw=x.dw, k0=x.k0, gamma=x.gamma,
+                          free_surface=x.free_surface) for x in batch_specs]
+
+    def served(on_device=True, batch_specs=specs):
+        for f in srv.submit_forward_many(fresh(batch_specs), on_device=on_device):
             f.result()
 
     for _ in range(args.warmup):
@@ -417,76 +549,55 @@ def main():
     served()  # warm the host-result path's staging and pages
     served(False)
     t_served_host = timed(lambda: served(False), 1, torch)
-    srv.stop()
 
     # ---- plain: one synchronous launch per request
-    def plain():
-        for s in specs:
-            engine.forward_grid(engine.layers_tensor(s.model, dev), torch.from_numpy(s.p).to(dev),
-                                s.k0, s.dw, s.gamma, s.nw, s.free_surface)
+    def plain(batch_specs=specs):
+        for x in batch_specs:
+            engine.forward_grid(engine.layers_tensor(x.model, dev), torch.from_numpy(x.p).to(dev),
+                                x.k0, x.dw, x.gamma, x.nw, x.free_surface)
             torch.cuda.synchronize()
 
-    def plain_host():
-        for s in specs:
-            kennett_reflectivity_batch(s.model, s.p, (s.k0 + np.arange(s.nw)) * s.dw,
-                                       s.free_surface)
+    def plain_host(batch_specs=specs):
+        for x in batch_specs:
+            kennett_reflectivity_batch(x.model, x.p, (x.k0 + np.arange(x.nw)) * x.dw,
+                                       x.free_surface)
 
     keep_busy(plain, torch)
     t_plain = timed(plain, 1, torch)
     t_plain_host = timed(plain_host, 1, torch)
 
-    # ---- open loop with inversions, results to the host
+    # ---- served / plain per size class of the same 400 requests
+    classes = {
+        "small: nl <= 20 and np*nw <= 2.6e5": lambda x: x.n_layers <= 20 and len(x.p) * x.nw <= 2.6e5,
+        "medium: the rest with updates <= 1e8": lambda x: not (x.n_layers <= 20 and len(x.p) * x.nw <= 2.6e5) and x.updates() <= 1e8,
+        "large: updates > 1e8": lambda x: x.updates() > 1e8,
+    }
+    size_classes = {}
+    for name, pred in classes.items():
+        sub = [x for x in specs if pred(x)]
+        if not sub:
+            continue
+        upd = sum(x.updates() for x in sub)
+        keep_busy(lambda: served(True, sub), torch, 0.2)
+        ts = timed(lambda: served(True, sub), 3, torch)
+        keep_busy(lambda: plain(sub), torch, 0.2)
+        tp = timed(lambda: plain(sub), 2, torch)
+        tph = timed(lambda: plain_host(sub), 1, torch)
+        tsh = timed(lambda: served(False, sub), 2, torch)
+        size_classes[name] = {
+            "requests": len(sub), "updates": upd,
+            "served_updates_per_s": upd / ts, "plain_updates_per_s": upd / tp,
+            "served_over_plain": tp / ts,
+            "served_host_updates_per_s": upd / tsh, "plain_host_updates_per_s": upd / tph,
+            "served_over_plain_host_results": tph / tsh}
+    srv.stop()
+
+    # ---- open loop with inversions, results to the host, over the arrival rates
     open_loop = None
     if args.open_seconds > 0:
-        n_open = max(1, int(args.rate * args.open_seconds))
-        stream = serving_stream(args.rate, n_open, seed=42, lm_fraction=0.2)
-        # 20-layer forward requests (slowness and frequency counts from the
-        # same distributions) arriving beside the mix at a tenth of its rate,
-        # so that the 20-layer latency percentiles rest on more than the few
-        # 20-layer draws of the mix itself
-        probe_rate = 0.1 * args.rate
-        probe = serving_stream(probe_rate, max(1, int(probe_rate * args.open_seconds)), seed=7,
-                               lm_fraction=0.0, nl_range=(20, 20))
-        stream = sorted(stream + probe, key=lambda it: it.t)
-        reqs = stream_requests(stream)
-        with KennettServer(backend=DeviceBackend(n_streams=3)) as osrv:
-            # untimed warm-up on the same server: the stream's first second,
-            # replayed at its own pace, populates the allocator's cached
-            # blocks of the server's streams, so the timed run sees steady
-            # state (a cold server's first second runs forward p99 several
-            # times higher)
-            t0 = time.perf_counter()
-            warm = []
-            for it, (kind, spec) in zip(stream, reqs):
-                if it.t > 1.0:
-                    break
-                d = it.t - (time.perf_counter() - t0)
-                if d > 0:
-                    time.sleep(d)
-                warm.append(osrv.submit_forward(spec) if kind == "forward"
-                            else osrv.submit_invert(spec))
-            for f in warm:
-                f.result()
-            osrv.reset_stats()
-            t0 = time.perf_counter()
-            futs = []
-            for it, (kind, spec) in zip(stream, reqs):
-                d = it.t - (time.perf_counter() - t0)
-                if d > 0:
-                    time.sleep(d)
-                futs.append(osrv.submit_forward(spec) if kind == "forward"
-                            else osrv.submit_invert(spec))
-            for f in futs:
-                f.result()
-        st = osrv.stats()
-        open_loop = {"rate_req_per_s": args.rate, "requests": n_open,
-                     "probe_20_layer_rate_req_per_s": probe_rate, "probe_requests": len(probe),
-                     "inversions": sum(1 for k, _ in reqs if k == "invert"),
-                     "forward_updates_per_s": st["forward_updates_per_s"],
-                     "batches": st["batches"], "failed": st["failed"],
-                     "latency_forward_ms": st["latency_forward"],
-                     "latency_forward_20_layers_ms": st["latency_forward_20_layers"],
-                     "latency_invert_ms": st["latency_invert"]}
+        open_loop = open_loop_sweep(torch, [float(r) for r in args.rates.split(",") if r],
+                                    args.open_seconds, args.forward_deadline,
+                                    args.invert_deadline)
 
     kernels = kernel_figures(torch, dev)
     kernels["c1_plain_5_layers"] = c1_figure(torch, dev, c1_cpu)
@@ -506,7 +617,9 @@ def main():
             ratio = json.load(f).get("traffic_over_algorithmic")
         if ratio:
             traffic = float(ratio) * 16.0 * out_per_launch
-    roofline = {"bound": "mfma", "pipe": "fp64-vector", "achieved": achieved,
+    roofline = {"bound": "fp64-vector", "pipe": "FP64 VALU (v_fma_f64 and friends), not MFMA "
+                "and not HBM: 340 FLOP per 16 output bytes per interface",
+                "achieved": achieved,
                 "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
                 "traffic": traffic, "kernel": f"kh::multi_kernel<{be.shape[1]}>",
                 "kernel_ms_spans": "the whole kh_forward_multi call: counter reset, "
@@ -547,6 +660,7 @@ def main():
         "served_host_results_updates_per_s": updates / t_served_host,
         "plain_host_results_updates_per_s": updates / t_plain_host,
         "served_over_plain_host_results": t_plain_host / t_served_host,
+        "size_classes": size_classes,
         "open_loop": open_loop,
         "kernels": kernels,
         "roofline": roofline,

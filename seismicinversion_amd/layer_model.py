@@ -124,6 +124,13 @@ def check_table(table, max_layers: int = MAX_LAYERS, values: bool = True) -> np.
         raise ValueError(f"model has {nl} layers; the kernels take at most {max_layers} ({cap})")
     if not values:
         return tab
+    # fast path: one column-wise minimum (NaN propagates through it) decides
+    # almost every table; the messages below are only built for a bad one
+    mins = tab.min(axis=0)
+    if (mins[0] > 0 and mins[1] >= 0 and mins[2] > 0 and mins[3] == mins[3] and mins[4] > 0
+            and mins[5] == mins[5]
+            and (mins[5] > 0 or bool(((tab[:, 5] > 0) | (tab[:, 1] <= 0)).all()))):
+        return tab
     if np.isnan(tab).any():
         raise ValueError("model contains NaN")
     alpha, beta, rho, _, q_a, q_b = tab.T
@@ -144,5 +151,6 @@ def pack_layers(model: LayerModel, max_layers: int = MAX_LAYERS) -> np.ndarray:
     Raises ``ValueError`` for inputs the kernels cannot take (``check_table``):
     fewer than two or more than ``max_layers`` layers, a non-positive Q, or a
     NaN anywhere."""
-    return check_table(np.stack([getattr(model, c) for c in LAYER_COLUMNS], axis=1),
+    return check_table(np.array([model.alpha, model.beta, model.rho, model.thickness,
+                                 model.Q_alpha, model.Q_beta], dtype=np.float64).T.copy(),
                        max_layers)

@@ -46,6 +46,7 @@ import gc
 import queue
 import threading
 import time
+from collections import deque
 from concurrent.futures import Future, ThreadPoolExecutor
 from dataclasses import dataclass, field
 
@@ -131,6 +132,12 @@ def percentile_stats(records, kind="forward", n_layers=None) -> dict:
     lat = np.asarray(lat) * 1e3
     return {"count": int(lat.size), "p50_ms": float(np.percentile(lat, 50)),
             "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean())}
+
+
+# Host results of a batch are copied down in contiguous groups: the first
+# of at most this many bytes, each further one this factor larger.
+COPY_GROUP_FIRST = 4 << 20
+COPY_GROUP_GROWTH = 4
 
 
 class DeviceBackend:
@@ -248,28 +255,47 @@ class DeviceBackend:
             cp.wait_event(kernel_done)
             with torch.cuda.stream(cp):
                 total = int(plan.r_off[-1])
+                counts = torch.empty(len(specs), dtype=torch.int32, pin_memory=True)
+                counts.copy_(plan.nonfinite[:len(specs)], non_blocking=True)
+                counts_ev = torch.cuda.Event()
+                counts_ev.record(cp)
                 host = None
+                groups = []  # (request indices, event) in copy order
                 if any(to_host):
                     if 16 * total <= self.max_batch_bytes:
                         host = self.staging[slot][:16 * total].view(torch.complex128)
                     else:
                         host = torch.empty(total, dtype=torch.complex128, pin_memory=True)
-                    if all(to_host):
-                        host.copy_(plan.R, non_blocking=True)
-                    else:
-                        for i, h in enumerate(to_host):
-                            if h:
-                                a, b = int(plan.r_off[i]), int(plan.r_off[i + 1])
-                                host[a:b].copy_(plan.R[a:b], non_blocking=True)
-                counts = torch.empty(len(specs), dtype=torch.int32, pin_memory=True)
-                counts.copy_(plan.nonfinite[:len(specs)], non_blocking=True)
+                    # host rows go down in a few contiguous copies of growing
+                    # size, each with its own event, so the requests of the
+                    # first copies are complete before the whole batch has
+                    # arrived (the server lists a batch smallest output first)
+                    cap = COPY_GROUP_FIRST
+                    run: list[int] = []
+                    a = b = 0
+                    for i in [i for i, h in enumerate(to_host) if h] + [None]:
+                        lo = int(plan.r_off[i]) if i is not None else -1
+                        if run and (i is None or lo != b or 16 * (b - a) >= cap):
+                            host[a:b].copy_(plan.R[a:b], non_blocking=True)
+                            e = torch.cuda.Event()
+                            e.record(cp)
+                            groups.append((run, e))
+                            cap *= COPY_GROUP_GROWTH
+                            run = []
+                        if i is None:
+                            break
+                        if not run:
+                            a = lo
+                        run.append(i)
+                        b = int(plan.r_off[i + 1])
                 ev = torch.cuda.Event()
                 ev.record(cp)
         except BaseException:
             self._release_slot(slot)
             raise
         return {"plan": plan, "host": host, "to_host": list(to_host), "counts": counts,
-                "event": ev, "slot": slot, "timing": t_ev,
+                "counts_event": counts_ev, "groups": groups, "event": ev, "slot": slot,
+                "timing": t_ev,
                 "updates": sum(x.updates() for x in specs)}
 
     def _release_slot(self, slot: int) -> None:
@@ -280,37 +306,61 @@ class DeviceBackend:
         return handle["event"].query()
 
     def results(self, handle):
-        """(per-request results, per-request NaN/Inf counts); frees the slot.
-        Host results are copied out of the staging buffer by the copy pool;
-        device results are views of the batch output."""
+        """(per-request results, per-request NaN/Inf counts); frees the slot."""
+        outs: list = [None] * len(handle["plan"].shapes)
+        counts: list = [0] * len(outs)
+
+        def put(i, out, count):
+            outs[i] = out
+            counts[i] = count
+
+        self.results_each(handle, put)
+        return outs, counts
+
+    def results_each(self, handle, deliver):
+        """Hands each request of a batch its result as soon as it is complete:
+        ``deliver(i, result, nonfinite_count)`` for every request i, device
+        results right after the kernel, then host results in their copy order
+        (smallest first), each after its own rows have arrived in the staging
+        buffer and been copied out into a host array by the copy pool.  Frees
+        the slot at the end."""
         from . import batch
 
         try:
-            handle["event"].synchronize()
             plan, to_host = handle["plan"], handle["to_host"]
-            dev_views = batch.split(plan)
-            outs = [np.empty(shape, np.complex128) if h else dev_views[i]
-                    for i, (shape, h) in enumerate(zip(plan.shapes, to_host))]
-            jobs = []
-            chunk = 1 << 19  # complex elements (8 MiB) per copy job
-            for i, (n_p, nw) in enumerate(plan.shapes):
-                if not to_host[i]:
-                    continue
-                off = int(plan.r_off[i])
-                rows = max(1, chunk // nw)
-                for r0 in range(0, n_p, rows):
-                    jobs.append((i, off, nw, r0, min(n_p, r0 + rows)))
-            if jobs:
-                flat = handle["host"].numpy()
-
-                def copy(job):
-                    i, off, nw, r0, r1 = job
-                    np.copyto(outs[i][r0:r1],
-                              flat[off + r0 * nw:off + r1 * nw].reshape(r1 - r0, nw))
-
-                for _ in self.pool.map(copy, jobs):
-                    pass
+            handle["counts_event"].synchronize()
             counts = batch.nonfinite_counts(plan, handle["counts"].numpy())
+            dev_views = batch.split(plan)
+            for i, h in enumerate(to_host):
+                if not h:
+                    deliver(i, dev_views[i], counts[i])
+            if handle["groups"]:
+                flat = handle["host"].numpy()
+                chunk = 1 << 19  # complex elements (8 MiB) per copy job
+                for run, e in handle["groups"]:
+                    outs = {i: np.empty(plan.shapes[i], np.complex128) for i in run}
+                    jobs = []
+                    for i in run:
+                        n_p, nw = plan.shapes[i]
+                        rows = max(1, chunk // nw)
+                        jobs += [(i, r0, min(n_p, r0 + rows)) for r0 in range(0, n_p, rows)]
+
+                    def copy(job, outs=outs):
+                        i, r0, r1 = job
+                        nw = plan.shapes[i][1]
+                        off = int(plan.r_off[i])
+                        np.copyto(outs[i][r0:r1], flat[off + r0 * nw:off + r1 * nw].reshape(r1 - r0, nw))
+
+                    e.synchronize()
+                    if len(jobs) <= 2:
+                        for job in jobs:
+                            copy(job)
+                    else:
+                        for _ in self.pool.map(copy, jobs):
+                            pass
+                    for i in run:
+                        deliver(i, outs[i], counts[i])
+            handle["event"].synchronize()
             if handle["timing"] is not None:
                 a, b = handle["timing"]
                 with self._lock:
@@ -318,7 +368,6 @@ class DeviceBackend:
                                             int(plan.r_off[-1])))
         finally:
             self._release_slot(handle["slot"])
-        return outs, counts
 
     def invert(self, spec: InvertSpec):
         from .taup_inversion import invert_taup
@@ -362,13 +411,15 @@ class KennettServer:
         self.max_batch_bytes = (max_batch_bytes if max_batch_bytes is not None
                                 else getattr(self.backend, "max_batch_bytes", None))
         self.incoming: queue.Queue = queue.Queue()
-        self.waiting: list[Request] = []
+        self.waiting: deque[Request] = deque()
         self.completions: queue.Queue = queue.Queue()
         self.n_inflight = 0
         self.outstanding = 0  # submitted and not yet finished
         self.records: list[dict] = []
         self.lock = threading.Lock()
-        self.waiting_inv: list[Request] = []
+        self.waiting_inv: deque[Request] = deque()
+        # longest FIFOs seen since the last reset_stats (forward, inversion)
+        self.waiting_max = [0, 0]
         self.lm_threads: list[threading.Thread] = []
         self.max_lm = max_lm_concurrency
         self.lm_running = 0
@@ -475,6 +526,7 @@ class KennettServer:
         with self.lock:
             self.records = []
             self.batches = 0
+            self.waiting_max = [0, 0]
 
     def stats(self) -> dict:
         with self.lock:
@@ -482,9 +534,30 @@ class KennettServer:
         done = [r for r in recs if r.get("status") == "ok"]
         fwd = [r for r in done if r["kind"] == "forward"]
         span = (max(r["end"] for r in done) - min(r["arrival"] for r in done)) if done else 0.0
+        rejected = {k: sum(1 for r in recs if r["kind"] == k and
+                           r.get("status") in ("TimeoutError", "MemoryError"))
+                    for k in ("forward", "invert")}
+        def pct(x):
+            return ({"p50_ms": float(np.percentile(x, 50) * 1e3),
+                     "p99_ms": float(np.percentile(x, 99) * 1e3)} if x else {})
+
+        queue_ms = {k: pct([r["admit"] - r["arrival"] for r in recs
+                            if r["kind"] == k and "admit" in r])
+                    for k in ("forward", "invert")}
+        # forward requests: admission to launch (waiting for a batch and a
+        # stream slot) and launch to result (transfer, kernel, copy-out)
+        phases = {"admit_to_launch": pct([r["launch"] - r["admit"] for r in fwd
+                                          if "launch" in r and "admit" in r]),
+                  "launch_to_result": pct([r["end"] - r["launch"] for r in fwd if "launch" in r])}
         return {
             "completed": len(done),
             "failed": sum(1 for r in recs if r.get("status") not in ("ok", "queued")),
+            "completed_by_kind": {k: sum(1 for r in done if r["kind"] == k)
+                                  for k in ("forward", "invert")},
+            "rejected_by_kind": rejected,
+            "admission_wait": queue_ms,
+            "forward_phases": phases,
+            "waiting_max": {"forward": self.waiting_max[0], "invert": self.waiting_max[1]},
             "batches": self.batches,
             "forward_updates": int(sum(r["updates"] for r in fwd)),
             "forward_updates_per_s": (sum(r["updates"] for r in fwd) / span) if span else 0.0,
@@ -546,7 +619,10 @@ class KennettServer:
         admitted only when an inversion slot is free and its workspace fits,
         so inversions waiting for a slot hold no memory and never hold up
         forward requests.  A request is validated when it reaches the head
-        of its FIFO; an unusable one fails alone."""
+        of its FIFO; an unusable one fails alone.  Deadlines are checked at
+        the head of each FIFO (requests with equal relative deadlines expire
+        in arrival order), so a call costs O(admitted + expired), however
+        long the queues are."""
         while True:
             try:
                 item = self.incoming.get_nowait()
@@ -554,18 +630,20 @@ class KennettServer:
                 break
             for req in (item if isinstance(item, list) else (item,)):
                 (self.waiting if req.kind == "forward" else self.waiting_inv).append(req)
+        self.waiting_max[0] = max(self.waiting_max[0], len(self.waiting))
+        self.waiting_max[1] = max(self.waiting_max[1], len(self.waiting_inv))
         admitted = []
         now = time.perf_counter()
         for fifo, is_inv in ((self.waiting, False), (self.waiting_inv, True)):
-            keep = []
             room = fwd_room
-            for k, req in enumerate(fifo):
-                if keep or (not is_inv and room <= 0):
-                    keep.extend(fifo[k:])  # FIFO: the rest waits behind
-                    break
+            while fifo:
+                req = fifo[0]
                 if req.deadline is not None and now > req.deadline:
+                    fifo.popleft()
                     self._finish(req, error=TimeoutError("deadline passed before admission"))
                     continue
+                if (is_inv and self.lm_running >= self.max_lm) or (not is_inv and room <= 0):
+                    break  # FIFO: the rest waits behind the head
                 if req.mem is None:
                     try:
                         self._validate(req)
@@ -574,19 +652,19 @@ class KennettServer:
                             raise MemoryError(f"request needs {req.mem} bytes, "
                                               f"budget is {self.budget}")
                     except (ValueError, MemoryError) as e:
+                        fifo.popleft()
                         self._finish(req, error=e)
                         continue
-                if (not is_inv or self.lm_running < self.max_lm) and self._reserve(req.mem):
-                    req.record["admit"] = now
-                    admitted.append(req)
-                    if is_inv:
-                        with self.lock:
-                            self.lm_running += 1
-                    else:
-                        room -= req.record["updates"]
+                if not self._reserve(req.mem):
+                    break
+                fifo.popleft()
+                req.record["admit"] = now
+                admitted.append(req)
+                if is_inv:
+                    with self.lock:
+                        self.lm_running += 1
                 else:
-                    keep.append(req)
-            fifo[:] = keep
+                    room -= req.record["updates"]
         return admitted
 
     def _reserve(self, nbytes: int) -> bool:
@@ -626,6 +704,9 @@ class KennettServer:
                     batch.append(r)
                     total += nxt.updates()
                     nbytes += b
+                # device-resident results first, then host results smallest
+                # first: the backend's copies to the host complete in that order
+                batch.sort(key=lambda r: (not r.on_device, len(r.spec.p) * r.spec.nw))
                 launched = time.perf_counter()
                 try:
                     handle = self.backend.launch_forward([r.spec for r in batch],
@@ -656,16 +737,28 @@ class KennettServer:
             if item is None:
                 return
             handle, batch = item
+            each = getattr(self.backend, "results_each", None)
+            delivered = [False] * len(batch)
             try:
-                outs, counts = self.backend.results(handle)
+                if each is not None:
+                    def deliver(i, out, count):
+                        delivered[i] = True
+                        self._release(batch[i].mem)
+                        self._finish(batch[i], result=out, nonfinite=count)
+
+                    each(handle, deliver)
+                else:
+                    outs, counts = self.backend.results(handle)
+                    for i, (r, out) in enumerate(zip(batch, outs)):
+                        delivered[i] = True
+                        self._release(r.mem)
+                        self._finish(r, result=out,
+                                     nonfinite=None if counts is None else counts[i])
             except Exception as e:
-                for r in batch:
-                    self._release(r.mem)
-                    self._finish(r, error=e)
-            else:
-                for i, (r, out) in enumerate(zip(batch, outs)):
-                    self._release(r.mem)
-                    self._finish(r, result=out, nonfinite=None if counts is None else counts[i])
+                for i, r in enumerate(batch):
+                    if not delivered[i]:
+                        self._release(r.mem)
+                        self._finish(r, error=e)
             with self.lock:
                 self.n_inflight -= 1
 

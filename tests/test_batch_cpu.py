@@ -122,6 +122,15 @@ def test_table_values_are_checked(column, value):
         batch.ModelSpec(model=tab, p=np.array([0.1]), nw=8).table()
 
 
+@pytest.mark.parametrize("column", range(6))
+def test_nan_anywhere_is_rejected(column):
+    """Also in the water layer, whose Q_beta the kernels never read."""
+    tab = _good_table(4)
+    tab[0, column] = np.nan
+    with pytest.raises(ValueError, match="NaN"):
+        batch.ModelSpec(model=tab, p=np.array([0.1]), nw=8).table()
+
+
 def test_empty_grids_are_rejected():
     """nw = 0 or no slowness: ValueError from describe() (not a division by
     zero in the tile arithmetic)."""
