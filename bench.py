@@ -535,8 +535,10 @@ def main():
                           free_surface=x.free_surface) for x in batch_specs]
 
     def served(on_device=True, batch_specs=specs):
-        for f in srv.submit_forward_many(fresh(batch_specs), on_device=on_device):
-            f.result()
+        futs = srv.submit_forward_many(fresh(batch_specs), on_device=on_device)
+        for k in range(len(futs)):  # each result is dropped once it has arrived
+            futs[k].result()
+            futs[k] = None
 
     for _ in range(args.warmup):
         served()

@@ -140,6 +140,96 @@ COPY_GROUP_FIRST = 4 << 20
 COPY_GROUP_GROWTH = 4
 
 
+class PinnedResultPool:
+    """Page-locked host memory handed out to callers as their results.
+
+    A batch leases one contiguous block for all of its host results; the
+    device copies the rows straight into it and each request gets a NumPy
+    view of its own part, so there is no second copy on the host and no
+    page faulting of fresh result arrays.  The block returns to the pool when
+    the last of its views is garbage (a finalizer on each view's tensor);
+    a batch that finds no free block of its size falls back to the staging
+    buffer and a copy into ordinary host arrays.  First fit over a free list
+    kept sorted by offset, with neighbours merged on release."""
+
+    GRANULE = 1 << 16
+
+    def __init__(self, nbytes: int, buf=None):
+        import torch
+
+        self.nbytes = int(nbytes) // self.GRANULE * self.GRANULE
+        # ``buf``: a uint8 tensor to carve up instead of a new page-locked one
+        self.buf = (buf[:self.nbytes] if buf is not None
+                    else torch.empty(self.nbytes, dtype=torch.uint8, pin_memory=True))
+        self.free = [(0, self.nbytes)] if self.nbytes > 0 else []
+        self.lock = threading.Lock()
+
+    def take(self, nbytes: int):
+        """(offset, size) of a free block of at least nbytes, or None."""
+        size = -(-int(nbytes) // self.GRANULE) * self.GRANULE
+        with self.lock:
+            for k, (off, n) in enumerate(self.free):
+                if n >= size:
+                    if n == size:
+                        del self.free[k]
+                    else:
+                        self.free[k] = (off + size, n - size)
+                    return off, size
+        return None
+
+    def give(self, off: int, size: int) -> None:
+        import bisect
+
+        with self.lock:
+            k = bisect.bisect_left(self.free, (off, 0))
+            self.free.insert(k, (off, size))
+            if k + 1 < len(self.free) and off + size == self.free[k + 1][0]:
+                self.free[k] = (off, size + self.free[k + 1][1])
+                del self.free[k + 1]
+            if k > 0 and self.free[k - 1][0] + self.free[k - 1][1] == off:
+                self.free[k - 1] = (self.free[k - 1][0], self.free[k - 1][1] + self.free[k][1])
+                del self.free[k]
+
+    def free_bytes(self) -> int:
+        with self.lock:
+            return sum(n for _, n in self.free)
+
+
+class _Lease:
+    """One batch's block of a PinnedResultPool, returned when its count of
+    holders (the batch itself plus every view handed out) reaches zero."""
+
+    def __init__(self, pool: PinnedResultPool, off: int, size: int):
+        self.pool, self.off, self.size = pool, off, size
+        self.holders = 1  # the batch, until all of its results are delivered
+        self.lock = threading.Lock()
+        import torch
+
+        self.c128 = pool.buf[off:off + size].view(torch.complex128)
+
+    def hold(self) -> None:
+        with self.lock:
+            self.holders += 1
+
+    def drop(self) -> None:
+        with self.lock:
+            self.holders -= 1
+            last = self.holders == 0
+        if last:
+            self.pool.give(self.off, self.size)
+
+    def array(self, a: int, b: int, shape):
+        """NumPy view of complex elements [a, b) of the block, holding it."""
+        import weakref
+
+        base = self.c128[a:b].numpy()
+        self.hold()
+        # every NumPy view of the result (the reshape below, a caller's
+        # slices) refers to `base`, so it dies with the last of them
+        weakref.finalize(base, self.drop)
+        return base.reshape(shape)
+
+
 class DeviceBackend:
     """HIP backend: persistent multi-model launches and device LM inversions.
 
@@ -151,15 +241,18 @@ class DeviceBackend:
     for them, and its outputs come back into the staging buffer on the copy
     stream once the kernel is done (pinned copies run at the link rate;
     pageable memory would be staged by the runtime at a fraction of it).
-    ``results`` copies them into one host array per request on
-    ``copy_threads`` threads while later batches run.  The server caps
+    Host results go straight into a block of the ``result_pool_bytes``
+    page-locked result pool that the caller's arrays then view
+    (``PinnedResultPool``); without a free block they come back into the
+    slot's staging buffer and ``results_each`` copies them into host arrays
+    on ``copy_threads`` threads while later batches run.  The server caps
     batches at ``max_batch_bytes`` (the staging size); a single request larger
     than that gets a one-off pinned buffer."""
 
     def __init__(self, n_streams: int = 3, shape=(64, 8), device=None,
                  staging_bytes: int = 512 << 20, copy_threads: int = 8,
                  kernel_timing: bool = False, lm_streams: int = 1,
-                 lm_launch_updates: float = 5e7):
+                 lm_launch_updates: float = 5e7, result_pool_bytes: int = 2 << 30):
         import torch
 
         from . import _abi
@@ -185,6 +278,7 @@ class DeviceBackend:
         self.max_batch_bytes = int(staging_bytes)
         self.staging = [torch.empty(self.max_batch_bytes, dtype=torch.uint8, pin_memory=True)
                         for _ in range(n_streams)]
+        self.result_pool = PinnedResultPool(result_pool_bytes) if result_pool_bytes > 0 else None
         self._free = list(range(n_streams))
         self._lock = threading.Lock()
         self.pool = ThreadPoolExecutor(max_workers=copy_threads,
@@ -260,11 +354,22 @@ class DeviceBackend:
                 counts_ev = torch.cuda.Event()
                 counts_ev.record(cp)
                 host = None
+                lease = None
+                h0 = 0  # first complex element of R that the host buffer holds
                 groups = []  # (request indices, event) in copy order
                 if any(to_host):
-                    if 16 * total <= self.max_batch_bytes:
+                    hosts = [i for i, h in enumerate(to_host) if h]
+                    h0 = int(plan.r_off[min(hosts)])
+                    span = int(plan.r_off[max(hosts) + 1]) - h0
+                    blk = self.result_pool.take(16 * span) if self.result_pool else None
+                    if blk is not None:
+                        lease = _Lease(self.result_pool, *blk)
+                        host = lease.c128
+                    elif 16 * total <= self.max_batch_bytes:
+                        h0 = 0
                         host = self.staging[slot][:16 * total].view(torch.complex128)
                     else:
+                        h0 = 0
                         host = torch.empty(total, dtype=torch.complex128, pin_memory=True)
                     # host rows go down in a few contiguous copies of growing
                     # size, each with its own event, so the requests of the
@@ -276,7 +381,7 @@ class DeviceBackend:
                     for i in [i for i, h in enumerate(to_host) if h] + [None]:
                         lo = int(plan.r_off[i]) if i is not None else -1
                         if run and (i is None or lo != b or 16 * (b - a) >= cap):
-                            host[a:b].copy_(plan.R[a:b], non_blocking=True)
+                            host[a - h0:b - h0].copy_(plan.R[a:b], non_blocking=True)
                             e = torch.cuda.Event()
                             e.record(cp)
                             groups.append((run, e))
@@ -293,8 +398,9 @@ class DeviceBackend:
         except BaseException:
             self._release_slot(slot)
             raise
-        return {"plan": plan, "host": host, "to_host": list(to_host), "counts": counts,
-                "counts_event": counts_ev, "groups": groups, "event": ev, "slot": slot,
+        return {"plan": plan, "host": host, "lease": lease, "h0": h0, "to_host": list(to_host),
+                "counts": counts, "counts_event": counts_ev, "groups": groups, "event": ev,
+                "slot": slot,
                 "timing": t_ev,
                 "updates": sum(x.updates() for x in specs)}
 
@@ -321,11 +427,13 @@ class DeviceBackend:
         """Hands each request of a batch its result as soon as it is complete:
         ``deliver(i, result, nonfinite_count)`` for every request i, device
         results right after the kernel, then host results in their copy order
-        (smallest first), each after its own rows have arrived in the staging
-        buffer and been copied out into a host array by the copy pool.  Frees
+        (smallest first), each once its copy group has arrived: as a view of
+        the batch's page-locked result block, or (no free block) after a copy
+        out of the staging buffer into a host array on the copy pool.  Frees
         the slot at the end."""
         from . import batch
 
+        lease = handle.get("lease")
         try:
             plan, to_host = handle["plan"], handle["to_host"]
             handle["counts_event"].synchronize()
@@ -334,7 +442,14 @@ class DeviceBackend:
             for i, h in enumerate(to_host):
                 if not h:
                     deliver(i, dev_views[i], counts[i])
-            if handle["groups"]:
+            h0 = handle["h0"]
+            if lease is not None:  # rows copied straight into callers' memory
+                for run, e in handle["groups"]:
+                    e.synchronize()
+                    for i in run:
+                        a, b = int(plan.r_off[i]) - h0, int(plan.r_off[i + 1]) - h0
+                        deliver(i, lease.array(a, b, plan.shapes[i]), counts[i])
+            elif handle["groups"]:
                 flat = handle["host"].numpy()
                 chunk = 1 << 19  # complex elements (8 MiB) per copy job
                 for run, e in handle["groups"]:
@@ -361,12 +476,18 @@ class DeviceBackend:
                     for i in run:
                         deliver(i, outs[i], counts[i])
             handle["event"].synchronize()
+            if lease is not None:
+                lease.drop()  # the batch's own hold; the views keep the block
+                lease = None
             if handle["timing"] is not None:
                 a, b = handle["timing"]
                 with self._lock:
                     self.kernel_log.append((a.elapsed_time(b), handle["updates"],
                                             int(plan.r_off[-1])))
         finally:
+            if lease is not None:  # an error before the batch let go of its block
+                handle["event"].synchronize()  # no copy may still target the block
+                lease.drop()
             self._release_slot(handle["slot"])
 
     def invert(self, spec: InvertSpec):

@@ -205,3 +205,40 @@ def test_inversion_above_the_derivative_cap_fails_alone():
         with pytest.raises(ValueError):
             f_bad.result(timeout=30)
         assert f_ok.result(timeout=30) == {"iterations": 5}
+
+
+def test_result_pool_blocks_return_when_the_last_view_dies():
+    """The page-locked result pool's bookkeeping on a plain host buffer: a
+    batch's block stays taken while any result view (or a NumPy view of
+    one) is alive, returns when the last one is garbage, and freed
+    neighbours merge back into one block."""
+    import gc
+
+    import torch
+
+    from seismicinversion_amd.serve import PinnedResultPool, _Lease
+
+    g = PinnedResultPool.GRANULE
+    pool = PinnedResultPool(8 * g, buf=torch.zeros(8 * g, dtype=torch.uint8))
+    b1, b2 = pool.take(3 * g - 5), pool.take(2 * g)
+    assert b1 == (0, 3 * g) and b2 == (3 * g, 2 * g) and pool.free_bytes() == 3 * g
+    assert pool.take(4 * g) is None  # no block that large is free
+    l1, l2 = _Lease(pool, *b1), _Lease(pool, *b2)
+    x = l1.array(0, 4, (2, 2))
+    y = l1.array(4, 10, (3, 2))
+    x[:] = 1 + 2j
+    assert np.all(l1.c128[:4].numpy() == 1 + 2j)  # the view is the block's memory
+    z = l2.array(0, 8, (8,))
+    l1.drop()
+    l2.drop()  # the batches let go; their views still hold the blocks
+    assert pool.free_bytes() == 3 * g
+    keep = y[1:]  # a NumPy view of a result keeps the block too
+    del x, y
+    gc.collect()
+    assert pool.free_bytes() == 3 * g
+    del keep
+    gc.collect()
+    assert pool.free_bytes() == 6 * g
+    del z
+    gc.collect()
+    assert pool.free == [(0, 8 * g)]  # merged back into one block
