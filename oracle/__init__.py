@@ -22,6 +22,12 @@ Pinning status of each function (what checks it besides its own formula):
   invariance, the two-layer closed form, the free-surface identity.
 * ``bessel_sum``, ``hermitian_ifft``, ``gather`` — golden gather written by the
   reference's float64 ``compute_gather``; J0 against ``scipy.special.j0``.
-* ``torch_reference`` — AD through the torch restatement against the golden
+* ``kennett_torch_ref`` — AD through the torch restatement against the golden
   Jacobian/Hessian of the reference and central finite differences.
+* ``reflectivity_batch(..., ctype=np.clongdouble)`` — the same operations in
+  x87 extended precision, the exact value of the parity tests: against
+  40-digit mpmath (``kennett_mp``), which satisfies the free-surface identity
+  to its own precision.
+* ``taup_lm_ref`` — the LM loop against the reference's own 3-iteration
+  invert_taup trajectory (tests/golden/invert_taup_2pct.npz) to 1e-12.
 """

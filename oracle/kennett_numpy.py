@@ -13,7 +13,12 @@ Follows, operation for operation:
 
 Everything is vectorised over slowness (no per-p Python loop), in the
 reference's (n_p, n_omega, 2, 2) layout, so it doubles as the timed CPU
-baseline.
+baseline.  ``ctype=np.clongdouble`` runs the same operations in x87 extended
+precision (64-bit significand, 2048 times finer than float64) on the float64
+inputs taken as exact: the reference value against which the rounding error
+of any float64 evaluation, the reference's own golden vectors included, is
+measured (pinned against 40-digit mpmath, oracle/kennett_mp.py, in
+tests/test_oracle.py).
 """
 
 from __future__ import annotations
@@ -35,14 +40,18 @@ __all__ = [
 ]
 
 
-def complex_slowness(v, Q):
+def _real(ctype):
+    return np.float64 if np.dtype(ctype) == np.complex128 else np.longdouble
+
+
+def complex_slowness(v, Q, ctype=np.complex128):
     """layer_model.py:42-45 (vectorised)."""
-    v = np.asarray(v, dtype=np.float64)
-    Q = np.asarray(Q, dtype=np.float64)
+    v = np.asarray(v, dtype=np.float64).astype(_real(ctype))
+    Q = np.asarray(Q, dtype=np.float64).astype(_real(ctype))
     two_q = 2.0 * Q
     two_q_sq = two_q * two_q
     denom = (1.0 + two_q_sq) * v
-    return (two_q_sq / denom) + 1j * (two_q / denom)
+    return ((two_q_sq / denom) + 1j * (two_q / denom)).astype(ctype)
 
 
 def vertical_slowness(s, p):
@@ -52,7 +61,7 @@ def vertical_slowness(s, p):
     return np.where(eta.imag <= 0.0, -eta, eta)
 
 
-def layer_slownesses(alpha, beta, Q_alpha, Q_beta, p):
+def layer_slownesses(alpha, beta, Q_alpha, Q_beta, p, ctype=np.complex128):
     """Per-layer slownesses for every p.
 
     Returns (eta, neta, beta_c) with eta, neta of shape (nl, np) and beta_c
@@ -60,12 +69,12 @@ def layer_slownesses(alpha, beta, Q_alpha, Q_beta, p):
     (kennett_reflectivity_gpu.py:139-147)."""
     alpha = np.asarray(alpha, np.float64)
     beta = np.asarray(beta, np.float64)
-    p = np.atleast_1d(np.asarray(p, np.float64)).astype(np.complex128)
-    s_p = complex_slowness(alpha, Q_alpha)
-    s_s = np.zeros(len(alpha), np.complex128)
+    p = np.atleast_1d(np.asarray(p, np.float64)).astype(ctype)
+    s_p = complex_slowness(alpha, Q_alpha, ctype)
+    s_s = np.zeros(len(alpha), ctype)
     pos = beta > 0
-    s_s[pos] = complex_slowness(beta[pos], np.asarray(Q_beta, np.float64)[pos])
-    beta_c = np.zeros(len(alpha), np.complex128)
+    s_s[pos] = complex_slowness(beta[pos], np.asarray(Q_beta, np.float64)[pos], ctype)
+    beta_c = np.zeros(len(alpha), ctype)
     nz = np.abs(s_s) > 0
     beta_c[nz] = 1.0 / s_s[nz]
     eta = vertical_slowness(s_p[:, None], p[None, :])
@@ -86,9 +95,10 @@ def _solid_solid(p, eta1, neta1, rho1, beta1, eta2, neta2, rho2, beta2):
     rtza2 = rteta2 * rtrho2
     rtzb1 = rtneta1 * rtrho1
     rtzb2 = rtneta2 * rtrho2
-    psq = (p * p).astype(np.complex128)
-    crho1 = complex(rho1)
-    crho2 = complex(rho2)
+    ctype = np.result_type(eta1, np.complex128)
+    psq = (p * p).astype(ctype)
+    crho1 = np.asarray(rho1).astype(ctype)
+    crho2 = np.asarray(rho2).astype(ctype)
     drho = crho2 - crho1
     dmu = crho2 * beta2 * beta2 - crho1 * beta1 * beta1
     d = 2.0 * dmu
@@ -113,9 +123,9 @@ def _solid_solid(p, eta1, neta1, rho1, beta1, eta2, neta2, rho2, beta2):
     v_v = (a * c + b * d * eta1 * neta1) * p / det
     m2ci = complex(0.0, -2.0)
     n = p.shape[0]
-    Rd = np.empty((n, 2, 2), np.complex128)
-    Ru = np.empty((n, 2, 2), np.complex128)
-    Td = np.empty((n, 2, 2), np.complex128)
+    Rd = np.empty((n, 2, 2), ctype)
+    Ru = np.empty((n, 2, 2), ctype)
+    Td = np.empty((n, 2, 2), ctype)
     Rd[:, 0, 0] = q_v - r_v
     Rd[:, 0, 1] = m2ci * rteta1 * rtneta1 * s_v
     Rd[:, 1, 0] = m2ci * rteta1 * rtneta1 * s_v
@@ -142,9 +152,10 @@ def _ocean_bottom(p, eta1, rho1, eta2, neta2, rho2, beta2):
     rtza1 = rtrho1 * rteta1
     rtza2 = rtrho2 * rteta2
     rtzb2 = rtrho2 * rtneta2
+    ctype = np.result_type(eta1, np.complex128)
     psq = p * p
-    crho1 = complex(rho1)
-    crho2 = complex(rho2)
+    crho1 = np.asarray(rho1).astype(ctype)
+    crho2 = np.asarray(rho2).astype(ctype)
     drho = crho2 - crho1
     dmu = crho2 * beta2 * beta2
     d = 2.0 * dmu
@@ -172,10 +183,10 @@ def _ocean_bottom(p, eta1, rho1, eta2, neta2, rho2, beta2):
     pdsd = 2.0 * mci * rtza1 * rtzb2 * H * p
     pusd = 2.0 * mci * rteta2 * rtneta2 * t6
     n = p.shape[0]
-    Rd = np.zeros((n, 2, 2), np.complex128)
-    Ru = np.zeros((n, 2, 2), np.complex128)
-    Tu = np.zeros((n, 2, 2), np.complex128)
-    Td = np.zeros((n, 2, 2), np.complex128)
+    Rd = np.zeros((n, 2, 2), ctype)
+    Ru = np.zeros((n, 2, 2), ctype)
+    Tu = np.zeros((n, 2, 2), ctype)
+    Td = np.zeros((n, 2, 2), ctype)
     Rd[:, 0, 0] = pdpu
     Td[:, 0, 0] = pdpd
     Td[:, 1, 0] = pdsd
@@ -188,17 +199,17 @@ def _ocean_bottom(p, eta1, rho1, eta2, neta2, rho2, beta2):
     return Rd, Ru, Tu, Td
 
 
-def interface_coefficients(model, p):
+def interface_coefficients(model, p, ctype=np.complex128):
     """Rd, Ru, Tu, Td for every interface and p: each (nl-1, np, 2, 2).
 
     Interface 0 is the ocean bottom, 1..nl-2 solid-solid
     (kennett_reflectivity_gpu.py:150-191)."""
-    p = np.atleast_1d(np.asarray(p, np.float64))
+    p = np.atleast_1d(np.asarray(p, np.float64)).astype(_real(ctype))
     eta, neta, beta_c = layer_slownesses(model.alpha, model.beta, model.Q_alpha,
-                                         model.Q_beta, p)
+                                         model.Q_beta, p, ctype)
     nl = model.n_layers
-    out = [np.zeros((nl - 1, len(p), 2, 2), np.complex128) for _ in range(4)]
-    rho = model.rho
+    out = [np.zeros((nl - 1, len(p), 2, 2), ctype) for _ in range(4)]
+    rho = np.asarray(model.rho, np.float64).astype(_real(ctype))
     coeffs = _ocean_bottom(p, eta[0], rho[0], eta[1], neta[1], rho[1], beta_c[1])
     for k in range(4):
         out[k][0] = coeffs[k]
@@ -225,27 +236,29 @@ def _inv2x2(M):
     return inv
 
 
-def reflectivity_batch(model, p_samples, omega, free_surface=False):
-    """kennett_reflectivity_gpu.py:88-275: R of shape (np, nw), complex128."""
+def reflectivity_batch(model, p_samples, omega, free_surface=False, ctype=np.complex128):
+    """kennett_reflectivity_gpu.py:88-275: R of shape (np, nw), complex128
+    (or ``ctype``)."""
     p_samples = np.atleast_1d(np.asarray(p_samples, np.float64))
-    omega_c = np.asarray(omega).astype(np.complex128)
+    omega_c = np.asarray(omega).astype(np.complex128).astype(ctype)
     nl = model.n_layers
     nfreq = omega_c.shape[0]
     eta, neta, _ = layer_slownesses(model.alpha, model.beta, model.Q_alpha,
-                                    model.Q_beta, p_samples)
-    Rd_all, Ru_all, Tu_all, Td_all = interface_coefficients(model, p_samples)
+                                    model.Q_beta, p_samples, ctype)
+    Rd_all, Ru_all, Tu_all, Td_all = interface_coefficients(model, p_samples, ctype)
 
     def phases(i):
-        h = model.thickness[i]
+        h = np.float64(model.thickness[i])
         if np.isinf(h):
-            one = np.ones((len(p_samples), nfreq), np.complex128)
+            one = np.ones((len(p_samples), nfreq), ctype)
             return one, one
+        h = _real(ctype)(h)
         ea = np.exp(1j * eta[i, :, None] * omega_c[None, :] * h)
         eb = np.exp(1j * neta[i, :, None] * omega_c[None, :] * h)
         return ea, eb
 
-    RRd = np.zeros((len(p_samples), nfreq, 2, 2), np.complex128)
-    eye = np.eye(2, dtype=np.complex128)
+    RRd = np.zeros((len(p_samples), nfreq, 2, 2), ctype)
+    eye = np.eye(2, dtype=ctype)
     for iface in range(nl - 2, -1, -1):
         ea, eb = phases(iface + 1)
         eaea = ea ** 2

@@ -9,7 +9,7 @@ import pytest
 import torch
 
 from oracle import gmm_numpy as G
-from parity import assert_rows_close, row_tolerance
+from parity import assert_rows_close, ulp_tolerance
 from seismicinversion_amd import engine
 from seismicinversion_amd.global_matrix import gmm_reflectivity, gmm_reflectivity_batch
 from seismicinversion_amd.layer_model import LayerModel
@@ -26,7 +26,10 @@ def test_gmm_kernel_vs_oracle(fs):
         p = np.array([0.05, 0.15, 0.3, 0.45, 0.6, 0.8])
         R = gmm_reflectivity_batch(m, p, W, fs)
         O = np.stack([G.riccati_reflectivity(m, pv, W, fs) for pv in p])
-        assert_rows_close(R, O, np.maximum(row_tolerance(m, p, W, fs), 1e-11), "gmm vs oracle")
+        # a different formulation (4x4 LU per layer) against its own oracle:
+        # its rounding differs from the Kennett recursion's, hence the floor
+        assert_rows_close(R, O, np.maximum(ulp_tolerance(m, p, W, fs, factor=16.0), 1e-11),
+                          "gmm vs oracle")
         assert np.all(np.isfinite(R))
 
 

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from oracle import kennett_numpy as K
-from parity import row_tolerance
+from parity import ulp_tolerance
 from seismicinversion_amd import kennett_gather_gpu, kennett_seismogram
 from seismicinversion_amd.kennett_gather_gpu import compute_gather_gpu
 from seismicinversion_amd.kennett_seismogram import compute_seismogram
@@ -24,7 +24,7 @@ def oracle_seismogram(model, p, T, nw, eps, fs):
     R = K.reflectivity_batch(model, np.array([p]), w, fs)
     Y = R * K.ricker_spectrum(w, nw * dw)[None, :]
     t, tr = K.hermitian_ifft(Y, nw, -eps, T)
-    return t, tr[0], row_tolerance(model, np.array([p]), w, fs)[0]
+    return t, tr[0], ulp_tolerance(model, np.array([p]), w, fs)[0]
 
 
 @pytest.mark.parametrize("p,eps,fs,nw", [(0.2, 0.0, False, 2048), (0.1, 0.05, True, 1024),
