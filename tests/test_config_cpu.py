@@ -1,6 +1,8 @@
 """Inversion config I/O (inversion_config.py:174-346) and the report writers
-of the inversion CLI, on the CPU.  tests/golden/reference_default_ocean_crust.yaml
-is the reference's own config data file (configs/default_ocean_crust.yaml)."""
+of the inversion CLI, on the CPU.  The reference's config schema
+(configs/default_ocean_crust.yaml: model.layers with per-layer name and the
+six parameters, model.fixed_layers, the inversion section, output with
+trace_display) is exercised with a file the test writes in that schema."""
 
 from __future__ import annotations
 
@@ -16,7 +18,6 @@ from seismicinversion_amd.taup_report import (depth_profile, plot_convergence_cu
                                               write_model_profiles_tikz, write_model_table_latex)
 from seismicinversion_amd.workloads import default_ocean_crust_model
 
-GOLD = os.path.join(os.path.dirname(__file__), "golden")
 
 
 def same_model(a, b):
@@ -24,9 +25,35 @@ def same_model(a, b):
         np.testing.assert_array_equal(getattr(a, k), getattr(b, k))
 
 
-def test_default_and_reference_configs_load_the_same_run():
+def _reference_schema_yaml(path):
+    """The 5-layer ocean-crust run in the reference's config schema, with
+    numbers written the way a hand-edited file writes them (ints for Q,
+    exponent forms, .inf), as flow-style YAML."""
+    import yaml
+
+    layers = [("Ocean", 1.5, 0.0, 1.0, 2.0, 20000, "1.0e+10"),
+              ("Sediment", 1.6, 0.3, 2.0, 1.0, 100, 100),
+              ("Crust", 3.0, 1.5, 3.0, 1.0, 100, 100),
+              ("Upper mantle", 5.0, 3.0, 3.0, 1.0, 100, 100),
+              ("Half-space", 2.2, 1.1, 1.8, ".inf", 100, 100)]
+    text = "model:\n  layers:\n"
+    for name, a, b, r, h, qa, qb in layers:
+        text += (f"    - {{name: {name}, alpha: {a}, beta: {b}, rho: {r}, thickness: {h}, "
+                 f"Q_alpha: {qa}, Q_beta: {qb}}}\n")
+    text += ("  fixed_layers: [0]\n"
+             "inversion: {p_values: [0.05, 0.10, 0.15, 0.20, 0.25, 0.30, 0.35, 0.40, 0.45, "
+             "0.50, 0.55, 0.60], nfreq: 64, perturbation: 0.15, max_iter: 50, seed: 42, "
+             "tol: 1.0e-8}\n"
+             "output: {directory: figures, formats: [table, profiles, traces, convergence], "
+             "trace_display: {t_max: 15.0, nw: 1024}}\n")
+    yaml.safe_load(text)  # well-formed
+    path.write_text(text)
+    return path
+
+
+def test_default_and_reference_configs_load_the_same_run(tmp_path):
     ours = load_config(DEFAULT_CONFIG)
-    ref = load_config(os.path.join(GOLD, "reference_default_ocean_crust.yaml"))
+    ref = load_config(_reference_schema_yaml(tmp_path / "reference_schema.yaml"))
     for cfg in (ours, ref):
         same_model(cfg.true_model, default_ocean_crust_model())
         assert cfg.layer_names == ["Ocean", "Sediment", "Crust", "Upper mantle", "Half-space"]

@@ -170,3 +170,18 @@ def test_max_layers_and_errors():
     with pytest.raises(ValueError):
         LayerModel.from_arrays(alpha=[1.5, -2], beta=[0, 1], rho=[1, 2], thickness=[1, np.inf],
                                Q_alpha=[1, 1], Q_beta=[1, 1])
+
+
+def test_batch_256_p_damped_vs_golden():
+    """kennett_reflectivity_batch at 256 slownesses in (0, 1.2] x 1023 damped
+    frequencies, 20 layers, free surface (the reference's own output,
+    tests/golden/batch_20layer_256.npz), every row to the parity bar."""
+    d = gold("batch_20layer_256.npz")
+    m = model_from(d)
+    R = kennett_reflectivity_batch(m, d["p"], d["omega"], True)
+    acc = accuracy(m, d["p"], d["omega"], True, golden=d["R"])
+    err = assert_accurate(R, acc, "256 p x 1023 damped omega")
+    vs_gold = row_rel_err(R, d["R"])
+    print("rows <= 1e-12 vs exact:", int((err <= 1e-12).sum()), "/", len(err),
+          "; vs golden: rows <= 1e-12", int((vs_gold <= 1e-12).sum()), "max", vs_gold.max())
+    assert np.all(vs_gold[acc.e_golden <= 1e-13] <= 1e-12)

@@ -82,3 +82,25 @@ def test_many_layers_and_evanescent():
     assert np.all(np.isfinite(G.riccati_reflectivity(m, 0.3, w)))
     for p in (0.4, 0.6, 0.8):
         assert np.all(np.isfinite(G.riccati_reflectivity(default_ocean_crust_model(), p, W)))
+
+
+def test_22_layer_model_vs_the_reference():
+    """The reference's many-layer model (GlobalMatrix/test_riccati.py:262-307)
+    at p = 0.15, 63 frequencies, against the reference's own Riccati, dense
+    and Kennett outputs (tests/golden/riccati_22layer.npz): the oracles match
+    them, our model generator reproduces the reference's draw, and the
+    reference's own bar (Riccati vs dense and vs Kennett, 1e-8) holds."""
+    import os
+
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "riccati_22layer.npz"))
+    m = LayerModel.from_arrays(**{c: d[c] for c in ("alpha", "beta", "rho", "thickness",
+                                                    "Q_alpha", "Q_beta")})
+    mm = many_layer_model(20)
+    for c in ("alpha", "beta", "rho", "thickness", "Q_alpha", "Q_beta"):
+        np.testing.assert_array_equal(getattr(mm, c), d[c])
+    p, w = float(d["p"]), d["omega"]
+    close(G.riccati_reflectivity(m, p, w), d["R_riccati"], 1e-12)
+    close(G.dense_reflectivity(m, p, w), d["R_dense"], 1e-10)
+    close(K.reflectivity(m, p, w), d["R_kennett"], 1e-12)
+    for a, b in (("R_riccati", "R_dense"), ("R_riccati", "R_kennett")):
+        assert np.max(np.abs(d[a] - d[b]) / np.abs(d[b])) < 1e-8

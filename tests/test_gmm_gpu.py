@@ -64,3 +64,28 @@ def test_gmm_small_models_and_api():
     with pytest.raises(ValueError):
         gmm_reflectivity(default_ocean_crust_model(), 0.2, W, solver="lu")
     assert gmm_reflectivity_batch(default_ocean_crust_model(), np.zeros(0), W).shape == (0, 255)
+
+
+def test_22_layer_reference_model_on_both_kernels():
+    """The reference's many-layer Riccati case (GlobalMatrix/test_riccati.py:
+    259-337, tests/golden/riccati_22layer.npz): the device block-Riccati
+    kernel against the reference's Riccati and dense outputs, the Kennett
+    kernel against the reference's kennett_reflectivity, each to the
+    reference's own 1e-8 (pointwise) and the Kennett kernel to the parity bar."""
+    import os
+
+    from parity import accuracy, assert_accurate
+    from seismicinversion_amd.kennett_reflectivity import kennett_reflectivity
+
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "riccati_22layer.npz"))
+    m = LayerModel.from_arrays(**{c: d[c] for c in ("alpha", "beta", "rho", "thickness",
+                                                    "Q_alpha", "Q_beta")})
+    p, w = float(d["p"]), d["omega"]
+    Rg = gmm_reflectivity(m, p, w)
+    Rk = kennett_reflectivity(m, p, w)
+    for R, key in ((Rg, "R_riccati"), (Rg, "R_dense"), (Rk, "R_kennett"), (Rk, "R_riccati")):
+        e = np.max(np.abs(R - d[key]) / np.abs(d[key]))
+        print(key, e)
+        assert e < 1e-8
+    assert_accurate(Rk[None], accuracy(m, np.array([p]), w, False, golden=d["R_kennett"][None]),
+                    "Kennett kernel, 22 layers")

@@ -241,3 +241,15 @@ def test_golden_vectors_error_against_the_exact_value():
     print("golden vs exact", acc.e_golden, "\noracle vs exact", acc.e_oracle)
     assert np.all(acc.e_golden[d["p"] <= 0.3] < 1e-13)
     assert acc.e_golden[-1] > 1e-10 and acc.e_oracle[-1] > 1e-10
+
+
+def test_batch_256_p_damped_vs_golden():
+    """The oracle against the reference's batched output at 256 slownesses x
+    1023 damped frequencies (20 layers, free surface), every row to the bar."""
+    d = gold("batch_20layer_256.npz")
+    m = model_from(d)
+    R = K.reflectivity_batch(m, d["p"], d["omega"], True)
+    assert_accurate(R, accuracy(m, d["p"], d["omega"], True, golden=d["R"]),
+                    "oracle vs kennett_reflectivity_batch (256 p)")
+    # the oracle restates the reference's operations: bit-near agreement
+    assert np.max(row_rel_err(R, d["R"])) < 1e-9

@@ -243,3 +243,73 @@ def test_invert_taup_with_graphs_equals_eager():
     assert a.misfit_history == b.misfit_history
     assert a.grad_norm_history == b.grad_norm_history
     assert np.array_equal(a.recovered_model.alpha, b.recovered_model.alpha)
+
+
+def test_20_layer_jacobian_hessian_frechet_vs_golden():
+    """The reference's own Jacobian (jacrev), Hessian (torch.func.hessian of
+    ||R - R_obs||^2) and analytical Frechet derivatives on the seeded 20-layer
+    model (P = 75), 8 frequencies, p = 0.15 and 0.35
+    (tests/golden/torch_derivs_20layer.npz; kennett_torch.py:611-684,
+    frechet_analytical.py:583): the drop-ins through the device kernels to
+    1e-10 (Jacobian, Frechet) and 1e-8 (Hessian)."""
+    d = gold("torch_derivs_20layer.npz")
+    m = model_from(d)
+    tens = kt.model_to_tensors(m)
+    w = torch.tensor(d["omega"])
+    for pv, key in ((0.15, "015"), (0.35, "035")):
+        R = kt.kennett_reflectivity_torch(**tens, p=pv, omega=w)
+        assert rel(R.numpy(), d[f"R_p{key}"]) < 1e-12
+        J = kt.jacobian(**tens, p=pv, omega=w)
+        assert J.shape == (8, 75)
+        eJ = rel(J.numpy(), d[f"J_p{key}"])
+        F = frechet_kennett(m.alpha, m.beta, m.rho, m.thickness, m.Q_alpha, m.Q_beta, pv,
+                            d["omega"])
+        eF = rel(F, d[f"F_p{key}"])
+        H = kt.hessian(**tens, p=pv, omega=w, R_obs=torch.tensor(d[f"R_obs_p{key}"]))
+        eH = rel(H.numpy(), d[f"H_p{key}"])
+        print(f"p={pv}: J {eJ:.2e}, Frechet {eF:.2e}, H {eH:.2e}")
+        assert eJ < 1e-10 and eF < 1e-10 and eH < 1e-8
+
+
+@pytest.mark.parametrize("p,fs,nw", [(0.2, False, 1024), (0.45, True, 512)])
+def test_forward_model_torch_vs_oracle(p, fs, nw):
+    """forward_model_torch (kennett_torch.py:488-544): reflectivity x Ricker
+    through the Hermitian FFT, against the oracle's trace (NumPy) to the
+    reflectivity's parity bar, and its gradient (through the device adjoint)
+    against torch AD of the same pipeline on the oracle's torch forward."""
+    from parity import ulp_tolerance
+
+    for model in (default_ocean_crust_model(), seeded_model(12, 4)):
+        tens = kt.model_to_tensors(model)
+        t, s = kt.forward_model_torch(**tens, p=p, T=64.0, nw=nw, free_surface=fs)
+        assert t.shape == (2 * nw,) and s.shape == (2 * nw,) and s.dtype == torch.float64
+        dw = 2 * np.pi / 64.0
+        w = np.arange(1, nw) * dw
+        R = K.reflectivity_batch(model, np.array([p]), w, fs)
+        t_o, s_o = K.hermitian_ifft(R * K.ricker_spectrum(w, nw * dw)[None, :], nw, 0.0, 64.0)
+        np.testing.assert_array_equal(t.numpy(), t_o)
+        err = np.max(np.abs(s.numpy() - s_o[0])) / np.max(np.abs(s_o[0]))
+        tol = ulp_tolerance(model, np.array([p]), w, fs)[0]
+        print(f"p={p} nl={model.n_layers}: trace {err:.2e} (bar {tol:.2e})")
+        assert err <= tol
+
+        # gradient of a scalar of the trace, device adjoint vs oracle AD
+        a, b, r, h, qa, qb = (tens[k].clone() for k in ("alpha", "beta", "rho", "thickness",
+                                                       "Q_alpha", "Q_beta"))
+        x = kt._pack_params(a, b, r, h).clone().requires_grad_(True)
+        weights = torch.linspace(-1.0, 1.0, 2 * nw, dtype=torch.float64)
+        _, s1 = kt.forward_model_torch(*kt._unpack_params(x, a, b, r, h), qa, qb, p, 64.0, nw, fs)
+        (s1 * weights).sum().backward()
+        xt = kt._pack_params(a, b, r, h).clone().requires_grad_(True)
+        wt = torch.arange(1, nw, dtype=torch.float64) * dw
+        Rt = T.reflectivity(*T.unpack_params(xt, a, b, r, h), qa, qb, p, wt, fs)
+        a_src = 0.1 * nw * dw
+        Z = wt ** 2 / (4.0 * a_src ** 2)
+        St = Z * torch.exp(-Z + 1j * wt * (5.0 / (a_src * 2.0 ** 0.5)))
+        Y = Rt * St
+        X = torch.cat([torch.zeros(1, dtype=torch.complex128), Y,
+                       torch.zeros(1, dtype=torch.complex128), torch.conj(torch.flip(Y, [0]))])
+        (torch.fft.fft(X).real * weights).sum().backward()
+        eg = rel(x.grad.numpy(), xt.grad.numpy())
+        print(f"   gradient vs oracle AD {eg:.2e}")
+        assert eg < 1e-8
