@@ -285,6 +285,23 @@ int kh_misfit_hvp(const double* layers, const double* dlayers, int ndir, int nl,
                   double* grad, double* hvp, double* dR, void* workspace,
                   size_t workspace_bytes, void* stream);
 
+/* kh_misfit_hvp computing, for each direction d > 0, only the entries of
+ * the layers 0 .. l_d + 1 of hvp[d], with l_d the deepest layer in which
+ * dlayers[d] is nonzero; the entries of deeper layers are written as 0
+ * (direction 0, whose sweep also yields grad, is complete).
+ * The reverse sweep stops at interface l_d + 1, so for the basis direction
+ * of a parameter of layer l the launch does the sweep work of layers 0..l+1
+ * only.  Since the Hessian is symmetric, a caller passing the basis
+ * direction of every parameter assembles the whole matrix: entry (j, k)
+ * with layer(j) > layer(k) + 1 is entry (k, j) of direction j.  chi, grad
+ * and dR are as kh_misfit_hvp's (complete).  Replaces, with that assembly,
+ * the torch.func.hessian of the misfit (Kennett_Reflectivity/taup_inversion.py:177). */
+int kh_misfit_hvp_partial(const double* layers, const double* dlayers, int ndir, int nl,
+                          const double* p, int np, const double* omega, int nw,
+                          int free_surface, const double* R_obs, int second_only, double* chi,
+                          double* grad, double* hvp, double* dR, void* workspace,
+                          size_t workspace_bytes, void* stream);
+
 /* Directional derivative of the vector-Jacobian product: for cotangent G
  * (np, nw) with tangents dG (ndir, np, nw) (or NULL = 0) and parameter
  * directions dlayers, dgrad[d] = d/dt grad(m + t dm_d, G + t dG_d)

@@ -45,6 +45,8 @@ EXPORTS: dict[str, list] = {
                        ctypes.c_size_t, _vp],
     "kh_misfit_hvp": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _vp, _c_int,
                       _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp],
+    "kh_misfit_hvp_partial": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _vp,
+                              _c_int, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp],
     "kh_vjp_hvp": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _vp, _vp, _vp,
                    _vp, _vp, _vp, ctypes.c_size_t, _vp],
     "kh_interfaces_direct": [_c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,

@@ -124,6 +124,18 @@ struct WaveSink {
   }
 };
 
+// The deepest layer in which direction row `drow` ((nl, 6)) is nonzero,
+// -1 for a zero direction, nl - 1 when there is no direction (every
+// tangent may be nonzero).  The same for every thread of the workgroup.
+__device__ __forceinline__ int deepest_moved_layer(const double* __restrict__ drow, int nl) {
+  if (drow == nullptr) return nl - 1;
+  for (int l = nl - 1; l >= 0; --l) {
+    for (int c = 0; c < NCOL; ++c)
+      if (drow[(size_t)l * NCOL + c] != 0.0) return l;
+  }
+  return -1;
+}
+
 template <class Z>
 struct GlobalStore {
   Z* base;
@@ -216,7 +228,7 @@ __global__ void __launch_bounds__(AD_BLOCK_MAX) adjoint_kernel(
     const cd* __restrict__ Gin, const cd* __restrict__ dGin, int second_only, Z* __restrict__ ws,
     long nlanes,
     IfaceZ<Z>* __restrict__ cot_if, LayerCot<Z>* __restrict__ cot_ly, Z* __restrict__ chi_part,
-    cd* __restrict__ R, cd* __restrict__ dR, int ntiles, int ntp, int red_split) {
+    cd* __restrict__ R, cd* __restrict__ dR, int ntiles, int ntp, int red_split, int partial) {
   extern __shared__ double smem_a[];
   LayerPh<Z>* s_layer = reinterpret_cast<LayerPh<Z>*>(smem_a);
   IfaceZ<Z>* s_if = reinterpret_cast<IfaceZ<Z>*>(s_layer + nl);
@@ -244,6 +256,13 @@ __global__ void __launch_bounds__(AD_BLOCK_MAX) adjoint_kernel(
   const cd gin = Gin[(size_t)ip * nw + wc];
   const cd dgin = dGin ? dGin[((size_t)dir * np + ip) * nw + wc] : cd{0.0, 0.0};
   Z chi = lift<Z>(0.0);
+  // tangents exist only down to the deepest layer the direction moves; with
+  // `partial` the reverse sweep stops one interface below it, except for
+  // direction 0, whose cotangents also give the gradient (all layers)
+  const int ltan = std::is_same<Z, cd>::value
+                       ? nl : deepest_moved_layer(dlayers ? dlayers + (size_t)dir * nl * NCOL
+                                                          : nullptr, nl);
+  const int lrev = (partial && dir != 0) ? ltan + 1 : nl;
   const Z r = lane_adjoint<Z>(s_layer, s_if, nl, om, free_surface, st, sink, [&](const Z& Rv) {
     if (!valid) return lift<Z>(0.0);
     if (MISFIT) {
@@ -256,7 +275,7 @@ __global__ void __launch_bounds__(AD_BLOCK_MAX) adjoint_kernel(
     Z g = lift_c<Z>(gin);
     if constexpr (!std::is_same<Z, cd>::value) g.d = dgin;
     return g;
-  });
+  }, ltan, lrev);
   if (wave_ok && lane == 0) {
     const Z zero = lift<Z>(0.0);
     sink.cly[nl - 1] = LayerCot<Z>{zero, zero, zero};
@@ -294,7 +313,7 @@ __global__ void __launch_bounds__(CH_BLOCK) chain_kernel(const double* __restric
                              const LayerCot<Z>* __restrict__ cot_ly,
                              const Z* __restrict__ chi_part, int ntiles, int ntp,
                              double* __restrict__ grad, double* __restrict__ hvp,
-                             double* __restrict__ chi) {
+                             double* __restrict__ chi, int partial) {
   extern __shared__ double smem_c[];
   IfaceZ<Z>* s_if = reinterpret_cast<IfaceZ<Z>*>(smem_c);
   LayerCot<Z>* s_ly = reinterpret_cast<LayerCot<Z>*>(s_if + (nl - 1));
@@ -324,11 +343,19 @@ __global__ void __launch_bounds__(CH_BLOCK) chain_kernel(const double* __restric
   const Z zero = lift<Z>(0.0);
   const IfaceZ<Z> none{zero, zero, zero, zero, zero, zero, zero, zero, zero, zero};
   const double* dl = dlayers ? dlayers + (size_t)dir * nl * NCOL : nullptr;
+  // with `partial`, the reverse sweep stopped one interface below the
+  // deepest moved layer (direction 0 excepted): only the layers above it
+  // have whole cotangents
+  const int lmax = (partial && dir != 0) ? deepest_moved_layer(dl, nl) + 1 : nl - 1;
   for (int k = threadIdx.x; k < nl * NCOL; k += blockDim.x) {
     const int l = k / NCOL, col = k - (k / NCOL) * NCOL;
+    const size_t g = ((size_t)ip * nl + l) * NCOL + col;
+    if (l > lmax) {
+      if (hvp) hvp[(size_t)dir * np * nl * NCOL + g] = 0.0;
+      continue;
+    }
     const Z v = chain_column<Z>(layers, dl, nl, l, p[ip], l >= 1 ? s_if[l - 1] : none,
                                 l <= nl - 2 ? s_if[l] : none, s_ly[l], col);
-    const size_t g = ((size_t)ip * nl + l) * NCOL + col;
     if (grad && dir == 0) grad[g] = prim(v).r;
     if (hvp) hvp[(size_t)dir * np * nl * NCOL + g] = tangent_re(v);
   }
@@ -381,7 +408,7 @@ static int run_adjoint(const char* what, const double* layers, const double* dla
                        const double* Gin, const double* dGin, int second_only, double* chi,
                        double* grad,
                        double* hvp, double* R, double* dR, void* workspace, size_t ws_bytes,
-                       hipStream_t st) {
+                       hipStream_t st, int partial = 0) {
   const DerivLayout d = deriv_layout<Z>(nl, np, nw, ndir);
   KH_REQUIRE(workspace && ws_bytes >= d.total, KH_ERR_INVALID_ARGUMENT,
              "%s: workspace of %zu bytes needed, %zu given", what, d.total, ws_bytes);
@@ -404,12 +431,12 @@ static int run_adjoint(const char* what, const double* layers, const double* dla
                      reinterpret_cast<const cd*>(Gin), reinterpret_cast<const cd*>(dGin), second_only,
                      reinterpret_cast<Z*>(wsb + d.ws), d.nlanes, cif, cly, chip,
                      reinterpret_cast<cd*>(R), reinterpret_cast<cd*>(dR), d.ntiles, d.ntp,
-                     split ? 1 : 0);
+                     split ? 1 : 0, partial);
   KH_TRY(check_launch(what));
   const size_t lds_c = (size_t)(nl - 1) * sizeof(IfaceZ<Z>) + (size_t)nl * sizeof(LayerCot<Z>);
   hipLaunchKernelGGL((chain_kernel<Z>), dim3((unsigned)((long)ndir * np)), dim3(CH_BLOCK), lds_c, st,
                      layers, dlayers, nl, p, np, ndir, cif, cly, MISFIT ? chip : nullptr,
-                     d.ntiles, d.ntp, grad, hvp, chi);
+                     d.ntiles, d.ntp, grad, hvp, chi, partial);
   return check_launch(what);
 }
 
@@ -484,6 +511,23 @@ extern "C" int kh_misfit_hvp(const double* layers, const double* dlayers, int nd
                                 free_surface, R_obs, nullptr, second_only, chi, grad, hvp, nullptr,
                                 dR,
                                 workspace, workspace_bytes, as_stream(stream));
+}
+
+extern "C" int kh_misfit_hvp_partial(const double* layers, const double* dlayers, int ndir,
+                                     int nl, const double* p, int np, const double* omega,
+                                     int nw, int free_surface, const double* R_obs,
+                                     int second_only, double* chi, double* grad, double* hvp,
+                                     double* dR, void* workspace, size_t workspace_bytes,
+                                     void* stream) {
+  clear_error();
+  KH_TRY(check_deriv_args("kh_misfit_hvp_partial", layers, nl, np, nw, ndir));
+  if (np == 0) return KH_OK;
+  KH_REQUIRE(dlayers && p && omega && R_obs && hvp, KH_ERR_INVALID_ARGUMENT,
+             "kh_misfit_hvp_partial: NULL pointer");
+  KH_REQUIRE(nw > 0, KH_ERR_INVALID_ARGUMENT, "kh_misfit_hvp_partial: nw must be > 0");
+  return run_adjoint<dcd, true>("kh_misfit_hvp_partial", layers, dlayers, ndir, nl, p, np, omega,
+                                nw, free_surface, R_obs, nullptr, second_only, chi, grad, hvp,
+                                nullptr, dR, workspace, workspace_bytes, as_stream(stream), 1);
 }
 
 extern "C" int kh_vjp_hvp(const double* layers, const double* dlayers, int ndir, int nl,

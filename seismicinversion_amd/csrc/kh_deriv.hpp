@@ -23,6 +23,8 @@
 // the coefficient formulas along each parameter (one dual level above Z).
 #pragma once
 
+#include <type_traits>
+
 #include "kh_adjoint.hpp"
 #include "kennett_hip.h"
 
@@ -91,15 +93,55 @@ __host__ __device__ __forceinline__ IfaceZ<Z> primal_uniform(const IfaceZ<Z>& f)
                    primal_uniform(f.td11)};
 }
 
+// Primal parts of an interface's coefficients.
+__host__ __device__ __forceinline__ const IfaceZ<cd>& prim_iface(const IfaceZ<cd>& f) { return f; }
+template <class T>
+__host__ __device__ __forceinline__ IfaceZ<cd> prim_iface(const IfaceZ<Dual<T>>& f) {
+  return IfaceZ<cd>{prim(f.rd00), prim(f.rd01), prim(f.rd11), prim(f.ru00), prim(f.ru01),
+                    prim(f.ru11), prim(f.td00), prim(f.td01), prim(f.td10), prim(f.td11)};
+}
+
+// A forward step whose inputs carry no tangent: every layer it reads lies
+// below the deepest layer the direction moves, so the state below, the
+// interface and the phases all have zero tangents.  The plain step on the
+// primal parts, lifted back with zero tangents: bitwise the primal of the
+// full step, and the tangents it leaves out are exactly zero.  The phases
+// are returned too (lifted), for the reverse sweep's buffer.
+template <class Z, class LZ>
+__host__ __device__ __forceinline__ M2<Z> step_fwd_flat(const M2<Z>& S, const LZ& lb, cd om,
+                                                        const IfaceZ<Z>& f, Z& ea, Z& eb) {
+  if constexpr (std::is_same<Z, cd>::value) {
+    phases_of(lb, om, ea, eb);
+    return step_fwd(S, ea, eb, f);
+  } else {
+    cd a{1.0, 0.0}, b{1.0, 0.0};
+    if (!lb.halfspace) {
+      a = phase_t(prim(lb.eta), om, prim(lb.h));
+      b = phase_t(prim(lb.neta), om, prim(lb.h));
+    }
+    const M2<cd> s{prim(S.a00), prim(S.a01), prim(S.a10), prim(S.a11)};
+    const M2<cd> r = step_fwd(s, a, b, prim_iface(f));
+    ea = lift_c<Z>(a);
+    eb = lift_c<Z>(b);
+    return M2<Z>{lift_c<Z>(r.a00), lift_c<Z>(r.a01), lift_c<Z>(r.a10), lift_c<Z>(r.a11)};
+  }
+}
+
 // Storage: st(e) returns a reference to Z-slot e of this lane.
 // Sink: sink.iface(i, IfaceZ<Z>), sink.layer(l, LayerCot<Z>).
 // G_of(R) -> cotangent of R.  Returns R.
 // L: per-layer records with eta, neta, h and halfspace (LayerZ<Z> or
 // LayerPh<Z>).
+// ltan: the deepest layer whose parameters carry a tangent (dual Z): forward
+// steps at interfaces i > ltan read only tangent-free layers and run on the
+// primal parts alone (step_fwd_flat; results unchanged).  lrev: the reverse
+// sweep stops after interface lrev, so the sink receives interfaces
+// 0..min(lrev, nl-2) and layers 0..min(lrev + 1, nl-1) only.
 template <class Z, class LZ, class Store, class Sink, class GFn>
 __host__ __device__ inline Z lane_adjoint(const LZ* __restrict__ L,
                                           const IfaceZ<Z>* __restrict__ F, int nl, cd om,
-                                          int free_surface, Store& st, Sink& sink, GFn&& G_of) {
+                                          int free_surface, Store& st, Sink& sink, GFn&& G_of,
+                                          int ltan = 1 << 30, int lrev = 1 << 30) {
   const int K = ckpt_stride(nl);
   const int nck = ckpt_count(nl, K);
   auto save = [&](int slot, const M2<Z>& S) {
@@ -119,8 +161,12 @@ __host__ __device__ inline Z lane_adjoint(const LZ* __restrict__ L,
   if ((nl - 2) % K == 0) save((nl - 2) / K, S);
   for (int i = nl - 3; i >= 0; --i) {
     Z ea, eb;
-    phases_of(L[i + 1], om, ea, eb);
-    S = step_fwd(S, ea, eb, F[i]);
+    if (i > ltan) {
+      S = step_fwd_flat(S, L[i + 1], om, F[i], ea, eb);
+    } else {
+      phases_of(L[i + 1], om, ea, eb);
+      S = step_fwd(S, ea, eb, F[i]);
+    }
     if (i % K == 0) save(i / K, S);
   }
 
@@ -145,8 +191,9 @@ __host__ __device__ inline Z lane_adjoint(const LZ* __restrict__ L,
     sink.layer(0, c0);
   }
 
-  // 3. reverse, segment by segment
-  for (int i0 = 0; i0 <= nl - 2; i0 += K) {
+  // 3. reverse, segment by segment (down to interface lrev)
+  const int ilast = (lrev < nl - 2) ? lrev : nl - 2;
+  for (int i0 = 0; i0 <= ilast; i0 += K) {
     const int i1 = (i0 + K - 1 < nl - 2) ? i0 + K - 1 : nl - 2;
     // recompute S_{i+1} (and the phases of layer i+1) for i = i1 .. i0
     M2<Z> T;
@@ -158,17 +205,22 @@ __host__ __device__ inline Z lane_adjoint(const LZ* __restrict__ L,
     for (int i = i1; i >= i0; --i) {
       if (i == nl - 2) continue;  // bottom interface: no step (S_{nl-2} = Rd)
       Z ea, eb;
-      phases_of(L[i + 1], om, ea, eb);
       const int k = i - i0;
       buf(k, 0) = T.a00;
       buf(k, 1) = T.a01;
       buf(k, 2) = T.a10;
       buf(k, 3) = T.a11;
+      if (i > ltan) {
+        T = step_fwd_flat(T, L[i + 1], om, F[i], ea, eb);
+      } else {
+        phases_of(L[i + 1], om, ea, eb);
+        T = step_fwd(T, ea, eb, F[i]);
+      }
       buf(k, 4) = ea;
       buf(k, 5) = eb;
-      T = step_fwd(T, ea, eb, F[i]);
     }
-    for (int i = i0; i <= i1; ++i) {
+    const int iend = (i1 < ilast) ? i1 : ilast;
+    for (int i = i0; i <= iend; ++i) {
       IfaceZ<Z> fbar;
       fbar.rd00 = fbar.rd01 = fbar.rd11 = zero_z<Z>();
       fbar.ru00 = fbar.ru01 = fbar.ru11 = zero_z<Z>();

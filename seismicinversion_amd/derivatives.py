@@ -17,7 +17,8 @@ import torch
 from . import _abi
 from .engine import check_precision
 
-__all__ = ["jvp", "vjp", "misfit_grad", "misfit_hvp", "vjp_hvp", "paced", "WORKSPACE_LIMIT"]
+__all__ = ["jvp", "vjp", "misfit_grad", "misfit_hvp", "vjp_hvp", "paced", "WORKSPACE_LIMIT",
+           "symmetric_from_partial"]
 
 # Directions per launch are capped so that one launch's workspace stays below this.
 WORKSPACE_LIMIT = 4 << 30
@@ -126,10 +127,14 @@ def misfit_grad(layers, p, omega, R_obs, free_surface=False):
 
 
 def misfit_hvp(layers, dlayers, p, omega, R_obs, free_surface=False, second_only=False,
-               want_dR=False):
+               want_dR=False, partial=False):
     """Exact misfit Hessian-vector products along each direction.
 
-    Returns (chi (np,), grad (np, nl, 6), hvp (ndir, np, nl, 6), dR (ndir, np, nw) or None)."""
+    Returns (chi (np,), grad (np, nl, 6), hvp (ndir, np, nl, 6), dR (ndir, np, nw) or None).
+    With ``partial`` (kh_misfit_hvp_partial) hvp[d] holds only the entries of
+    the layers down to one below the deepest layer direction d moves (the
+    rest are 0); ``symmetric_from_partial`` assembles the Hessian of a
+    parameter basis from such rows."""
     dev = layers.device
     p = _f64(p, dev)
     omega = _c128(omega, dev)
@@ -146,7 +151,8 @@ def misfit_hvp(layers, dlayers, p, omega, R_obs, free_surface=False, second_only
         ws = _workspace(nl, n_p, nw, nd, True, dev)
         chi_c = torch.empty((nd, n_p, 2), dtype=torch.float64, device=dev)
         _gate()
-        rc = _abi.lib().kh_misfit_hvp(
+        fn = _abi.lib().kh_misfit_hvp_partial if partial else _abi.lib().kh_misfit_hvp
+        rc = fn(
             _abi.ptr(layers), _abi.ptr(dlayers[s:e]), nd, nl, _abi.ptr(p), n_p, _abi.ptr(omega),
             nw, int(bool(free_surface)), _abi.ptr(R_obs), int(bool(second_only)),
             _abi.ptr(chi_c), _abi.ptr(grad) if s == 0 else None, _abi.ptr(hvp[s:e]),
@@ -156,6 +162,16 @@ def misfit_hvp(layers, dlayers, p, omega, R_obs, free_surface=False, second_only
         if chi is None:
             chi = chi_c[0, :, 0]
     return chi, grad, hvp, dR
+
+
+def symmetric_from_partial(rows: torch.Tensor, li: torch.Tensor) -> torch.Tensor:
+    """The (P, P) Hessian from partial rows: rows[k, j] = entry j of the
+    Hessian-vector product of the basis direction of parameter k, computed
+    where layer li[j] <= li[k] + 1 (kh_misfit_hvp_partial).  Entry (j, k)
+    is rows[k, j] there and rows[j, k] otherwise (the Hessian is
+    symmetric)."""
+    valid = li[:, None] <= li[None, :] + 1  # valid[j, k]: rows[k, j] was computed
+    return torch.where(valid, rows.T, rows)
 
 
 def vjp_hvp(layers, dlayers, p, omega, G, dG=None, free_surface=False):

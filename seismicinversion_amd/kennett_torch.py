@@ -391,7 +391,8 @@ def hessian(alpha, beta, rho, thickness, Q_alpha, Q_beta, p, omega, R_obs=None,
             free_surface: bool = False) -> torch.Tensor:
     """H (P, P) of chi = ||R - R_obs||^2 in the packed parameters
     (kennett_torch.py:644-684): P exact Hessian-vector products
-    (tangent-over-adjoint) in one kh_misfit_hvp launch."""
+    (tangent-over-adjoint) in one kh_misfit_hvp_partial launch, each down to
+    one layer below its parameter's layer, completed by symmetry."""
     tab = _layer_table(alpha, beta, rho, thickness, Q_alpha, Q_beta).detach()
     nl = tab.shape[0]
     om = _omega_c(omega)
@@ -401,11 +402,11 @@ def hessian(alpha, beta, rho, thickness, Q_alpha, Q_beta, p, omega, R_obs=None,
     _, _, hvp, _ = D.misfit_hvp(engine.layers_tensor(tab, dev), _basis(nl),
                                 torch.tensor([_pval(p)], dtype=_FDTYPE), om,
                                 torch.as_tensor(R_obs).reshape(1, -1),
-                                free_surface=free_surface)
+                                free_surface=free_surface, partial=True)
     idx = packed_index(nl)
     li = torch.tensor([l for l, _ in idx], device=hvp.device)
     ci = torch.tensor([c for _, c in idx], device=hvp.device)
-    H = hvp[:, 0][:, li, ci].T  # column k = H e_k
+    H = D.symmetric_from_partial(hvp[:, 0][:, li, ci], li)
     return H.contiguous().to(torch.as_tensor(alpha).device)
 
 

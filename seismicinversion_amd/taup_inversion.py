@@ -157,16 +157,22 @@ class _Problem:
         return float(np.sum(chi.cpu().numpy()))
 
     def _grad_hessian_dev(self, layers: torch.Tensor, mode: str):
+        # rows of the basis directions down to one layer below each
+        # direction's own (the reverse sweep stops there); the rest of the
+        # symmetric matrix comes from the deeper layers' rows
         if mode == "hvp":
             chi, grad, hvp, _ = derivatives.misfit_hvp(layers, self.basis, self.p, self.omega,
-                                                       self.R_obs, self.free_surface)
-            H = hvp.sum(1)[:, self.li, self.ci].T
+                                                       self.R_obs, self.free_surface,
+                                                       partial=True)
+            H = derivatives.symmetric_from_partial(hvp.sum(1)[:, self.li, self.ci], self.li)
         elif mode == "gram":
             chi, grad, S, dR = derivatives.misfit_hvp(layers, self.basis, self.p, self.omega,
                                                       self.R_obs, self.free_surface,
-                                                      second_only=True, want_dR=True)
+                                                      second_only=True, want_dR=True,
+                                                      partial=True)
             X = torch.view_as_real(dR).reshape(self.P, -1)
-            H = engine.gram(X, 2.0) + S.sum(1)[:, self.li, self.ci].T
+            H = engine.gram(X, 2.0) + derivatives.symmetric_from_partial(
+                S.sum(1)[:, self.li, self.ci], self.li)
         else:
             raise ValueError(f"unknown hessian mode {mode!r}")
         g = grad.sum(0)[self.li, self.ci]

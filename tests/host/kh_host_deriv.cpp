@@ -120,3 +120,60 @@ extern "C" int host_misfit_hvp(const double* layers, const double* dlayers, int 
   }
   return 0;
 }
+
+// host_misfit_hvp with the kernels' shortcuts: mode 1 runs the forward steps
+// below the deepest moved layer on primal parts only (step_fwd_flat), mode 2
+// also stops the reverse sweep one interface below it and leaves the deeper
+// layers' entries of hvp at 0 (kh_misfit_hvp_partial).  Mode 0 is
+// host_misfit_hvp.
+extern "C" int host_misfit_hvp_mode(const double* layers, const double* dlayers, int nl,
+                                    double p, const double* omega, int nw, int fs,
+                                    const double* Robs, int second_only, int mode, double* chi,
+                                    double* grad, double* hvp, double* dR) {
+  int ltan = nl - 1;
+  for (; ltan >= 0; --ltan) {
+    bool moved = false;
+    for (int c = 0; c < NCOL; ++c) moved = moved || dlayers[ltan * NCOL + c] != 0.0;
+    if (moved) break;
+  }
+  const int lt = mode >= 1 ? ltan : (1 << 30);
+  const int lrev = mode >= 2 ? ltan + 1 : (1 << 30);
+  std::vector<LayerZ<dcd>> L(nl);
+  std::vector<IfaceZ<dcd>> F(nl - 1);
+  stage<dcd>(layers, dlayers, nl, p, L, F);
+  Sink<dcd> sink(nl);
+  Store<dcd> st(lane_slots(nl));
+  dcd chis = lift<dcd>(0.0);
+  for (int w = 0; w < nw; ++w) {
+    const cd om{omega[2 * w], omega[2 * w + 1]};
+    const dcd ro = lift_c<dcd>(cd{Robs[2 * w], Robs[2 * w + 1]});
+    const dcd r = lane_adjoint<dcd>(L.data(), F.data(), nl, om, fs, st, sink, [&](const dcd& R) {
+      const dcd res = R - ro;
+      chis = chis + re_part(conj(res) * res);
+      dcd g = res * 2.0;
+      if (second_only) g.d = cd{0.0, 0.0};
+      return g;
+    }, lt, lrev);
+    dR[2 * w] = r.d.r;
+    dR[2 * w + 1] = r.d.i;
+  }
+  chi[0] = chis.v.r;
+  chi[1] = chis.d.r;
+  const dcd z = lift<dcd>(0.0);
+  const IfaceZ<dcd> none{z, z, z, z, z, z, z, z, z, z};
+  const int lmax = mode >= 2 ? ltan + 1 : nl - 1;
+  for (int l = 0; l < nl; ++l) {
+    dcd out[NCOL];
+    if (l > lmax) {
+      for (int c = 0; c < NCOL; ++c) hvp[l * NCOL + c] = 0.0;
+      continue;
+    }
+    chain_layer<dcd>(layers, dlayers, nl, l, p, l >= 1 ? sink.cif[l - 1] : none,
+                     l <= nl - 2 ? sink.cif[l] : none, sink.cly[l], out);
+    for (int c = 0; c < NCOL; ++c) {
+      grad[l * NCOL + c] = out[c].v.r;
+      hvp[l * NCOL + c] = out[c].d.r;
+    }
+  }
+  return 0;
+}

@@ -102,3 +102,52 @@ def test_host_vjp_and_hvp_vs_torch(lib, model, p, fs):
     assert rel(hv, hvt) < 1e-9
     _, dRt = torch.func.jvp(f, (t0,), (torch.tensor(dm),))
     assert rel(dR, dRt.numpy()) < 1e-9
+
+
+@pytest.mark.parametrize("fs,second_only", [(False, True), (True, False)])
+def test_flat_forward_and_partial_reverse(lib, fs, second_only):
+    """The derivative kernels' two shortcuts on the host build: forward steps
+    below the deepest moved layer on primal parts only give bitwise the full
+    dual sweep's chi, gradient, Hessian-vector product and dR; stopping the
+    reverse one interface below it gives bitwise the same entries for the
+    layers above, and the Hessian assembled from those rows by symmetry
+    equals the full one."""
+    lib.host_misfit_hvp_mode.argtypes = [vp, vp, ctypes.c_int, ctypes.c_double, vp, ctypes.c_int,
+                                         ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, vp, vp, vp,
+                                         vp]
+    from seismicinversion_amd.kennett_torch import packed_index
+
+    model = seeded_model(9, 5)
+    tab = np.ascontiguousarray(pack_layers(model))
+    nl, p = len(tab), 0.27
+    w = np.ascontiguousarray((np.arange(1, 41) * 2 * np.pi / 64.0).astype(np.complex128))
+    Robs = np.ascontiguousarray(np.exp(1j * np.arange(40) * 0.1) * 0.05)
+    idx = packed_index(nl)
+    P = len(idx)
+    rows = {m: np.zeros((P, nl, 6)) for m in (0, 1, 2)}
+    for k, (l, c) in enumerate(idx):
+        d = np.zeros((nl, 6))
+        d[l, c] = 1.0
+        outs = {}
+        for mode in (0, 1, 2):
+            chi, grad, hvp = np.zeros(2), np.zeros((nl, 6)), np.zeros((nl, 6))
+            dR = np.zeros(40, np.complex128)
+            lib.host_misfit_hvp_mode(tab.ctypes.data, d.ctypes.data, nl, p, w.ctypes.data, 40,
+                                     int(fs), Robs.ctypes.data, int(second_only), mode,
+                                     chi.ctypes.data, grad.ctypes.data, hvp.ctypes.data,
+                                     dR.ctypes.data)
+            outs[mode] = (chi, grad, hvp, dR)
+            rows[mode][k] = hvp
+        for a, b in zip(outs[0], outs[1]):
+            np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(outs[0][0], outs[2][0])
+        np.testing.assert_array_equal(outs[0][3], outs[2][3])
+        np.testing.assert_array_equal(outs[2][2][:l + 2], outs[0][2][:l + 2])
+        assert np.all(outs[2][2][l + 2:] == 0)
+    li = np.array([l for l, _ in idx])
+    ci = np.array([c for _, c in idx])
+    M = rows[2][:, li, ci]        # M[k, j] = entry j of direction k (layers <= l_k + 1)
+    full = rows[0][:, li, ci]
+    valid = li[None, :] <= li[:, None] + 1
+    H = np.where(valid, M, M.T)
+    assert np.max(np.abs(H - full)) <= 1e-12 * np.max(np.abs(full))
