@@ -137,13 +137,16 @@ __host__ __device__ __forceinline__ M2<Z> step_fwd_flat(const M2<Z>& S, const LZ
 // primal parts alone (step_fwd_flat; results unchanged).  lrev: the reverse
 // sweep stops after interface lrev, so the sink receives interfaces
 // 0..min(lrev, nl-2) and layers 0..min(lrev + 1, nl-1) only.
-template <class Z, class LZ, class Store, class Sink, class GFn>
-__host__ __device__ inline Z lane_adjoint(const LZ* __restrict__ L,
-                                          const IfaceZ<Z>* __restrict__ F, int nl, cd om,
-                                          int free_surface, Store& st, Sink& sink, GFn&& G_of,
-                                          int ltan = 1 << 30, int lrev = 1 << 30) {
-  const int K = ckpt_stride(nl);
-  const int nck = ckpt_count(nl, K);
+// Checkpoints and buffer in separate storage: st holds the states S_j at
+// every K-th level (slots 0 .. ckpt_count(nl, K) * 4), bst the K-level
+// buffer of a segment being walked back (K * 6 slots).  K = ckpt_stride(nl)
+// balances the two; a smaller K keeps the buffer small enough for on-chip
+// storage (the reverse reads it right after the recompute writes it).
+template <class Z, class LZ, class Store, class BufStore, class Sink, class GFn>
+__host__ __device__ inline Z lane_adjoint_k(const LZ* __restrict__ L,
+                                            const IfaceZ<Z>* __restrict__ F, int nl, cd om,
+                                            int free_surface, Store& st, BufStore& bst, int K,
+                                            Sink& sink, GFn&& G_of, int ltan, int lrev) {
   auto save = [&](int slot, const M2<Z>& S) {
     st(slot * 4 + 0) = S.a00;
     st(slot * 4 + 1) = S.a01;
@@ -153,12 +156,16 @@ __host__ __device__ inline Z lane_adjoint(const LZ* __restrict__ L,
   auto load = [&](int slot) {
     return M2<Z>{st(slot * 4 + 0), st(slot * 4 + 1), st(slot * 4 + 2), st(slot * 4 + 3)};
   };
-  auto buf = [&](int k, int q) -> Z& { return st(nck * 4 + k * 6 + q); };
+  auto buf = [&](int k, int q) -> Z& { return bst(k * 6 + q); };
+  // the reverse sweep ends at interface ilast: checkpoints below the one
+  // its last segment starts from are never read
+  const int ilast = (lrev < nl - 2) ? lrev : nl - 2;
+  const int ck_max = ilast / K + 1;
 
   // 1. forward with checkpoints
   const IfaceZ<Z>& fb = F[nl - 2];
   M2<Z> S{fb.rd00, fb.rd01, fb.rd01, fb.rd11};
-  if ((nl - 2) % K == 0) save((nl - 2) / K, S);
+  if ((nl - 2) % K == 0 && (nl - 2) / K <= ck_max) save((nl - 2) / K, S);
   for (int i = nl - 3; i >= 0; --i) {
     Z ea, eb;
     if (i > ltan) {
@@ -167,7 +174,7 @@ __host__ __device__ inline Z lane_adjoint(const LZ* __restrict__ L,
       phases_of(L[i + 1], om, ea, eb);
       S = step_fwd(S, ea, eb, F[i]);
     }
-    if (i % K == 0) save(i / K, S);
+    if (i % K == 0 && i / K <= ck_max) save(i / K, S);
   }
 
   // 2. surface value and its cotangent
@@ -192,7 +199,6 @@ __host__ __device__ inline Z lane_adjoint(const LZ* __restrict__ L,
   }
 
   // 3. reverse, segment by segment (down to interface lrev)
-  const int ilast = (lrev < nl - 2) ? lrev : nl - 2;
   for (int i0 = 0; i0 <= ilast; i0 += K) {
     const int i1 = (i0 + K - 1 < nl - 2) ? i0 + K - 1 : nl - 2;
     // recompute S_{i+1} (and the phases of layer i+1) for i = i1 .. i0
@@ -252,6 +258,25 @@ __host__ __device__ inline Z lane_adjoint(const LZ* __restrict__ L,
     }
   }
   return R;
+}
+
+// One storage for both: the checkpoints, then the buffer, K = ckpt_stride(nl)
+// (lane_slots(nl) slots).
+template <class Z, class Store>
+struct BufAfter {
+  Store& st;
+  int off;
+  __host__ __device__ Z& operator()(int e) { return st(off + e); }
+};
+
+template <class Z, class LZ, class Store, class Sink, class GFn>
+__host__ __device__ inline Z lane_adjoint(const LZ* __restrict__ L,
+                                          const IfaceZ<Z>* __restrict__ F, int nl, cd om,
+                                          int free_surface, Store& st, Sink& sink, GFn&& G_of,
+                                          int ltan = 1 << 30, int lrev = 1 << 30) {
+  const int K = ckpt_stride(nl);
+  BufAfter<Z, Store> bst{st, ckpt_count(nl, K) * 4};
+  return lane_adjoint_k<Z>(L, F, nl, om, free_surface, st, bst, K, sink, G_of, ltan, lrev);
 }
 
 // One dual level above a scalar type, with the seeding of its inputs.

@@ -20,8 +20,9 @@ from .engine import check_precision
 __all__ = ["jvp", "vjp", "misfit_grad", "misfit_hvp", "vjp_hvp", "paced", "WORKSPACE_LIMIT",
            "symmetric_from_partial"]
 
-# Directions per launch are capped so that one launch's workspace stays below this.
-WORKSPACE_LIMIT = 4 << 30
+# Directions per launch are capped so that one launch's workspace stays below
+# this (the 20-layer LM's 75 directions at 128 x 512 need 7.8 GB: one launch).
+WORKSPACE_LIMIT = 8 << 30
 
 
 def _f64(x, dev):

@@ -127,10 +127,17 @@ class _Problem:
         self.P = len(idx)
         self.li = torch.tensor([l for l, _ in idx], device=self.dev)
         self.ci = torch.tensor([c for _, c in idx], device=self.dev)
+        # the basis directions deepest layer first: with partial reverse
+        # sweeps their cost falls with the layer, so the launch starts its
+        # longest workgroups first (the first one, whose sweep also gives the
+        # gradient, is complete either way)
+        order = sorted(range(self.P), key=lambda k: -idx[k][0])
         basis = torch.zeros((self.P, self.nl, 6), dtype=_F)
-        for k, (l, c) in enumerate(idx):
+        for k, kk in enumerate(order):
+            l, c = idx[kk]
             basis[k, l, c] = 1.0
         self.basis = basis.to(self.dev)
+        self.unorder = torch.tensor(np.argsort(order), device=self.dev)  # packed k -> row
         self.use_graphs = use_graphs
         self._graphs: dict[str, _Graphed] = {}
 
@@ -164,15 +171,16 @@ class _Problem:
             chi, grad, hvp, _ = derivatives.misfit_hvp(layers, self.basis, self.p, self.omega,
                                                        self.R_obs, self.free_surface,
                                                        partial=True)
-            H = derivatives.symmetric_from_partial(hvp.sum(1)[:, self.li, self.ci], self.li)
+            rows = hvp.sum(1)[self.unorder][:, self.li, self.ci]
+            H = derivatives.symmetric_from_partial(rows, self.li)
         elif mode == "gram":
             chi, grad, S, dR = derivatives.misfit_hvp(layers, self.basis, self.p, self.omega,
                                                       self.R_obs, self.free_surface,
                                                       second_only=True, want_dR=True,
                                                       partial=True)
-            X = torch.view_as_real(dR).reshape(self.P, -1)
+            X = torch.view_as_real(dR[self.unorder]).reshape(self.P, -1)
             H = engine.gram(X, 2.0) + derivatives.symmetric_from_partial(
-                S.sum(1)[:, self.li, self.ci], self.li)
+                S.sum(1)[self.unorder][:, self.li, self.ci], self.li)
         else:
             raise ValueError(f"unknown hessian mode {mode!r}")
         g = grad.sum(0)[self.li, self.ci]

@@ -138,22 +138,29 @@ extern "C" int host_misfit_hvp_mode(const double* layers, const double* dlayers,
   }
   const int lt = mode >= 1 ? ltan : (1 << 30);
   const int lrev = mode >= 2 ? ltan + 1 : (1 << 30);
+  // mode 3 is mode 2 with checkpoints every 2 levels and the buffer in a
+  // store of its own
   std::vector<LayerZ<dcd>> L(nl);
   std::vector<IfaceZ<dcd>> F(nl - 1);
   stage<dcd>(layers, dlayers, nl, p, L, F);
   Sink<dcd> sink(nl);
-  Store<dcd> st(lane_slots(nl));
+  Store<dcd> st(lane_slots(nl) + 64);
+  Store<dcd> bst(2 * 6);  // mode 3: two-level segments, buffer apart (the LDS layout)
   dcd chis = lift<dcd>(0.0);
   for (int w = 0; w < nw; ++w) {
     const cd om{omega[2 * w], omega[2 * w + 1]};
     const dcd ro = lift_c<dcd>(cd{Robs[2 * w], Robs[2 * w + 1]});
-    const dcd r = lane_adjoint<dcd>(L.data(), F.data(), nl, om, fs, st, sink, [&](const dcd& R) {
+    auto G = [&](const dcd& R) {
       const dcd res = R - ro;
       chis = chis + re_part(conj(res) * res);
       dcd g = res * 2.0;
       if (second_only) g.d = cd{0.0, 0.0};
       return g;
-    }, lt, lrev);
+    };
+    const dcd r = mode == 3
+                      ? lane_adjoint_k<dcd>(L.data(), F.data(), nl, om, fs, st, bst, 2, sink, G,
+                                            ltan, ltan + 1)
+                      : lane_adjoint<dcd>(L.data(), F.data(), nl, om, fs, st, sink, G, lt, lrev);
     dR[2 * w] = r.d.r;
     dR[2 * w + 1] = r.d.i;
   }
