@@ -362,10 +362,12 @@ def kernel_figures(torch, dev):
 
 def lm_figures(torch, dev):
     """20-layer Newton-LM: P = 75 packed parameters, 128 p in (0, 0.6] x 512
-    omega, start model perturbed 5 %, 10 iterations; eager and graph-replayed
-    (the graph run includes the captures, which run each computation three
-    times), plus one misfit/gradient/exact-Hessian evaluation timed on its
-    own, eager and replayed from its captured graph."""
+    omega, start model perturbed 5 %, 10 iterations, eager and with HIP graphs
+    (each computation runs eagerly on its first call, is captured on its
+    second and replayed from then on); one misfit/gradient/exact-Hessian
+    evaluation timed on its own, eager and replayed.  Beside it the served
+    inversion size (17 layers, 32 p x 128 omega, 5 iterations), where the
+    kernels are short and launch overhead counts, eager and with graphs."""
     from seismicinversion_amd import engine, kennett_torch as kt
     from seismicinversion_amd.taup_inversion import _Problem, invert_taup
     from seismicinversion_amd.workloads import seeded_model, uniform_slownesses
@@ -374,13 +376,16 @@ def lm_figures(torch, dev):
     p = list(uniform_slownesses(128, 0.6))
     invert_taup(m, p[:4], nfreq=16, max_iter=1, tol=0.0)  # warm-up
     runs = {}
-    for graphs in (False, True):
+    for graphs in (False, True, False, True):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         res = invert_taup(m, p, nfreq=512, perturbation=0.05, max_iter=10, tol=0.0,
                           use_graphs=graphs)
         torch.cuda.synchronize()
-        runs[graphs] = (time.perf_counter() - t0, res)
+        runs.setdefault(graphs, []).append((time.perf_counter() - t0, res))
+    t_e = min(t for t, _ in runs[False])
+    t_g = min(t for t, _ in runs[True])
+    res, res_g = runs[False][0][1], runs[True][0][1]
     tens = kt.model_to_tensors(m)
     refs = [tens[k] for k in ("alpha", "beta", "rho", "thickness", "Q_alpha", "Q_beta")]
     w = torch.arange(1, 513, dtype=torch.float64) * (2 * np.pi / 64)
@@ -389,21 +394,36 @@ def lm_figures(torch, dev):
     phys = kt._pack_params(*refs[:4]).numpy() * 1.02
     prob.grad_hessian(phys)
     t_gh = timed(lambda: prob.grad_hessian(phys), 3, torch)
-    # the same evaluation replayed from its captured graph (capture excluded)
     prob_g = _Problem(*refs, p, w, obs, use_graphs=True)
-    prob_g.grad_hessian(phys)
+    prob_g.grad_hessian(phys)  # eager
+    prob_g.grad_hessian(phys)  # captured
     t_gh_graph = timed(lambda: prob_g.grad_hessian(phys), 3, torch)
-    res = runs[False][1]
+    # the served inversion size: 5 requests of 5 iterations, eager and graphs
+    small = [(seeded_model(17, 100 + k), list(uniform_slownesses(32, 0.8))) for k in range(5)]
+    t_small = {}
+    for graphs in (False, True, False, True):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        its = 0
+        for mm, pp in small:
+            r = invert_taup(mm, pp, nfreq=128, perturbation=0.05, max_iter=5, tol=0.0,
+                            use_graphs=graphs)
+            its += r.n_iterations
+        torch.cuda.synchronize()
+        t_small.setdefault(graphs, []).append((time.perf_counter() - t0) * 1e3 / its)
     return {"P": prob.P, "n_p": 128, "n_omega": 512, "iterations": res.n_iterations,
-            "eager_s": runs[False][0], "graphs_s": runs[True][0],
-            "eager_ms_per_iteration": runs[False][0] * 1e3 / res.n_iterations,
-            "graphs_ms_per_iteration": runs[True][0] * 1e3 / runs[True][1].n_iterations,
-            "same_trajectory": runs[True][1].misfit_history == res.misfit_history,
+            "eager_s": t_e, "graphs_s": t_g,
+            "eager_ms_per_iteration": t_e * 1e3 / res.n_iterations,
+            "graphs_ms_per_iteration": t_g * 1e3 / res_g.n_iterations,
+            "same_trajectory": res_g.misfit_history == res.misfit_history,
             "misfit_first_last": [res.misfit_history[0], res.misfit_history[-1]],
             "param_error_last": res.param_error_history[-1],
             "grad_hessian_ms": t_gh * 1e3,
             "grad_hessian_graph_replay_ms": t_gh_graph * 1e3,
-            "tangent_over_adjoint_lane_updates": prob.P * 128 * 512 * 19}
+            "tangent_over_adjoint_lane_updates": prob.P * 128 * 512 * 19,
+            "served_size_17_layers_32p_128w": {
+                "eager_ms_per_iteration": min(t_small[False]),
+                "graphs_ms_per_iteration": min(t_small[True])}}
 
 
 def open_loop_sweep(torch, rates, seconds, fwd_deadline, inv_deadline):

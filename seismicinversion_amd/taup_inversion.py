@@ -66,11 +66,14 @@ class _Graphed:
     """A device computation of a (nl, 6) layer table, captured once as a HIP
     graph (torch.cuda.CUDAGraph) and replayed for every new table.
 
-    The table is copied into a static input before each replay; the outputs
-    are static tensors overwritten by the next replay, so callers copy them
-    out first.  Capture is thread-local (other threads may use the device
-    meanwhile).  The captured work is this library's kernels and small torch
-    reductions along short axes; none of it records a memset node."""
+    The caller runs the computation eagerly once first (its allocations and
+    any lazy set-up are then settled); the capture itself executes nothing,
+    so no computation runs more often than it is asked for.  The table is
+    copied into a static input before each replay; the outputs are static
+    tensors overwritten by the next replay, so callers copy them out first.
+    Capture is thread-local (other threads may use the device meanwhile).
+    The captured work is this library's kernels and small torch reductions
+    along short axes; none of it records a memset node."""
 
     def __init__(self, fn, layers: torch.Tensor):
         dev = layers.device
@@ -80,8 +83,6 @@ class _Graphed:
         side.wait_stream(cur)
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.stream(side):
-            for _ in range(2):  # settle allocations before capture
-                fn(self.inp)
             # capture on the side stream; unlike the torch.cuda.graph context
             # this neither synchronises the whole device nor empties the
             # allocator cache, so other threads' work is not held up
@@ -140,6 +141,7 @@ class _Problem:
         self.unorder = torch.tensor(np.argsort(order), device=self.dev)  # packed k -> row
         self.use_graphs = use_graphs
         self._graphs: dict[str, _Graphed] = {}
+        self._eager_once: set[str] = set()
 
     def table(self, phys: np.ndarray) -> torch.Tensor:
         a, b, r, h = _unpack_params(torch.as_tensor(phys, dtype=_F), *self.refs[:4])
@@ -148,10 +150,15 @@ class _Problem:
         return engine.layers_tensor(tab, self.dev)
 
     def _run(self, key, fn, layers):
+        """fn(layers); with graphs: eagerly on the first call, captured on
+        the second and replayed from then on."""
         if not self.use_graphs:
             return fn(layers)
         g = self._graphs.get(key)
         if g is None:
+            if key not in self._eager_once:
+                self._eager_once.add(key)
+                return fn(layers)
             g = self._graphs[key] = _Graphed(fn, layers)
         return g(layers)
 

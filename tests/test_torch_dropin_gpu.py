@@ -228,7 +228,7 @@ def test_lm_graph_replays_match_eager(mode):
         chi_e, g_e, H_e = eager.grad_hessian(phys, mode)
         mis_e = eager.misfit(phys * 1.01)
         torch.cuda.synchronize()
-        chi_r, g_r, H_r = graphed.grad_hessian(phys, mode)
+        chi_r, g_r, H_r = graphed.grad_hessian(phys, mode)  # eager, then captured, then replays
         mis_r = graphed.misfit(phys * 1.01)
         assert chi_r == chi_e and mis_r == mis_e, f"replay {it}"
         assert np.array_equal(g_r, g_e) and np.array_equal(H_r, H_e), f"replay {it}"
