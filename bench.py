@@ -80,9 +80,10 @@ def parse():
                     help="processes of the all-core CPU baseline (the box's share is 16 cores)")
     ap.add_argument("--warmup-seconds", type=float, default=2.0,
                     help="untimed forward launches before anything is timed (clock ramp)")
-    ap.add_argument("--section", choices=("all", "lm", "open"), default="all",
+    ap.add_argument("--section", choices=("all", "lm", "open", "served"), default="all",
                     help="'lm': only the 20-layer Newton-LM figures (for profiling its kernels); "
-                         "'open': only the open-loop rate sweep")
+                         "'open': only the open-loop rate sweep; 'served': only the headline "
+                         "served steps and their roofline")
     ap.add_argument("--traffic-json", default=os.path.join(HERE, "profiles", "traffic_multi_served.json"),
                     help="HBM bytes per launch of the multi-model kernel from rocprofv3 --pmc")
     return ap.parse_args()
@@ -503,6 +504,37 @@ def open_loop_sweep(torch, rates, seconds, fwd_deadline, inv_deadline):
     return {"deadlines_s": {"forward": fwd_deadline, "invert": inv_deadline}, "rates": out}
 
 
+def served_roofline(log, traffic_json, wpt):
+    """FP64 roofline of the persistent multi-model kernel over the timed
+    served launches: algorithmic FLOPs (340 per update) per launch over the
+    mean launch duration (HIP events on the server's compute stream)."""
+    kern_ms = float(np.mean([m for m, _, _ in log]))
+    upd_per_launch = float(np.mean([u for _, u, _ in log]))
+    out_per_launch = float(np.mean([e for _, _, e in log]))
+    achieved = FLOP_PER_UPDATE * upd_per_launch / (kern_ms * 1e-3) / 1e12
+    # HBM bytes per launch: the ratio of measured (rocprofv3 FETCH_SIZE and
+    # WRITE_SIZE, calibrated on a device copy) to algorithmic bytes over the
+    # served dispatches of the counter runs, times this run's algorithmic
+    # bytes per launch
+    traffic = None
+    if os.path.exists(traffic_json):
+        with open(traffic_json) as f:
+            ratio = json.load(f).get("traffic_over_algorithmic")
+        if ratio:
+            traffic = float(ratio) * 16.0 * out_per_launch
+    return {"bound": "fp64-vector", "pipe": "FP64 VALU (v_fma_f64 and friends), not MFMA "
+            "and not HBM: 340 FLOP per 16 output bytes per interface",
+            "achieved": achieved,
+            "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+            "traffic": traffic, "kernel": f"kh::multi_kernel<{wpt}>",
+            "kernel_ms_spans": "the whole kh_forward_multi call: counter reset, "
+                               "kh::stage_tables_kernel and the persistent sweep",
+            "launches": len(log),
+            "kernel_ms": kern_ms, "updates_per_launch": upd_per_launch,
+            "flop_per_update": FLOP_PER_UPDATE,
+            "algorithmic_bytes_per_launch": 16.0 * out_per_launch}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -537,7 +569,9 @@ def main():
     items = serving_stream(0.0, args.requests, seed=42, lm_fraction=0.0)
     specs = [s for _, s in stream_requests(items)]
     updates = sum(s.updates() for s in specs)
-    cpu, gather_cpu, cpu_traces, c1_cpu = cpu_baselines(specs, args.cpu_workers)
+    full = args.section == "all"
+    if full:
+        cpu, gather_cpu, cpu_traces, c1_cpu = cpu_baselines(specs, args.cpu_workers)
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
@@ -568,6 +602,13 @@ def main():
     t_served = timed(served, args.steps, torch)
     batches_per_step = (srv.batches - b0) / args.steps
     log = list(be.kernel_log)
+    if not full:
+        srv.stop()
+        r = served_roofline(log, args.traffic_json, be.shape[1])
+        print(json.dumps({"section": "served", "value": updates / t_served,
+                          "ms_per_step": t_served * 1e3, "batches_per_step": batches_per_step,
+                          "roofline": r}))
+        return
     served()  # warm the host-result path's staging and pages
     served(False)
     t_served_host = timed(lambda: served(False), 1, torch)
@@ -625,31 +666,7 @@ def main():
     kernels["c1_plain_5_layers"] = c1_figure(torch, dev, c1_cpu)
 
     # ---- roofline of the persistent multi-model kernel over the timed served steps
-    kern_ms = float(np.mean([m for m, _, _ in log]))
-    upd_per_launch = float(np.mean([u for _, u, _ in log]))
-    out_per_launch = float(np.mean([e for _, _, e in log]))
-    achieved = FLOP_PER_UPDATE * upd_per_launch / (kern_ms * 1e-3) / 1e12
-    # HBM bytes per launch: the ratio of measured (rocprofv3 FETCH_SIZE and
-    # WRITE_SIZE, calibrated on a device copy) to algorithmic bytes over the
-    # served dispatches of the counter runs, times this run's algorithmic
-    # bytes per launch
-    traffic = None
-    if os.path.exists(args.traffic_json):
-        with open(args.traffic_json) as f:
-            ratio = json.load(f).get("traffic_over_algorithmic")
-        if ratio:
-            traffic = float(ratio) * 16.0 * out_per_launch
-    roofline = {"bound": "fp64-vector", "pipe": "FP64 VALU (v_fma_f64 and friends), not MFMA "
-                "and not HBM: 340 FLOP per 16 output bytes per interface",
-                "achieved": achieved,
-                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
-                "traffic": traffic, "kernel": f"kh::multi_kernel<{be.shape[1]}>",
-                "kernel_ms_spans": "the whole kh_forward_multi call: counter reset, "
-                                   "kh::stage_tables_kernel and the persistent sweep",
-                "launches": len(log),
-                "kernel_ms": kern_ms, "updates_per_launch": upd_per_launch,
-                "flop_per_update": FLOP_PER_UPDATE,
-                "algorithmic_bytes_per_launch": 16.0 * out_per_launch}
+    roofline = served_roofline(log, args.traffic_json, be.shape[1])
 
     g = kernels["gather"]
     g["cpu_all_cores"] = gather_cpu

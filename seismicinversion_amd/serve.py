@@ -574,20 +574,18 @@ class KennettServer:
         return self._submit("invert", spec, deadline_s)
 
     def _make(self, kind, spec, deadline_s, on_device=False) -> Request:
-        """A Request for `spec`; when the spec is unusable its future already
-        holds the error (the request fails alone, the service goes on)."""
+        """A Request for `spec` (validated later, at admission: an unusable
+        one fails alone and the service goes on)."""
         fut: Future = Future()
         now = time.perf_counter()
         req = Request(kind=kind, spec=spec, future=fut, arrival=now,
                       deadline=None if deadline_s is None else now + deadline_s,
                       on_device=bool(on_device))
+        # layer count and updates are filled in when the scheduler validates
+        # the request (one batch at a time, while earlier batches run), not
+        # here in the caller's thread
         req.record = {"kind": kind, "arrival": now, "n_layers": 0, "updates": 0,
                       "status": "queued"}
-        try:
-            req.record["n_layers"] = spec.n_layers
-            req.record["updates"] = spec.updates()
-        except ValueError as e:
-            self._finish(req, error=e)
         return req
 
     def _enqueue(self, reqs: list[Request]) -> None:
@@ -729,6 +727,8 @@ class KennettServer:
                 pack_layers(m, MAX_LAYERS_DERIV)
             if len(spec.p) == 0 or spec.nfreq <= 0 or spec.iterations < 0:
                 raise ValueError("an inversion needs at least one slowness and one frequency")
+        req.record["n_layers"] = spec.n_layers
+        req.record["updates"] = spec.updates()
 
     def _admit(self, fwd_room: float = float("inf")) -> list[Request]:
         """Arrivals join one of two FIFOs; requests leave them admitted.
