@@ -56,15 +56,29 @@ struct Flat<dcd> {
 };
 
 // Row stride (doubles) of the reduction buffer: one pad element spreads the
-// rows that different lanes sum over the LDS banks.  A record of up to 10
-// scalars is reduced in one pass (one row per double), or, when `split` is
-// set, 5 scalars at a time: with deep models the LDS per workgroup, not the
-// registers, decides how many workgroups share a CU, and the split halves
-// the buffer at the cost of a second pass.
+// rows that different lanes sum over the LDS banks.  Reduction modes, by the
+// LDS a workgroup can have (with deep models the LDS per workgroup, not the
+// registers, decides how many workgroups share a CU):
+//   RED_LEVEL  an interface's 10 scalars and the layer below it (3) in one
+//              pass (one row per double);
+//   RED_RECORD the interface, then the layer: two passes;
+//   RED_HALF   the interface 5 scalars at a time, then the layer: three
+//              passes in half the buffer.
 constexpr int RED_STRIDE = AD_WAVE + 1;
+enum { RED_LEVEL = 0, RED_RECORD = 1, RED_HALF = 2 };
 template <class Z>
-constexpr int red_rows(bool split) {
-  return (split ? 5 : 10) * Flat<Z>::N;
+constexpr int red_rows(int mode) {
+  return (mode == RED_LEVEL ? 13 : mode == RED_RECORD ? 10 : 5) * Flat<Z>::N;
+}
+
+// Workgroup barrier that orders LDS only: the reductions exchange data
+// through LDS alone, so the wave's outstanding global loads and stores
+// (checkpoints, the segment buffer, cotangent records) stay in flight
+// across it instead of being drained at every level.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 // Per-wave sums of the per-lane cotangent records, written per tile.  Each
@@ -76,15 +90,16 @@ template <class Z>
 struct WaveSink {
   IfaceZ<Z>* cif;
   LayerCot<Z>* cly;
-  double* red;  // LDS, red_rows<Z>(split) x RED_STRIDE, this wave's own
-  bool split;
+  double* red;  // LDS, red_rows<Z>(mode) x RED_STRIDE, this wave's own
+  int mode;
   bool store;  // false for a padding wave (no tile): it only keeps the barriers
 
-  // one pass over scalars [F0, F1) of the record (every wave of the
-  // workgroup reduces its own record at the same level: the barriers are
-  // shared)
+  // one pass over scalars [F0, F1) of the record v; the sums of its first n0
+  // doubles go to d0, the rest to d1 (every wave of the workgroup reduces
+  // its own record at the same level: the barriers are shared)
   template <int F0, int F1, int NF>
-  __device__ void reduce_pass(const Z (&v)[NF], double* __restrict__ dst) {
+  __device__ void reduce_pass(const Z (&v)[NF], double* __restrict__ d0, int n0,
+                              double* __restrict__ d1) {
     constexpr int N = Flat<Z>::N;
     const int lane = threadIdx.x % AD_WAVE;
 #pragma unroll
@@ -92,7 +107,7 @@ struct WaveSink {
 #pragma unroll
       for (int k = 0; k < N; ++k) red[((f - F0) * N + k) * RED_STRIDE + lane] = Flat<Z>::get(v[f], k);
     }
-    __syncthreads();
+    lds_barrier();
     if (store && lane < (F1 - F0) * N) {
       const double* row = red + lane * RED_STRIDE;
       double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
@@ -103,24 +118,37 @@ struct WaveSink {
         s2 += row[j + 2];
         s3 += row[j + 3];
       }
-      dst[F0 * N + lane] = (s0 + s1) + (s2 + s3);
+      const int e = F0 * N + lane;
+      *(e < n0 ? d0 + e : d1 + (e - n0)) = (s0 + s1) + (s2 + s3);
     }
-    __syncthreads();  // the buffer is rewritten by the next pass
+    lds_barrier();  // the buffer is rewritten by the next pass
   }
   __device__ void iface(int i, const IfaceZ<Z>& f) {
     const Z v[10] = {f.rd00, f.rd01, f.rd11, f.ru00, f.ru01,
                      f.ru11, f.td00, f.td01, f.td10, f.td11};
     double* dst = reinterpret_cast<double*>(cif + i);
-    if (split) {
-      reduce_pass<0, 5>(v, dst);
-      reduce_pass<5, 10>(v, dst);
+    if (mode == RED_HALF) {
+      reduce_pass<0, 5>(v, dst, 1 << 30, nullptr);
+      reduce_pass<5, 10>(v, dst, 1 << 30, nullptr);
     } else {
-      reduce_pass<0, 10>(v, dst);
+      reduce_pass<0, 10>(v, dst, 1 << 30, nullptr);
     }
   }
   __device__ void layer(int l, const LayerCot<Z>& c) {
     const Z v[3] = {c.eta, c.neta, c.h};
-    reduce_pass<0, 3>(v, reinterpret_cast<double*>(cly + l));
+    reduce_pass<0, 3>(v, reinterpret_cast<double*>(cly + l), 1 << 30, nullptr);
+  }
+  // interface i and layer i + 1
+  __device__ void level(int i, const IfaceZ<Z>& f, const LayerCot<Z>& c) {
+    if (mode != RED_LEVEL) {
+      iface(i, f);
+      layer(i + 1, c);
+      return;
+    }
+    const Z v[13] = {f.rd00, f.rd01, f.rd11, f.ru00, f.ru01, f.ru11, f.td00,
+                     f.td01, f.td10, f.td11, c.eta,  c.neta, c.h};
+    reduce_pass<0, 13>(v, reinterpret_cast<double*>(cif + i), 10 * Flat<Z>::N,
+                       reinterpret_cast<double*>(cly + i + 1));
   }
 };
 
@@ -228,14 +256,14 @@ __global__ void __launch_bounds__(AD_BLOCK_MAX) adjoint_kernel(
     const cd* __restrict__ Gin, const cd* __restrict__ dGin, int second_only, Z* __restrict__ ws,
     long nlanes,
     IfaceZ<Z>* __restrict__ cot_if, LayerCot<Z>* __restrict__ cot_ly, Z* __restrict__ chi_part,
-    cd* __restrict__ R, cd* __restrict__ dR, int ntiles, int ntp, int red_split, int partial) {
+    cd* __restrict__ R, cd* __restrict__ dR, int ntiles, int ntp, int red_mode, int partial) {
   extern __shared__ double smem_a[];
   LayerPh<Z>* s_layer = reinterpret_cast<LayerPh<Z>*>(smem_a);
   IfaceZ<Z>* s_if = reinterpret_cast<IfaceZ<Z>*>(s_layer + nl);
   const int wpb = (int)(blockDim.x / AD_WAVE);
   const int wave = (int)(threadIdx.x / AD_WAVE), lane = (int)(threadIdx.x % AD_WAVE);
   double* s_red = reinterpret_cast<double*>(s_if + (nl - 1)) +
-                  (size_t)wave * red_rows<Z>(red_split != 0) * RED_STRIDE;
+                  (size_t)wave * red_rows<Z>(red_mode) * RED_STRIDE;
   const int ngroups = ntp / wpb;
   int b = blockIdx.x;
   const int grp = b % ngroups;
@@ -252,7 +280,7 @@ __global__ void __launch_bounds__(AD_BLOCK_MAX) adjoint_kernel(
   const cd om = omega[wc];
   const size_t slot = ((size_t)dir * np + ip) * ntp + tile;  // (dir, p, tile)
   GlobalStore<Z> st{ws + (long)slot * AD_WAVE + lane, nlanes};
-  WaveSink<Z> sink{cot_if + slot * (nl - 1), cot_ly + slot * nl, s_red, red_split != 0, wave_ok};
+  WaveSink<Z> sink{cot_if + slot * (nl - 1), cot_ly + slot * nl, s_red, red_mode, wave_ok};
   const cd gin = Gin[(size_t)ip * nw + wc];
   const cd dgin = dGin ? dGin[((size_t)dir * np + ip) * nw + wc] : cd{0.0, 0.0};
   Z chi = lift<Z>(0.0);
@@ -415,14 +443,17 @@ static int run_adjoint(const char* what, const double* layers, const double* dla
   char* wsb = reinterpret_cast<char*>(workspace);
   const long nblocks = (long)ndir * np * (d.ntp / d.wpb);
   KH_REQUIRE(nblocks <= 0x7fffffffL, KH_ERR_UNSUPPORTED, "%s: grid too large", what);
-  // split the reductions only when whole-record buffers would keep four
-  // waves (the register limit of the dual kernel: one per SIMD) off a CU's
-  // 160 KB of LDS
+  // the fewest reduction passes whose buffers still let four waves (the
+  // register limit of the dual kernel: one per SIMD) share a CU's 160 KB of
+  // LDS
   const size_t tables = (size_t)nl * sizeof(LayerPh<Z>) + (size_t)(nl - 1) * sizeof(IfaceZ<Z>);
-  const size_t red_full = (size_t)red_rows<Z>(false) * RED_STRIDE * sizeof(double);
   const int blocks_per_cu = 4 / d.wpb;
-  const bool split = (size_t)blocks_per_cu * (tables + d.wpb * red_full) > (size_t)160 * 1024;
-  const size_t lds = tables + (size_t)d.wpb * red_rows<Z>(split) * RED_STRIDE * sizeof(double);
+  auto lds_of = [&](int mode) {
+    return tables + (size_t)d.wpb * red_rows<Z>(mode) * RED_STRIDE * sizeof(double);
+  };
+  int mode = RED_LEVEL;
+  while (mode < RED_HALF && (size_t)blocks_per_cu * lds_of(mode) > (size_t)160 * 1024) ++mode;
+  const size_t lds = lds_of(mode);
   auto* cif = reinterpret_cast<IfaceZ<Z>*>(wsb + d.cif);
   auto* cly = reinterpret_cast<LayerCot<Z>*>(wsb + d.cly);
   auto* chip = reinterpret_cast<Z*>(wsb + d.chi);
@@ -431,7 +462,7 @@ static int run_adjoint(const char* what, const double* layers, const double* dla
                      reinterpret_cast<const cd*>(Gin), reinterpret_cast<const cd*>(dGin), second_only,
                      reinterpret_cast<Z*>(wsb + d.ws), d.nlanes, cif, cly, chip,
                      reinterpret_cast<cd*>(R), reinterpret_cast<cd*>(dR), d.ntiles, d.ntp,
-                     split ? 1 : 0, partial);
+                     mode, partial);
   KH_TRY(check_launch(what));
   const size_t lds_c = (size_t)(nl - 1) * sizeof(IfaceZ<Z>) + (size_t)nl * sizeof(LayerCot<Z>);
   hipLaunchKernelGGL((chain_kernel<Z>), dim3((unsigned)((long)ndir * np)), dim3(CH_BLOCK), lds_c, st,

@@ -128,7 +128,8 @@ __host__ __device__ __forceinline__ M2<Z> step_fwd_flat(const M2<Z>& S, const LZ
 }
 
 // Storage: st(e) returns a reference to Z-slot e of this lane.
-// Sink: sink.iface(i, IfaceZ<Z>), sink.layer(l, LayerCot<Z>).
+// Sink: sink.iface(i, IfaceZ<Z>), sink.layer(l, LayerCot<Z>) and
+// sink.level(i, IfaceZ<Z>, LayerCot<Z>) for interface i and layer i + 1 together.
 // G_of(R) -> cotangent of R.  Returns R.
 // L: per-layer records with eta, neta, h and halfspace (LayerZ<Z> or
 // LayerPh<Z>).
@@ -244,7 +245,6 @@ __host__ __device__ inline Z lane_adjoint_k(const LZ* __restrict__ L,
       M2<Z> rb;
       Z eab, ebb;
       step_rev(Sin, ea, eb, primal_uniform(F[i]), sb, rb, fbar, eab, ebb);
-      sink.iface(i, fbar);
       LayerCot<Z> c{zero_z<Z>(), zero_z<Z>(), zero_z<Z>()};
       const LZ& lb = L[i + 1];
       if (!lb.halfspace) {
@@ -253,7 +253,7 @@ __host__ __device__ inline Z lane_adjoint_k(const LZ* __restrict__ L,
         phase_rev(lb.neta, om, lb.h, eb, ebb, c.neta, hb2);
         c.h = c.h + hb2;
       }
-      sink.layer(i + 1, c);
+      sink.level(i, fbar, c);  // interface i and layer i + 1
       sb = rb;
     }
   }

@@ -33,6 +33,10 @@ struct Sink {
     cly[l].neta = cly[l].neta + c.neta;
     cly[l].h = cly[l].h + c.h;
   }
+  void level(int i, const IfaceZ<Z>& f, const LayerCot<Z>& c) {
+    iface(i, f);
+    layer(i + 1, c);
+  }
 };
 
 template <class Z>
