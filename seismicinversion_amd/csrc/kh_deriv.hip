@@ -395,6 +395,172 @@ __global__ void __launch_bounds__(CH_BLOCK) chain_kernel(const double* __restric
   }
 }
 
+// ---- the chain of a partial Hessian, split by what a direction reaches ----
+//
+// The second-order part of a pair (layer l, column c) of direction d comes
+// from the curvature of the coefficient formulas along d at the interfaces
+// around layer l, which read layers l - 1, l and l + 1 only.  Where d does
+// not move any of them, that curvature is exactly zero and the pair's
+// Hessian-vector product is the first-order chain of the cotangents'
+// tangent parts (one dual level instead of two).  The few pairs d reaches
+// run the full dual-of-dual chain, packed across (direction, slowness) pairs
+// so that their waves are full.
+
+// The shallowest layer in which direction row `drow` ((nl, 6)) is nonzero,
+// nl for a zero direction.
+__device__ __forceinline__ int shallowest_moved_layer(const double* __restrict__ drow, int nl) {
+  for (int l = 0; l < nl; ++l) {
+    for (int c = 0; c < NCOL; ++c)
+      if (drow[(size_t)l * NCOL + c] != 0.0) return l;
+  }
+  return nl;
+}
+
+// Layers [lo, hi] whose pairs direction d reaches at second order (empty
+// when lo > hi), and the deepest layer with a whole cotangent.
+struct Reach {
+  int lo, hi, lmax;
+};
+__device__ __forceinline__ Reach reach_of(const double* __restrict__ dl, int nl, int dir,
+                                          int partial) {
+  const int deep = deepest_moved_layer(dl, nl);
+  const int shallow = shallowest_moved_layer(dl, nl);
+  Reach r;
+  r.lmax = (partial && dir != 0) ? deep + 1 : nl - 1;
+  r.lo = shallow - 1 > 0 ? shallow - 1 : 0;
+  const int top = r.lmax < nl - 1 ? r.lmax : nl - 1;
+  r.hi = deep + 1 < top ? deep + 1 : top;
+  return r;
+}
+
+__device__ __forceinline__ IfaceZ<cd> tangent_iface(const IfaceZ<dcd>& f) {
+  return IfaceZ<cd>{f.rd00.d, f.rd01.d, f.rd11.d, f.ru00.d, f.ru01.d,
+                    f.ru11.d, f.td00.d, f.td01.d, f.td10.d, f.td11.d};
+}
+
+// Per (dir, p): the tiles' cotangents summed in tile order over tile 0's
+// slot (each thread reads every tile of its double before writing it), and
+// chi from the tile partials.
+template <class Z>
+__global__ void __launch_bounds__(CH_BLOCK) cot_sum_kernel(int nl, int np,
+                                                           IfaceZ<Z>* __restrict__ cot_if,
+                                                           LayerCot<Z>* __restrict__ cot_ly,
+                                                           const Z* __restrict__ chi_part,
+                                                           int ntiles, int ntp,
+                                                           double* __restrict__ chi) {
+  const int ip = (int)(blockIdx.x % (unsigned)np);
+  const int dir = (int)(blockIdx.x / (unsigned)np);
+  const size_t base = ((size_t)dir * np + ip) * ntp;
+  constexpr int NI = (int)(sizeof(IfaceZ<Z>) / sizeof(double));
+  constexpr int NLY = (int)(sizeof(LayerCot<Z>) / sizeof(double));
+  const int nif = (nl - 1) * NI;
+  const int nall = nif + nl * NLY;
+  double* gif = reinterpret_cast<double*>(cot_if + base * (nl - 1));
+  double* gly = reinterpret_cast<double*>(cot_ly + base * nl);
+  for (int e = threadIdx.x; e < nall; e += blockDim.x) {
+    double acc = 0.0;
+    if (e < nif) {
+      for (int t = 0; t < ntiles; ++t) acc += gif[(size_t)t * nif + e];
+      gif[e] = acc;
+    } else {
+      const int e2 = e - nif;
+      for (int t = 0; t < ntiles; ++t) acc += gly[(size_t)t * (nall - nif) + e2];
+      gly[e2] = acc;
+    }
+  }
+  if (chi && chi_part && threadIdx.x == 0) {
+    Z sz = lift<Z>(0.0);
+    for (int t = 0; t < ntiles; ++t) sz = sz + chi_part[base + t];
+    chi[((size_t)dir * np + ip) * 2] = prim(sz).r;
+    chi[((size_t)dir * np + ip) * 2 + 1] = tangent_re(sz);
+  }
+}
+
+// Per (dir, p): the pairs the direction does not reach (first-order chain of
+// the tangent parts; for direction 0 also the gradient, from the primal
+// parts) and zeros beyond lmax.  Reads the tile sums at tile 0's slot.
+__global__ void __launch_bounds__(CH_BLOCK) chain_far_kernel(
+    const double* __restrict__ layers, const double* __restrict__ dlayers, int nl,
+    const double* __restrict__ p, int np, const IfaceZ<dcd>* __restrict__ cot_if,
+    const LayerCot<dcd>* __restrict__ cot_ly, int ntp, double* __restrict__ grad,
+    double* __restrict__ hvp, int partial) {
+  extern __shared__ double smem_f[];
+  IfaceZ<dcd>* s_if = reinterpret_cast<IfaceZ<dcd>*>(smem_f);
+  LayerCot<dcd>* s_ly = reinterpret_cast<LayerCot<dcd>*>(s_if + (nl - 1));
+  const int ip = (int)(blockIdx.x % (unsigned)np);
+  const int dir = (int)(blockIdx.x / (unsigned)np);
+  const size_t base = ((size_t)dir * np + ip) * ntp;
+  constexpr int NI = (int)(sizeof(IfaceZ<dcd>) / sizeof(double));
+  constexpr int NLY = (int)(sizeof(LayerCot<dcd>) / sizeof(double));
+  const int nif = (nl - 1) * NI;
+  const int nall = nif + nl * NLY;
+  const double* gif = reinterpret_cast<const double*>(cot_if + base * (nl - 1));
+  const double* gly = reinterpret_cast<const double*>(cot_ly + base * nl);
+  double* sif = reinterpret_cast<double*>(s_if);
+  double* sly = reinterpret_cast<double*>(s_ly);
+  for (int e = threadIdx.x; e < nall; e += blockDim.x) {
+    if (e < nif) sif[e] = gif[e];
+    else sly[e - nif] = gly[e - nif];
+  }
+  __syncthreads();
+  const double* dl = dlayers + (size_t)dir * nl * NCOL;
+  const Reach rc = reach_of(dl, nl, dir, partial);
+  const cd zc{0.0, 0.0};
+  const IfaceZ<cd> none{zc, zc, zc, zc, zc, zc, zc, zc, zc, zc};
+  for (int k = threadIdx.x; k < nl * NCOL; k += blockDim.x) {
+    const int l = k / NCOL, col = k - (k / NCOL) * NCOL;
+    const size_t g = ((size_t)ip * nl + l) * NCOL + col;
+    double* hv = hvp + (size_t)dir * np * nl * NCOL + g;
+    if (l > rc.lmax) {
+      *hv = 0.0;
+      continue;
+    }
+    if (l >= rc.lo && l <= rc.hi) continue;  // chain_near_kernel's
+    const LayerCot<dcd>& lc = s_ly[l];
+    *hv = chain_column<cd>(layers, nullptr, nl, l, p[ip],
+                           l >= 1 ? tangent_iface(s_if[l - 1]) : none,
+                           l <= nl - 2 ? tangent_iface(s_if[l]) : none,
+                           LayerCot<cd>{lc.eta.d, lc.neta.d, lc.h.d}, col).r;
+    if (grad && dir == 0)
+      grad[g] = chain_column<cd>(layers, nullptr, nl, l, p[ip],
+                                 l >= 1 ? prim_iface(s_if[l - 1]) : none,
+                                 l <= nl - 2 ? prim_iface(s_if[l]) : none,
+                                 LayerCot<cd>{lc.eta.v, lc.neta.v, lc.h.v}, col).r;
+  }
+}
+
+// The pairs each direction reaches, NEAR_THREADS threads per (dir, p)
+// (three layers' columns: a basis direction's reach), packed into full
+// waves; a wider reach is covered by the same threads in turn.
+constexpr int NEAR_THREADS = 3 * NCOL;
+
+__global__ void __launch_bounds__(CH_BLOCK) chain_near_kernel(
+    const double* __restrict__ layers, const double* __restrict__ dlayers, int nl,
+    const double* __restrict__ p, int np, int ndir, const IfaceZ<dcd>* __restrict__ cot_if,
+    const LayerCot<dcd>* __restrict__ cot_ly, int ntp, double* __restrict__ grad,
+    double* __restrict__ hvp, int partial) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long pr = t / NEAR_THREADS;
+  if (pr >= (long)ndir * np) return;
+  const int j0 = (int)(t - pr * NEAR_THREADS);
+  const int ip = (int)(pr % np), dir = (int)(pr / np);
+  const size_t base = ((size_t)dir * np + ip) * ntp;
+  const IfaceZ<dcd>* cif = cot_if + base * (nl - 1);
+  const LayerCot<dcd>* cly = cot_ly + base * nl;
+  const double* dl = dlayers + (size_t)dir * nl * NCOL;
+  const Reach rc = reach_of(dl, nl, dir, partial);
+  const dcd zero = lift<dcd>(0.0);
+  const IfaceZ<dcd> none{zero, zero, zero, zero, zero, zero, zero, zero, zero, zero};
+  for (int j = j0; rc.lo + j / NCOL <= rc.hi; j += NEAR_THREADS) {
+    const int l = rc.lo + j / NCOL, col = j - (j / NCOL) * NCOL;
+    const size_t g = ((size_t)ip * nl + l) * NCOL + col;
+    const dcd v = chain_column<dcd>(layers, dl, nl, l, p[ip], l >= 1 ? cif[l - 1] : none,
+                                    l <= nl - 2 ? cif[l] : none, cly[l], col);
+    if (grad && dir == 0) grad[g] = prim(v).r;
+    hvp[(size_t)dir * np * nl * NCOL + g] = tangent_re(v);
+  }
+}
+
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct DerivLayout {
@@ -465,6 +631,22 @@ static int run_adjoint(const char* what, const double* layers, const double* dla
                      mode, partial);
   KH_TRY(check_launch(what));
   const size_t lds_c = (size_t)(nl - 1) * sizeof(IfaceZ<Z>) + (size_t)nl * sizeof(LayerCot<Z>);
+  if constexpr (std::is_same<Z, dcd>::value) {
+    if (partial && dlayers && hvp) {
+      hipLaunchKernelGGL((cot_sum_kernel<Z>), dim3((unsigned)((long)ndir * np)), dim3(CH_BLOCK),
+                         0, st, nl, np, cif, cly, MISFIT ? chip : nullptr, d.ntiles, d.ntp, chi);
+      KH_TRY(check_launch(what));
+      hipLaunchKernelGGL(chain_far_kernel, dim3((unsigned)((long)ndir * np)), dim3(CH_BLOCK),
+                         lds_c, st, layers, dlayers, nl, p, np, cif, cly, d.ntp, grad, hvp,
+                         partial);
+      KH_TRY(check_launch(what));
+      const long near_threads = (long)ndir * np * NEAR_THREADS;
+      hipLaunchKernelGGL(chain_near_kernel, dim3((unsigned)((near_threads + CH_BLOCK - 1) / CH_BLOCK)),
+                         dim3(CH_BLOCK), 0, st, layers, dlayers, nl, p, np, ndir, cif, cly, d.ntp,
+                         grad, hvp, partial);
+      return check_launch(what);
+    }
+  }
   hipLaunchKernelGGL((chain_kernel<Z>), dim3((unsigned)((long)ndir * np)), dim3(CH_BLOCK), lds_c, st,
                      layers, dlayers, nl, p, np, ndir, cif, cly, MISFIT ? chip : nullptr,
                      d.ntiles, d.ntp, grad, hvp, chi, partial);
