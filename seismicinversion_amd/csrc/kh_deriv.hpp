@@ -3,12 +3,15 @@
 //
 // lane_adjoint<Z>: one (p, omega) lane.
 //   1. forward sweep from the deepest interface, saving the state S_j every
-//      K = ceil(sqrt(nl-1)) levels (j % K == 0) in per-lane storage;
+//      K levels (j % K == 0) in per-lane storage: K = 1 (the whole tape) for
+//      models of up to TAPE_LAYERS layers, else K = ceil(sqrt(nl-1));
 //   2. R at the surface and its cotangent G (given, or 2 (R - R_obs) for the
 //      misfit |R - R_obs|^2);
-//   3. reverse sweep top-down in segments of K interfaces: each segment's
-//      states are recomputed from the checkpoint below it (with the phase
-//      factors) into a K-entry buffer, then walked back with step_rev.
+//   3. reverse sweep top-down: with the whole tape each level reads its
+//      state and recomputes its phase factors; with K > 1, in segments of K
+//      interfaces, each segment's states are recomputed from the checkpoint
+//      below it (with the phase factors) into a K-entry buffer, then walked
+//      back with step_rev.
 //   The cotangents of every interface's coefficients and of every layer's
 //   (eta, nu, h) are handed to a sink (the kernel reduces them over the
 //   wavefront; the host harness sums them).
@@ -35,16 +38,29 @@ struct LayerCot {
   Z eta, neta, h;
 };
 
-__host__ __device__ __forceinline__ int ckpt_stride(int nl) {
+// Models of up to this many layers keep the whole forward tape (every
+// level's state, K = 1): the reverse sweep then recomputes nothing and
+// reads one state per level.  Deeper models keep sqrt(N) checkpoints.
+constexpr int TAPE_LAYERS = 40;
+
+// Levels between checkpoints: 1 up to TAPE_LAYERS layers, else
+// ceil(sqrt(nl - 1)).
+__host__ __device__ __forceinline__ int ckpt_sqrt(int nl) {
   int k = 1;
   while (k * k < nl - 1) ++k;
   return k;
 }
+__host__ __device__ __forceinline__ int ckpt_stride(int nl) {
+  return nl <= TAPE_LAYERS ? 1 : ckpt_sqrt(nl);
+}
 __host__ __device__ __forceinline__ int ckpt_count(int nl, int K) { return (nl - 2) / K + 1; }
-// Z-slots of per-lane storage: nck checkpoints of 4 + K buffer entries of 6.
+// Z-slots of per-lane storage at stride K: the checkpoints (4 each), then,
+// for K > 1, the K-level buffer of the segment being walked back (6 each).
+__host__ __device__ __forceinline__ int lane_slots_k(int nl, int K) {
+  return ckpt_count(nl, K) * 4 + (K > 1 ? K * 6 : 0);
+}
 __host__ __device__ __forceinline__ int lane_slots(int nl) {
-  const int K = ckpt_stride(nl);
-  return ckpt_count(nl, K) * 4 + K * 6;
+  return lane_slots_k(nl, ckpt_stride(nl));
 }
 
 template <class Z>
@@ -101,6 +117,23 @@ __host__ __device__ __forceinline__ IfaceZ<cd> prim_iface(const IfaceZ<Dual<T>>&
                     prim(f.ru11), prim(f.td00), prim(f.td01), prim(f.td10), prim(f.td11)};
 }
 
+// The phases of a layer whose parameters carry no tangent: those of the
+// primal parts, lifted with zero tangents.
+template <class Z, class LZ>
+__host__ __device__ __forceinline__ void phases_flat(const LZ& lb, cd om, Z& ea, Z& eb) {
+  if constexpr (std::is_same<Z, cd>::value) {
+    phases_of(lb, om, ea, eb);
+  } else {
+    cd a{1.0, 0.0}, b{1.0, 0.0};
+    if (!lb.halfspace) {
+      a = phase_t(prim(lb.eta), om, prim(lb.h));
+      b = phase_t(prim(lb.neta), om, prim(lb.h));
+    }
+    ea = lift_c<Z>(a);
+    eb = lift_c<Z>(b);
+  }
+}
+
 // A forward step whose inputs carry no tangent: every layer it reads lies
 // below the deepest layer the direction moves, so the state below, the
 // interface and the phases all have zero tangents.  The plain step on the
@@ -110,19 +143,12 @@ __host__ __device__ __forceinline__ IfaceZ<cd> prim_iface(const IfaceZ<Dual<T>>&
 template <class Z, class LZ>
 __host__ __device__ __forceinline__ M2<Z> step_fwd_flat(const M2<Z>& S, const LZ& lb, cd om,
                                                         const IfaceZ<Z>& f, Z& ea, Z& eb) {
+  phases_flat(lb, om, ea, eb);
   if constexpr (std::is_same<Z, cd>::value) {
-    phases_of(lb, om, ea, eb);
     return step_fwd(S, ea, eb, f);
   } else {
-    cd a{1.0, 0.0}, b{1.0, 0.0};
-    if (!lb.halfspace) {
-      a = phase_t(prim(lb.eta), om, prim(lb.h));
-      b = phase_t(prim(lb.neta), om, prim(lb.h));
-    }
     const M2<cd> s{prim(S.a00), prim(S.a01), prim(S.a10), prim(S.a11)};
-    const M2<cd> r = step_fwd(s, a, b, prim_iface(f));
-    ea = lift_c<Z>(a);
-    eb = lift_c<Z>(b);
+    const M2<cd> r = step_fwd(s, prim(ea), prim(eb), prim_iface(f));
     return M2<Z>{lift_c<Z>(r.a00), lift_c<Z>(r.a01), lift_c<Z>(r.a10), lift_c<Z>(r.a11)};
   }
 }
@@ -199,7 +225,51 @@ __host__ __device__ inline Z lane_adjoint_k(const LZ* __restrict__ L,
     sink.layer(0, c0);
   }
 
-  // 3. reverse, segment by segment (down to interface lrev)
+  // 3. reverse, top-down to interface ilast
+  // the bottom interface: RRd = Rd, no step
+  auto bottom = [&](int i) {
+    IfaceZ<Z> fbar;
+    fbar.ru00 = fbar.ru01 = fbar.ru11 = zero_z<Z>();
+    fbar.td00 = fbar.td01 = fbar.td10 = fbar.td11 = zero_z<Z>();
+    fbar.rd00 = sb.a00;
+    fbar.rd01 = sb.a01 + sb.a10;
+    fbar.rd11 = sb.a11;
+    sink.iface(i, fbar);
+  };
+  // interface i, from the state below it and the phases of layer i + 1
+  auto level = [&](int i, const M2<Z>& Sin, const Z& ea, const Z& eb) {
+    IfaceZ<Z> fbar;
+    fbar.rd00 = fbar.rd01 = fbar.rd11 = zero_z<Z>();
+    fbar.ru00 = fbar.ru01 = fbar.ru11 = zero_z<Z>();
+    fbar.td00 = fbar.td01 = fbar.td10 = fbar.td11 = zero_z<Z>();
+    M2<Z> rb;
+    Z eab, ebb;
+    step_rev(Sin, ea, eb, primal_uniform(F[i]), sb, rb, fbar, eab, ebb);
+    LayerCot<Z> c{zero_z<Z>(), zero_z<Z>(), zero_z<Z>()};
+    const LZ& lb = L[i + 1];
+    if (!lb.halfspace) {
+      phase_rev(lb.eta, om, lb.h, ea, eab, c.eta, c.h);
+      Z hb2 = zero_z<Z>();
+      phase_rev(lb.neta, om, lb.h, eb, ebb, c.neta, hb2);
+      c.h = c.h + hb2;
+    }
+    sink.level(i, fbar, c);  // interface i and layer i + 1
+    sb = rb;
+  };
+  if (K == 1) {  // the whole tape: S_{i+1} is checkpoint i + 1
+    for (int i = 0; i <= ilast; ++i) {
+      if (i == nl - 2) {
+        bottom(i);
+        continue;
+      }
+      Z ea, eb;
+      if (i > ltan) phases_flat(L[i + 1], om, ea, eb);
+      else phases_of(L[i + 1], om, ea, eb);
+      level(i, load(i + 1), ea, eb);
+    }
+    return R;
+  }
+  // segment by segment of K levels
   for (int i0 = 0; i0 <= ilast; i0 += K) {
     const int i1 = (i0 + K - 1 < nl - 2) ? i0 + K - 1 : nl - 2;
     // recompute S_{i+1} (and the phases of layer i+1) for i = i1 .. i0
@@ -217,7 +287,10 @@ __host__ __device__ inline Z lane_adjoint_k(const LZ* __restrict__ L,
       buf(k, 1) = T.a01;
       buf(k, 2) = T.a10;
       buf(k, 3) = T.a11;
-      if (i > ltan) {
+      if (i == i0) {  // the state this step would give lies above the segment
+        if (i > ltan) phases_flat(L[i + 1], om, ea, eb);
+        else phases_of(L[i + 1], om, ea, eb);
+      } else if (i > ltan) {
         T = step_fwd_flat(T, L[i + 1], om, F[i], ea, eb);
       } else {
         phases_of(L[i + 1], om, ea, eb);
@@ -228,33 +301,12 @@ __host__ __device__ inline Z lane_adjoint_k(const LZ* __restrict__ L,
     }
     const int iend = (i1 < ilast) ? i1 : ilast;
     for (int i = i0; i <= iend; ++i) {
-      IfaceZ<Z> fbar;
-      fbar.rd00 = fbar.rd01 = fbar.rd11 = zero_z<Z>();
-      fbar.ru00 = fbar.ru01 = fbar.ru11 = zero_z<Z>();
-      fbar.td00 = fbar.td01 = fbar.td10 = fbar.td11 = zero_z<Z>();
       if (i == nl - 2) {
-        fbar.rd00 = sb.a00;
-        fbar.rd01 = sb.a01 + sb.a10;
-        fbar.rd11 = sb.a11;
-        sink.iface(i, fbar);
+        bottom(i);
         continue;
       }
       const int k = i - i0;
-      const M2<Z> Sin{buf(k, 0), buf(k, 1), buf(k, 2), buf(k, 3)};
-      const Z ea = buf(k, 4), eb = buf(k, 5);
-      M2<Z> rb;
-      Z eab, ebb;
-      step_rev(Sin, ea, eb, primal_uniform(F[i]), sb, rb, fbar, eab, ebb);
-      LayerCot<Z> c{zero_z<Z>(), zero_z<Z>(), zero_z<Z>()};
-      const LZ& lb = L[i + 1];
-      if (!lb.halfspace) {
-        phase_rev(lb.eta, om, lb.h, ea, eab, c.eta, c.h);
-        Z hb2 = zero_z<Z>();
-        phase_rev(lb.neta, om, lb.h, eb, ebb, c.neta, hb2);
-        c.h = c.h + hb2;
-      }
-      sink.level(i, fbar, c);  // interface i and layer i + 1
-      sb = rb;
+      level(i, M2<Z>{buf(k, 0), buf(k, 1), buf(k, 2), buf(k, 3)}, buf(k, 4), buf(k, 5));
     }
   }
   return R;

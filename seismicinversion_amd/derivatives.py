@@ -21,8 +21,9 @@ __all__ = ["jvp", "vjp", "misfit_grad", "misfit_hvp", "vjp_hvp", "paced", "WORKS
            "symmetric_from_partial"]
 
 # Directions per launch are capped so that one launch's workspace stays below
-# this (the 20-layer LM's 75 directions at 128 x 512 need 7.8 GB: one launch).
-WORKSPACE_LIMIT = 8 << 30
+# this (the 20-layer LM's 75 directions at 128 x 512 keep their whole forward
+# tape in 12.5 GB: one launch).
+WORKSPACE_LIMIT = 16 << 30
 
 
 def _f64(x, dev):

@@ -143,12 +143,13 @@ extern "C" int host_misfit_hvp_mode(const double* layers, const double* dlayers,
   const int lt = mode >= 1 ? ltan : (1 << 30);
   const int lrev = mode >= 2 ? ltan + 1 : (1 << 30);
   // mode 3 is mode 2 with checkpoints every 2 levels and the buffer in a
-  // store of its own
+  // store of its own; mode 4 is mode 2 with sqrt(N) checkpoints and the
+  // buffer after them (the layout of models deeper than TAPE_LAYERS)
   std::vector<LayerZ<dcd>> L(nl);
   std::vector<IfaceZ<dcd>> F(nl - 1);
   stage<dcd>(layers, dlayers, nl, p, L, F);
   Sink<dcd> sink(nl);
-  Store<dcd> st(lane_slots(nl) + 64);
+  Store<dcd> st(lane_slots(nl) + lane_slots_k(nl, ckpt_sqrt(nl)) + 64);
   Store<dcd> bst(2 * 6);  // mode 3: two-level segments, buffer apart (the LDS layout)
   dcd chis = lift<dcd>(0.0);
   for (int w = 0; w < nw; ++w) {
@@ -161,10 +162,18 @@ extern "C" int host_misfit_hvp_mode(const double* layers, const double* dlayers,
       if (second_only) g.d = cd{0.0, 0.0};
       return g;
     };
-    const dcd r = mode == 3
-                      ? lane_adjoint_k<dcd>(L.data(), F.data(), nl, om, fs, st, bst, 2, sink, G,
-                                            ltan, ltan + 1)
-                      : lane_adjoint<dcd>(L.data(), F.data(), nl, om, fs, st, sink, G, lt, lrev);
+    dcd r;
+    if (mode == 3) {
+      r = lane_adjoint_k<dcd>(L.data(), F.data(), nl, om, fs, st, bst, 2, sink, G, ltan,
+                              ltan + 1);
+    } else if (mode == 4) {
+      const int K = ckpt_sqrt(nl);
+      BufAfter<dcd, Store<dcd>> after{st, ckpt_count(nl, K) * 4};
+      r = lane_adjoint_k<dcd>(L.data(), F.data(), nl, om, fs, st, after, K, sink, G, ltan,
+                              ltan + 1);
+    } else {
+      r = lane_adjoint<dcd>(L.data(), F.data(), nl, om, fs, st, sink, G, lt, lrev);
+    }
     dR[2 * w] = r.d.r;
     dR[2 * w + 1] = r.d.i;
   }

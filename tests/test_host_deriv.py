@@ -111,7 +111,8 @@ def test_flat_forward_and_partial_reverse(lib, fs, second_only):
     dual sweep's chi, gradient, Hessian-vector product and dR; stopping the
     reverse one interface below it gives bitwise the same entries for the
     layers above, and the Hessian assembled from those rows by symmetry
-    equals the full one."""
+    equals the full one.  The whole forward tape (the stride of this model),
+    two-level segments and sqrt(N) segments give the same numbers."""
     lib.host_misfit_hvp_mode.argtypes = [vp, vp, ctypes.c_int, ctypes.c_double, vp, ctypes.c_int,
                                          ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, vp, vp, vp,
                                          vp]
@@ -124,12 +125,12 @@ def test_flat_forward_and_partial_reverse(lib, fs, second_only):
     Robs = np.ascontiguousarray(np.exp(1j * np.arange(40) * 0.1) * 0.05)
     idx = packed_index(nl)
     P = len(idx)
-    rows = {m: np.zeros((P, nl, 6)) for m in (0, 1, 2, 3)}
+    rows = {m: np.zeros((P, nl, 6)) for m in (0, 1, 2, 3, 4)}
     for k, (l, c) in enumerate(idx):
         d = np.zeros((nl, 6))
         d[l, c] = 1.0
         outs = {}
-        for mode in (0, 1, 2, 3):
+        for mode in (0, 1, 2, 3, 4):
             chi, grad, hvp = np.zeros(2), np.zeros((nl, 6)), np.zeros((nl, 6))
             dR = np.zeros(40, np.complex128)
             lib.host_misfit_hvp_mode(tab.ctypes.data, d.ctypes.data, nl, p, w.ctypes.data, 40,
@@ -144,9 +145,10 @@ def test_flat_forward_and_partial_reverse(lib, fs, second_only):
         np.testing.assert_array_equal(outs[0][3], outs[2][3])
         np.testing.assert_array_equal(outs[2][2][:l + 2], outs[0][2][:l + 2])
         assert np.all(outs[2][2][l + 2:] == 0)
-        for a, b in zip(outs[2], outs[3]):  # two-level segments: the same numbers
-            np.testing.assert_array_equal(a[:l + 2] if a.ndim == 2 else a,
-                                          b[:l + 2] if b.ndim == 2 else b)
+        for m in (3, 4):  # two-level segments, sqrt(N) segments: the same numbers
+            for a, b in zip(outs[2], outs[m]):
+                np.testing.assert_array_equal(a[:l + 2] if a.ndim == 2 else a,
+                                              b[:l + 2] if b.ndim == 2 else b)
     li = np.array([l for l, _ in idx])
     ci = np.array([c for _, c in idx])
     M = rows[2][:, li, ci]        # M[k, j] = entry j of direction k (layers <= l_k + 1)
