@@ -80,10 +80,13 @@ def parse():
                     help="processes of the all-core CPU baseline (the box's share is 16 cores)")
     ap.add_argument("--warmup-seconds", type=float, default=2.0,
                     help="untimed forward launches before anything is timed (clock ramp)")
-    ap.add_argument("--section", choices=("all", "lm", "open", "served"), default="all",
-                    help="'lm': only the 20-layer Newton-LM figures (for profiling its kernels); "
-                         "'open': only the open-loop rate sweep; 'served': only the headline "
-                         "served steps and their roofline")
+    ap.add_argument("--section", choices=("all", "lm", "open", "served", "hessian"),
+                    default="all",
+                    help="'lm': only the 20-layer Newton-LM figures; 'hessian': only --steps "
+                         "misfit + gradient + exact-Hessian evaluations of that LM after "
+                         "--warmup untimed ones (for profiling its kernels); 'open': only the "
+                         "open-loop rate sweep; 'served': only the headline served steps and "
+                         "their roofline")
     ap.add_argument("--traffic-json", default=os.path.join(HERE, "profiles", "traffic_multi_served.json"),
                     help="HBM bytes per launch of the multi-model kernel from rocprofv3 --pmc")
     return ap.parse_args()
@@ -369,7 +372,6 @@ def lm_figures(torch, dev):
     evaluation timed on its own, eager and replayed.  Beside it the served
     inversion size (17 layers, 32 p x 128 omega, 5 iterations), where the
     kernels are short and launch overhead counts, eager and with graphs."""
-    from seismicinversion_amd import engine, kennett_torch as kt
     from seismicinversion_amd.taup_inversion import _Problem, invert_taup
     from seismicinversion_amd.workloads import seeded_model, uniform_slownesses
 
@@ -387,15 +389,11 @@ def lm_figures(torch, dev):
     t_e = min(t for t, _ in runs[False])
     t_g = min(t for t, _ in runs[True])
     res, res_g = runs[False][0][1], runs[True][0][1]
-    tens = kt.model_to_tensors(m)
-    refs = [tens[k] for k in ("alpha", "beta", "rho", "thickness", "Q_alpha", "Q_beta")]
-    w = torch.arange(1, 513, dtype=torch.float64) * (2 * np.pi / 64)
-    obs = engine.forward(engine.layers_tensor(m, dev), torch.tensor(p), w.to(torch.complex128))
-    prob = _Problem(*refs, p, w, obs)
-    phys = kt._pack_params(*refs[:4]).numpy() * 1.02
+    refs, p_, w, obs, phys = lm_problem(torch, dev)
+    prob = _Problem(*refs, p_, w, obs)
     prob.grad_hessian(phys)
     t_gh = timed(lambda: prob.grad_hessian(phys), 3, torch)
-    prob_g = _Problem(*refs, p, w, obs, use_graphs=True)
+    prob_g = _Problem(*refs, p_, w, obs, use_graphs=True)
     prob_g.grad_hessian(phys)  # eager
     prob_g.grad_hessian(phys)  # captured
     t_gh_graph = timed(lambda: prob_g.grad_hessian(phys), 3, torch)
@@ -425,6 +423,43 @@ def lm_figures(torch, dev):
             "served_size_17_layers_32p_128w": {
                 "eager_ms_per_iteration": min(t_small[False]),
                 "graphs_ms_per_iteration": min(t_small[True])}}
+
+
+def lm_problem(torch, dev):
+    """The 20-layer Newton-LM's device problem (P = 75, 128 p in (0, 0.6] x
+    512 omega, observations of the seeded model) and the start model scaled
+    by 1.02, where its evaluations are timed."""
+    from seismicinversion_amd import engine, kennett_torch as kt
+    from seismicinversion_amd.taup_inversion import _Problem
+    from seismicinversion_amd.workloads import seeded_model, uniform_slownesses
+
+    m = seeded_model(20, 42)
+    p = list(uniform_slownesses(128, 0.6))
+    tens = kt.model_to_tensors(m)
+    refs = [tens[k] for k in ("alpha", "beta", "rho", "thickness", "Q_alpha", "Q_beta")]
+    w = torch.arange(1, 513, dtype=torch.float64) * (2 * np.pi / 64)
+    obs = engine.forward(engine.layers_tensor(m, dev), torch.tensor(p), w.to(torch.complex128))
+    return refs, p, w, obs, kt._pack_params(*refs[:4]).numpy() * 1.02
+
+
+def hessian_figures(torch, dev, steps, warmup):
+    """Misfit + gradient + exact Hessian of the 20-layer LM: `warmup`
+    untimed evaluations, then `steps` timed one by one."""
+    from seismicinversion_amd.taup_inversion import _Problem
+
+    refs, p, w, obs, phys = lm_problem(torch, dev)
+    prob = _Problem(*refs, p, w, obs)
+    for _ in range(max(1, warmup)):
+        chi, g, H = prob.grad_hessian(phys)
+    ms = []
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        chi, g, H = prob.grad_hessian(phys)
+        ms.append((time.perf_counter() - t0) * 1e3)
+    return {"P": prob.P, "n_p": len(p), "n_omega": len(w), "evaluations_ms": ms,
+            "min_ms": min(ms), "misfit": chi, "grad_norm": float(np.linalg.norm(g)),
+            "hessian_trace": float(np.trace(H))}
 
 
 def open_loop_sweep(torch, rates, seconds, fwd_deadline, inv_deadline):
@@ -550,6 +585,12 @@ def main():
         torch.cuda.set_device(dev)
         warm_clocks(torch, dev, args.warmup_seconds)
         print(json.dumps({"section": "lm", "lm_20_layers": lm_figures(torch, dev)}))
+        return
+    if args.section == "hessian":
+        dev = torch.device("cuda", 0)
+        torch.cuda.set_device(dev)
+        print(json.dumps({"section": "hessian", "lm_20_layers_evaluation": hessian_figures(
+            torch, dev, args.steps, args.warmup)}))
         return
     if args.section == "open":
         dev = torch.device("cuda", 0)
