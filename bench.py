@@ -169,6 +169,20 @@ def _cpu_gather_part(rng):
     return K.bessel_sum(R, ps, g["w"], g["offsets"], g["dp"], p_max=g["p"][-1])
 
 
+def host_cpu():
+    """The host's CPU model and logical core count (from /proc/cpuinfo)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cores": os.cpu_count()}
+
+
 def cpu_baselines(specs, workers):
     """The oracle timed on the host before the GPU is touched (workers are
     forked): one process on the middle slowness row of every request, all
@@ -228,7 +242,8 @@ def cpu_baselines(specs, workers):
                      "itself would be slower and GPU/CPU ratios are conservative",
            "single_core": {"value": upd1 / dt1, "cores": 1,
                            "sample": f"middle slowness row of each request ({upd1:.3e} updates), "
-                                     f"{dt1:.1f} s"}}
+                                     f"{dt1:.1f} s"},
+           "host": host_cpu()}
     gather_cpu = {"s": dtg, "cores": workers,
                   "what": "whole gather workload: reflectivity + wavenumber sum per 32-slowness "
                           "chunk in the workers, chunks added in order, FFT in the parent"}

@@ -11,8 +11,8 @@ baseline. The product package ``seismicinversion_amd`` never imports it.
 Pinning status of each function (what checks it besides its own formula):
 
 * ``complex_slowness``, ``vertical_slowness`` — closed form and the
-  eta^2 = (s+p)(s-p), Im(eta) > 0 invariants (test_package.py:34-96), and the
-  golden vectors.
+  eta^2 = (s+p)(s-p), Im(eta) > 0 invariants (tests/test_oracle.py:81-97), and
+  the golden vectors.
 * ``interface_coefficients`` — Tu = Td^T, Rd/Ru symmetry, the ocean-bottom
   zero pattern, the normal-incidence impedance formula at p = 0, identical
   media (Rd = Ru = 0, Td = Tu = I), and energy-flux unitarity for elastic
