@@ -232,3 +232,48 @@ def test_hessian_vs_central_differences():
     e = rel(hvp.cpu().numpy()[0, 0], fd)
     print("HVP vs central differences of the gradient (step 1e-6):", e)
     assert e < 1e-6
+
+
+def test_partial_rows_equal_full_rows_for_general_directions():
+    """kh_misfit_hvp_partial with directions that are not a parameter basis:
+    one moving two separated layers (a second-order reach of several layers,
+    covered in turn by the packed dual-of-dual chain), one moving the
+    half-space, a zero direction and a dense one.  For every direction d the
+    rows of the layers down to one below its deepest moved layer equal the
+    full kh_misfit_hvp rows to rounding (the first-order chain of the pairs a
+    direction cannot reach at second order and the dual-of-dual chain of the
+    full call contract their products differently), the deeper rows are 0,
+    chi and dR equal the full call's bit for bit and the gradient to
+    rounding."""
+    model = seeded_model(14, 8)
+    tab = pack_layers(model)
+    nl = len(tab)
+    rng = np.random.default_rng(21)
+    scale = np.array([0.1, 0.1, 0.1, 0.1, 5.0, 5.0])
+    dm = np.zeros((5, nl, 6))
+    dm[0, [2, 7]] = rng.standard_normal((2, 6)) * scale          # two separated layers
+    dm[1, nl - 1] = rng.standard_normal(6) * scale               # the half-space
+    dm[3] = rng.standard_normal((nl, 6)) * scale                 # every layer
+    dm[4, 5, 0] = 0.2                                            # one parameter
+    dm[:, ~np.isfinite(tab[:, 3]), 3] = 0.0                      # (dm[2] stays zero)
+    layers = engine.layers_tensor(model)
+    ps = torch.tensor([0.12, 0.31, 0.55], dtype=torch.float64)
+    w = torch.from_numpy((np.arange(1, 201) * 2 * np.pi / 64.0 + 1j * np.pi / 64).astype(complex))
+    R_obs = engine.forward(layers, ps.to(layers.device), w.to(layers.device)) * (1 - 0.03j)
+    d_t = torch.from_numpy(dm)
+    full = D.misfit_hvp(layers, d_t, ps, w, R_obs, free_surface=True, want_dR=True)
+    part = D.misfit_hvp(layers, d_t, ps, w, R_obs, free_surface=True, want_dR=True, partial=True)
+    chi_f, grad_f, hvp_f, dR_f = (x.cpu().numpy() for x in full)
+    chi_p, grad_p, hvp_p, dR_p = (x.cpu().numpy() for x in part)
+    np.testing.assert_array_equal(chi_p, chi_f)
+    np.testing.assert_array_equal(dR_p, dR_f)
+    assert rel(grad_p, grad_f) <= 1e-12
+    for d in range(len(dm)):
+        moved = np.nonzero(np.any(dm[d] != 0.0, axis=1))[0]
+        lmax = (moved.max() + 1) if (d > 0 and len(moved)) else (0 if d > 0 else nl - 1)
+        top = min(lmax, nl - 1)
+        if np.max(np.abs(hvp_f[d, :, :top + 1])) > 0:
+            assert rel(hvp_p[d, :, :top + 1], hvp_f[d, :, :top + 1]) <= 1e-12, d
+        else:
+            assert np.all(hvp_p[d, :, :top + 1] == 0.0)
+        assert np.all(hvp_p[d, :, top + 1:] == 0.0), d
