@@ -241,13 +241,24 @@ int kh_hermitian_traces(const double* U, int nr, int nw, double gamma, double T,
  * convention for complex outputs of real inputs:
  *     grad_x = Re( sum_w conj(G_w) dR_w/dx ).
  * Thickness derivatives of half-space layers (h = inf) are 0.  nl must not
- * exceed KH_MAX_LAYERS_DERIV.  The adjoint entry points keep the forward
- * states every ceil(sqrt(nl-1)) interfaces in the workspace and recompute the
- * rest; per-frequency cotangents are reduced in a fixed order. */
+ * exceed KH_MAX_LAYERS_DERIV.  The adjoint entry points keep the whole
+ * forward tape (every interface's state) in the workspace when it is large
+ * enough, and otherwise the states every ceil(sqrt(nl-1)) interfaces,
+ * recomputing the rest (the same numbers to rounding); per-frequency
+ * cotangents are reduced in a fixed order. */
 
 /* Workspace bytes for kh_vjp / kh_misfit_grad (dual = 0) or kh_misfit_hvp /
- * kh_vjp_hvp (dual = 1) with ndir directions. */
+ * kh_vjp_hvp (dual = 1) with ndir directions, keeping the whole forward
+ * tape (the faster layout). */
 size_t kh_deriv_workspace_bytes(int nl, int np, int nw, int ndir, int dual);
+
+/* The smallest workspace the same calls accept: sqrt(N) checkpoints (or the
+ * tape, where that is smaller).  A workspace of at least
+ * kh_deriv_workspace_bytes gets the tape, a smaller one of at least this
+ * size the checkpoints.  Replaces, with kh_deriv_workspace_bytes, the
+ * memory the reference's torch autograd keeps for the backward of
+ * Kennett_Reflectivity/kennett_torch.py:312-480. */
+size_t kh_deriv_workspace_bytes_min(int nl, int np, int nw, int ndir, int dual);
 
 /* Tangent-linear (forward-mode) products dR[d] = J dlayers[d], (ndir, np, nw)
  * complex128; R (np, nw) is written too when non-NULL.  One launch gives the
@@ -285,10 +296,11 @@ int kh_misfit_hvp(const double* layers, const double* dlayers, int ndir, int nl,
                   double* grad, double* hvp, double* dR, void* workspace,
                   size_t workspace_bytes, void* stream);
 
-/* kh_misfit_hvp computing, for each direction d > 0, only the entries of
- * the layers 0 .. l_d + 1 of hvp[d], with l_d the deepest layer in which
- * dlayers[d] is nonzero; the entries of deeper layers are written as 0
- * (direction 0, whose sweep also yields grad, is complete).
+/* kh_misfit_hvp computing, for each direction d, only the entries of the
+ * layers 0 .. l_d + 1 of hvp[d], with l_d the deepest layer in which
+ * dlayers[d] is nonzero; the entries of deeper layers are written as 0.
+ * When grad is given, direction 0 (whose sweep also yields grad) is
+ * complete.
  * The reverse sweep stops at interface l_d + 1, so for the basis direction
  * of a parameter of layer l the launch does the sweep work of layers 0..l+1
  * only.  Since the Hessian is symmetric, a caller passing the basis

@@ -6,7 +6,8 @@
 //   jvp_kernel            tangent-linear sweep (Z = Dual<cd>): dR = J dm for
 //                         one parameter direction per grid slice
 //   adjoint_kernel<Z, M>  one wave per (direction, p, 64-frequency tile):
-//                         forward with sqrt(N) checkpoints in the workspace,
+//                         forward keeping the whole tape (or sqrt(N)
+//                         checkpoints, when the workspace holds only those),
 //                         reverse sweep, per-level cotangents reduced over the
 //                         wave (fixed butterfly) and written per tile.
 //                         Z = cd: vector-Jacobian product (M = 0, given
@@ -152,6 +153,15 @@ struct WaveSink {
   }
 };
 
+// Partial modes: PARTIAL_OFF runs every direction's reverse sweep to the
+// bottom; PARTIAL_REST stops it one interface below each direction's deepest
+// moved layer except for direction 0 of the call, whose cotangents also give
+// the gradient; PARTIAL_ALL stops it for every direction (no gradient asked).
+enum { PARTIAL_OFF = 0, PARTIAL_REST = 1, PARTIAL_ALL = 2 };
+__device__ __forceinline__ bool stops_early(int partial, int dir) {
+  return partial == PARTIAL_ALL || (partial == PARTIAL_REST && dir != 0);
+}
+
 // The deepest layer in which direction row `drow` ((nl, 6)) is nonzero,
 // -1 for a zero direction, nl - 1 when there is no direction (every
 // tangent may be nonzero).  The same for every thread of the workgroup.
@@ -256,7 +266,8 @@ __global__ void __launch_bounds__(AD_BLOCK_MAX) adjoint_kernel(
     const cd* __restrict__ Gin, const cd* __restrict__ dGin, int second_only, Z* __restrict__ ws,
     long nlanes,
     IfaceZ<Z>* __restrict__ cot_if, LayerCot<Z>* __restrict__ cot_ly, Z* __restrict__ chi_part,
-    cd* __restrict__ R, cd* __restrict__ dR, int ntiles, int ntp, int red_mode, int partial) {
+    cd* __restrict__ R, cd* __restrict__ dR, int ntiles, int ntp, int red_mode, int partial,
+    int kstride) {
   extern __shared__ double smem_a[];
   LayerPh<Z>* s_layer = reinterpret_cast<LayerPh<Z>*>(smem_a);
   IfaceZ<Z>* s_if = reinterpret_cast<IfaceZ<Z>*>(s_layer + nl);
@@ -290,7 +301,7 @@ __global__ void __launch_bounds__(AD_BLOCK_MAX) adjoint_kernel(
   const int ltan = std::is_same<Z, cd>::value
                        ? nl : deepest_moved_layer(dlayers ? dlayers + (size_t)dir * nl * NCOL
                                                           : nullptr, nl);
-  const int lrev = (partial && dir != 0) ? ltan + 1 : nl;
+  const int lrev = stops_early(partial, dir) ? ltan + 1 : nl;
   const Z r = lane_adjoint<Z>(s_layer, s_if, nl, om, free_surface, st, sink, [&](const Z& Rv) {
     if (!valid) return lift<Z>(0.0);
     if (MISFIT) {
@@ -303,7 +314,7 @@ __global__ void __launch_bounds__(AD_BLOCK_MAX) adjoint_kernel(
     Z g = lift_c<Z>(gin);
     if constexpr (!std::is_same<Z, cd>::value) g.d = dgin;
     return g;
-  }, ltan, lrev);
+  }, ltan, lrev, kstride);
   if (wave_ok && lane == 0) {
     const Z zero = lift<Z>(0.0);
     sink.cly[nl - 1] = LayerCot<Z>{zero, zero, zero};
@@ -374,7 +385,7 @@ __global__ void __launch_bounds__(CH_BLOCK) chain_kernel(const double* __restric
   // with `partial`, the reverse sweep stopped one interface below the
   // deepest moved layer (direction 0 excepted): only the layers above it
   // have whole cotangents
-  const int lmax = (partial && dir != 0) ? deepest_moved_layer(dl, nl) + 1 : nl - 1;
+  const int lmax = stops_early(partial, dir) ? deepest_moved_layer(dl, nl) + 1 : nl - 1;
   for (int k = threadIdx.x; k < nl * NCOL; k += blockDim.x) {
     const int l = k / NCOL, col = k - (k / NCOL) * NCOL;
     const size_t g = ((size_t)ip * nl + l) * NCOL + col;
@@ -426,7 +437,7 @@ __device__ __forceinline__ Reach reach_of(const double* __restrict__ dl, int nl,
   const int deep = deepest_moved_layer(dl, nl);
   const int shallow = shallowest_moved_layer(dl, nl);
   Reach r;
-  r.lmax = (partial && dir != 0) ? deep + 1 : nl - 1;
+  r.lmax = stops_early(partial, dir) ? deep + 1 : nl - 1;
   r.lo = shallow - 1 > 0 ? shallow - 1 : 0;
   const int top = r.lmax < nl - 1 ? r.lmax : nl - 1;
   r.hi = deep + 1 < top ? deep + 1 : top;
@@ -569,21 +580,37 @@ struct DerivLayout {
   int ntiles;  // frequency tiles of AD_WAVE lanes
   int wpb;     // waves (tiles) per workgroup
   int ntp;     // tiles with storage: ntiles rounded up to a multiple of wpb
+  int kstride;  // levels between checkpoints: 1 (whole tape) or ckpt_sqrt(nl)
 };
 
+// K = 1: the whole forward tape; K = ckpt_sqrt(nl): sqrt(N) checkpoints.
 template <class Z>
-static DerivLayout deriv_layout(int nl, int np, int nw, int ndir) {
+static DerivLayout deriv_layout(int nl, int np, int nw, int ndir, int K) {
   DerivLayout d;
+  d.kstride = K;
   d.ntiles = (nw + AD_WAVE - 1) / AD_WAVE;
   d.wpb = d.ntiles >= 2 ? AD_WAVES_MAX : 1;
   d.ntp = (d.ntiles + d.wpb - 1) / d.wpb * d.wpb;
   const long nblocks = (long)ndir * np * d.ntp;  // (dir, p, tile) slots
   d.nlanes = nblocks * AD_WAVE;
   d.ws = 0;
-  d.cif = align256(d.ws + (size_t)d.nlanes * lane_slots(nl) * sizeof(Z));
+  d.cif = align256(d.ws + (size_t)d.nlanes * lane_slots_k(nl, K) * sizeof(Z));
   d.cly = align256(d.cif + (size_t)nblocks * (nl - 1) * sizeof(IfaceZ<Z>));
   d.chi = align256(d.cly + (size_t)nblocks * nl * sizeof(LayerCot<Z>));
   d.total = align256(d.chi + (size_t)nblocks * sizeof(Z));
+  return d;
+}
+
+// The layout a call's workspace holds: the whole tape when it fits, else
+// sqrt(N) checkpoints (ok = false when neither fits).
+template <class Z>
+static DerivLayout layout_for(int nl, int np, int nw, int ndir, size_t ws_bytes, bool& ok) {
+  DerivLayout d = deriv_layout<Z>(nl, np, nw, ndir, 1);
+  ok = ws_bytes >= d.total;
+  if (!ok) {
+    d = deriv_layout<Z>(nl, np, nw, ndir, ckpt_sqrt(nl));
+    ok = ws_bytes >= d.total;
+  }
   return d;
 }
 
@@ -603,9 +630,13 @@ static int run_adjoint(const char* what, const double* layers, const double* dla
                        double* grad,
                        double* hvp, double* R, double* dR, void* workspace, size_t ws_bytes,
                        hipStream_t st, int partial = 0) {
-  const DerivLayout d = deriv_layout<Z>(nl, np, nw, ndir);
-  KH_REQUIRE(workspace && ws_bytes >= d.total, KH_ERR_INVALID_ARGUMENT,
-             "%s: workspace of %zu bytes needed, %zu given", what, d.total, ws_bytes);
+  // direction 0 keeps its whole sweep only when the call asks for the gradient
+  if (partial) partial = grad != nullptr ? PARTIAL_REST : PARTIAL_ALL;
+  bool fits = false;
+  const DerivLayout d = layout_for<Z>(nl, np, nw, ndir, ws_bytes, fits);
+  KH_REQUIRE(workspace && fits, KH_ERR_INVALID_ARGUMENT,
+             "%s: workspace of %zu bytes (sqrt(N) checkpoints) or %zu (whole tape) needed, "
+             "%zu given", what, d.total, deriv_layout<Z>(nl, np, nw, ndir, 1).total, ws_bytes);
   char* wsb = reinterpret_cast<char*>(workspace);
   const long nblocks = (long)ndir * np * (d.ntp / d.wpb);
   KH_REQUIRE(nblocks <= 0x7fffffffL, KH_ERR_UNSUPPORTED, "%s: grid too large", what);
@@ -628,7 +659,7 @@ static int run_adjoint(const char* what, const double* layers, const double* dla
                      reinterpret_cast<const cd*>(Gin), reinterpret_cast<const cd*>(dGin), second_only,
                      reinterpret_cast<Z*>(wsb + d.ws), d.nlanes, cif, cly, chip,
                      reinterpret_cast<cd*>(R), reinterpret_cast<cd*>(dR), d.ntiles, d.ntp,
-                     mode, partial);
+                     mode, partial, d.kstride);
   KH_TRY(check_launch(what));
   const size_t lds_c = (size_t)(nl - 1) * sizeof(IfaceZ<Z>) + (size_t)nl * sizeof(LayerCot<Z>);
   if constexpr (std::is_same<Z, dcd>::value) {
@@ -659,7 +690,17 @@ using namespace kh;
 
 extern "C" size_t kh_deriv_workspace_bytes(int nl, int np, int nw, int ndir, int dual) {
   if (nl < 2 || np <= 0 || nw <= 0 || ndir <= 0) return 0;
-  return dual ? deriv_layout<dcd>(nl, np, nw, ndir).total : deriv_layout<cd>(nl, np, nw, ndir).total;
+  return dual ? deriv_layout<dcd>(nl, np, nw, ndir, 1).total
+              : deriv_layout<cd>(nl, np, nw, ndir, 1).total;
+}
+
+extern "C" size_t kh_deriv_workspace_bytes_min(int nl, int np, int nw, int ndir, int dual) {
+  if (nl < 2 || np <= 0 || nw <= 0 || ndir <= 0) return 0;
+  const int K = ckpt_sqrt(nl);
+  const size_t sq = dual ? deriv_layout<dcd>(nl, np, nw, ndir, K).total
+                         : deriv_layout<cd>(nl, np, nw, ndir, K).total;
+  const size_t tape = kh_deriv_workspace_bytes(nl, np, nw, ndir, dual);
+  return sq < tape ? sq : tape;  // very shallow models: the tape is the smaller
 }
 
 extern "C" int kh_jvp(const double* layers, const double* dlayers, int ndir, int nl,

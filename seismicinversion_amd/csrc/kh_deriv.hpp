@@ -3,8 +3,8 @@
 //
 // lane_adjoint<Z>: one (p, omega) lane.
 //   1. forward sweep from the deepest interface, saving the state S_j every
-//      K levels (j % K == 0) in per-lane storage: K = 1 (the whole tape) for
-//      models of up to TAPE_LAYERS layers, else K = ceil(sqrt(nl-1));
+//      K levels (j % K == 0) in per-lane storage: K = 1 (the whole tape)
+//      where the workspace holds it, else K = ceil(sqrt(nl-1));
 //   2. R at the surface and its cotangent G (given, or 2 (R - R_obs) for the
 //      misfit |R - R_obs|^2);
 //   3. reverse sweep top-down: with the whole tape each level reads its
@@ -38,29 +38,20 @@ struct LayerCot {
   Z eta, neta, h;
 };
 
-// Models of up to this many layers keep the whole forward tape (every
-// level's state, K = 1): the reverse sweep then recomputes nothing and
-// reads one state per level.  Deeper models keep sqrt(N) checkpoints.
-constexpr int TAPE_LAYERS = 40;
-
-// Levels between checkpoints: 1 up to TAPE_LAYERS layers, else
-// ceil(sqrt(nl - 1)).
+// Levels between checkpoints: K = 1 keeps the whole forward tape (every
+// level's state; the reverse sweep recomputes nothing and reads one state
+// per level), K = ckpt_sqrt(nl) = ceil(sqrt(nl - 1)) keeps sqrt(N)
+// checkpoints and recomputes each segment of K levels once.
 __host__ __device__ __forceinline__ int ckpt_sqrt(int nl) {
   int k = 1;
   while (k * k < nl - 1) ++k;
   return k;
-}
-__host__ __device__ __forceinline__ int ckpt_stride(int nl) {
-  return nl <= TAPE_LAYERS ? 1 : ckpt_sqrt(nl);
 }
 __host__ __device__ __forceinline__ int ckpt_count(int nl, int K) { return (nl - 2) / K + 1; }
 // Z-slots of per-lane storage at stride K: the checkpoints (4 each), then,
 // for K > 1, the K-level buffer of the segment being walked back (6 each).
 __host__ __device__ __forceinline__ int lane_slots_k(int nl, int K) {
   return ckpt_count(nl, K) * 4 + (K > 1 ? K * 6 : 0);
-}
-__host__ __device__ __forceinline__ int lane_slots(int nl) {
-  return lane_slots_k(nl, ckpt_stride(nl));
 }
 
 template <class Z>
@@ -312,8 +303,8 @@ __host__ __device__ inline Z lane_adjoint_k(const LZ* __restrict__ L,
   return R;
 }
 
-// One storage for both: the checkpoints, then the buffer, K = ckpt_stride(nl)
-// (lane_slots(nl) slots).
+// One storage for both: the checkpoints, then the buffer (lane_slots_k(nl, K)
+// slots).
 template <class Z, class Store>
 struct BufAfter {
   Store& st;
@@ -325,8 +316,7 @@ template <class Z, class LZ, class Store, class Sink, class GFn>
 __host__ __device__ inline Z lane_adjoint(const LZ* __restrict__ L,
                                           const IfaceZ<Z>* __restrict__ F, int nl, cd om,
                                           int free_surface, Store& st, Sink& sink, GFn&& G_of,
-                                          int ltan = 1 << 30, int lrev = 1 << 30) {
-  const int K = ckpt_stride(nl);
+                                          int ltan = 1 << 30, int lrev = 1 << 30, int K = 1) {
   BufAfter<Z, Store> bst{st, ckpt_count(nl, K) * 4};
   return lane_adjoint_k<Z>(L, F, nl, om, free_surface, st, bst, K, sink, G_of, ltan, lrev);
 }

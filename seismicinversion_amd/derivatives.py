@@ -36,8 +36,25 @@ def _c128(x, dev):
     return x.detach().to(device=dev, dtype=torch.complex128).contiguous()
 
 
-def _workspace(nl, n_p, nw, ndir, dual, dev):
-    nbytes = int(_abi.lib().kh_deriv_workspace_bytes(nl, n_p, nw, ndir, int(dual)))
+def _ws_bytes(nl, n_p, nw, ndir, dual, tape):
+    """Workspace bytes keeping the whole forward tape, or (tape False) the
+    smallest workspace, sqrt(N) checkpoints."""
+    lib = _abi.lib()
+    fn = lib.kh_deriv_workspace_bytes if tape else lib.kh_deriv_workspace_bytes_min
+    return int(fn(nl, n_p, nw, ndir, int(dual)))
+
+
+def _tape_fits(nl, n_p, nw, dual):
+    """Whether one direction's whole-tape workspace stays under WORKSPACE_LIMIT
+    (otherwise the calls run on sqrt(N) checkpoints; the numbers agree to
+    rounding)."""
+    return _ws_bytes(nl, n_p, nw, 1, dual, True) <= WORKSPACE_LIMIT
+
+
+def _workspace(nl, n_p, nw, ndir, dual, dev, tape=None):
+    if tape is None:
+        tape = _tape_fits(nl, n_p, nw, dual)
+    nbytes = _ws_bytes(nl, n_p, nw, ndir, dual, tape)
     return torch.empty(max(nbytes, 1), dtype=torch.uint8, device=dev)
 
 
@@ -67,12 +84,16 @@ def _gate():
 
 
 def _dir_chunks(nl, n_p, nw, ndir):
-    one = int(_abi.lib().kh_deriv_workspace_bytes(nl, n_p, nw, 1, 1)) or 1
+    """Direction chunks whose workspace stays under WORKSPACE_LIMIT (and, when
+    paced, whose launches stay under the pacing cap), and whether they keep
+    the whole tape."""
+    tape = _tape_fits(nl, n_p, nw, True)
+    one = _ws_bytes(nl, n_p, nw, 1, True, tape) or 1
     step = max(1, min(ndir, WORKSPACE_LIMIT // one))
     cfg = getattr(_pacing, "cfg", None)
     if cfg is not None:
         step = max(1, min(step, int(cfg[0] // max(1, n_p * nw * (nl - 1)))))
-    return [(s, min(ndir, s + step)) for s in range(0, ndir, step)]
+    return [(s, min(ndir, s + step)) for s in range(0, ndir, step)], tape
 
 
 def jvp(layers, dlayers, p, omega, free_surface=False):
@@ -135,8 +156,8 @@ def misfit_hvp(layers, dlayers, p, omega, R_obs, free_surface=False, second_only
     Returns (chi (np,), grad (np, nl, 6), hvp (ndir, np, nl, 6), dR (ndir, np, nw) or None).
     With ``partial`` (kh_misfit_hvp_partial) hvp[d] holds only the entries of
     the layers down to one below the deepest layer direction d moves (the
-    rest are 0); ``symmetric_from_partial`` assembles the Hessian of a
-    parameter basis from such rows."""
+    rest are 0), except hvp[0], which is complete; ``symmetric_from_partial``
+    assembles the Hessian of a parameter basis from such rows."""
     dev = layers.device
     p = _f64(p, dev)
     omega = _c128(omega, dev)
@@ -148,9 +169,10 @@ def misfit_hvp(layers, dlayers, p, omega, R_obs, free_surface=False, second_only
     hvp = torch.empty((ndir, n_p, nl, 6), dtype=torch.float64, device=dev)
     dR = torch.empty((ndir, n_p, nw), dtype=torch.complex128, device=dev) if want_dR else None
     chi = None
-    for s, e in _dir_chunks(nl, n_p, nw, ndir):
+    chunks, tape = _dir_chunks(nl, n_p, nw, ndir)
+    for s, e in chunks:
         nd = e - s
-        ws = _workspace(nl, n_p, nw, nd, True, dev)
+        ws = _workspace(nl, n_p, nw, nd, True, dev, tape)
         chi_c = torch.empty((nd, n_p, 2), dtype=torch.float64, device=dev)
         _gate()
         fn = _abi.lib().kh_misfit_hvp_partial if partial else _abi.lib().kh_misfit_hvp
@@ -190,9 +212,10 @@ def vjp_hvp(layers, dlayers, p, omega, G, dG=None, free_surface=False):
         dG = _c128(dG, dev).reshape(ndir, n_p, nw)
     grad = torch.empty((n_p, nl, 6), dtype=torch.float64, device=dev)
     dgrad = torch.empty((ndir, n_p, nl, 6), dtype=torch.float64, device=dev)
-    for s, e in _dir_chunks(nl, n_p, nw, ndir):
+    chunks, tape = _dir_chunks(nl, n_p, nw, ndir)
+    for s, e in chunks:
         nd = e - s
-        ws = _workspace(nl, n_p, nw, nd, True, dev)
+        ws = _workspace(nl, n_p, nw, nd, True, dev, tape)
         _gate()
         rc = _abi.lib().kh_vjp_hvp(
             _abi.ptr(layers), _abi.ptr(dlayers[s:e]), nd, nl, _abi.ptr(p), n_p, _abi.ptr(omega),

@@ -64,7 +64,7 @@ extern "C" int host_vjp(const double* layers, int nl, double p, const double* om
   std::vector<IfaceZ<cd>> F(nl - 1);
   stage<cd>(layers, nullptr, nl, p, L, F);
   Sink<cd> sink(nl);
-  Store<cd> st(lane_slots(nl));
+  Store<cd> st(lane_slots_k(nl, 1));
   for (int w = 0; w < nw; ++w) {
     const cd om{omega[2 * w], omega[2 * w + 1]};
     const cd g{G[2 * w], G[2 * w + 1]};
@@ -94,7 +94,7 @@ extern "C" int host_misfit_hvp(const double* layers, const double* dlayers, int 
   std::vector<IfaceZ<dcd>> F(nl - 1);
   stage<dcd>(layers, dlayers, nl, p, L, F);
   Sink<dcd> sink(nl);
-  Store<dcd> st(lane_slots(nl));
+  Store<dcd> st(lane_slots_k(nl, 1));
   dcd chis = lift<dcd>(0.0);
   for (int w = 0; w < nw; ++w) {
     const cd om{omega[2 * w], omega[2 * w + 1]};
@@ -144,12 +144,12 @@ extern "C" int host_misfit_hvp_mode(const double* layers, const double* dlayers,
   const int lrev = mode >= 2 ? ltan + 1 : (1 << 30);
   // mode 3 is mode 2 with checkpoints every 2 levels and the buffer in a
   // store of its own; mode 4 is mode 2 with sqrt(N) checkpoints and the
-  // buffer after them (the layout of models deeper than TAPE_LAYERS)
+  // buffer after them (the layout of a workspace too small for the tape)
   std::vector<LayerZ<dcd>> L(nl);
   std::vector<IfaceZ<dcd>> F(nl - 1);
   stage<dcd>(layers, dlayers, nl, p, L, F);
   Sink<dcd> sink(nl);
-  Store<dcd> st(lane_slots(nl) + lane_slots_k(nl, ckpt_sqrt(nl)) + 64);
+  Store<dcd> st(lane_slots_k(nl, 1) + lane_slots_k(nl, ckpt_sqrt(nl)) + 64);
   Store<dcd> bst(2 * 6);  // mode 3: two-level segments, buffer apart (the LDS layout)
   dcd chis = lift<dcd>(0.0);
   for (int w = 0; w < nw; ++w) {

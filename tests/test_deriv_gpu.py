@@ -277,3 +277,42 @@ def test_partial_rows_equal_full_rows_for_general_directions():
         else:
             assert np.all(hvp_p[d, :, :top + 1] == 0.0)
         assert np.all(hvp_p[d, :, top + 1:] == 0.0), d
+
+
+@pytest.mark.parametrize("nl", [12, 61])
+def test_whole_tape_and_sqrt_checkpoints_give_the_same_numbers(monkeypatch, nl):
+    """A workspace that holds the whole forward tape and the smallest one
+    (sqrt(N) checkpoints, recomputing each segment) give the same chi,
+    gradient, Hessian-vector products and dR to rounding (a recomputed state
+    may round differently from the stored one: the two are compiled at
+    different sites), for the misfit HVP (partial and full) and the
+    first-order misfit gradient; the partial rows are zero beyond the same
+    layers in both."""
+    model = seeded_model(nl, 30 + nl)
+    tab = pack_layers(model)
+    rng = np.random.default_rng(nl)
+    dm = rng.standard_normal((3, nl, 6)) * np.array([0.1, 0.1, 0.1, 0.1, 5.0, 5.0])
+    dm[1, nl // 2 + 1:] = 0.0
+    dm[:, ~np.isfinite(tab[:, 3]), 3] = 0.0
+    layers = engine.layers_tensor(model)
+    ps = torch.tensor([0.1, 0.4], dtype=torch.float64)
+    w = torch.from_numpy((np.arange(1, 97) * 2 * np.pi / 64.0 + 1j * np.pi / 64).astype(complex))
+    R_obs = engine.forward(layers, ps.to(layers.device), w.to(layers.device)) * (1 + 0.02j)
+    d_t = torch.from_numpy(dm)
+
+    def run():
+        out = [D.misfit_hvp(layers, d_t, ps, w, R_obs, free_surface=True, want_dR=True,
+                            partial=part) for part in (False, True)]
+        out.append(D.misfit_grad(layers, ps, w, R_obs, free_surface=True))
+        return [x.cpu().numpy() for o in out for x in o]
+
+    assert D._tape_fits(nl, 2, 96, True)
+    tape = run()
+    monkeypatch.setattr(D, "WORKSPACE_LIMIT", 1)  # no whole tape fits: sqrt(N) checkpoints
+    assert not D._tape_fits(nl, 2, 96, True)
+    sqrt_n = run()
+    for a, b in zip(tape, sqrt_n):
+        assert a.shape == b.shape
+        assert np.array_equal(a == 0, b == 0)
+        if np.any(b != 0):
+            assert rel(a, b) <= 1e-11
