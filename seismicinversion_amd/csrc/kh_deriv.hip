@@ -72,14 +72,17 @@ constexpr int red_rows(int mode) {
   return (mode == RED_LEVEL ? 13 : mode == RED_RECORD ? 10 : 5) * Flat<Z>::N;
 }
 
-// Workgroup barrier that orders LDS only: the reductions exchange data
-// through LDS alone, so the wave's outstanding global loads and stores
-// (checkpoints, the segment buffer, cotangent records) stay in flight
-// across it instead of being drained at every level.
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+// Orders one wave's LDS accesses around a reduction pass.  The buffer is the
+// wave's own and the LDS executes a wave's instructions in order, so its
+// lanes' writes are seen by the reads that follow and the next pass's
+// writes come after this pass's reads, without a workgroup barrier: the
+// wavefront-scope fences only keep the compiler from moving LDS accesses
+// across the point (no instruction, no wait on the wave's outstanding
+// global loads and stores, no waiting for the other wave of the workgroup).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
 }
 
 // Per-wave sums of the per-lane cotangent records, written per tile.  Each
@@ -93,11 +96,10 @@ struct WaveSink {
   LayerCot<Z>* cly;
   double* red;  // LDS, red_rows<Z>(mode) x RED_STRIDE, this wave's own
   int mode;
-  bool store;  // false for a padding wave (no tile): it only keeps the barriers
+  bool store;  // false for a wave with no tile
 
   // one pass over scalars [F0, F1) of the record v; the sums of its first n0
-  // doubles go to d0, the rest to d1 (every wave of the workgroup reduces
-  // its own record at the same level: the barriers are shared)
+  // doubles go to d0, the rest to d1
   template <int F0, int F1, int NF>
   __device__ void reduce_pass(const Z (&v)[NF], double* __restrict__ d0, int n0,
                               double* __restrict__ d1) {
@@ -108,7 +110,7 @@ struct WaveSink {
 #pragma unroll
       for (int k = 0; k < N; ++k) red[((f - F0) * N + k) * RED_STRIDE + lane] = Flat<Z>::get(v[f], k);
     }
-    lds_barrier();
+    wave_lds_sync();
     if (store && lane < (F1 - F0) * N) {
       const double* row = red + lane * RED_STRIDE;
       double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
@@ -122,7 +124,7 @@ struct WaveSink {
       const int e = F0 * N + lane;
       *(e < n0 ? d0 + e : d1 + (e - n0)) = (s0 + s1) + (s2 + s3);
     }
-    lds_barrier();  // the buffer is rewritten by the next pass
+    wave_lds_sync();  // the buffer is rewritten by the next pass
   }
   __device__ void iface(int i, const IfaceZ<Z>& f) {
     const Z v[10] = {f.rd00, f.rd01, f.rd11, f.ru00, f.ru01,
@@ -253,12 +255,10 @@ __device__ __forceinline__ void stage_ph(const double* __restrict__ layers,
   __syncthreads();
 }
 
-// grid.x = ndir * np * ntiles, blockDim = 64 (one wave per block).
 // grid.x = ndir * np * ngroups, blockDim = 64 * wpb: wave k of group g
 // sweeps frequency tile g * wpb + k of (dir, p), all waves sharing one staged
 // table.  ntp = ngroups * wpb tiles have workspace and cotangent slots; a
-// wave whose tile is >= ntiles (padding) runs with every lane invalid and
-// stores nothing.
+// wave whose tile is >= ntiles (padding) helps stage the table and returns.
 template <class Z, bool MISFIT>
 __global__ void __launch_bounds__(AD_BLOCK_MAX) adjoint_kernel(
     const double* __restrict__ layers, const double* __restrict__ dlayers, int nl,
@@ -285,6 +285,7 @@ __global__ void __launch_bounds__(AD_BLOCK_MAX) adjoint_kernel(
               s_if);
   const int tile = grp * wpb + wave;
   const bool wave_ok = tile < ntiles;
+  if (!wave_ok) return;  // no workgroup barrier follows the staging
   const int w = tile * AD_WAVE + lane;
   const bool valid = wave_ok && w < nw;
   const int wc = valid ? w : nw - 1;
