@@ -518,7 +518,8 @@ class DeviceBackend:
 class KennettServer:
     def __init__(self, backend=None, hbm_budget: int | None = None, max_batch_updates: float = 1e9,
                  max_lm_concurrency: int = 1, idle_sleep: float = 2e-4,
-                 max_inflight: int | None = None, max_batch_bytes: int | None = None):
+                 max_inflight: int | None = None, max_batch_bytes: int | None = None,
+                 first_batch_updates: float = 2.5e8):
         self.backend = (backend if backend is not None
                         else DeviceBackend(lm_streams=max(1, max_lm_concurrency)))
         # batches in flight at once (one per stream): further arrivals wait for
@@ -529,6 +530,12 @@ class KennettServer:
         self.budget = int(hbm_budget)
         self.in_use = 0
         self.max_batch_updates = max_batch_updates
+        # with nothing in flight the device is idle: the first batch of a
+        # burst is cut short so that its launch does not wait for a whole
+        # batch to be validated (the next batches fill up while it runs)
+        self.first_batch_updates = min(first_batch_updates, max_batch_updates)
+        # set by arrivals, completions and releases: the scheduler sleeps on it
+        self._wake = threading.Event()
         self.max_batch_bytes = (max_batch_bytes if max_batch_bytes is not None
                                 else getattr(self.backend, "max_batch_bytes", None))
         self.incoming: queue.Queue = queue.Queue()
@@ -596,6 +603,7 @@ class KennettServer:
         for r in reqs:
             r.record["counted"] = True
         self.incoming.put(reqs if len(reqs) > 1 else reqs[0])
+        self._wake.set()
 
     def _submit(self, kind, spec, deadline_s, on_device=False) -> Future:
         req = self._make(kind, spec, deadline_s, on_device)
@@ -626,6 +634,7 @@ class KennettServer:
                         break
                 time.sleep(1e-3)
         self._stop.set()
+        self._wake.set()
         if self._thread is not None:
             self._thread.join()
         if self._completer is not None:
@@ -798,13 +807,17 @@ class KennettServer:
     def _release(self, nbytes: int) -> None:
         with self.lock:
             self.in_use -= nbytes
+        self._wake.set()
 
     def _loop(self):
         pending_fwd: list[Request] = []
         while not self._stop.is_set():
             progressed = False
             queued = sum(r.record["updates"] for r in pending_fwd)
-            for req in self._admit(self.max_batch_updates - queued):
+            room = self.max_batch_updates - queued
+            if self.n_inflight == 0 and not pending_fwd:
+                room = self.first_batch_updates
+            for req in self._admit(room):
                 progressed = True
                 if req.kind == "forward":
                     pending_fwd.append(req)
@@ -813,12 +826,14 @@ class KennettServer:
                     self.lm_threads.append(t)
                     t.start()
             if pending_fwd and self.n_inflight < self.max_inflight:
+                first = self.n_inflight == 0
+                limit = self.first_batch_updates if first else self.max_batch_updates
                 batch, total, nbytes = [], 0.0, 0
                 cap = self.max_batch_bytes
                 while pending_fwd:
                     nxt = pending_fwd[0].spec
                     b = 0 if pending_fwd[0].on_device else 16 * len(nxt.p) * nxt.nw
-                    if batch and (total + nxt.updates() > self.max_batch_updates or
+                    if batch and (total + nxt.updates() > limit or
                                   (cap is not None and nbytes + b > cap)):
                         break
                     r = pending_fwd.pop(0)
@@ -847,7 +862,11 @@ class KennettServer:
                 progressed = True
             self.lm_threads = [t for t in self.lm_threads if t.is_alive()]
             if not progressed:
-                time.sleep(self.idle_sleep)
+                # an arrival, completion or release wakes the loop at once;
+                # a wake-up that comes between the wait and the clear is not
+                # lost: the next pass runs right after the clear
+                self._wake.wait(self.idle_sleep)
+                self._wake.clear()
 
     def _complete_loop(self):
         """Completion thread: waits for each launched batch in launch order,
@@ -882,6 +901,7 @@ class KennettServer:
                         self._finish(r, error=e)
             with self.lock:
                 self.n_inflight -= 1
+            self._wake.set()
 
     def _run_invert(self, req: Request):
         req.record["launch"] = time.perf_counter()
@@ -895,3 +915,4 @@ class KennettServer:
         with self.lock:
             self.lm_running -= 1
         self._finish(req, result=res, error=err)
+        self._wake.set()
