@@ -30,7 +30,8 @@ Beside it, on the same requests:
   inverse FFT, 100 offsets, nt 2048, and the reference CLI's -n 2048 shape,
   nt 4096), the 256-model uneven batch (persistent launch vs one launch per
   model) and the 20-layer Newton-LM inversion (75 parameters, 128 p x 512
-  omega, 10 iterations, eager and from HIP graphs).
+  omega, 10 iterations: the device loop eager and from a captured HIP graph,
+  and the host loop).
 
 ``roofline``: the persistent multi-model kernel of the served steps, FP64
 algorithmic FLOPs (340 per update) per launch over the average launch
@@ -381,29 +382,32 @@ def kernel_figures(torch, dev):
 
 def lm_figures(torch, dev):
     """20-layer Newton-LM: P = 75 packed parameters, 128 p in (0, 0.6] x 512
-    omega, start model perturbed 5 %, 10 iterations, eager and with HIP graphs
-    (each computation runs eagerly on its first call, is captured on its
-    second and replayed from then on); one misfit/gradient/exact-Hessian
-    evaluation timed on its own, eager and replayed.  Beside it the served
-    inversion size (17 layers, 32 p x 128 omega, 5 iterations), where the
-    kernels are short and launch overhead counts, eager and with graphs."""
+    omega, start model perturbed 5 %, 10 iterations, three ways: the device
+    loop eager (the default: the whole iteration, trials and decision
+    included, enqueued with no host round trip), the device loop replayed
+    from one captured HIP graph per iteration, and the host loop that solves
+    and decides on the host one trial at a time as the reference does.  One
+    misfit/gradient/exact-Hessian evaluation is timed on its own.  Beside it
+    the served inversion size (17 layers, 32 p x 128 omega, 5 iterations),
+    where the kernels are short and launch overhead counts."""
     from seismicinversion_amd.taup_inversion import _Problem, invert_taup
     from seismicinversion_amd.workloads import seeded_model, uniform_slownesses
 
+    variants = {"eager": {}, "graphs": {"use_graphs": True}, "host_loop": {"device_loop": False}}
     m = seeded_model(20, 42)
     p = list(uniform_slownesses(128, 0.6))
-    invert_taup(m, p[:4], nfreq=16, max_iter=1, tol=0.0)  # warm-up
+    for kw in variants.values():  # warm-up
+        invert_taup(m, p[:4], nfreq=16, max_iter=3, tol=0.0, **kw)
     runs = {}
-    for graphs in (False, True, False, True):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        res = invert_taup(m, p, nfreq=512, perturbation=0.05, max_iter=10, tol=0.0,
-                          use_graphs=graphs)
-        torch.cuda.synchronize()
-        runs.setdefault(graphs, []).append((time.perf_counter() - t0, res))
-    t_e = min(t for t, _ in runs[False])
-    t_g = min(t for t, _ in runs[True])
-    res, res_g = runs[False][0][1], runs[True][0][1]
+    for _ in range(2):
+        for name, kw in variants.items():
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            res = invert_taup(m, p, nfreq=512, perturbation=0.05, max_iter=10, tol=0.0, **kw)
+            torch.cuda.synchronize()
+            runs.setdefault(name, []).append((time.perf_counter() - t0, res))
+    best = {k: min(t for t, _ in v) for k, v in runs.items()}
+    res = {k: v[0][1] for k, v in runs.items()}
     refs, p_, w, obs, phys = lm_problem(torch, dev)
     prob = _Problem(*refs, p_, w, obs)
     prob.grad_hessian(phys)
@@ -412,32 +416,40 @@ def lm_figures(torch, dev):
     prob_g.grad_hessian(phys)  # eager
     prob_g.grad_hessian(phys)  # captured
     t_gh_graph = timed(lambda: prob_g.grad_hessian(phys), 3, torch)
-    # the served inversion size: 5 requests of 5 iterations, eager and graphs
+    # the served inversion size: 5 requests of 5 iterations
     small = [(seeded_model(17, 100 + k), list(uniform_slownesses(32, 0.8))) for k in range(5)]
     t_small = {}
-    for graphs in (False, True, False, True):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        its = 0
-        for mm, pp in small:
-            r = invert_taup(mm, pp, nfreq=128, perturbation=0.05, max_iter=5, tol=0.0,
-                            use_graphs=graphs)
-            its += r.n_iterations
-        torch.cuda.synchronize()
-        t_small.setdefault(graphs, []).append((time.perf_counter() - t0) * 1e3 / its)
-    return {"P": prob.P, "n_p": 128, "n_omega": 512, "iterations": res.n_iterations,
-            "eager_s": t_e, "graphs_s": t_g,
-            "eager_ms_per_iteration": t_e * 1e3 / res.n_iterations,
-            "graphs_ms_per_iteration": t_g * 1e3 / res_g.n_iterations,
-            "same_trajectory": res_g.misfit_history == res.misfit_history,
-            "misfit_first_last": [res.misfit_history[0], res.misfit_history[-1]],
-            "param_error_last": res.param_error_history[-1],
+    for _ in range(2):
+        for name, kw in variants.items():
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            its = 0
+            for mm, pp in small:
+                r = invert_taup(mm, pp, nfreq=128, perturbation=0.05, max_iter=5, tol=0.0, **kw)
+                its += r.n_iterations
+            torch.cuda.synchronize()
+            t_small.setdefault(name, []).append((time.perf_counter() - t0) * 1e3 / its)
+    n_it = res["eager"].n_iterations
+    return {"P": prob.P, "n_p": 128, "n_omega": 512, "iterations": n_it,
+            "eager_s": best["eager"], "graphs_s": best["graphs"],
+            "host_loop_s": best["host_loop"],
+            "eager_ms_per_iteration": best["eager"] * 1e3 / n_it,
+            "graphs_ms_per_iteration": best["graphs"] * 1e3 / res["graphs"].n_iterations,
+            "host_loop_ms_per_iteration": best["host_loop"] * 1e3 / res["host_loop"].n_iterations,
+            "loops": "eager and graphs: the device loop (trials and decision on the device); "
+                     "host_loop: solve and decision on the host, one trial at a time",
+            "same_trajectory": res["graphs"].misfit_history == res["eager"].misfit_history,
+            "host_loop_max_rel_misfit_diff": float(np.max(np.abs(
+                np.array(res["host_loop"].misfit_history) - res["eager"].misfit_history)
+                / np.array(res["eager"].misfit_history))),
+            "misfit_first_last": [res["eager"].misfit_history[0],
+                                  res["eager"].misfit_history[-1]],
+            "param_error_last": res["eager"].param_error_history[-1],
             "grad_hessian_ms": t_gh * 1e3,
             "grad_hessian_graph_replay_ms": t_gh_graph * 1e3,
             "tangent_over_adjoint_lane_updates": prob.P * 128 * 512 * 19,
             "served_size_17_layers_32p_128w": {
-                "eager_ms_per_iteration": min(t_small[False]),
-                "graphs_ms_per_iteration": min(t_small[True])}}
+                f"{k}_ms_per_iteration": min(v) for k, v in t_small.items()}}
 
 
 def lm_problem(torch, dev):

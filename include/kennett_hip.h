@@ -363,6 +363,75 @@ int kh_gmm_forward_grid(const double* layers, int nl, const double* p, int np, d
 int kh_residual_norms(const double* R, const double* R_obs, int np, int nw, double* chi,
                       void* stream);
 
+/* ---- Device-resident Levenberg-Marquardt iteration ----
+ *
+ * The host loop of Kennett_Reflectivity/taup_inversion.py:389-439 as four
+ * kernels that follow the misfit / gradient / Hessian evaluation of the
+ * current iterate, so an iteration needs no host round trip (and can be
+ * captured as one HIP graph).  Packed log-parameters x (P = 4 nl - 5):
+ * m = exp(x) = [alpha_1..N, beta_1..N, rho_1..N, h_1..N-1], N = nl - 1
+ * (kennett_torch.py:552-608).
+ *
+ * Loop state: KH_LM_STATE_LEN float64 values in device memory,
+ *   [0] lambda (set it < 0 before the first iteration: kh_lm_prepare then
+ *       sets 0.01 max|diag H_log|, floor 1e-10, taup_inversion.py:406-407)
+ *   [1] misfit at the current iterate   [2] misfit at the previous iterate
+ *   [3] iterations evaluated            [4] status: 0 running, 1 converged
+ *       (gradient or relative-misfit test), 2 line search failed
+ *   [5] trial accepted in the last iteration (-1: none)   [6..7] unused.
+ * Every kernel returns at once when the status is not 0, so iterations
+ * enqueued after the loop has stopped change nothing. */
+#define KH_LM_STATE_LEN 8
+
+/* tables (nsets, nl, 6) of packed log-parameters x (nsets, P): columns 0-3
+ * of layers 1..nl-1 from exp(x) (the half-space keeps its thickness), the
+ * water layer and the Q columns from ref_layers; thicknesses >= 1e30 become
+ * +inf.  `state` may be NULL (always run).  Replaces np.exp and
+ * _unpack_params (taup_inversion.py:88-95, 147, 203-215). */
+int kh_lm_tables(const double* ref_layers, int nl, const double* x, int nsets, double* tables,
+                 const double* state, void* stream);
+
+/* One workgroup: chi = sum of chi_rows[p * chi_stride] over p ascending;
+ * g_log = m g, H_log = (m_i m_j) H_ij + delta_ij g_j m_j (P x P row-major);
+ * history[iter] = [chi, |g_log|, |m - x_true| / |x_true|] while
+ * iter < max_hist; first lambda; status 1 when |g_log| < tol or, after the
+ * first iteration, |chi_prev - chi| / max(|chi|, 1e-30) < tol.  Replaces
+ * taup_inversion.py:180-185 and 389-420. */
+int kh_lm_prepare(int nl, int np, const double* chi_rows, int chi_stride, const double* g,
+                  const double* H, const double* log_x, const double* x_true, double tol,
+                  int max_hist, double* state, double* g_log, double* H_log, double* history,
+                  void* stream);
+
+/* Workspace bytes of kh_lm_trials (0 when P <= 89: the systems fit in LDS). */
+size_t kh_lm_trials_workspace_bytes(int nl, int ntrials);
+
+/* Trials t = 0..ntrials-1 of the damping schedule at once, one workgroup
+ * each: (H_log + lambda 5^t I) dx = -g_log by LU with partial pivoting,
+ * trial_x[t] = log_x + dx (ntrials, P) and trial_tables[t] (ntrials, nl, 6)
+ * as kh_lm_tables.  A singular system yields non-finite entries (a trial
+ * that is never accepted) where np.linalg.solve raises.  Replaces
+ * newton_lm_step (taup_inversion.py:238-254) inside the trial loop
+ * (:425-435). */
+int kh_lm_trials(int nl, int ntrials, const double* H_log, const double* g_log,
+                 const double* log_x, const double* ref_layers, const double* state,
+                 double* trial_x, double* trial_tables, void* workspace, size_t workspace_bytes,
+                 void* stream);
+
+/* chi_trials (ntrials, np): per trial and slowness sum_w |R - R_obs|^2 of
+ * the trial table, equal bit for bit to kh_forward followed by
+ * kh_residual_norms.  R_obs (np, nw) complex128, omega (nw,) complex128.
+ * Replaces _eval_misfit_only (taup_inversion.py:190-230) for every trial. */
+int kh_lm_trial_misfits(const double* trial_tables, int nl, int ntrials, const double* p, int np,
+                        const double* omega, int nw, int free_surface, const double* R_obs,
+                        const double* state, double* chi_trials, void* stream);
+
+/* Accepts the first trial whose misfit (rows summed over p ascending) is
+ * below the current one: log_x = trial_x[t], lambda = max(lambda 5^t / 2,
+ * 1e-15); if none is, lambda 5^ntrials and status 2.  Replaces
+ * taup_inversion.py:425-439. */
+int kh_lm_decide(int nl, int ntrials, int np, const double* chi_trials, const double* trial_x,
+                 double* state, double* log_x, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

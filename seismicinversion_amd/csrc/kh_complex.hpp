@@ -28,6 +28,12 @@ __host__ __device__ __forceinline__ cd operator*(cd a, cd b) {
 __host__ __device__ __forceinline__ cd operator*(cd a, double s) { return cd{a.r * s, a.i * s}; }
 __host__ __device__ __forceinline__ cd operator*(double s, cd a) { return cd{a.r * s, a.i * s}; }
 __host__ __device__ __forceinline__ cd conj(cd a) { return cd{a.r, -a.i}; }
+// acc + |a - b|^2: the one association every misfit sum of the library uses,
+// so that sums over the same terms in the same order agree bit for bit.
+__host__ __device__ __forceinline__ double add_sq_diff(double acc, cd a, cd b) {
+  const double dr = a.r - b.r, di = a.i - b.i;
+  return acc + (dr * dr + di * di);
+}
 // c + a*b and c - a*b as two fused multiply-adds per component.
 __host__ __device__ __forceinline__ cd cfma(cd a, cd b, cd c) {
   return cd{fma(a.r, b.r, fma(-a.i, b.i, c.r)), fma(a.r, b.i, fma(a.i, b.r, c.i))};

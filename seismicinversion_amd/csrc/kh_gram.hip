@@ -89,10 +89,7 @@ __global__ void __launch_bounds__(RN_BLOCK) residual_norm_kernel(const cd* __res
   const cd* a = R + row * nw;
   const cd* b = R_obs + row * nw;
   double acc = 0.0;
-  for (int w = threadIdx.x; w < nw; w += RN_BLOCK) {
-    const double dr = a[w].r - b[w].r, di = a[w].i - b[w].i;
-    acc += dr * dr + di * di;
-  }
+  for (int w = threadIdx.x; w < nw; w += RN_BLOCK) acc = add_sq_diff(acc, a[w], b[w]);
   s[threadIdx.x] = acc;
   __syncthreads();
   for (int o = RN_BLOCK / 2; o > 0; o >>= 1) {

@@ -1,0 +1,407 @@
+// Device-resident Levenberg-Marquardt iteration of the tau-p inversion
+// (Kennett_Reflectivity/taup_inversion.py:389-439), so that an iteration
+// needs no host round trip and can be captured whole as one HIP graph.
+//
+// One iteration, after the misfit / gradient / exact Hessian evaluation of
+// the current iterate (the derivative entry points):
+//   lm_prepare_kernel   the log-parameter transform of g and H
+//                       (taup_inversion.py:180-185), the history row, the
+//                       first damping value and the two convergence tests
+//                       (:389-420);
+//   lm_trials_kernel    every trial of the damping schedule at once: trial t
+//                       uses lambda * 5^t (the schedule multiplies by 5 after
+//                       each rejection, :433), so the trials are independent
+//                       and one workgroup per trial solves (H + lambda_t I)
+//                       dx = -g by LU with partial pivoting (np.linalg.solve,
+//                       :238-254) and writes the trial's layer table;
+//   lm_trial_misfit_kernel  the misfit of every trial model
+//                       (_eval_misfit_only, :190-230), one workgroup per
+//                       (trial, slowness) row, with the arithmetic and the
+//                       summation order of kh_forward + kh_residual_norms;
+//   lm_decide_kernel    the first trial whose misfit is below the current
+//                       one is accepted (lambda halved, floor 1e-15),
+//                       otherwise the line search has failed (:425-439).
+// The sequential loop's result follows: it accepts the same first trial
+// with the same damping.  Every kernel reads the loop status first and does
+// nothing once the loop has stopped, so iterations launched past the end
+// change nothing.
+//
+// Loop state: KH_LM_STATE_LEN float64 values (see kennett_hip.h).
+#include "kh_physics.hpp"
+#include "kh_runtime.hpp"
+#include "kh_sweep.hpp"
+
+namespace kh {
+
+enum { LM_LAM = 0, LM_CHI, LM_CHI_PREV, LM_ITER, LM_STATUS, LM_TRIAL };
+
+constexpr int LM_MAX_P = 4 * KH_MAX_LAYERS_DERIV - 5;  // packed parameters at the layer cap
+constexpr int LM_BLOCK = 512;
+constexpr int LM_LDS_MAX_P = 89;                       // P x (P + 1) doubles within 64 KB of LDS
+
+static_assert(KH_LM_STATE_LEN >= LM_TRIAL + 1, "loop state layout");
+
+__host__ __device__ __forceinline__ int lm_params(int nl) { return 4 * nl - 5; }
+
+// Column c of layer l of the table of the packed physical parameters
+// m = exp(x) = [alpha_1..N, beta_1..N, rho_1..N, h_1..N-1], N = nl - 1
+// (kennett_torch.py:552-608): the water layer, the Q columns and the
+// half-space thickness come from the reference table; thicknesses of
+// 1e30 or more mean a half-space (+inf), as the host table builder maps them.
+__device__ __forceinline__ double lm_table_entry(const double* __restrict__ ref, int nl,
+                                                 const double* __restrict__ x, int l, int c) {
+  const int n = nl - 1;
+  double v = ref[l * NCOL + c];
+  if (l > 0 && c < COL_QA && !(c == COL_H && l == nl - 1)) v = exp(x[c * n + (l - 1)]);
+  if (c == COL_H && !(v < 1e30)) v = INFINITY;
+  return v;
+}
+
+__global__ void lm_tables_kernel(const double* __restrict__ ref, int nl,
+                                 const double* __restrict__ x, int nsets,
+                                 double* __restrict__ tables, const double* __restrict__ state) {
+  if (state != nullptr && state[LM_STATUS] != 0.0) return;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long per = (long)nl * NCOL;
+  if (idx >= per * nsets) return;
+  const int s = (int)(idx / per);
+  const int r = (int)(idx - (long)s * per);
+  tables[idx] = lm_table_entry(ref, nl, x + (size_t)s * lm_params(nl), r / NCOL, r % NCOL);
+}
+
+// One workgroup.  g_log = m g, H_log = (m_i m_j) H_ij + delta_ij g_j m_j;
+// chi summed over the slowness rows in ascending order; history row
+// [chi, |g_log|, |m - m_true| / |m_true|].
+__global__ void __launch_bounds__(LM_BLOCK) lm_prepare_kernel(
+    int P, int np, const double* __restrict__ chi_rows, int chi_stride,
+    const double* __restrict__ g, const double* __restrict__ H, const double* __restrict__ log_x,
+    const double* __restrict__ x_true, double tol, int max_hist, double* __restrict__ state,
+    double* __restrict__ g_log, double* __restrict__ H_log, double* __restrict__ hist) {
+  __shared__ double s_m[LM_MAX_P];
+  const bool run = state[LM_STATUS] == 0.0;
+  __syncthreads();  // every wave has read the status before thread 0 may change it
+  if (!run) return;
+  for (int j = threadIdx.x; j < P; j += blockDim.x) {
+    const double m = exp(log_x[j]);
+    s_m[j] = m;
+    g_log[j] = m * g[j];
+  }
+  __syncthreads();
+  for (long idx = threadIdx.x; idx < (long)P * P; idx += blockDim.x) {
+    const int i = (int)(idx / P), j = (int)(idx - (long)i * P);
+    double v = (s_m[i] * s_m[j]) * H[idx];
+    if (i == j) v += g[j] * s_m[j];
+    H_log[idx] = v;
+  }
+  if (threadIdx.x != 0) return;
+  double chi = 0.0;
+  for (int p = 0; p < np; ++p) chi += chi_rows[(size_t)p * chi_stride];
+  double gg = 0.0, ee = 0.0, tt = 0.0, dmax = 0.0;
+  for (int j = 0; j < P; ++j) {
+    const double m = s_m[j];
+    const double gl = m * g[j];
+    gg += gl * gl;
+    const double e = m - x_true[j];
+    ee += e * e;
+    tt += x_true[j] * x_true[j];
+    double d = (m * m) * H[(size_t)j * P + j];
+    d += g[j] * m;
+    dmax = fmax(dmax, fabs(d));
+  }
+  const double gnorm = sqrt(gg);
+  const double perr = sqrt(ee) / sqrt(tt);
+  const int it = (int)state[LM_ITER];
+  if (it < max_hist) {
+    hist[3 * it + 0] = chi;
+    hist[3 * it + 1] = gnorm;
+    hist[3 * it + 2] = perr;
+  }
+  if (state[LM_LAM] < 0.0) state[LM_LAM] = 0.01 * fmax(dmax, 1e-10);
+  double status = 0.0;
+  if (gnorm < tol) status = 1.0;
+  else if (it > 0 && fabs(state[LM_CHI_PREV] - chi) / fmax(fabs(chi), 1e-30) < tol) status = 1.0;
+  state[LM_CHI_PREV] = chi;
+  state[LM_CHI] = chi;
+  state[LM_ITER] = (double)(it + 1);
+  state[LM_STATUS] = status;
+}
+
+// One workgroup per trial t: A = [H_log + lambda_t I | -g_log] (row-major,
+// leading dimension P + 1) in LDS or in the caller's workspace, LU with
+// partial pivoting (the first row of largest magnitude in the column), the
+// right-hand side eliminated along, then back substitution.  Writes
+// trial_x[t] = log_x + dx and the trial's layer table.  A zero pivot gives
+// inf/NaN entries, whose trial misfit is NaN and is never accepted.
+template <bool SHARED>
+__global__ void __launch_bounds__(LM_BLOCK) lm_trials_kernel(
+    int nl, const double* __restrict__ H, const double* __restrict__ g,
+    const double* __restrict__ log_x, const double* __restrict__ ref,
+    const double* __restrict__ state, double* __restrict__ trial_x,
+    double* __restrict__ tables, double* __restrict__ ws) {
+  extern __shared__ double smem[];
+  __shared__ int s_piv;
+  __shared__ double s_xi;
+  if (state[LM_STATUS] != 0.0) return;  // this kernel never writes the state
+  const int P = lm_params(nl);
+  const int ld = P + 1;
+  const int t = blockIdx.x;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  double* A = SHARED ? smem : ws + (size_t)t * P * ld;
+  double lam = state[LM_LAM];
+  for (int k = 0; k < t; ++k) lam *= 5.0;
+
+  for (long idx = tid; idx < (long)P * ld; idx += nthr) {
+    const int i = (int)(idx / ld), j = (int)(idx - (long)i * ld);
+    double v;
+    if (j < P) {
+      v = H[(size_t)i * P + j];
+      if (i == j) v += lam;
+    } else {
+      v = -g[i];
+    }
+    A[idx] = v;
+  }
+  __syncthreads();
+
+  for (int k = 0; k < P; ++k) {
+    if (tid < 64) {  // pivot search in the first wave
+      double best = -1.0;
+      int bi = P;
+      for (int i = k + tid; i < P; i += 64) {
+        const double v = fabs(A[(size_t)i * ld + k]);
+        if (v > best) {
+          best = v;
+          bi = i;
+        }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(best, off, 64);
+        const int oi = __shfl_xor(bi, off, 64);
+        if (ob > best || (ob == best && oi < bi)) {
+          best = ob;
+          bi = oi;
+        }
+      }
+      if (tid == 0) s_piv = bi < P ? bi : k;  // an all-NaN column keeps its row
+    }
+    __syncthreads();
+    const int piv = s_piv;
+    if (piv != k) {
+      for (int j = k + tid; j < ld; j += nthr) {
+        const double a = A[(size_t)k * ld + j];
+        A[(size_t)k * ld + j] = A[(size_t)piv * ld + j];
+        A[(size_t)piv * ld + j] = a;
+      }
+      __syncthreads();
+    }
+    const double pivot = A[(size_t)k * ld + k];
+    for (int i = k + 1 + tid; i < P; i += nthr) A[(size_t)i * ld + k] /= pivot;
+    __syncthreads();
+    const int rows = P - 1 - k, cols = ld - 1 - k;
+    for (long idx = tid; idx < (long)rows * cols; idx += nthr) {
+      const int i = k + 1 + (int)(idx / cols), j = k + 1 + (int)(idx % cols);
+      A[(size_t)i * ld + j] = fma(-A[(size_t)i * ld + k], A[(size_t)k * ld + j],
+                                  A[(size_t)i * ld + j]);
+    }
+    __syncthreads();
+  }
+
+  for (int i = P - 1; i >= 0; --i) {  // x overwrites the right-hand side column
+    if (tid == 0) {
+      s_xi = A[(size_t)i * ld + P] / A[(size_t)i * ld + i];
+      A[(size_t)i * ld + P] = s_xi;
+    }
+    __syncthreads();
+    const double xi = s_xi;
+    for (int j = tid; j < i; j += nthr)
+      A[(size_t)j * ld + P] = fma(-A[(size_t)j * ld + i], xi, A[(size_t)j * ld + P]);
+    __syncthreads();
+  }
+
+  double* xt = trial_x + (size_t)t * P;
+  for (int j = tid; j < P; j += nthr) xt[j] = log_x[j] + A[(size_t)j * ld + P];
+  __syncthreads();
+  double* tab = tables + (size_t)t * nl * NCOL;
+  for (int r = tid; r < nl * NCOL; r += nthr) tab[r] = lm_table_entry(ref, nl, xt, r / NCOL, r % NCOL);
+}
+
+// chi[t, p] = sum_w |R_t(p, w) - R_obs(p, w)|^2 for trial table t: the lane
+// arithmetic of kh_forward (explicit frequencies) and the summation order of
+// kh_residual_norms (work-item j adds frequencies j, j + 256, ... in order,
+// then a fixed LDS tree), so each row equals that pair of launches bit for bit.
+constexpr int LM_MISFIT_BLOCK = 256;
+
+__global__ void __launch_bounds__(LM_MISFIT_BLOCK) lm_trial_misfit_kernel(
+    const double* __restrict__ tables, int nl, const double* __restrict__ p, int np,
+    const cd* __restrict__ omega, int nw, int free_surface, const cd* __restrict__ R_obs,
+    const double* __restrict__ state, double* __restrict__ chi) {
+  extern __shared__ double smem_d[];
+  __shared__ double s_red[LM_MISFIT_BLOCK];
+  if (state[LM_STATUS] != 0.0) return;
+  LayerPt* s_layer = reinterpret_cast<LayerPt*>(smem_d);
+  Iface* s_if = reinterpret_cast<Iface*>(s_layer + nl);
+  const int t = blockIdx.x / np;
+  const int ip = blockIdx.x - t * np;
+  stage_slowness(tables + (size_t)t * nl * NCOL, nl, p[ip], s_layer, s_if);
+  const cd* obs = R_obs + (size_t)ip * nw;
+  double acc = 0.0;
+  for (int w = threadIdx.x; w < nw; w += LM_MISFIT_BLOCK) {
+    const cd om[1] = {omega[w]};
+    cd out[1];
+    sweep_lanes<1, false>(s_layer, s_if, static_cast<const LayerStep*>(nullptr), nl, om,
+                          free_surface, out);
+    acc = add_sq_diff(acc, out[0], obs[w]);
+  }
+  s_red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = LM_MISFIT_BLOCK / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s_red[threadIdx.x] += s_red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) chi[blockIdx.x] = s_red[0];
+}
+
+// One workgroup: the first trial whose misfit (rows summed in ascending
+// slowness order) is below the current misfit becomes the iterate.
+__global__ void __launch_bounds__(256) lm_decide_kernel(int P, int ntrials, int np,
+                                                        const double* __restrict__ chi_trials,
+                                                        const double* __restrict__ trial_x,
+                                                        double* __restrict__ state,
+                                                        double* __restrict__ log_x) {
+  __shared__ int s_acc;
+  const bool run = state[LM_STATUS] == 0.0;
+  __syncthreads();
+  if (!run) return;
+  if (threadIdx.x == 0) {
+    const double chi = state[LM_CHI];
+    double lam = state[LM_LAM];
+    int acc = -1;
+    for (int t = 0; t < ntrials; ++t) {
+      double s = 0.0;
+      for (int q = 0; q < np; ++q) s += chi_trials[(size_t)t * np + q];
+      if (s < chi) {
+        acc = t;
+        break;
+      }
+      lam *= 5.0;
+    }
+    if (acc >= 0) lam = fmax(lam / 2.0, 1e-15);
+    else state[LM_STATUS] = 2.0;
+    state[LM_LAM] = lam;
+    state[LM_TRIAL] = (double)acc;
+    s_acc = acc;
+  }
+  __syncthreads();
+  const int acc = s_acc;
+  if (acc < 0) return;
+  for (int j = threadIdx.x; j < P; j += blockDim.x) log_x[j] = trial_x[(size_t)acc * P + j];
+}
+
+static int check_lm_layers(const char* what, int nl) {
+  KH_REQUIRE(nl >= 2, KH_ERR_INVALID_ARGUMENT, "%s: need at least 2 layers, got %d", what, nl);
+  KH_REQUIRE(nl <= KH_MAX_LAYERS_DERIV, KH_ERR_UNSUPPORTED,
+             "%s: %d layers exceeds KH_MAX_LAYERS_DERIV=%d", what, nl, KH_MAX_LAYERS_DERIV);
+  return KH_OK;
+}
+
+}  // namespace kh
+
+using namespace kh;
+
+extern "C" int kh_lm_tables(const double* ref_layers, int nl, const double* x, int nsets,
+                            double* tables, const double* state, void* stream) {
+  clear_error();
+  KH_TRY(check_lm_layers("kh_lm_tables", nl));
+  KH_REQUIRE(nsets >= 0, KH_ERR_INVALID_ARGUMENT, "kh_lm_tables: nsets < 0");
+  if (nsets == 0) return KH_OK;
+  KH_REQUIRE(ref_layers && x && tables, KH_ERR_INVALID_ARGUMENT, "kh_lm_tables: NULL pointer");
+  const long n = (long)nsets * nl * NCOL;
+  hipLaunchKernelGGL(lm_tables_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     as_stream(stream), ref_layers, nl, x, nsets, tables, state);
+  return check_launch("kh_lm_tables");
+}
+
+extern "C" int kh_lm_prepare(int nl, int np, const double* chi_rows, int chi_stride,
+                             const double* g, const double* H, const double* log_x,
+                             const double* x_true, double tol, int max_hist, double* state,
+                             double* g_log, double* H_log, double* history, void* stream) {
+  clear_error();
+  KH_TRY(check_lm_layers("kh_lm_prepare", nl));
+  KH_REQUIRE(np >= 1 && chi_stride >= 1 && max_hist >= 0, KH_ERR_INVALID_ARGUMENT,
+             "kh_lm_prepare: np %d, chi_stride %d, max_hist %d", np, chi_stride, max_hist);
+  KH_REQUIRE(chi_rows && g && H && log_x && x_true && state && g_log && H_log &&
+                 (history || max_hist == 0),
+             KH_ERR_INVALID_ARGUMENT, "kh_lm_prepare: NULL pointer");
+  hipLaunchKernelGGL(lm_prepare_kernel, dim3(1), dim3(LM_BLOCK), 0, as_stream(stream),
+                     lm_params(nl), np, chi_rows, chi_stride, g, H, log_x, x_true, tol, max_hist,
+                     state, g_log, H_log, history);
+  return check_launch("kh_lm_prepare");
+}
+
+extern "C" size_t kh_lm_trials_workspace_bytes(int nl, int ntrials) {
+  if (nl < 2 || ntrials <= 0) return 0;
+  const int P = lm_params(nl);
+  if (P <= LM_LDS_MAX_P) return 0;
+  return (size_t)ntrials * P * (P + 1) * sizeof(double);
+}
+
+extern "C" int kh_lm_trials(int nl, int ntrials, const double* H_log, const double* g_log,
+                            const double* log_x, const double* ref_layers, const double* state,
+                            double* trial_x, double* trial_tables, void* workspace,
+                            size_t workspace_bytes, void* stream) {
+  clear_error();
+  KH_TRY(check_lm_layers("kh_lm_trials", nl));
+  KH_REQUIRE(ntrials >= 0, KH_ERR_INVALID_ARGUMENT, "kh_lm_trials: ntrials < 0");
+  if (ntrials == 0) return KH_OK;
+  KH_REQUIRE(H_log && g_log && log_x && ref_layers && state && trial_x && trial_tables,
+             KH_ERR_INVALID_ARGUMENT, "kh_lm_trials: NULL pointer");
+  const int P = lm_params(nl);
+  hipStream_t st = as_stream(stream);
+  if (P <= LM_LDS_MAX_P) {
+    const size_t lds = (size_t)P * (P + 1) * sizeof(double);
+    hipLaunchKernelGGL(lm_trials_kernel<true>, dim3((unsigned)ntrials), dim3(LM_BLOCK), lds, st,
+                       nl, H_log, g_log, log_x, ref_layers, state, trial_x, trial_tables,
+                       static_cast<double*>(nullptr));
+  } else {
+    const size_t need = kh_lm_trials_workspace_bytes(nl, ntrials);
+    KH_REQUIRE(workspace && workspace_bytes >= need, KH_ERR_INVALID_ARGUMENT,
+               "kh_lm_trials: workspace of %zu bytes needed, %zu given", need, workspace_bytes);
+    hipLaunchKernelGGL(lm_trials_kernel<false>, dim3((unsigned)ntrials), dim3(LM_BLOCK), 0, st,
+                       nl, H_log, g_log, log_x, ref_layers, state, trial_x, trial_tables,
+                       static_cast<double*>(workspace));
+  }
+  return check_launch("kh_lm_trials");
+}
+
+extern "C" int kh_lm_trial_misfits(const double* trial_tables, int nl, int ntrials,
+                                   const double* p, int np, const double* omega, int nw,
+                                   int free_surface, const double* R_obs, const double* state,
+                                   double* chi_trials, void* stream) {
+  clear_error();
+  KH_TRY(check_lm_layers("kh_lm_trial_misfits", nl));
+  KH_REQUIRE(ntrials >= 0 && np >= 0 && nw >= 0, KH_ERR_INVALID_ARGUMENT,
+             "kh_lm_trial_misfits: negative size");
+  if (ntrials == 0 || np == 0) return KH_OK;
+  KH_REQUIRE(trial_tables && p && state && chi_trials && (nw == 0 || (omega && R_obs)),
+             KH_ERR_INVALID_ARGUMENT, "kh_lm_trial_misfits: NULL pointer");
+  const long nblocks = (long)ntrials * np;
+  KH_REQUIRE(nblocks <= 0x7fffffffL, KH_ERR_UNSUPPORTED, "kh_lm_trial_misfits: grid too large");
+  const size_t lds = (size_t)nl * sizeof(LayerPt) + (size_t)(nl - 1) * sizeof(Iface);
+  hipLaunchKernelGGL(lm_trial_misfit_kernel, dim3((unsigned)nblocks), dim3(LM_MISFIT_BLOCK), lds,
+                     as_stream(stream), trial_tables, nl, p, np,
+                     reinterpret_cast<const cd*>(omega), nw, free_surface,
+                     reinterpret_cast<const cd*>(R_obs), state, chi_trials);
+  return check_launch("kh_lm_trial_misfits");
+}
+
+extern "C" int kh_lm_decide(int nl, int ntrials, int np, const double* chi_trials,
+                            const double* trial_x, double* state, double* log_x, void* stream) {
+  clear_error();
+  KH_TRY(check_lm_layers("kh_lm_decide", nl));
+  KH_REQUIRE(ntrials >= 0 && np >= 1, KH_ERR_INVALID_ARGUMENT, "kh_lm_decide: bad size");
+  KH_REQUIRE(state && log_x && (ntrials == 0 || (chi_trials && trial_x)), KH_ERR_INVALID_ARGUMENT,
+             "kh_lm_decide: NULL pointer");
+  hipLaunchKernelGGL(lm_decide_kernel, dim3(1), dim3(256), 0, as_stream(stream), lm_params(nl),
+                     ntrials, np, chi_trials, trial_x, state, log_x);
+  return check_launch("kh_lm_decide");
+}

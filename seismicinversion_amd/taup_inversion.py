@@ -12,10 +12,13 @@ exact Hessian in the packed parameters:
   a Gram matrix on the FP64 matrix cores (kh_gram).
 * hessian_mode "hvp": the P exact Hessian-vector products H e_k directly.
 
-The host keeps what the reference keeps there: the P x P damped solve
-(np.linalg.solve), the damping schedule (lambda_0 = 0.01 max|diag H|,
-x5 on rejection, /2 on acceptance, at most 10 trials) and the convergence
-tests, so the iteration sequence follows taup_inversion.py:389-439.
+The rest of the iteration follows taup_inversion.py:389-439: the P x P
+damped solve, the damping schedule (lambda_0 = 0.01 max|diag H|, x5 on
+rejection, /2 on acceptance, at most 10 trials) and the convergence tests.
+invert_taup runs it on the device by default (_DeviceLoop: all trials of
+an iteration solved and evaluated at once, the first improving one
+accepted), or on the host one trial at a time as the reference does
+(device_loop=False, with np.linalg.solve).
 """
 
 from __future__ import annotations
@@ -62,36 +65,41 @@ class InversionResult:
     converged: bool = False
 
 
+def _capture(fn, dev):
+    """(graph, fn()) with fn's device work captured as a HIP graph
+    (torch.cuda.CUDAGraph), not executed.  Capture runs on a side stream:
+    unlike the torch.cuda.graph context it neither synchronises the whole
+    device nor empties the allocator cache, and it is thread-local, so other
+    threads' work is not held up."""
+    cur = torch.cuda.current_stream(dev)
+    side = torch.cuda.Stream(device=dev)
+    side.wait_stream(cur)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(side):
+        graph.capture_begin(capture_error_mode="thread_local")
+        try:
+            out = fn()
+        finally:
+            graph.capture_end()
+    cur.wait_stream(side)
+    return graph, out
+
+
 class _Graphed:
     """A device computation of a (nl, 6) layer table, captured once as a HIP
-    graph (torch.cuda.CUDAGraph) and replayed for every new table.
+    graph and replayed for every new table.
 
     The caller runs the computation eagerly once first (its allocations and
     any lazy set-up are then settled); the capture itself executes nothing,
     so no computation runs more often than it is asked for.  The table is
     copied into a static input before each replay; the outputs are static
     tensors overwritten by the next replay, so callers copy them out first.
-    Capture is thread-local (other threads may use the device meanwhile).
     The captured work is this library's kernels and small torch reductions
     along short axes; none of it records a memset node."""
 
     def __init__(self, fn, layers: torch.Tensor):
-        dev = layers.device
         self.inp = layers.detach().clone()
-        cur = torch.cuda.current_stream(dev)
-        side = torch.cuda.Stream(device=dev)
-        side.wait_stream(cur)
-        self.graph = torch.cuda.CUDAGraph()
-        with torch.cuda.stream(side):
-            # capture on the side stream; unlike the torch.cuda.graph context
-            # this neither synchronises the whole device nor empties the
-            # allocator cache, so other threads' work is not held up
-            self.graph.capture_begin(capture_error_mode="thread_local")
-            try:
-                self.out = fn(self.inp)
-            finally:
-                self.graph.capture_end()
-        cur.wait_stream(side)
+        self.graph, self.out = _capture(lambda: fn(self.inp), layers.device)
 
     def __call__(self, layers: torch.Tensor):
         self.inp.copy_(layers)
@@ -201,6 +209,100 @@ class _Problem:
         return float(np.sum(chi.cpu().numpy())), g.cpu().numpy(), H.cpu().numpy()
 
 
+class _DeviceLoop:
+    """The LM loop of taup_inversion.py:389-439 resident on the device.
+
+    One iteration is a fixed sequence of launches on the current stream,
+    with no host round trip:
+      kh_lm_tables (the iterate's layer table) -> the misfit / gradient /
+      exact Hessian evaluation -> kh_lm_prepare (log transform, history,
+      first damping, convergence tests) -> kh_lm_trials (all ``max_trials``
+      trials of the damping schedule, one LU solve per workgroup) ->
+      kh_lm_trial_misfits -> kh_lm_decide (the first trial below the current
+      misfit is accepted, as the sequential loop would).
+    The loop state, history and buffers are static tensors, so with
+    ``use_graphs`` the second iteration is captured as one HIP graph and
+    replayed for the rest (the first runs eagerly and settles every
+    allocation).  The host only reads the 8-byte loop status, one iteration
+    behind (so the device never waits for it); at most one evaluation past
+    the stop is computed and changes nothing."""
+
+    def __init__(self, prob: _Problem, true_params: np.ndarray, init_params: np.ndarray,
+                 max_iter: int, tol: float, hessian_mode: str, max_trials: int):
+        if hessian_mode not in ("hvp", "gram"):
+            raise ValueError(f"unknown hessian mode {hessian_mode!r}")
+        dev = self.dev = prob.dev
+        self.prob, self.mode = prob, hessian_mode
+        self.nl, self.P, self.np = prob.nl, prob.P, len(prob.p_values)
+        self.ntr, self.max_iter, self.tol = int(max_trials), int(max_iter), float(tol)
+        f64 = dict(dtype=_F, device=dev)
+        self.ref = prob.table(np.asarray(true_params, dtype=np.float64))
+        self.x_true = torch.tensor(true_params, **f64)
+        self.log_x = torch.tensor(np.log(init_params), **f64)
+        self.state = torch.tensor([-1.0, 0.0, 0.0, 0.0, 0.0, -1.0, 0.0, 0.0], **f64)
+        self.hist = torch.zeros((max(self.max_iter, 1), 3), **f64)
+        self.table = torch.empty((self.nl, 6), **f64)
+        self.g_log = torch.empty((self.P,), **f64)
+        self.H_log = torch.empty((self.P, self.P), **f64)
+        self.trial_x = torch.empty((max(self.ntr, 1), self.P), **f64)
+        self.trial_tab = torch.empty((max(self.ntr, 1), self.nl, 6), **f64)
+        self.trial_chi = torch.empty((max(self.ntr, 1), self.np), **f64)
+        self.ws_bytes = int(_abi.lib().kh_lm_trials_workspace_bytes(self.nl, self.ntr))
+        self.ws = torch.empty((max(self.ws_bytes, 1),), dtype=torch.uint8, device=dev)
+        self.status = torch.zeros((max(self.max_iter, 1),), dtype=_F, pin_memory=True)
+
+    def step(self) -> None:
+        """Enqueue one iteration on the current stream."""
+        L, ptr, prob = _abi.lib(), _abi.ptr, self.prob
+        s = _abi.stream_handle(self.dev)
+        _abi.check(L.kh_lm_tables(ptr(self.ref), self.nl, ptr(self.log_x), 1, ptr(self.table),
+                                  None, s), "kh_lm_tables")
+        chi, g, H = prob._grad_hessian_dev(self.table, self.mode)
+        g, H = g.contiguous(), H.contiguous()
+        _abi.check(L.kh_lm_prepare(self.nl, self.np, chi.data_ptr(), chi.stride(0), ptr(g),
+                                   ptr(H), ptr(self.log_x), ptr(self.x_true), self.tol,
+                                   self.max_iter, ptr(self.state), ptr(self.g_log),
+                                   ptr(self.H_log), ptr(self.hist), s), "kh_lm_prepare")
+        if self.ntr > 0:
+            _abi.check(L.kh_lm_trials(self.nl, self.ntr, ptr(self.H_log), ptr(self.g_log),
+                                      ptr(self.log_x), ptr(self.ref), ptr(self.state),
+                                      ptr(self.trial_x), ptr(self.trial_tab), ptr(self.ws),
+                                      self.ws.numel(), s), "kh_lm_trials")
+            _abi.check(L.kh_lm_trial_misfits(ptr(self.trial_tab), self.nl, self.ntr, ptr(prob.p),
+                                             self.np, ptr(prob.omega), prob.omega.numel(),
+                                             int(bool(prob.free_surface)), ptr(prob.R_obs),
+                                             ptr(self.state), ptr(self.trial_chi), s),
+                       "kh_lm_trial_misfits")
+        _abi.check(L.kh_lm_decide(self.nl, self.ntr, self.np, ptr(self.trial_chi),
+                                  ptr(self.trial_x), ptr(self.state), ptr(self.log_x), s),
+                   "kh_lm_decide")
+
+    def run(self, use_graphs: bool = False):
+        """Run up to max_iter iterations; returns (history (n, 3), final log_x,
+        status) on the host."""
+        graph = None
+        events = []
+        for it in range(self.max_iter):
+            if use_graphs and it > 0:
+                if graph is None:
+                    graph, _ = _capture(self.step, self.dev)
+                graph.replay()
+            else:
+                self.step()
+            self.status[it:it + 1].copy_(self.state[4:5], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record()
+            events.append(ev)
+            if it > 0:
+                events[it - 1].synchronize()
+                if self.status[it - 1] != 0.0:
+                    break
+        torch.cuda.current_stream(self.dev).synchronize()
+        state = self.state.cpu().numpy()
+        n = int(state[3])
+        return self.hist[:n].cpu().numpy(), self.log_x.cpu().numpy(), int(state[4])
+
+
 def _log_transform(m, g, H):
     """g_log = m g, H_log = m m^T H + diag(g m) (taup_inversion.py:180-185)."""
     return m * g, np.outer(m, m) * H + np.diag(g * m)
@@ -241,10 +343,15 @@ def _to_model(tab_cols, ref_model: LayerModel) -> LayerModel:
 def invert_taup(true_model: LayerModel, p_values: list[float], nfreq: int = 64,
                 perturbation: float = 0.15, max_iter: int = 100, seed: int = 42,
                 tol: float = 1e-8, hessian_mode: str = "gram",
-                max_trials: int = 10, use_graphs: bool = False) -> InversionResult:
+                max_trials: int = 10, use_graphs: bool = False,
+                device_loop: bool = True) -> InversionResult:
     """Newton-LM inversion from a perturbed start model (taup_inversion.py:262-470).
-    ``use_graphs`` replays the iteration's device work from captured HIP
-    graphs (same numbers, fewer launches from the host)."""
+
+    ``device_loop`` (default) runs the whole iteration on the device: the
+    damped solves of all trials, their misfits and the accept / reject
+    decision included (_DeviceLoop); otherwise the host solves and decides
+    as the reference does, one trial at a time.  ``use_graphs`` replays the
+    iterations from a captured HIP graph (same numbers, fewer launches)."""
     t = model_to_tensors(true_model)
     refs = (t["alpha"], t["beta"], t["rho"], t["thickness"], t["Q_alpha"], t["Q_beta"])
     dw = 2.0 * np.pi / 64.0
@@ -268,10 +375,22 @@ def invert_taup(true_model: LayerModel, p_values: list[float], nfreq: int = 64,
     init_params = true_params * perturb
     initial_model = _to_model(_unpack_params(torch.tensor(init_params), *refs[:4]), true_model)
 
+    if device_loop:
+        loop = _DeviceLoop(prob, true_params, init_params, max_iter, tol, hessian_mode, max_trials)
+        hist, log_x, status = loop.run(use_graphs)
+        misfits = [float(v) for v in hist[:, 0]]
+        gnorms = [float(v) for v in hist[:, 1]]
+        perrs = [float(v) for v in hist[:, 2]]
+        for iteration, (chi, gn, pe) in enumerate(hist):
+            logger.info("iter %3d  chi=%.4e  ||g||=%.4e  err=%.4e", iteration, chi, gn, pe)
+        if status == 2:
+            logger.warning("Line search failed at iteration %d", len(misfits) - 1)
+        return _result(true_model, initial_model, refs, log_x, misfits, gnorms, perrs)
+
     log_x = np.log(init_params)
-    misfits: list[float] = []
-    gnorms: list[float] = []
-    perrs: list[float] = []
+    misfits = []
+    gnorms = []
+    perrs = []
     lam = None
     for iteration in range(max_iter):
         m = np.exp(log_x)
@@ -303,6 +422,11 @@ def invert_taup(true_model: LayerModel, p_values: list[float], nfreq: int = 64,
             logger.warning("Line search failed at iteration %d", iteration)
             break
 
+    return _result(true_model, initial_model, refs, log_x, misfits, gnorms, perrs)
+
+
+def _result(true_model, initial_model, refs, log_x, misfits, gnorms, perrs) -> InversionResult:
+    """The recovered model and history (taup_inversion.py:441-470)."""
     recovered = _to_model(_unpack_params(torch.tensor(np.exp(log_x)), *refs[:4]), true_model)
     return InversionResult(true_model=true_model, initial_model=initial_model,
                            recovered_model=recovered, misfit_history=misfits,

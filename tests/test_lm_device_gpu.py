@@ -1,0 +1,252 @@
+"""The device-resident Levenberg-Marquardt iteration (kh_lm_*): each kernel
+against what the reference's host loop computes (taup_inversion.py:180-185,
+238-254, 389-439), and the whole device loop against the host loop and the
+reference's own trajectory."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from seismicinversion_amd import _abi, engine
+from seismicinversion_amd import kennett_torch as kt
+from seismicinversion_amd.taup_inversion import _log_transform, _Problem, invert_taup
+from seismicinversion_amd.workloads import seeded_model
+
+pytestmark = pytest.mark.gpu
+F = dict(dtype=torch.float64, device="cuda")
+
+
+def problem(nl, seed, n_p=6, nfreq=40):
+    m = seeded_model(nl, seed)
+    t = kt.model_to_tensors(m)
+    refs = (t["alpha"], t["beta"], t["rho"], t["thickness"], t["Q_alpha"], t["Q_beta"])
+    p = list(np.linspace(0.05, 0.5, n_p))
+    w = torch.arange(1, nfreq + 1, dtype=torch.float64) * (2 * np.pi / 64)
+    obs = engine.forward(engine.layers_tensor(m), torch.tensor(p), w.to(torch.complex128))
+    prob = _Problem(*refs, p, w, obs)
+    return prob, kt._pack_params(*refs[:4]).numpy()
+
+
+def state(lam=-1.0, chi=0.0, chi_prev=0.0, it=0, status=0.0):
+    return torch.tensor([lam, chi, chi_prev, float(it), status, -1.0, 0.0, 0.0], **F)
+
+
+def call(name, *args):
+    _abi.check(getattr(_abi.lib(), name)(*args, _abi.stream_handle(torch.device("cuda"))), name)
+
+
+def ptr(t):
+    return _abi.ptr(t)
+
+
+@pytest.mark.parametrize("nl", [20, 27])  # P = 75 in LDS, P = 103 in the workspace
+def test_trial_solves_match_numpy(nl):
+    """Every trial t solves (H + lambda 5^t I) dx = -g as np.linalg.solve does,
+    on an indefinite symmetric H whose pivots need row exchanges."""
+    prob, xt = problem(nl, 3)
+    P = prob.P
+    rng = np.random.default_rng(nl)
+    Q, _ = np.linalg.qr(rng.standard_normal((P, P)))
+    H = Q @ np.diag(rng.uniform(0.5, 50.0, P) * rng.choice([-1.0, 1.0], P)) @ Q.T
+    H = 0.5 * (H + H.T)
+    H[0, 0] = 0.0  # a small first pivot candidate: partial pivoting exchanges rows
+    g = rng.standard_normal(P)
+    log_x = np.log(xt) + 0.01 * rng.standard_normal(P)
+    lam, ntr = 0.37, 6
+    tx = torch.empty((ntr, P), **F)
+    tab = torch.empty((ntr, nl, 6), **F)
+    nws = int(_abi.lib().kh_lm_trials_workspace_bytes(nl, ntr))
+    assert (nws == 0) == (P <= 89)
+    ws = torch.empty(max(nws, 1), dtype=torch.uint8, device="cuda")
+    ref = prob.table(xt)
+    Hd, gd, xd, st = (torch.tensor(H, **F), torch.tensor(g, **F), torch.tensor(log_x, **F),
+                      state(lam))  # held until the kernel has read them
+    call("kh_lm_trials", nl, ntr, ptr(Hd), ptr(gd), ptr(xd), ptr(ref), ptr(st), ptr(tx), ptr(tab),
+         ptr(ws), ws.numel())
+    tx, tab = tx.cpu().numpy(), tab.cpu().numpy()
+    lt = lam
+    for t in range(ntr):
+        A = H + lt * np.eye(P)
+        dx = np.linalg.solve(A, -g)
+        err = np.max(np.abs((tx[t] - log_x) - dx)) / np.max(np.abs(dx))
+        bar = 1e-13 * np.linalg.cond(A)
+        print(f"trial {t}: lambda {lt:.3g} rel err {err:.2e} bar {bar:.2e}", flush=True)
+        assert err < bar
+        host = prob.table(np.exp(tx[t])).cpu().numpy()
+        np.testing.assert_allclose(tab[t], host, rtol=5e-16, atol=0)
+        lt *= 5.0
+
+
+def test_tables_match_host_builder():
+    prob, xt = problem(9, 4)
+    rng = np.random.default_rng(0)
+    X = np.log(xt)[None, :] + 0.05 * rng.standard_normal((3, prob.P))
+    out = torch.empty((3, prob.nl, 6), **F)
+    ref, Xd = prob.table(xt), torch.tensor(X, **F)
+    call("kh_lm_tables", ptr(ref), prob.nl, ptr(Xd), 3, ptr(out), None)
+    for s in range(3):
+        host = prob.table(np.exp(X[s])).cpu().numpy()
+        np.testing.assert_allclose(out[s].cpu().numpy(), host, rtol=5e-16, atol=0)
+    assert np.all(np.isinf(out[:, -1, 3].cpu().numpy()))
+
+
+def test_trial_misfits_equal_forward_and_residual_norms():
+    """Bit for bit: the fused trial kernel is kh_forward + kh_residual_norms."""
+    prob, xt = problem(12, 5, n_p=9, nfreq=300)
+    rng = np.random.default_rng(1)
+    tabs = torch.stack([prob.table(xt * rng.uniform(0.97, 1.03, xt.size)) for _ in range(4)])
+    chi = torch.empty((4, 9), **F)
+    st = state()
+    call("kh_lm_trial_misfits", ptr(tabs), prob.nl, 4, ptr(prob.p), 9, ptr(prob.omega),
+         prob.omega.numel(), 0, ptr(prob.R_obs), ptr(st), ptr(chi))
+    for t in range(4):
+        R = engine.forward(tabs[t], prob.p, prob.omega)
+        assert torch.equal(chi[t], engine.residual_norms(R, prob.R_obs)), f"trial {t}"
+
+
+def test_prepare_matches_log_transform_and_history():
+    prob, xt = problem(8, 6)
+    P, n_p = prob.P, 5
+    rng = np.random.default_rng(2)
+    B = rng.standard_normal((P, P))
+    H = B + B.T
+    g = rng.standard_normal(P)
+    log_x = np.log(xt) + 0.02 * rng.standard_normal(P)
+    chi_rows = rng.uniform(1.0, 2.0, (n_p, 2))  # read with stride 2, as the adjoint writes them
+    st = state(chi_prev=float(chi_rows[:, 0].sum()) * (1 + 1e-3), it=1)
+    g_log, H_log = torch.empty(P, **F), torch.empty((P, P), **F)
+    hist = torch.zeros((4, 3), **F)
+    args = [torch.tensor(a, **F) for a in (chi_rows, g, H, log_x, xt)]  # held across the calls
+
+    def prepare(st):
+        c, gd, Hd, xd, td = args
+        call("kh_lm_prepare", prob.nl, n_p, ptr(c), 2, ptr(gd), ptr(Hd), ptr(xd), ptr(td), 1e-6, 4,
+             ptr(st), ptr(g_log), ptr(H_log), ptr(hist))
+
+    prepare(st)
+    m = np.exp(log_x)
+    gl, Hl = _log_transform(m, g, H)
+    np.testing.assert_allclose(g_log.cpu().numpy(), gl, rtol=1e-15, atol=0)
+    # exp on the device and on the host may differ by an ulp; the diagonal
+    # adds two terms, so its error is bounded against the matrix's scale
+    np.testing.assert_allclose(H_log.cpu().numpy(), Hl, rtol=0, atol=1e-15 * np.abs(Hl).max())
+    s = st.cpu().numpy()
+    h = hist.cpu().numpy()
+    chi = chi_rows[:, 0].sum()
+    np.testing.assert_allclose(h[1], [chi, np.linalg.norm(gl),
+                                      np.linalg.norm(m - xt) / np.linalg.norm(xt)], rtol=1e-14)
+    assert np.all(h[[0, 2, 3]] == 0)
+    np.testing.assert_allclose(s[0], 0.01 * np.abs(np.diag(Hl)).max(), rtol=1e-14)
+    assert s[3] == 2 and s[4] == 0 and s[1] == s[2] == h[1, 0]
+    # a relative misfit change below tol stops the loop; nothing changes after
+    st2 = state(lam=1.0, chi_prev=chi * (1 + 1e-9), it=1)
+    prepare(st2)
+    s2 = st2.cpu().numpy().copy()
+    assert s2[4] == 1 and s2[0] == 1.0
+    prepare(st2)
+    assert np.array_equal(st2.cpu().numpy(), s2)
+
+
+def test_decide_accepts_the_first_improving_trial():
+    P, n_p, ntr = 11, 3, 5  # nl = 4
+    rng = np.random.default_rng(3)
+    tx = torch.tensor(rng.standard_normal((ntr, P)), **F)
+    chi = np.full((ntr, n_p), 1.0)
+    chi[0, 0] = np.nan  # a non-finite trial is rejected
+    chi[2] = [0.5, 0.5, 0.99]  # 1.99 < 2.0: the first accepted
+    chi[3] = [0.1, 0.1, 0.1]
+    log_x = torch.zeros(P, **F)
+    st = state(lam=0.3, chi=2.0)
+    chid = torch.tensor(chi, **F)
+    call("kh_lm_decide", 4, ntr, n_p, ptr(chid), ptr(tx), ptr(st), ptr(log_x))
+    s = st.cpu().numpy()
+    assert torch.equal(log_x, tx[2]) and s[5] == 2 and s[4] == 0
+    assert s[0] == max(0.3 * 5.0 * 5.0 / 2.0, 1e-15)
+    # none below: status 2, lambda x 5 per trial, iterate unchanged
+    st = state(lam=0.3, chi=0.01)
+    before = log_x.clone()
+    call("kh_lm_decide", 4, ntr, n_p, ptr(chid), ptr(tx), ptr(st), ptr(log_x))
+    s = st.cpu().numpy()
+    lam = 0.3
+    for _ in range(ntr):
+        lam *= 5.0
+    assert s[4] == 2 and s[5] == -1 and s[0] == lam and torch.equal(log_x, before)
+
+
+@pytest.mark.parametrize("mode", ["gram", "hvp"])
+def test_device_loop_follows_host_loop(mode):
+    """Same evaluations, accept / reject rule and damping; only the solve
+    (LU here, LAPACK there) and the order of a few sums differ."""
+    m = seeded_model(6, 8)
+    p = list(np.linspace(0.05, 0.45, 10))
+    kw = dict(nfreq=48, perturbation=0.05, max_iter=6, tol=0.0, hessian_mode=mode)
+    dev = invert_taup(m, p, **kw)
+    host = invert_taup(m, p, device_loop=False, **kw)
+    print("device", dev.misfit_history, "\nhost  ", host.misfit_history, flush=True)
+    assert dev.n_iterations == host.n_iterations == 6
+    np.testing.assert_allclose(dev.misfit_history, host.misfit_history, rtol=1e-9)
+    np.testing.assert_allclose(dev.grad_norm_history, host.grad_norm_history, rtol=1e-7)
+    np.testing.assert_allclose(dev.param_error_history, host.param_error_history, rtol=1e-9)
+    np.testing.assert_allclose(dev.recovered_model.alpha, host.recovered_model.alpha, rtol=1e-10)
+
+
+def _stop_iteration(tol, g, r):
+    """First iteration at which the reference's tests stop the loop
+    (taup_inversion.py:410-418), or None."""
+    for i in range(len(g)):
+        if g[i] < tol or (i >= 1 and r[i] < tol):
+            return i
+    return None
+
+
+def test_device_loop_stops_like_the_reference():
+    """With a tolerance that stops the loop part-way (picked from the host
+    loop's own gradient norms and relative misfit changes, away from every
+    one of them), both loops stop after the same iteration; the iterations
+    the device ran past the stop change nothing."""
+    m = seeded_model(6, 8)
+    p = list(np.linspace(0.05, 0.45, 10))
+    kw = dict(nfreq=48, perturbation=0.05)
+    free = invert_taup(m, p, max_iter=8, tol=0.0, device_loop=False, **kw)
+    g = free.grad_norm_history
+    c = free.misfit_history
+    r = [np.inf] + [abs(c[i - 1] - c[i]) / max(abs(c[i]), 1e-30) for i in range(1, len(c))]
+    vals = sorted(set(g) | set(r[1:]))
+    tol = k = None
+    for lo, hi in zip(vals, vals[1:]):
+        if hi > 1.05 * lo:
+            t = float(np.sqrt(lo * hi))
+            k = _stop_iteration(t, g, r)
+            if k is not None and 1 <= k < len(g) - 1:
+                tol = t
+                break
+    assert tol is not None, (g, r)
+    for device_loop in (True, False):
+        res = invert_taup(m, p, max_iter=len(g) + 2, tol=tol, device_loop=device_loop, **kw)
+        assert res.n_iterations == k + 1, (device_loop, res.n_iterations, k)
+        np.testing.assert_allclose(res.misfit_history, c[:k + 1], rtol=1e-9)
+    stop0 = invert_taup(m, p, max_iter=5, tol=1e30, **kw)  # gradient test at the start
+    assert stop0.n_iterations == 1
+    np.testing.assert_allclose(stop0.recovered_model.alpha, stop0.initial_model.alpha,
+                               rtol=1e-15)
+    failed = invert_taup(m, p, max_iter=5, tol=0.0, max_trials=0, **kw)  # no trial: fails
+    assert failed.n_iterations == 1
+
+
+def test_device_loop_graph_replays_equal_eager():
+    """The captured iteration, replayed for every iteration after the first,
+    gives the eager device loop's numbers bit for bit (with eager work of
+    another inversion in between)."""
+    m = seeded_model(9, 12)
+    p = list(np.linspace(0.05, 0.5, 7))
+    kw = dict(nfreq=40, perturbation=0.05, max_iter=5, tol=0.0)
+    a = invert_taup(m, p, **kw)
+    b = invert_taup(m, p, use_graphs=True, **kw)
+    invert_taup(seeded_model(5, 1), [0.2], nfreq=16, max_iter=2, tol=0.0)
+    c = invert_taup(m, p, use_graphs=True, **kw)
+    for r in (b, c):
+        assert r.misfit_history == a.misfit_history
+        assert r.grad_norm_history == a.grad_norm_history
+        assert np.array_equal(r.recovered_model.beta, a.recovered_model.beta)

@@ -235,11 +235,13 @@ def test_lm_graph_replays_match_eager(mode):
     assert set(graphed._graphs) == {"gh_" + mode, "misfit"}
 
 
-def test_invert_taup_with_graphs_equals_eager():
+@pytest.mark.parametrize("device_loop", [True, False])
+def test_invert_taup_with_graphs_equals_eager(device_loop):
     m = seeded_model(6, 8)
     p = list(np.linspace(0.05, 0.45, 10))
-    a = invert_taup(m, p, nfreq=48, perturbation=0.05, max_iter=4, tol=0.0)
-    b = invert_taup(m, p, nfreq=48, perturbation=0.05, max_iter=4, tol=0.0, use_graphs=True)
+    kw = dict(nfreq=48, perturbation=0.05, max_iter=4, tol=0.0, device_loop=device_loop)
+    a = invert_taup(m, p, **kw)
+    b = invert_taup(m, p, use_graphs=True, **kw)
     assert a.misfit_history == b.misfit_history
     assert a.grad_norm_history == b.grad_norm_history
     assert np.array_equal(a.recovered_model.alpha, b.recovered_model.alpha)
