@@ -388,7 +388,8 @@ def lm_figures(torch, dev):
     from one captured HIP graph per iteration, and the host loop that solves
     and decides on the host one trial at a time as the reference does.  One
     misfit/gradient/exact-Hessian evaluation is timed on its own.  Beside it
-    the served inversion size (17 layers, 32 p x 128 omega, 5 iterations),
+    the served inversion size (17 layers, 32 p x 128 omega, 5 iterations)
+    and the reference CLI's default inversion (5 layers, 12 p x 64 omega),
     where the kernels are short and launch overhead counts."""
     from seismicinversion_amd.taup_inversion import _Problem, invert_taup
     from seismicinversion_amd.workloads import seeded_model, uniform_slownesses
@@ -429,6 +430,22 @@ def lm_figures(torch, dev):
                 its += r.n_iterations
             torch.cuda.synchronize()
             t_small.setdefault(name, []).append((time.perf_counter() - t0) * 1e3 / its)
+    # the reference CLI's default inversion (configs/default_ocean_crust.yaml:
+    # 5 layers, 12 p, 64 omega), 20 iterations: kernels of a few microseconds,
+    # where launching them is the cost
+    from seismicinversion_amd.inversion_config import DEFAULT_CONFIG, load_config
+
+    cfg = load_config(DEFAULT_CONFIG)
+    t_cli = {}
+    for _ in range(2):
+        for name, kw in variants.items():
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = invert_taup(cfg.true_model, cfg.p_values, nfreq=cfg.nfreq,
+                            perturbation=cfg.perturbation, max_iter=20, seed=cfg.seed, tol=0.0,
+                            **kw)
+            torch.cuda.synchronize()
+            t_cli.setdefault(name, []).append((time.perf_counter() - t0) * 1e3 / r.n_iterations)
     n_it = res["eager"].n_iterations
     return {"P": prob.P, "n_p": 128, "n_omega": 512, "iterations": n_it,
             "eager_s": best["eager"], "graphs_s": best["graphs"],
@@ -449,7 +466,9 @@ def lm_figures(torch, dev):
             "grad_hessian_graph_replay_ms": t_gh_graph * 1e3,
             "tangent_over_adjoint_lane_updates": prob.P * 128 * 512 * 19,
             "served_size_17_layers_32p_128w": {
-                f"{k}_ms_per_iteration": min(v) for k, v in t_small.items()}}
+                f"{k}_ms_per_iteration": min(v) for k, v in t_small.items()},
+            "reference_cli_5_layers_12p_64w": {
+                f"{k}_ms_per_iteration": min(v) for k, v in t_cli.items()}}
 
 
 def lm_problem(torch, dev):

@@ -378,7 +378,7 @@ int kh_residual_norms(const double* R, const double* R_obs, int np, int nw, doub
  *   [1] misfit at the current iterate   [2] misfit at the previous iterate
  *   [3] iterations evaluated            [4] status: 0 running, 1 converged
  *       (gradient or relative-misfit test), 2 line search failed
- *   [5] trial accepted in the last iteration (-1: none)   [6..7] unused.
+ *   [5] trial accepted in this iteration (-1: none yet)   [6..7] unused.
  * Every kernel returns at once when the status is not 0, so iterations
  * enqueued after the loop has stopped change nothing. */
 #define KH_LM_STATE_LEN 8
@@ -391,6 +391,17 @@ int kh_residual_norms(const double* R, const double* R_obs, int np, int nw, doub
 int kh_lm_tables(const double* ref_layers, int nl, const double* x, int nsets, double* tables,
                  const double* state, void* stream);
 
+/* The packed gradient g (P,) and Hessian H (P, P) summed over slownesses
+ * (p ascending) from kh_misfit_hvp_partial over the P basis directions:
+ * rows (P, np, nl, 6) are its hvp output, grad (np, nl, 6) its gradient,
+ * and direction order[k] (int64) moves packed parameter k.  Where a row is
+ * partial the entry comes from its mirror (H is symmetric).  G (P, P), the
+ * Gauss-Newton block 2 Re(J^H J) over the directions (kh_gram of their
+ * dR), is added when not NULL.  Replaces the per-slowness accumulation of
+ * Kennett_Reflectivity/taup_inversion.py:163-178. */
+int kh_lm_assemble(int nl, int np, const double* rows, const double* grad, const double* G,
+                   const long long* order, double* g, double* H, void* stream);
+
 /* One workgroup: chi = sum of chi_rows[p * chi_stride] over p ascending;
  * g_log = m g, H_log = (m_i m_j) H_ij + delta_ij g_j m_j (P x P row-major);
  * history[iter] = [chi, |g_log|, |m - x_true| / |x_true|] while
@@ -402,35 +413,41 @@ int kh_lm_prepare(int nl, int np, const double* chi_rows, int chi_stride, const 
                   int max_hist, double* state, double* g_log, double* H_log, double* history,
                   void* stream);
 
-/* Workspace bytes of kh_lm_trials (0 when P <= 89: the systems fit in LDS). */
+/* Workspace bytes of kh_lm_trials for a launch of ntrials trials (0 when
+ * P <= 85: the systems fit in LDS). */
 size_t kh_lm_trials_workspace_bytes(int nl, int ntrials);
 
-/* Trials t = 0..ntrials-1 of the damping schedule at once, one workgroup
- * each: (H_log + lambda 5^t I) dx = -g_log by LU with partial pivoting,
- * trial_x[t] = log_x + dx (ntrials, P) and trial_tables[t] (ntrials, nl, 6)
- * as kh_lm_tables.  A singular system yields non-finite entries (a trial
- * that is never accepted) where np.linalg.solve raises.  Replaces
- * newton_lm_step (taup_inversion.py:238-254) inside the trial loop
- * (:425-435). */
-int kh_lm_trials(int nl, int ntrials, const double* H_log, const double* g_log,
+/* Trials t = first..first+ntrials-1 of the damping schedule at once, one
+ * workgroup each: (H_log + lambda 5^t I) dx = -g_log by LU with partial
+ * pivoting, trial_x[t] = log_x + dx and trial_tables[t] as kh_lm_tables
+ * (trial_x (T, P) and trial_tables (T, nl, 6) hold all T trials of the
+ * schedule).  With first > 0 nothing runs once a trial of this iteration
+ * has been accepted, so a caller can launch trial 0 alone and the rest
+ * after it.  A singular system yields non-finite entries (a trial that is
+ * never accepted) where np.linalg.solve raises.  Replaces newton_lm_step
+ * (taup_inversion.py:238-254) inside the trial loop (:425-435). */
+int kh_lm_trials(int nl, int first, int ntrials, const double* H_log, const double* g_log,
                  const double* log_x, const double* ref_layers, const double* state,
                  double* trial_x, double* trial_tables, void* workspace, size_t workspace_bytes,
                  void* stream);
 
-/* chi_trials (ntrials, np): per trial and slowness sum_w |R - R_obs|^2 of
- * the trial table, equal bit for bit to kh_forward followed by
- * kh_residual_norms.  R_obs (np, nw) complex128, omega (nw,) complex128.
- * Replaces _eval_misfit_only (taup_inversion.py:190-230) for every trial. */
-int kh_lm_trial_misfits(const double* trial_tables, int nl, int ntrials, const double* p, int np,
-                        const double* omega, int nw, int free_surface, const double* R_obs,
-                        const double* state, double* chi_trials, void* stream);
+/* chi_trials (T, np), rows first..first+ntrials-1: per trial and slowness
+ * sum_w |R - R_obs|^2 of the trial table, equal bit for bit to kh_forward
+ * followed by kh_residual_norms.  R_obs (np, nw) complex128, omega (nw,)
+ * complex128; skipped as kh_lm_trials.  Replaces _eval_misfit_only
+ * (taup_inversion.py:190-230) for those trials. */
+int kh_lm_trial_misfits(const double* trial_tables, int nl, int first, int ntrials,
+                        const double* p, int np, const double* omega, int nw, int free_surface,
+                        const double* R_obs, const double* state, double* chi_trials,
+                        void* stream);
 
-/* Accepts the first trial whose misfit (rows summed over p ascending) is
- * below the current one: log_x = trial_x[t], lambda = max(lambda 5^t / 2,
- * 1e-15); if none is, lambda 5^ntrials and status 2.  Replaces
- * taup_inversion.py:425-439. */
-int kh_lm_decide(int nl, int ntrials, int np, const double* chi_trials, const double* trial_x,
-                 double* state, double* log_x, void* stream);
+/* Accepts the first of trials first..first+ntrials-1 whose misfit (rows
+ * summed over p ascending) is below the current one: log_x = trial_x[t],
+ * lambda = max(lambda 5^t / 2, 1e-15).  If none is and `last` is nonzero
+ * (the schedule ends with these trials): lambda 5^(first+ntrials) and
+ * status 2.  Skipped as kh_lm_trials.  Replaces taup_inversion.py:425-439. */
+int kh_lm_decide(int nl, int first, int ntrials, int last, int np, const double* chi_trials,
+                 const double* trial_x, double* state, double* log_x, void* stream);
 
 #ifdef __cplusplus
 }

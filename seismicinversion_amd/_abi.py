@@ -67,14 +67,15 @@ EXPORTS: dict[str, list] = {
     "kh_forward_multi_static": [_vp, _c_int, ctypes.c_longlong, _c_int, _vp, _vp, _vp, _c_int,
                                 _c_int, _c_int, ctypes.c_longlong, _c_int, _vp, _vp],
     "kh_lm_tables": [_vp, _c_int, _vp, _c_int, _vp, _vp, _vp],
+    "kh_lm_assemble": [_c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "kh_lm_prepare": [_c_int, _c_int, _vp, _c_int, _vp, _vp, _vp, _vp, _c_double, _c_int, _vp,
                       _vp, _vp, _vp, _vp],
     "kh_lm_trials_workspace_bytes": [_c_int, _c_int],
-    "kh_lm_trials": [_c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_size_t,
-                     _vp],
-    "kh_lm_trial_misfits": [_vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _vp, _vp,
-                            _vp, _vp],
-    "kh_lm_decide": [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp],
+    "kh_lm_trials": [_c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                     ctypes.c_size_t, _vp],
+    "kh_lm_trial_misfits": [_vp, _c_int, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _vp,
+                            _vp, _vp, _vp],
+    "kh_lm_decide": [_c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp],
 }
 _RESTYPES = {"kh_multi_items": ctypes.c_longlong, "kh_multi_table_bytes": ctypes.c_size_t,
              "kh_deriv_workspace_bytes": ctypes.c_size_t,

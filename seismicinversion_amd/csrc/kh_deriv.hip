@@ -488,14 +488,29 @@ __global__ void __launch_bounds__(CH_BLOCK) cot_sum_kernel(int nl, int np,
   }
 }
 
+// The first-order factors of the far pairs, one thread per (p, layer,
+// column other than h): J[(ip * nl + l) * NJCOL + ci].
+constexpr int NJCOL = NCOL - 1;
+
+__global__ void __launch_bounds__(CH_BLOCK) local_jac_kernel(const double* __restrict__ layers,
+                                                             int nl, const double* __restrict__ p,
+                                                             int np, LocalJac* __restrict__ J) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)np * nl * NJCOL) return;
+  const int ci = (int)(t % NJCOL);
+  const long r = t / NJCOL;
+  const int l = (int)(r % nl), ip = (int)(r / nl);
+  J[t] = local_jac(layers, nl, l, p[ip], ci < COL_H ? ci : ci + 1);
+}
+
 // Per (dir, p): the pairs the direction does not reach (first-order chain of
 // the tangent parts; for direction 0 also the gradient, from the primal
-// parts) and zeros beyond lmax.  Reads the tile sums at tile 0's slot.
+// parts) and zeros beyond lmax.  Reads the tile sums at tile 0's slot and
+// the factors J of local_jac_kernel, shared by all directions.
 __global__ void __launch_bounds__(CH_BLOCK) chain_far_kernel(
-    const double* __restrict__ layers, const double* __restrict__ dlayers, int nl,
-    const double* __restrict__ p, int np, const IfaceZ<dcd>* __restrict__ cot_if,
-    const LayerCot<dcd>* __restrict__ cot_ly, int ntp, double* __restrict__ grad,
-    double* __restrict__ hvp, int partial) {
+    const double* __restrict__ dlayers, int nl, int np, const IfaceZ<dcd>* __restrict__ cot_if,
+    const LayerCot<dcd>* __restrict__ cot_ly, const LocalJac* __restrict__ J, int ntp,
+    double* __restrict__ grad, double* __restrict__ hvp, int partial) {
   extern __shared__ double smem_f[];
   IfaceZ<dcd>* s_if = reinterpret_cast<IfaceZ<dcd>*>(smem_f);
   LayerCot<dcd>* s_ly = reinterpret_cast<LayerCot<dcd>*>(s_if + (nl - 1));
@@ -519,6 +534,7 @@ __global__ void __launch_bounds__(CH_BLOCK) chain_far_kernel(
   const Reach rc = reach_of(dl, nl, dir, partial);
   const cd zc{0.0, 0.0};
   const IfaceZ<cd> none{zc, zc, zc, zc, zc, zc, zc, zc, zc, zc};
+  const LocalJac* Jp = J + (size_t)ip * nl * NJCOL;
   for (int k = threadIdx.x; k < nl * NCOL; k += blockDim.x) {
     const int l = k / NCOL, col = k - (k / NCOL) * NCOL;
     const size_t g = ((size_t)ip * nl + l) * NCOL + col;
@@ -529,15 +545,14 @@ __global__ void __launch_bounds__(CH_BLOCK) chain_far_kernel(
     }
     if (l >= rc.lo && l <= rc.hi) continue;  // chain_near_kernel's
     const LayerCot<dcd>& lc = s_ly[l];
-    *hv = chain_column<cd>(layers, nullptr, nl, l, p[ip],
-                           l >= 1 ? tangent_iface(s_if[l - 1]) : none,
-                           l <= nl - 2 ? tangent_iface(s_if[l]) : none,
-                           LayerCot<cd>{lc.eta.d, lc.neta.d, lc.h.d}, col).r;
+    const LocalJac jc = col == COL_H ? LocalJac{} : Jp[l * NJCOL + (col < COL_H ? col : col - 1)];
+    *hv = chain_from_jac(jc, l, nl, l >= 1 ? tangent_iface(s_if[l - 1]) : none,
+                         l <= nl - 2 ? tangent_iface(s_if[l]) : none,
+                         LayerCot<cd>{lc.eta.d, lc.neta.d, lc.h.d}, col).r;
     if (grad && dir == 0)
-      grad[g] = chain_column<cd>(layers, nullptr, nl, l, p[ip],
-                                 l >= 1 ? prim_iface(s_if[l - 1]) : none,
-                                 l <= nl - 2 ? prim_iface(s_if[l]) : none,
-                                 LayerCot<cd>{lc.eta.v, lc.neta.v, lc.h.v}, col).r;
+      grad[g] = chain_from_jac(jc, l, nl, l >= 1 ? prim_iface(s_if[l - 1]) : none,
+                               l <= nl - 2 ? prim_iface(s_if[l]) : none,
+                               LayerCot<cd>{lc.eta.v, lc.neta.v, lc.h.v}, col).r;
   }
 }
 
@@ -576,7 +591,7 @@ __global__ void __launch_bounds__(CH_BLOCK) chain_near_kernel(
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct DerivLayout {
-  size_t ws, cif, cly, chi, total;
+  size_t ws, cif, cly, chi, jac, total;
   long nlanes;
   int ntiles;  // frequency tiles of AD_WAVE lanes
   int wpb;     // waves (tiles) per workgroup
@@ -598,7 +613,10 @@ static DerivLayout deriv_layout(int nl, int np, int nw, int ndir, int K) {
   d.cif = align256(d.ws + (size_t)d.nlanes * lane_slots_k(nl, K) * sizeof(Z));
   d.cly = align256(d.cif + (size_t)nblocks * (nl - 1) * sizeof(IfaceZ<Z>));
   d.chi = align256(d.cly + (size_t)nblocks * nl * sizeof(LayerCot<Z>));
-  d.total = align256(d.chi + (size_t)nblocks * sizeof(Z));
+  d.jac = align256(d.chi + (size_t)nblocks * sizeof(Z));
+  // the far-chain factors of the partial Hessian (dual calls only)
+  const size_t jac = std::is_same<Z, dcd>::value ? (size_t)np * nl * NJCOL * sizeof(LocalJac) : 0;
+  d.total = align256(d.jac + jac);
   return d;
 }
 
@@ -668,9 +686,13 @@ static int run_adjoint(const char* what, const double* layers, const double* dla
       hipLaunchKernelGGL((cot_sum_kernel<Z>), dim3((unsigned)((long)ndir * np)), dim3(CH_BLOCK),
                          0, st, nl, np, cif, cly, MISFIT ? chip : nullptr, d.ntiles, d.ntp, chi);
       KH_TRY(check_launch(what));
+      auto* jac = reinterpret_cast<LocalJac*>(wsb + d.jac);
+      const long njac = (long)np * nl * NJCOL;
+      hipLaunchKernelGGL(local_jac_kernel, dim3((unsigned)((njac + CH_BLOCK - 1) / CH_BLOCK)),
+                         dim3(CH_BLOCK), 0, st, layers, nl, p, np, jac);
+      KH_TRY(check_launch(what));
       hipLaunchKernelGGL(chain_far_kernel, dim3((unsigned)((long)ndir * np)), dim3(CH_BLOCK),
-                         lds_c, st, layers, dlayers, nl, p, np, cif, cly, d.ntp, grad, hvp,
-                         partial);
+                         lds_c, st, dlayers, nl, np, cif, cly, jac, d.ntp, grad, hvp, partial);
       KH_TRY(check_launch(what));
       const long near_threads = (long)ndir * np * NEAR_THREADS;
       hipLaunchKernelGGL(chain_near_kernel, dim3((unsigned)((near_threads + CH_BLOCK - 1) / CH_BLOCK)),

@@ -405,6 +405,48 @@ __host__ __device__ inline Z chain_column(const double* layers, const double* dl
   return acc;
 }
 
+// The first-order factors of chain_column<cd> for parameter column col
+// (not COL_H) of layer l at slowness p: d eta, d neta and the derivatives of
+// the interfaces above (up, zero for l = 0) and below (dn, zero for
+// l = nl-1).  They depend on neither the direction nor the cotangents, so
+// one evaluation serves every direction of a launch.
+struct LocalJac {
+  cd eta, neta;
+  IfaceZ<cd> up, dn;
+};
+
+__host__ __device__ inline LocalJac local_jac(const double* layers, int nl, int l, double p,
+                                              int col) {
+  using W = dcd;
+  auto layer = [&](int ll) {  // as chain_column<cd> seeds it
+    W x[NCOL];
+    for (int c = 0; c < NCOL; ++c)
+      x[c] = Up<cd>::seed(layers[(size_t)ll * NCOL + c], 0.0, (ll == l && c == col) ? 1.0 : 0.0);
+    return layer_t<W>(x, ll, p);
+  };
+  const cd z{0.0, 0.0};
+  const IfaceZ<cd> none{z, z, z, z, z, z, z, z, z, z};
+  LocalJac j;
+  const LayerZ<W> mid = layer(l);
+  j.eta = Up<cd>::outer(mid.eta);
+  j.neta = Up<cd>::outer(mid.neta);
+  j.up = l >= 1 ? outer_iface<W, cd>(interface_t<W>(l - 1, p, layer(l - 1), mid)) : none;
+  j.dn = l <= nl - 2 ? outer_iface<W, cd>(interface_t<W>(l, p, mid, layer(l + 1))) : none;
+  return j;
+}
+
+// chain_column<cd> from its factors: the same products summed in the same
+// order, so the two agree bit for bit.
+__host__ __device__ inline cd chain_from_jac(const LocalJac& j, int l, int nl,
+                                             const IfaceZ<cd>& cb_up, const IfaceZ<cd>& cb_dn,
+                                             const LayerCot<cd>& lc, int col) {
+  if (col == COL_H) return lc.h;
+  cd acc = re_dot(j.eta, lc.eta) + re_dot(j.neta, lc.neta);
+  if (l >= 1) acc = acc + iface_dot(j.up, cb_up);
+  if (l <= nl - 2) acc = acc + iface_dot(j.dn, cb_dn);
+  return acc;
+}
+
 // All six parameter cotangents of layer l (out[c] for c in COL_*).
 template <class Z>
 __host__ __device__ inline void chain_layer(const double* layers, const double* dlayers, int nl,

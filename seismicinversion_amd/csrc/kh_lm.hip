@@ -37,7 +37,7 @@ enum { LM_LAM = 0, LM_CHI, LM_CHI_PREV, LM_ITER, LM_STATUS, LM_TRIAL };
 
 constexpr int LM_MAX_P = 4 * KH_MAX_LAYERS_DERIV - 5;  // packed parameters at the layer cap
 constexpr int LM_BLOCK = 512;
-constexpr int LM_LDS_MAX_P = 89;                       // P x (P + 1) doubles within 64 KB of LDS
+constexpr int LM_LDS_MAX_P = 85;  // P x (P + 1) doubles and the pivot tables within 64 KB of LDS
 
 static_assert(KH_LM_STATE_LEN >= LM_TRIAL + 1, "loop state layout");
 
@@ -67,6 +67,55 @@ __global__ void lm_tables_kernel(const double* __restrict__ ref, int nl,
   const int s = (int)(idx / per);
   const int r = (int)(idx - (long)s * per);
   tables[idx] = lm_table_entry(ref, nl, x + (size_t)s * lm_params(nl), r / NCOL, r % NCOL);
+}
+
+// Parameter k of the packed vector is column c of layer l
+// (kennett_torch.py:552-571: all alphas, all betas, all rhos, then the
+// thicknesses of layers 1..nl-2).
+__device__ __forceinline__ void packed_lc(int k, int n, int& l, int& c) {
+  c = k / n;
+  l = k - c * n + 1;
+}
+
+// The gradient and Hessian in the packed parameters, summed over the
+// slownesses in ascending order, from the per-slowness outputs of
+// kh_misfit_hvp_partial over a basis (rows (P, np, nl, 6); direction
+// order[k] moves packed parameter k):
+//   g[k]    = sum_p grad[p, l_k, c_k]
+//   H[a][b] = G[order[a]][order[b]] + r(b, a)  if l_a <= l_b + 1
+//           = G[order[a]][order[b]] + r(a, b)  otherwise,
+//   r(k, j) = sum_p rows[order[k], p, l_j, c_j]
+// (r(k, j) is whole where l_j <= l_k + 1; the other half comes from the
+// symmetry of H.)  G (the Gauss-Newton block over the directions) may be
+// NULL.
+__global__ void lm_assemble_kernel(int nl, int np, const double* __restrict__ rows,
+                                   const double* __restrict__ grad, const double* __restrict__ G,
+                                   const long long* __restrict__ order, double* __restrict__ g,
+                                   double* __restrict__ H) {
+  const int n = nl - 1, P = lm_params(nl);
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long PP = (long)P * P;
+  const size_t per = (size_t)nl * NCOL;
+  if (t < PP) {
+    const int a = (int)(t / P), b = (int)(t - (long)(t / P) * P);
+    int la, ca, lb, cb;
+    packed_lc(a, n, la, ca);
+    packed_lc(b, n, lb, cb);
+    const bool mirror = la <= lb + 1;
+    const int k = mirror ? b : a;
+    const int lj = mirror ? la : lb, cj = mirror ? ca : cb;
+    const double* src = rows + (size_t)order[k] * np * per + (size_t)lj * NCOL + cj;
+    double s = 0.0;
+    for (int q = 0; q < np; ++q) s += src[(size_t)q * per];
+    H[t] = G != nullptr ? G[(size_t)order[a] * P + order[b]] + s : s;
+  } else if (t < PP + P) {
+    const int k = (int)(t - PP);
+    int l, c;
+    packed_lc(k, n, l, c);
+    double s = 0.0;
+    for (int q = 0; q < np; ++q) s += grad[(size_t)q * per + (size_t)l * NCOL + c];
+    g[k] = s;
+  }
 }
 
 // One workgroup.  g_log = m g, H_log = (m_i m_j) H_ij + delta_ij g_j m_j;
@@ -123,30 +172,42 @@ __global__ void __launch_bounds__(LM_BLOCK) lm_prepare_kernel(
   state[LM_CHI_PREV] = chi;
   state[LM_CHI] = chi;
   state[LM_ITER] = (double)(it + 1);
+  state[LM_TRIAL] = -1.0;
   state[LM_STATUS] = status;
 }
 
-// One workgroup per trial t: A = [H_log + lambda_t I | -g_log] (row-major,
-// leading dimension P + 1) in LDS or in the caller's workspace, LU with
-// partial pivoting (the first row of largest magnitude in the column), the
-// right-hand side eliminated along, then back substitution.  Writes
-// trial_x[t] = log_x + dx and the trial's layer table.  A zero pivot gives
-// inf/NaN entries, whose trial misfit is NaN and is never accepted.
+// Whether trial kernels of a launch starting at trial `first` have work:
+// the loop runs, and no earlier trial of this iteration was accepted.
+__device__ __forceinline__ bool lm_trials_open(const double* state, int first) {
+  return state[LM_STATUS] == 0.0 && (first == 0 || state[LM_TRIAL] < 0.0);
+}
+
+// One workgroup per trial t = first + blockIdx.x: A = [H_log + lambda_t I |
+// -g_log] (row-major, leading dimension P + 1) in LDS or in the caller's
+// workspace, LU with partial pivoting (the first row of largest magnitude
+// in the column), the right-hand side eliminated along, then back
+// substitution.  Rows are exchanged through a permutation in LDS rather
+// than moved, so a column step is a pivot search in one wave and an update
+// by all, two barriers; the multipliers use one reciprocal of the pivot per
+// step, as LAPACK's dgetf2 scales the column.  Back substitution runs in
+// the first wave alone.  Writes trial_x[t] = log_x + dx and the trial's
+// layer table.  A zero pivot gives inf/NaN entries, whose trial misfit is
+// NaN and is never accepted.
 template <bool SHARED>
 __global__ void __launch_bounds__(LM_BLOCK) lm_trials_kernel(
-    int nl, const double* __restrict__ H, const double* __restrict__ g,
+    int nl, int first, const double* __restrict__ H, const double* __restrict__ g,
     const double* __restrict__ log_x, const double* __restrict__ ref,
     const double* __restrict__ state, double* __restrict__ trial_x,
     double* __restrict__ tables, double* __restrict__ ws) {
   extern __shared__ double smem[];
-  __shared__ int s_piv;
-  __shared__ double s_xi;
-  if (state[LM_STATUS] != 0.0) return;  // this kernel never writes the state
+  __shared__ int s_perm[LM_MAX_P];
+  __shared__ double s_x[LM_MAX_P];
+  if (!lm_trials_open(state, first)) return;  // this kernel never writes the state
   const int P = lm_params(nl);
   const int ld = P + 1;
-  const int t = blockIdx.x;
+  const int t = first + (int)blockIdx.x;
   const int tid = threadIdx.x, nthr = blockDim.x;
-  double* A = SHARED ? smem : ws + (size_t)t * P * ld;
+  double* A = SHARED ? smem : ws + (size_t)blockIdx.x * P * ld;
   double lam = state[LM_LAM];
   for (int k = 0; k < t; ++k) lam *= 5.0;
 
@@ -161,14 +222,15 @@ __global__ void __launch_bounds__(LM_BLOCK) lm_trials_kernel(
     }
     A[idx] = v;
   }
+  for (int i = tid; i < P; i += nthr) s_perm[i] = i;
   __syncthreads();
 
   for (int k = 0; k < P; ++k) {
-    if (tid < 64) {  // pivot search in the first wave
+    if (tid < 64) {  // pivot search over the logical rows k.. in the first wave
       double best = -1.0;
       int bi = P;
       for (int i = k + tid; i < P; i += 64) {
-        const double v = fabs(A[(size_t)i * ld + k]);
+        const double v = fabs(A[(size_t)s_perm[i] * ld + k]);
         if (v > best) {
           best = v;
           bi = i;
@@ -182,47 +244,42 @@ __global__ void __launch_bounds__(LM_BLOCK) lm_trials_kernel(
           bi = oi;
         }
       }
-      if (tid == 0) s_piv = bi < P ? bi : k;  // an all-NaN column keeps its row
-    }
-    __syncthreads();
-    const int piv = s_piv;
-    if (piv != k) {
-      for (int j = k + tid; j < ld; j += nthr) {
-        const double a = A[(size_t)k * ld + j];
-        A[(size_t)k * ld + j] = A[(size_t)piv * ld + j];
-        A[(size_t)piv * ld + j] = a;
+      if (tid == 0 && bi < P && bi != k) {  // an all-NaN column keeps its row
+        const int r = s_perm[k];
+        s_perm[k] = s_perm[bi];
+        s_perm[bi] = r;
       }
-      __syncthreads();
     }
-    const double pivot = A[(size_t)k * ld + k];
-    for (int i = k + 1 + tid; i < P; i += nthr) A[(size_t)i * ld + k] /= pivot;
     __syncthreads();
+    const double* rowk = A + (size_t)s_perm[k] * ld;
+    const double inv = 1.0 / rowk[k];
     const int rows = P - 1 - k, cols = ld - 1 - k;
     for (long idx = tid; idx < (long)rows * cols; idx += nthr) {
-      const int i = k + 1 + (int)(idx / cols), j = k + 1 + (int)(idx % cols);
-      A[(size_t)i * ld + j] = fma(-A[(size_t)i * ld + k], A[(size_t)k * ld + j],
-                                  A[(size_t)i * ld + j]);
+      const int r = (int)(idx / cols);
+      double* rowi = A + (size_t)s_perm[k + 1 + r] * ld;
+      const int j = k + 1 + (int)(idx - (long)r * cols);
+      rowi[j] = fma(-(rowi[k] * inv), rowk[j], rowi[j]);
     }
     __syncthreads();
   }
+  if (tid >= 64) return;  // back substitution and the outputs: the first wave
 
-  for (int i = P - 1; i >= 0; --i) {  // x overwrites the right-hand side column
-    if (tid == 0) {
-      s_xi = A[(size_t)i * ld + P] / A[(size_t)i * ld + i];
-      A[(size_t)i * ld + P] = s_xi;
+  for (int i = P - 1; i >= 0; --i) {
+    const double* rowi = A + (size_t)s_perm[i] * ld;
+    const double xi = rowi[P] / rowi[i];
+    if (tid == 0) s_x[i] = xi;
+    for (int j = tid; j < i; j += 64) {
+      double* rowj = A + (size_t)s_perm[j] * ld;
+      rowj[P] = fma(-rowj[i], xi, rowj[P]);
     }
-    __syncthreads();
-    const double xi = s_xi;
-    for (int j = tid; j < i; j += nthr)
-      A[(size_t)j * ld + P] = fma(-A[(size_t)j * ld + i], xi, A[(size_t)j * ld + P]);
-    __syncthreads();
+    __syncthreads();  // only the first wave is left: orders its LDS traffic
   }
 
   double* xt = trial_x + (size_t)t * P;
-  for (int j = tid; j < P; j += nthr) xt[j] = log_x[j] + A[(size_t)j * ld + P];
+  for (int j = tid; j < P; j += 64) xt[j] = log_x[j] + s_x[j];
   __syncthreads();
   double* tab = tables + (size_t)t * nl * NCOL;
-  for (int r = tid; r < nl * NCOL; r += nthr) tab[r] = lm_table_entry(ref, nl, xt, r / NCOL, r % NCOL);
+  for (int r = tid; r < nl * NCOL; r += 64) tab[r] = lm_table_entry(ref, nl, xt, r / NCOL, r % NCOL);
 }
 
 // chi[t, p] = sum_w |R_t(p, w) - R_obs(p, w)|^2 for trial table t: the lane
@@ -232,16 +289,16 @@ __global__ void __launch_bounds__(LM_BLOCK) lm_trials_kernel(
 constexpr int LM_MISFIT_BLOCK = 256;
 
 __global__ void __launch_bounds__(LM_MISFIT_BLOCK) lm_trial_misfit_kernel(
-    const double* __restrict__ tables, int nl, const double* __restrict__ p, int np,
+    const double* __restrict__ tables, int nl, int first, const double* __restrict__ p, int np,
     const cd* __restrict__ omega, int nw, int free_surface, const cd* __restrict__ R_obs,
     const double* __restrict__ state, double* __restrict__ chi) {
   extern __shared__ double smem_d[];
   __shared__ double s_red[LM_MISFIT_BLOCK];
-  if (state[LM_STATUS] != 0.0) return;
+  if (!lm_trials_open(state, first)) return;
   LayerPt* s_layer = reinterpret_cast<LayerPt*>(smem_d);
   Iface* s_if = reinterpret_cast<Iface*>(s_layer + nl);
-  const int t = blockIdx.x / np;
-  const int ip = blockIdx.x - t * np;
+  const int t = first + (int)(blockIdx.x / np);
+  const int ip = (int)(blockIdx.x % np);
   stage_slowness(tables + (size_t)t * nl * NCOL, nl, p[ip], s_layer, s_if);
   const cd* obs = R_obs + (size_t)ip * nw;
   double acc = 0.0;
@@ -258,25 +315,31 @@ __global__ void __launch_bounds__(LM_MISFIT_BLOCK) lm_trial_misfit_kernel(
     if ((int)threadIdx.x < o) s_red[threadIdx.x] += s_red[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) chi[blockIdx.x] = s_red[0];
+  if (threadIdx.x == 0) chi[(size_t)t * np + ip] = s_red[0];
 }
 
-// One workgroup: the first trial whose misfit (rows summed in ascending
-// slowness order) is below the current misfit becomes the iterate.
-__global__ void __launch_bounds__(256) lm_decide_kernel(int P, int ntrials, int np,
+// One workgroup: among trials first..first+ntrials-1, the first whose
+// misfit (rows summed in ascending slowness order) is below the current
+// misfit becomes the iterate, with lambda 5^t / 2 (floor 1e-15) from the
+// iteration's lambda.  If none is and `last` is set (the schedule ends with
+// this batch), lambda 5^(first+ntrials) and status 2; otherwise the state
+// is left for the next batch.
+__global__ void __launch_bounds__(256) lm_decide_kernel(int P, int first, int ntrials, int last,
+                                                        int np,
                                                         const double* __restrict__ chi_trials,
                                                         const double* __restrict__ trial_x,
                                                         double* __restrict__ state,
                                                         double* __restrict__ log_x) {
   __shared__ int s_acc;
-  const bool run = state[LM_STATUS] == 0.0;
-  __syncthreads();
+  const bool run = lm_trials_open(state, first);
+  __syncthreads();  // every wave has read the state before thread 0 may change it
   if (!run) return;
   if (threadIdx.x == 0) {
     const double chi = state[LM_CHI];
     double lam = state[LM_LAM];
+    for (int k = 0; k < first; ++k) lam *= 5.0;
     int acc = -1;
-    for (int t = 0; t < ntrials; ++t) {
+    for (int t = first; t < first + ntrials; ++t) {
       double s = 0.0;
       for (int q = 0; q < np; ++q) s += chi_trials[(size_t)t * np + q];
       if (s < chi) {
@@ -285,10 +348,13 @@ __global__ void __launch_bounds__(256) lm_decide_kernel(int P, int ntrials, int 
       }
       lam *= 5.0;
     }
-    if (acc >= 0) lam = fmax(lam / 2.0, 1e-15);
-    else state[LM_STATUS] = 2.0;
-    state[LM_LAM] = lam;
-    state[LM_TRIAL] = (double)acc;
+    if (acc >= 0) {
+      state[LM_LAM] = fmax(lam / 2.0, 1e-15);
+      state[LM_TRIAL] = (double)acc;
+    } else if (last) {
+      state[LM_LAM] = lam;
+      state[LM_STATUS] = 2.0;
+    }
     s_acc = acc;
   }
   __syncthreads();
@@ -321,6 +387,21 @@ extern "C" int kh_lm_tables(const double* ref_layers, int nl, const double* x, i
   return check_launch("kh_lm_tables");
 }
 
+extern "C" int kh_lm_assemble(int nl, int np, const double* rows, const double* grad,
+                              const double* G, const long long* order, double* g, double* H,
+                              void* stream) {
+  clear_error();
+  KH_TRY(check_lm_layers("kh_lm_assemble", nl));
+  KH_REQUIRE(np >= 1, KH_ERR_INVALID_ARGUMENT, "kh_lm_assemble: np %d", np);
+  KH_REQUIRE(rows && grad && order && g && H, KH_ERR_INVALID_ARGUMENT,
+             "kh_lm_assemble: NULL pointer");
+  const int P = lm_params(nl);
+  const long n = (long)P * P + P;
+  hipLaunchKernelGGL(lm_assemble_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     as_stream(stream), nl, np, rows, grad, G, order, g, H);
+  return check_launch("kh_lm_assemble");
+}
+
 extern "C" int kh_lm_prepare(int nl, int np, const double* chi_rows, int chi_stride,
                              const double* g, const double* H, const double* log_x,
                              const double* x_true, double tol, int max_hist, double* state,
@@ -345,13 +426,14 @@ extern "C" size_t kh_lm_trials_workspace_bytes(int nl, int ntrials) {
   return (size_t)ntrials * P * (P + 1) * sizeof(double);
 }
 
-extern "C" int kh_lm_trials(int nl, int ntrials, const double* H_log, const double* g_log,
-                            const double* log_x, const double* ref_layers, const double* state,
-                            double* trial_x, double* trial_tables, void* workspace,
-                            size_t workspace_bytes, void* stream) {
+extern "C" int kh_lm_trials(int nl, int first, int ntrials, const double* H_log,
+                            const double* g_log, const double* log_x, const double* ref_layers,
+                            const double* state, double* trial_x, double* trial_tables,
+                            void* workspace, size_t workspace_bytes, void* stream) {
   clear_error();
   KH_TRY(check_lm_layers("kh_lm_trials", nl));
-  KH_REQUIRE(ntrials >= 0, KH_ERR_INVALID_ARGUMENT, "kh_lm_trials: ntrials < 0");
+  KH_REQUIRE(first >= 0 && ntrials >= 0, KH_ERR_INVALID_ARGUMENT,
+             "kh_lm_trials: first %d, ntrials %d", first, ntrials);
   if (ntrials == 0) return KH_OK;
   KH_REQUIRE(H_log && g_log && log_x && ref_layers && state && trial_x && trial_tables,
              KH_ERR_INVALID_ARGUMENT, "kh_lm_trials: NULL pointer");
@@ -360,26 +442,26 @@ extern "C" int kh_lm_trials(int nl, int ntrials, const double* H_log, const doub
   if (P <= LM_LDS_MAX_P) {
     const size_t lds = (size_t)P * (P + 1) * sizeof(double);
     hipLaunchKernelGGL(lm_trials_kernel<true>, dim3((unsigned)ntrials), dim3(LM_BLOCK), lds, st,
-                       nl, H_log, g_log, log_x, ref_layers, state, trial_x, trial_tables,
+                       nl, first, H_log, g_log, log_x, ref_layers, state, trial_x, trial_tables,
                        static_cast<double*>(nullptr));
   } else {
     const size_t need = kh_lm_trials_workspace_bytes(nl, ntrials);
     KH_REQUIRE(workspace && workspace_bytes >= need, KH_ERR_INVALID_ARGUMENT,
                "kh_lm_trials: workspace of %zu bytes needed, %zu given", need, workspace_bytes);
     hipLaunchKernelGGL(lm_trials_kernel<false>, dim3((unsigned)ntrials), dim3(LM_BLOCK), 0, st,
-                       nl, H_log, g_log, log_x, ref_layers, state, trial_x, trial_tables,
+                       nl, first, H_log, g_log, log_x, ref_layers, state, trial_x, trial_tables,
                        static_cast<double*>(workspace));
   }
   return check_launch("kh_lm_trials");
 }
 
-extern "C" int kh_lm_trial_misfits(const double* trial_tables, int nl, int ntrials,
+extern "C" int kh_lm_trial_misfits(const double* trial_tables, int nl, int first, int ntrials,
                                    const double* p, int np, const double* omega, int nw,
                                    int free_surface, const double* R_obs, const double* state,
                                    double* chi_trials, void* stream) {
   clear_error();
   KH_TRY(check_lm_layers("kh_lm_trial_misfits", nl));
-  KH_REQUIRE(ntrials >= 0 && np >= 0 && nw >= 0, KH_ERR_INVALID_ARGUMENT,
+  KH_REQUIRE(first >= 0 && ntrials >= 0 && np >= 0 && nw >= 0, KH_ERR_INVALID_ARGUMENT,
              "kh_lm_trial_misfits: negative size");
   if (ntrials == 0 || np == 0) return KH_OK;
   KH_REQUIRE(trial_tables && p && state && chi_trials && (nw == 0 || (omega && R_obs)),
@@ -388,20 +470,22 @@ extern "C" int kh_lm_trial_misfits(const double* trial_tables, int nl, int ntria
   KH_REQUIRE(nblocks <= 0x7fffffffL, KH_ERR_UNSUPPORTED, "kh_lm_trial_misfits: grid too large");
   const size_t lds = (size_t)nl * sizeof(LayerPt) + (size_t)(nl - 1) * sizeof(Iface);
   hipLaunchKernelGGL(lm_trial_misfit_kernel, dim3((unsigned)nblocks), dim3(LM_MISFIT_BLOCK), lds,
-                     as_stream(stream), trial_tables, nl, p, np,
+                     as_stream(stream), trial_tables, nl, first, p, np,
                      reinterpret_cast<const cd*>(omega), nw, free_surface,
                      reinterpret_cast<const cd*>(R_obs), state, chi_trials);
   return check_launch("kh_lm_trial_misfits");
 }
 
-extern "C" int kh_lm_decide(int nl, int ntrials, int np, const double* chi_trials,
-                            const double* trial_x, double* state, double* log_x, void* stream) {
+extern "C" int kh_lm_decide(int nl, int first, int ntrials, int last, int np,
+                            const double* chi_trials, const double* trial_x, double* state,
+                            double* log_x, void* stream) {
   clear_error();
   KH_TRY(check_lm_layers("kh_lm_decide", nl));
-  KH_REQUIRE(ntrials >= 0 && np >= 1, KH_ERR_INVALID_ARGUMENT, "kh_lm_decide: bad size");
+  KH_REQUIRE(first >= 0 && ntrials >= 0 && np >= 1, KH_ERR_INVALID_ARGUMENT,
+             "kh_lm_decide: bad size");
   KH_REQUIRE(state && log_x && (ntrials == 0 || (chi_trials && trial_x)), KH_ERR_INVALID_ARGUMENT,
              "kh_lm_decide: NULL pointer");
   hipLaunchKernelGGL(lm_decide_kernel, dim3(1), dim3(256), 0, as_stream(stream), lm_params(nl),
-                     ntrials, np, chi_trials, trial_x, state, log_x);
+                     first, ntrials, last, np, chi_trials, trial_x, state, log_x);
   return check_launch("kh_lm_decide");
 }

@@ -25,6 +25,7 @@ __all__ = [
     "interfaces_direct",
     "residual_norms",
     "gram",
+    "lm_assemble",
     "gmm_forward",
     "gmm_forward_grid",
 ]
@@ -267,6 +268,24 @@ def residual_norms(R: torch.Tensor, R_obs: torch.Tensor,
                                       _abi.stream_handle(R.device))
     _abi.check(rc, "kh_residual_norms")
     return out
+
+
+def lm_assemble(nl: int, rows: torch.Tensor, grad: torch.Tensor, G: torch.Tensor | None,
+                order: torch.Tensor):
+    """(g (P,), H (P, P)) in the packed parameters, summed over slownesses, from
+    kh_misfit_hvp_partial's outputs over a basis (rows (P, np, nl, 6), grad
+    (np, nl, 6); direction order[k] moves packed parameter k) plus the
+    Gauss-Newton block G over the directions when given (kh_lm_assemble)."""
+    dev = rows.device
+    P, n_p = rows.shape[0], rows.shape[1]
+    g = torch.empty((P,), dtype=torch.float64, device=dev)
+    H = torch.empty((P, P), dtype=torch.float64, device=dev)
+    order = order.to(device=dev, dtype=torch.int64).contiguous()
+    rc = _abi.lib().kh_lm_assemble(nl, n_p, _abi.ptr(rows.contiguous()), _abi.ptr(grad),
+                                   _abi.ptr(G) if G is not None else None, _abi.ptr(order),
+                                   _abi.ptr(g), _abi.ptr(H), _abi.stream_handle(dev))
+    _abi.check(rc, "kh_lm_assemble")
+    return g, H
 
 
 def gram(X: torch.Tensor, scale: float = 1.0, out: torch.Tensor | None = None) -> torch.Tensor:

@@ -134,8 +134,6 @@ class _Problem:
         self.free_surface = free_surface
         idx = packed_index(self.nl)
         self.P = len(idx)
-        self.li = torch.tensor([l for l, _ in idx], device=self.dev)
-        self.ci = torch.tensor([c for _, c in idx], device=self.dev)
         # the basis directions deepest layer first: with partial reverse
         # sweeps their cost falls with the layer, so the launch starts its
         # longest workgroups first (the first one, whose sweep also gives the
@@ -146,7 +144,8 @@ class _Problem:
             l, c = idx[kk]
             basis[k, l, c] = 1.0
         self.basis = basis.to(self.dev)
-        self.unorder = torch.tensor(np.argsort(order), device=self.dev)  # packed k -> row
+        # packed parameter k -> its basis row
+        self.unorder = torch.tensor(np.argsort(order), dtype=torch.int64, device=self.dev)
         self.use_graphs = use_graphs
         self._graphs: dict[str, _Graphed] = {}
         self._eager_once: set[str] = set()
@@ -180,25 +179,23 @@ class _Problem:
 
     def _grad_hessian_dev(self, layers: torch.Tensor, mode: str):
         # rows of the basis directions down to one layer below each
-        # direction's own (the reverse sweep stops there); the rest of the
-        # symmetric matrix comes from the deeper layers' rows
+        # direction's own (the reverse sweep stops there); kh_lm_assemble
+        # sums them over slowness and takes the rest of the symmetric matrix
+        # from the deeper layers' rows
         if mode == "hvp":
-            chi, grad, hvp, _ = derivatives.misfit_hvp(layers, self.basis, self.p, self.omega,
-                                                       self.R_obs, self.free_surface,
-                                                       partial=True)
-            rows = hvp.sum(1)[self.unorder][:, self.li, self.ci]
-            H = derivatives.symmetric_from_partial(rows, self.li)
+            chi, grad, rows, _ = derivatives.misfit_hvp(layers, self.basis, self.p, self.omega,
+                                                        self.R_obs, self.free_surface,
+                                                        partial=True)
+            G = None
         elif mode == "gram":
-            chi, grad, S, dR = derivatives.misfit_hvp(layers, self.basis, self.p, self.omega,
-                                                      self.R_obs, self.free_surface,
-                                                      second_only=True, want_dR=True,
-                                                      partial=True)
-            X = torch.view_as_real(dR[self.unorder]).reshape(self.P, -1)
-            H = engine.gram(X, 2.0) + derivatives.symmetric_from_partial(
-                S.sum(1)[self.unorder][:, self.li, self.ci], self.li)
+            chi, grad, rows, dR = derivatives.misfit_hvp(layers, self.basis, self.p, self.omega,
+                                                         self.R_obs, self.free_surface,
+                                                         second_only=True, want_dR=True,
+                                                         partial=True)
+            G = engine.gram(torch.view_as_real(dR).reshape(self.P, -1), 2.0)
         else:
             raise ValueError(f"unknown hessian mode {mode!r}")
-        g = grad.sum(0)[self.li, self.ci]
+        g, H = engine.lm_assemble(self.nl, rows, grad, G, self.unorder)
         return chi, g, H
 
     def grad_hessian(self, phys: np.ndarray, mode: str = "gram"):
@@ -215,11 +212,13 @@ class _DeviceLoop:
     One iteration is a fixed sequence of launches on the current stream,
     with no host round trip:
       kh_lm_tables (the iterate's layer table) -> the misfit / gradient /
-      exact Hessian evaluation -> kh_lm_prepare (log transform, history,
-      first damping, convergence tests) -> kh_lm_trials (all ``max_trials``
-      trials of the damping schedule, one LU solve per workgroup) ->
-      kh_lm_trial_misfits -> kh_lm_decide (the first trial below the current
-      misfit is accepted, as the sequential loop would).
+      exact Hessian evaluation (kh_lm_assemble sums it) -> kh_lm_prepare
+      (log transform, history, first damping, convergence tests) ->
+      kh_lm_trials, kh_lm_trial_misfits, kh_lm_decide for trial 0 of the
+      damping schedule -> the same three for trials 1..max_trials-1 at once
+      (one LU solve per workgroup), which return at once when trial 0 was
+      accepted.  The first trial below the current misfit is accepted, as
+      the sequential loop would.
     The loop state, history and buffers are static tensors, so with
     ``use_graphs`` the second iteration is captured as one HIP graph and
     replayed for the rest (the first runs eagerly and settles every
@@ -263,19 +262,24 @@ class _DeviceLoop:
                                    ptr(H), ptr(self.log_x), ptr(self.x_true), self.tol,
                                    self.max_iter, ptr(self.state), ptr(self.g_log),
                                    ptr(self.H_log), ptr(self.hist), s), "kh_lm_prepare")
-        if self.ntr > 0:
-            _abi.check(L.kh_lm_trials(self.nl, self.ntr, ptr(self.H_log), ptr(self.g_log),
-                                      ptr(self.log_x), ptr(self.ref), ptr(self.state),
-                                      ptr(self.trial_x), ptr(self.trial_tab), ptr(self.ws),
-                                      self.ws.numel(), s), "kh_lm_trials")
-            _abi.check(L.kh_lm_trial_misfits(ptr(self.trial_tab), self.nl, self.ntr, ptr(prob.p),
-                                             self.np, ptr(prob.omega), prob.omega.numel(),
-                                             int(bool(prob.free_surface)), ptr(prob.R_obs),
-                                             ptr(self.state), ptr(self.trial_chi), s),
-                       "kh_lm_trial_misfits")
-        _abi.check(L.kh_lm_decide(self.nl, self.ntr, self.np, ptr(self.trial_chi),
-                                  ptr(self.trial_x), ptr(self.state), ptr(self.log_x), s),
-                   "kh_lm_decide")
+        # trial 0 first; the rest of the schedule only when it was rejected
+        # (their kernels return at once otherwise)
+        batches = [(0, 1), (1, self.ntr - 1)] if self.ntr > 1 else [(0, self.ntr)]
+        for first, n in batches:
+            last = int(first + n == self.ntr)
+            if n > 0:
+                _abi.check(L.kh_lm_trials(self.nl, first, n, ptr(self.H_log), ptr(self.g_log),
+                                          ptr(self.log_x), ptr(self.ref), ptr(self.state),
+                                          ptr(self.trial_x), ptr(self.trial_tab), ptr(self.ws),
+                                          self.ws.numel(), s), "kh_lm_trials")
+                _abi.check(L.kh_lm_trial_misfits(ptr(self.trial_tab), self.nl, first, n,
+                                                 ptr(prob.p), self.np, ptr(prob.omega),
+                                                 prob.omega.numel(), int(bool(prob.free_surface)),
+                                                 ptr(prob.R_obs), ptr(self.state),
+                                                 ptr(self.trial_chi), s), "kh_lm_trial_misfits")
+            _abi.check(L.kh_lm_decide(self.nl, first, n, last, self.np, ptr(self.trial_chi),
+                                      ptr(self.trial_x), ptr(self.state), ptr(self.log_x), s),
+                       "kh_lm_decide")
 
     def run(self, use_graphs: bool = False):
         """Run up to max_iter iterations; returns (history (n, 3), final log_x,

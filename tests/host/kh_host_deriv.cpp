@@ -197,3 +197,33 @@ extern "C" int host_misfit_hvp_mode(const double* layers, const double* dlayers,
   }
   return 0;
 }
+
+// Every (layer, column) of chain_column<cd> directly and from the shared
+// first-order factors (local_jac, then chain_from_jac), for the cotangents
+// cif ((nl - 1) interfaces x 10 complex) and cly (nl layers x 3 complex);
+// outputs (nl, 6) complex each.
+extern "C" int host_chain_factored(const double* layers, int nl, double p, const double* cif,
+                                   const double* cly, double* direct, double* factored) {
+  const cd* ci = reinterpret_cast<const cd*>(cif);
+  const cd* cl = reinterpret_cast<const cd*>(cly);
+  auto iface = [&](int i) {
+    const cd* c = ci + (size_t)i * 10;
+    return IfaceZ<cd>{c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7], c[8], c[9]};
+  };
+  const cd z{0.0, 0.0};
+  const IfaceZ<cd> none{z, z, z, z, z, z, z, z, z, z};
+  for (int l = 0; l < nl; ++l) {
+    const IfaceZ<cd> up = l >= 1 ? iface(l - 1) : none, dn = l <= nl - 2 ? iface(l) : none;
+    const LayerCot<cd> lc{cl[3 * l], cl[3 * l + 1], cl[3 * l + 2]};
+    for (int c = 0; c < NCOL; ++c) {
+      const cd a = chain_column<cd>(layers, nullptr, nl, l, p, up, dn, lc, c);
+      const cd b = chain_from_jac(c == COL_H ? LocalJac{} : local_jac(layers, nl, l, p, c), l, nl,
+                                  up, dn, lc, c);
+      direct[2 * (l * NCOL + c)] = a.r;
+      direct[2 * (l * NCOL + c) + 1] = a.i;
+      factored[2 * (l * NCOL + c)] = b.r;
+      factored[2 * (l * NCOL + c) + 1] = b.i;
+    }
+  }
+  return 0;
+}

@@ -156,3 +156,24 @@ def test_flat_forward_and_partial_reverse(lib, fs, second_only):
     valid = li[None, :] <= li[:, None] + 1
     H = np.where(valid, M, M.T)
     assert np.max(np.abs(H - full)) <= 1e-12 * np.max(np.abs(full))
+
+
+@pytest.mark.parametrize("model,p", [(default_ocean_crust_model(), 0.3), (seeded_model(12, 3), 0.55),
+                                     (seeded_model(27, 11), 0.08)])
+def test_far_chain_from_shared_factors_equals_direct_chain(lib, model, p):
+    """The far pairs of the partial Hessian contract one set of first-order
+    factors per (slowness, layer, column) with each direction's cotangents
+    (local_jac, chain_from_jac); that is the per-direction chain rule
+    (chain_column) regrouped, bit for bit."""
+    tab = np.ascontiguousarray(pack_layers(model))
+    nl = tab.shape[0]
+    rng = np.random.default_rng(nl)
+    cif = rng.standard_normal((max(nl - 1, 1), 10, 2))
+    cly = rng.standard_normal((nl, 3, 2))
+    direct = np.zeros((nl, 6, 2))
+    fact = np.zeros((nl, 6, 2))
+    lib.host_chain_factored.argtypes = [vp, ctypes.c_int, ctypes.c_double, vp, vp, vp, vp]
+    lib.host_chain_factored(tab.ctypes.data, nl, p, cif.ctypes.data, cly.ctypes.data,
+                            direct.ctypes.data, fact.ctypes.data)
+    assert np.all(np.isfinite(direct)) and np.any(direct != 0)
+    np.testing.assert_array_equal(fact, direct)

@@ -41,7 +41,9 @@ def ptr(t):
     return _abi.ptr(t)
 
 
-@pytest.mark.parametrize("nl", [20, 27])  # P = 75 in LDS, P = 103 in the workspace
+# P = 75: one wave, LDS; 91: one wave, workspace; 103 and 475 (the layer cap):
+# a 512-wide workgroup, workspace
+@pytest.mark.parametrize("nl", [20, 24, 27, 120])
 def test_trial_solves_match_numpy(nl):
     """Every trial t solves (H + lambda 5^t I) dx = -g as np.linalg.solve does,
     on an indefinite symmetric H whose pivots need row exchanges."""
@@ -63,8 +65,10 @@ def test_trial_solves_match_numpy(nl):
     ref = prob.table(xt)
     Hd, gd, xd, st = (torch.tensor(H, **F), torch.tensor(g, **F), torch.tensor(log_x, **F),
                       state(lam))  # held until the kernel has read them
-    call("kh_lm_trials", nl, ntr, ptr(Hd), ptr(gd), ptr(xd), ptr(ref), ptr(st), ptr(tx), ptr(tab),
-         ptr(ws), ws.numel())
+    call("kh_lm_trials", nl, 0, 1, ptr(Hd), ptr(gd), ptr(xd), ptr(ref), ptr(st), ptr(tx),
+         ptr(tab), ptr(ws), ws.numel())  # trial 0, then the rest of the schedule
+    call("kh_lm_trials", nl, 1, ntr - 1, ptr(Hd), ptr(gd), ptr(xd), ptr(ref), ptr(st), ptr(tx),
+         ptr(tab), ptr(ws), ws.numel())
     tx, tab = tx.cpu().numpy(), tab.cpu().numpy()
     lt = lam
     for t in range(ntr):
@@ -77,6 +81,32 @@ def test_trial_solves_match_numpy(nl):
         host = prob.table(np.exp(tx[t])).cpu().numpy()
         np.testing.assert_allclose(tab[t], host, rtol=5e-16, atol=0)
         lt *= 5.0
+
+
+def test_assemble_matches_the_torch_formulation():
+    """kh_lm_assemble against the sums, reorder and mirror written in torch:
+    g = sum_p grad[p, l_k, c_k]; H = G[order][:, order] + mirror(rows)."""
+    from seismicinversion_amd import derivatives
+    from seismicinversion_amd.kennett_torch import packed_index
+
+    nl, n_p = 9, 7
+    idx = packed_index(nl)
+    P = len(idx)
+    li = torch.tensor([l for l, _ in idx], device="cuda")
+    ci = torch.tensor([c for _, c in idx], device="cuda")
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    rows = torch.randn((P, n_p, nl, 6), generator=gen, **F)
+    grad = torch.randn((n_p, nl, 6), generator=gen, **F)
+    G = torch.randn((P, P), generator=gen, **F)
+    order = torch.randperm(P, device="cuda")
+    for with_gram in (True, False):
+        g, H = engine.lm_assemble(nl, rows, grad, G if with_gram else None, order)
+        r = rows.sum(1)[order][:, li, ci]
+        expect = derivatives.symmetric_from_partial(r, li)
+        if with_gram:
+            expect = G[order][:, order] + expect
+        torch.testing.assert_close(H, expect, rtol=1e-14, atol=1e-14)
+        torch.testing.assert_close(g, grad.sum(0)[li, ci], rtol=1e-14, atol=1e-14)
 
 
 def test_tables_match_host_builder():
@@ -99,7 +129,7 @@ def test_trial_misfits_equal_forward_and_residual_norms():
     tabs = torch.stack([prob.table(xt * rng.uniform(0.97, 1.03, xt.size)) for _ in range(4)])
     chi = torch.empty((4, 9), **F)
     st = state()
-    call("kh_lm_trial_misfits", ptr(tabs), prob.nl, 4, ptr(prob.p), 9, ptr(prob.omega),
+    call("kh_lm_trial_misfits", ptr(tabs), prob.nl, 0, 4, ptr(prob.p), 9, ptr(prob.omega),
          prob.omega.numel(), 0, ptr(prob.R_obs), ptr(st), ptr(chi))
     for t in range(4):
         R = engine.forward(tabs[t], prob.p, prob.omega)
@@ -157,22 +187,40 @@ def test_decide_accepts_the_first_improving_trial():
     chi[0, 0] = np.nan  # a non-finite trial is rejected
     chi[2] = [0.5, 0.5, 0.99]  # 1.99 < 2.0: the first accepted
     chi[3] = [0.1, 0.1, 0.1]
+    chid = torch.tensor(chi, **F)
+    for batches in ([(0, ntr)], [(0, 1), (1, ntr - 1)]):  # all at once, or trial 0 first
+        log_x = torch.zeros(P, **F)
+        st = state(lam=0.3, chi=2.0)
+        for first, n in batches:
+            call("kh_lm_decide", 4, first, n, int(first + n == ntr), n_p, ptr(chid), ptr(tx),
+                 ptr(st), ptr(log_x))
+        s = st.cpu().numpy()
+        assert torch.equal(log_x, tx[2]) and s[5] == 2 and s[4] == 0
+        assert s[0] == max(0.3 * 5.0 * 5.0 / 2.0, 1e-15)
+        # none below: status 2 after the last batch, lambda x 5 per trial, iterate unchanged
+        st = state(lam=0.3, chi=0.01)
+        before = log_x.clone()
+        for first, n in batches:
+            call("kh_lm_decide", 4, first, n, int(first + n == ntr), n_p, ptr(chid), ptr(tx),
+                 ptr(st), ptr(log_x))
+            if first + n < ntr:  # not the last batch: nothing decided yet
+                assert np.array_equal(st.cpu().numpy(), state(lam=0.3, chi=0.01).cpu().numpy())
+        s = st.cpu().numpy()
+        lam = 0.3
+        for _ in range(ntr):
+            lam *= 5.0
+        assert s[4] == 2 and s[5] == -1 and s[0] == lam and torch.equal(log_x, before)
+    # trial 0 accepted: the later batch changes nothing
+    chi0 = chi.copy()
+    chi0[0] = [0.1, 0.1, 0.1]
+    chi0d = torch.tensor(chi0, **F)
     log_x = torch.zeros(P, **F)
     st = state(lam=0.3, chi=2.0)
-    chid = torch.tensor(chi, **F)
-    call("kh_lm_decide", 4, ntr, n_p, ptr(chid), ptr(tx), ptr(st), ptr(log_x))
-    s = st.cpu().numpy()
-    assert torch.equal(log_x, tx[2]) and s[5] == 2 and s[4] == 0
-    assert s[0] == max(0.3 * 5.0 * 5.0 / 2.0, 1e-15)
-    # none below: status 2, lambda x 5 per trial, iterate unchanged
-    st = state(lam=0.3, chi=0.01)
-    before = log_x.clone()
-    call("kh_lm_decide", 4, ntr, n_p, ptr(chid), ptr(tx), ptr(st), ptr(log_x))
-    s = st.cpu().numpy()
-    lam = 0.3
-    for _ in range(ntr):
-        lam *= 5.0
-    assert s[4] == 2 and s[5] == -1 and s[0] == lam and torch.equal(log_x, before)
+    call("kh_lm_decide", 4, 0, 1, 0, n_p, ptr(chi0d), ptr(tx), ptr(st), ptr(log_x))
+    after0 = st.cpu().numpy().copy()
+    call("kh_lm_decide", 4, 1, ntr - 1, 1, n_p, ptr(chi0d), ptr(tx), ptr(st), ptr(log_x))
+    assert np.array_equal(st.cpu().numpy(), after0) and after0[5] == 0
+    assert torch.equal(log_x, tx[0]) and after0[0] == 0.15
 
 
 @pytest.mark.parametrize("mode", ["gram", "hvp"])

@@ -120,7 +120,8 @@ def test_scattering_matrices_vs_golden():
                 assert np.max(np.abs(M - ref_m)) <= 1e-11 * np.max(np.abs(ref_m))
 
 
-@pytest.mark.parametrize("P,K", [(1, 7), (15, 64), (17, 4099), (75, 131072), (33, 10000)])
+@pytest.mark.parametrize("P,K", [(1, 7), (15, 64), (17, 4099), (75, 131072), (33, 10000),
+                                 (63, 8192), (130, 40), (395, 8192)])
 def test_mfma_gram_matches_matmul(P, K):
     rng = np.random.default_rng(P * 1000 + K)
     X = rng.standard_normal((P, K))
