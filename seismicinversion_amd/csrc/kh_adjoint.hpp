@@ -16,6 +16,8 @@
 // contributions are folded into Td's).
 #pragma once
 
+#include <type_traits>
+
 #include "kh_physics_t.hpp"
 
 namespace kh {
@@ -28,16 +30,30 @@ struct M2 {
 // Intermediates of one step (shared by the forward step and its adjoint so
 // that the reverse pass recomputes exactly the forward values).  Sums of
 // products are fused multiply-add chains, as in kennett_step.
-template <class Z>
+//
+// The phases (type EZ) and the coefficients (IfaceZ<FZ>) may be plain cd
+// while the state is dual (Z = Dual<cd>): at a level whose layers the
+// direction does not move they have no tangent, and the mixed operations
+// (kh_dual.hpp) give the full dual step's numbers bit for bit with a third
+// fewer products in every coefficient and phase product.
+template <class Z, class EZ = Z>
 struct StepVals {
-  Z eaea, ebeb, eaeb, m00, m01, m10, m11, a00, a01, a10, a11, rdet, u00, u01, u10, u11;
+  EZ eaea, ebeb, eaeb;
+  Z m00, m01, m10, m11, a00, a01, a10, a11, rdet, u00, u01, u10, u11;
   Z x00, x01, x10, x11, y00, y01, y10, y11;
 };
 
-template <class Z>
-__host__ __device__ __forceinline__ StepVals<Z> step_vals(const M2<Z>& r, const Z& ea,
-                                                          const Z& eb, const IfaceZ<Z>& f) {
-  StepVals<Z> v;
+// A coefficient as the state's type (zero tangent when lifted).
+template <class Z, class X>
+__host__ __device__ __forceinline__ Z as_z(const X& x) {
+  if constexpr (std::is_same<X, Z>::value) return x;
+  else return lift_c<Z>(x);
+}
+
+template <class Z, class EZ, class FZ>
+__host__ __device__ __forceinline__ StepVals<Z, EZ> step_vals(const M2<Z>& r, const EZ& ea,
+                                                              const EZ& eb, const IfaceZ<FZ>& f) {
+  StepVals<Z, EZ> v;
   v.eaea = ea * ea;
   v.ebeb = eb * eb;
   v.eaeb = ea * eb;
@@ -66,26 +82,26 @@ __host__ __device__ __forceinline__ StepVals<Z> step_vals(const M2<Z>& r, const 
   return v;
 }
 
-template <class Z>
-__host__ __device__ __forceinline__ M2<Z> step_fwd(const M2<Z>& r, const Z& ea, const Z& eb,
-                                                   const IfaceZ<Z>& f) {
-  const StepVals<Z> v = step_vals(r, ea, eb, f);
+template <class Z, class EZ, class FZ>
+__host__ __device__ __forceinline__ M2<Z> step_fwd(const M2<Z>& r, const EZ& ea, const EZ& eb,
+                                                   const IfaceZ<FZ>& f) {
+  const StepVals<Z, EZ> v = step_vals(r, ea, eb, f);
   M2<Z> s;
-  s.a00 = cfma(v.y01, f.td10, cfma(v.y00, f.td00, f.rd00));
-  s.a01 = cfma(v.y01, f.td11, cfma(v.y00, f.td01, f.rd01));
-  s.a10 = cfma(v.y11, f.td10, cfma(v.y10, f.td00, f.rd01));
-  s.a11 = cfma(v.y11, f.td11, cfma(v.y10, f.td01, f.rd11));
+  s.a00 = cfma(v.y01, f.td10, cfma(v.y00, f.td00, as_z<Z>(f.rd00)));
+  s.a01 = cfma(v.y01, f.td11, cfma(v.y00, f.td01, as_z<Z>(f.rd01)));
+  s.a10 = cfma(v.y11, f.td10, cfma(v.y10, f.td00, as_z<Z>(f.rd01)));
+  s.a11 = cfma(v.y11, f.td11, cfma(v.y10, f.td01, as_z<Z>(f.rd11)));
   return s;
 }
 
 // Adjoint of step_fwd.  Given r = S_{i+1}, the phases, f and sb = S_i_bar:
 // returns rb = S_{i+1}_bar, adds the coefficient cotangents into fb and
 // returns the phase cotangents eab, ebb.
-template <class Z>
-__host__ __device__ __forceinline__ void step_rev(const M2<Z>& r, const Z& ea, const Z& eb,
-                                                  const IfaceZ<Z>& f, const M2<Z>& sb,
+template <class Z, class EZ, class FZ>
+__host__ __device__ __forceinline__ void step_rev(const M2<Z>& r, const EZ& ea, const EZ& eb,
+                                                  const IfaceZ<FZ>& f, const M2<Z>& sb,
                                                   M2<Z>& rb, IfaceZ<Z>& fb, Z& eab, Z& ebb) {
-  const StepVals<Z> v = step_vals(r, ea, eb, f);  // recompute the forward intermediates
+  const StepVals<Z, EZ> v = step_vals(r, ea, eb, f);  // recompute the forward intermediates
 
   // S = Rd + Y Td
   fb.rd00 = fb.rd00 + sb.a00;
@@ -156,13 +172,15 @@ __host__ __device__ __forceinline__ void step_rev(const M2<Z>& r, const Z& ea, c
 
 // Cotangents of eta (complex) and h (real, returned in .r with 0 imaginary
 // parts at every dual level) of ea = exp(((i eta) omega) h), given ea_bar.
-template <class Z>
-__host__ __device__ __forceinline__ void phase_rev(const Z& eta, cd omega, const Z& h, const Z& ea,
-                                                   const Z& eab, Z& etab, Z& hb) {
+// eta, h and ea may be plain cd (a layer without tangent) while the
+// cotangents are dual.
+template <class Z, class EZ>
+__host__ __device__ __forceinline__ void phase_rev(const EZ& eta, cd omega, const EZ& h,
+                                                   const EZ& ea, const Z& eab, Z& etab, Z& hb) {
   const Z zb = conj(ea) * eab;  // z = ((i eta) omega) h, ea = exp(z)
-  const Z iw = lift_c<Z>(mul_i(omega));
+  const EZ iw = lift_c<EZ>(mul_i(omega));
   etab = etab + conj(iw * h) * zb;
-  hb = hb + re_part(conj(mul_i(eta) * lift_c<Z>(omega)) * zb);
+  hb = hb + re_part(conj(mul_i(eta) * lift_c<EZ>(omega)) * zb);
 }
 
 }  // namespace kh

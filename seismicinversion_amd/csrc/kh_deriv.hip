@@ -176,6 +176,17 @@ __device__ __forceinline__ int deepest_moved_layer(const double* __restrict__ dr
   return -1;
 }
 
+// The shallowest layer in which direction row `drow` ((nl, 6)) is nonzero,
+// nl for a zero direction, 0 when there is no direction.
+__device__ __forceinline__ int shallowest_moved_layer(const double* __restrict__ drow, int nl) {
+  if (drow == nullptr) return 0;
+  for (int l = 0; l < nl; ++l) {
+    for (int c = 0; c < NCOL; ++c)
+      if (drow[(size_t)l * NCOL + c] != 0.0) return l;
+  }
+  return nl;
+}
+
 template <class Z>
 struct GlobalStore {
   Z* base;
@@ -303,6 +314,11 @@ __global__ void __launch_bounds__(AD_BLOCK_MAX) adjoint_kernel(
                        ? nl : deepest_moved_layer(dlayers ? dlayers + (size_t)dir * nl * NCOL
                                                           : nullptr, nl);
   const int lrev = stops_early(partial, dir) ? ltan + 1 : nl;
+  // and down from the shallowest moved layer: levels above it take the
+  // coefficients and phases without tangent
+  const int lsh = std::is_same<Z, cd>::value
+                      ? 0 : shallowest_moved_layer(dlayers ? dlayers + (size_t)dir * nl * NCOL
+                                                           : nullptr, nl);
   const Z r = lane_adjoint<Z>(s_layer, s_if, nl, om, free_surface, st, sink, [&](const Z& Rv) {
     if (!valid) return lift<Z>(0.0);
     if (MISFIT) {
@@ -315,7 +331,7 @@ __global__ void __launch_bounds__(AD_BLOCK_MAX) adjoint_kernel(
     Z g = lift_c<Z>(gin);
     if constexpr (!std::is_same<Z, cd>::value) g.d = dgin;
     return g;
-  }, ltan, lrev, kstride);
+  }, ltan, lrev, kstride, lsh);
   if (wave_ok && lane == 0) {
     const Z zero = lift<Z>(0.0);
     sink.cly[nl - 1] = LayerCot<Z>{zero, zero, zero};
@@ -417,16 +433,6 @@ __global__ void __launch_bounds__(CH_BLOCK) chain_kernel(const double* __restric
 // tangent parts (one dual level instead of two).  The few pairs d reaches
 // run the full dual-of-dual chain, packed across (direction, slowness) pairs
 // so that their waves are full.
-
-// The shallowest layer in which direction row `drow` ((nl, 6)) is nonzero,
-// nl for a zero direction.
-__device__ __forceinline__ int shallowest_moved_layer(const double* __restrict__ drow, int nl) {
-  for (int l = 0; l < nl; ++l) {
-    for (int c = 0; c < NCOL; ++c)
-      if (drow[(size_t)l * NCOL + c] != 0.0) return l;
-  }
-  return nl;
-}
 
 // Layers [lo, hi] whose pairs direction d reaches at second order (empty
 // when lo > hi), and the deepest layer with a whole cotangent.

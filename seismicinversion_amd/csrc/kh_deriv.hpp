@@ -152,7 +152,10 @@ __host__ __device__ __forceinline__ M2<Z> step_fwd_flat(const M2<Z>& S, const LZ
 // LayerPh<Z>).
 // ltan: the deepest layer whose parameters carry a tangent (dual Z): forward
 // steps at interfaces i > ltan read only tangent-free layers and run on the
-// primal parts alone (step_fwd_flat; results unchanged).  lrev: the reverse
+// primal parts alone (step_fwd_flat; results unchanged).  lsh: the
+// shallowest such layer (0 when unknown); steps at interfaces whose two
+// layers lie outside [lsh, ltan] take the coefficients and phases as plain
+// cd (results unchanged).  lrev: the reverse
 // sweep stops after interface lrev, so the sink receives interfaces
 // 0..min(lrev, nl-2) and layers 0..min(lrev + 1, nl-1) only.
 // Checkpoints and buffer in separate storage: st holds the states S_j at
@@ -164,7 +167,8 @@ template <class Z, class LZ, class Store, class BufStore, class Sink, class GFn>
 __host__ __device__ inline Z lane_adjoint_k(const LZ* __restrict__ L,
                                             const IfaceZ<Z>* __restrict__ F, int nl, cd om,
                                             int free_surface, Store& st, BufStore& bst, int K,
-                                            Sink& sink, GFn&& G_of, int ltan, int lrev) {
+                                            Sink& sink, GFn&& G_of, int ltan, int lrev,
+                                            int lsh = 0) {
   auto save = [&](int slot, const M2<Z>& S) {
     st(slot * 4 + 0) = S.a00;
     st(slot * 4 + 1) = S.a01;
@@ -179,6 +183,34 @@ __host__ __device__ inline Z lane_adjoint_k(const LZ* __restrict__ L,
   // its last segment starts from are never read
   const int ilast = (lrev < nl - 2) ? lrev : nl - 2;
   const int ck_max = ilast / K + 1;
+  // dual Z: interface i and the phases of layer i + 1 carry no tangent when
+  // neither layer i nor i + 1 lies in [lsh, ltan], the layers the direction
+  // moves; the step then takes them as plain cd (mixed operations, same
+  // numbers, a third fewer products with them)
+  constexpr bool DUAL = !std::is_same<Z, cd>::value;
+  auto flat = [&](int i) { return DUAL && (i + 1 < lsh || i > ltan); };
+  auto phases_prim = [&](int l, cd& ea, cd& eb) {
+    ea = cd{1.0, 0.0};
+    eb = cd{1.0, 0.0};
+    if (!L[l].halfspace) {
+      ea = phase_t(prim(L[l].eta), om, prim(L[l].h));
+      eb = phase_t(prim(L[l].neta), om, prim(L[l].h));
+    }
+  };
+  // a forward step at level i <= ltan (the state carries a tangent)
+  auto step_dual = [&](const M2<Z>& T, int i, Z& ea, Z& eb) {
+    if constexpr (DUAL) {
+      if (flat(i)) {
+        cd a, b;
+        phases_prim(i + 1, a, b);
+        ea = lift_c<Z>(a);
+        eb = lift_c<Z>(b);
+        return step_fwd(T, a, b, primal_uniform(prim_iface(F[i])));
+      }
+    }
+    phases_of(L[i + 1], om, ea, eb);
+    return step_fwd(T, ea, eb, F[i]);
+  };
 
   // 1. forward with checkpoints
   const IfaceZ<Z>& fb = F[nl - 2];
@@ -186,12 +218,8 @@ __host__ __device__ inline Z lane_adjoint_k(const LZ* __restrict__ L,
   if ((nl - 2) % K == 0 && (nl - 2) / K <= ck_max) save((nl - 2) / K, S);
   for (int i = nl - 3; i >= 0; --i) {
     Z ea, eb;
-    if (i > ltan) {
-      S = step_fwd_flat(S, L[i + 1], om, F[i], ea, eb);
-    } else {
-      phases_of(L[i + 1], om, ea, eb);
-      S = step_fwd(S, ea, eb, F[i]);
-    }
+    if (i > ltan) S = step_fwd_flat(S, L[i + 1], om, F[i], ea, eb);
+    else S = step_dual(S, i, ea, eb);
     if (i % K == 0 && i / K <= ck_max) save(i / K, S);
   }
 
@@ -227,36 +255,64 @@ __host__ __device__ inline Z lane_adjoint_k(const LZ* __restrict__ L,
     fbar.rd11 = sb.a11;
     sink.iface(i, fbar);
   };
-  // interface i, from the state below it and the phases of layer i + 1
-  auto level = [&](int i, const M2<Z>& Sin, const Z& ea, const Z& eb) {
+  // interface i, from the state below it and the phases of layer i + 1;
+  // then() runs once the step's arithmetic is done (the whole-tape sweep
+  // issues the next level's state load there, so that its latency overlaps
+  // this level's cotangent reduction)
+  // (phases of type Z, or plain cd at a flat level)
+  auto level = [&](int i, const M2<Z>& Sin, const auto& ea, const auto& eb, auto&& then) {
+    using EZ = std::decay_t<decltype(ea)>;
     IfaceZ<Z> fbar;
     fbar.rd00 = fbar.rd01 = fbar.rd11 = zero_z<Z>();
     fbar.ru00 = fbar.ru01 = fbar.ru11 = zero_z<Z>();
     fbar.td00 = fbar.td01 = fbar.td10 = fbar.td11 = zero_z<Z>();
     M2<Z> rb;
     Z eab, ebb;
-    step_rev(Sin, ea, eb, primal_uniform(F[i]), sb, rb, fbar, eab, ebb);
+    if constexpr (std::is_same<EZ, Z>::value)
+      step_rev(Sin, ea, eb, primal_uniform(F[i]), sb, rb, fbar, eab, ebb);
+    else
+      step_rev(Sin, ea, eb, primal_uniform(prim_iface(F[i])), sb, rb, fbar, eab, ebb);
+    then();
     LayerCot<Z> c{zero_z<Z>(), zero_z<Z>(), zero_z<Z>()};
     const LZ& lb = L[i + 1];
     if (!lb.halfspace) {
-      phase_rev(lb.eta, om, lb.h, ea, eab, c.eta, c.h);
       Z hb2 = zero_z<Z>();
-      phase_rev(lb.neta, om, lb.h, eb, ebb, c.neta, hb2);
+      if constexpr (std::is_same<EZ, Z>::value) {
+        phase_rev(lb.eta, om, lb.h, ea, eab, c.eta, c.h);
+        phase_rev(lb.neta, om, lb.h, eb, ebb, c.neta, hb2);
+      } else {
+        phase_rev(prim(lb.eta), om, prim(lb.h), ea, eab, c.eta, c.h);
+        phase_rev(prim(lb.neta), om, prim(lb.h), eb, ebb, c.neta, hb2);
+      }
       c.h = c.h + hb2;
     }
     sink.level(i, fbar, c);  // interface i and layer i + 1
     sb = rb;
   };
   if (K == 1) {  // the whole tape: S_{i+1} is checkpoint i + 1
+    M2<Z> next;
+    if (ilast >= 0 && nl - 2 > 0) next = load(1);
     for (int i = 0; i <= ilast; ++i) {
       if (i == nl - 2) {
         bottom(i);
         continue;
       }
+      const M2<Z> cur = next;
+      auto prefetch = [&] {
+        if (i + 1 <= ilast && i + 1 != nl - 2) next = load(i + 2);
+      };
+      if constexpr (DUAL) {
+        if (flat(i)) {
+          cd a, b;
+          phases_prim(i + 1, a, b);
+          level(i, cur, a, b, prefetch);
+          continue;
+        }
+      }
       Z ea, eb;
       if (i > ltan) phases_flat(L[i + 1], om, ea, eb);
       else phases_of(L[i + 1], om, ea, eb);
-      level(i, load(i + 1), ea, eb);
+      level(i, cur, ea, eb, prefetch);
     }
     return R;
   }
@@ -284,8 +340,7 @@ __host__ __device__ inline Z lane_adjoint_k(const LZ* __restrict__ L,
       } else if (i > ltan) {
         T = step_fwd_flat(T, L[i + 1], om, F[i], ea, eb);
       } else {
-        phases_of(L[i + 1], om, ea, eb);
-        T = step_fwd(T, ea, eb, F[i]);
+        T = step_dual(T, i, ea, eb);
       }
       buf(k, 4) = ea;
       buf(k, 5) = eb;
@@ -297,7 +352,14 @@ __host__ __device__ inline Z lane_adjoint_k(const LZ* __restrict__ L,
         continue;
       }
       const int k = i - i0;
-      level(i, M2<Z>{buf(k, 0), buf(k, 1), buf(k, 2), buf(k, 3)}, buf(k, 4), buf(k, 5));
+      const M2<Z> Sin{buf(k, 0), buf(k, 1), buf(k, 2), buf(k, 3)};
+      if constexpr (DUAL) {
+        if (flat(i)) {  // the buffered phases have no tangent here
+          level(i, Sin, prim(buf(k, 4)), prim(buf(k, 5)), [] {});
+          continue;
+        }
+      }
+      level(i, Sin, buf(k, 4), buf(k, 5), [] {});
     }
   }
   return R;
@@ -316,9 +378,10 @@ template <class Z, class LZ, class Store, class Sink, class GFn>
 __host__ __device__ inline Z lane_adjoint(const LZ* __restrict__ L,
                                           const IfaceZ<Z>* __restrict__ F, int nl, cd om,
                                           int free_surface, Store& st, Sink& sink, GFn&& G_of,
-                                          int ltan = 1 << 30, int lrev = 1 << 30, int K = 1) {
+                                          int ltan = 1 << 30, int lrev = 1 << 30, int K = 1,
+                                          int lsh = 0) {
   BufAfter<Z, Store> bst{st, ckpt_count(nl, K) * 4};
-  return lane_adjoint_k<Z>(L, F, nl, om, free_surface, st, bst, K, sink, G_of, ltan, lrev);
+  return lane_adjoint_k<Z>(L, F, nl, om, free_surface, st, bst, K, sink, G_of, ltan, lrev, lsh);
 }
 
 // One dual level above a scalar type, with the seeding of its inputs.

@@ -49,6 +49,29 @@ __host__ __device__ __forceinline__ Dual<T> cfms(const Dual<T>& a, const Dual<T>
                                                  const Dual<T>& c) {
   return {cfms(a.v, b.v, c.v), cfms(a.v, b.d, cfms(a.d, b.v, c.d))};
 }
+// A dual times (or fused with) a complex value without tangent: the full
+// dual operation with that tangent 0, less the products by 0.  Each tangent
+// is the expression the full operation reduces to (an fma by 0 returns its
+// addend), so for finite operands the results are the full operation's bit
+// for bit (only the sign of an exact zero may differ).
+__host__ __device__ __forceinline__ dcd operator*(const dcd& a, const cd& b) {
+  return {a.v * b, a.d * b};
+}
+__host__ __device__ __forceinline__ dcd operator*(const cd& a, const dcd& b) {
+  return {a * b.v, cfma(a, b.d, cd{0.0, 0.0})};
+}
+__host__ __device__ __forceinline__ dcd cfma(const dcd& a, const cd& b, const dcd& c) {
+  return {cfma(a.v, b, c.v), cfma(a.d, b, c.d)};
+}
+__host__ __device__ __forceinline__ dcd cfma(const cd& a, const dcd& b, const dcd& c) {
+  return {cfma(a, b.v, c.v), cfma(a, b.d, c.d)};
+}
+__host__ __device__ __forceinline__ dcd cfms(const dcd& a, const cd& b, const dcd& c) {
+  return {cfms(a.v, b, c.v), cfms(a.d, b, c.d)};
+}
+__host__ __device__ __forceinline__ dcd cfms(const cd& a, const dcd& b, const dcd& c) {
+  return {cfms(a, b.v, c.v), cfms(a, b.d, c.d)};
+}
 template <class T>
 __host__ __device__ __forceinline__ Dual<T> operator*(const Dual<T>& a, double s) {
   return {a.v * s, a.d * s};

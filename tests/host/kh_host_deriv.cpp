@@ -129,7 +129,8 @@ extern "C" int host_misfit_hvp(const double* layers, const double* dlayers, int 
 // below the deepest moved layer on primal parts only (step_fwd_flat), mode 2
 // also stops the reverse sweep one interface below it and leaves the deeper
 // layers' entries of hvp at 0 (kh_misfit_hvp_partial).  Mode 0 is
-// host_misfit_hvp.
+// host_misfit_hvp.  Modes 5, 6, 7 are modes 2, 3, 4 with the steps above the
+// shallowest moved layer on plain-cd coefficients and phases.
 extern "C" int host_misfit_hvp_mode(const double* layers, const double* dlayers, int nl,
                                     double p, const double* omega, int nw, int fs,
                                     const double* Robs, int second_only, int mode, double* chi,
@@ -140,6 +141,14 @@ extern "C" int host_misfit_hvp_mode(const double* layers, const double* dlayers,
     for (int c = 0; c < NCOL; ++c) moved = moved || dlayers[ltan * NCOL + c] != 0.0;
     if (moved) break;
   }
+  int lsh = 0;  // modes 5-7: modes 2-4 with the plain-cd steps above the shallowest moved layer
+  for (; lsh < nl; ++lsh) {
+    bool moved = false;
+    for (int c = 0; c < NCOL; ++c) moved = moved || dlayers[lsh * NCOL + c] != 0.0;
+    if (moved) break;
+  }
+  const int lsh_used = mode >= 5 ? lsh : 0;
+  if (mode >= 5) mode -= 3;
   const int lt = mode >= 1 ? ltan : (1 << 30);
   const int lrev = mode >= 2 ? ltan + 1 : (1 << 30);
   // mode 3 is mode 2 with checkpoints every 2 levels and the buffer in a
@@ -165,14 +174,14 @@ extern "C" int host_misfit_hvp_mode(const double* layers, const double* dlayers,
     dcd r;
     if (mode == 3) {
       r = lane_adjoint_k<dcd>(L.data(), F.data(), nl, om, fs, st, bst, 2, sink, G, ltan,
-                              ltan + 1);
+                              ltan + 1, lsh_used);
     } else if (mode == 4) {
       const int K = ckpt_sqrt(nl);
       BufAfter<dcd, Store<dcd>> after{st, ckpt_count(nl, K) * 4};
       r = lane_adjoint_k<dcd>(L.data(), F.data(), nl, om, fs, st, after, K, sink, G, ltan,
-                              ltan + 1);
+                              ltan + 1, lsh_used);
     } else {
-      r = lane_adjoint<dcd>(L.data(), F.data(), nl, om, fs, st, sink, G, lt, lrev);
+      r = lane_adjoint<dcd>(L.data(), F.data(), nl, om, fs, st, sink, G, lt, lrev, 1, lsh_used);
     }
     dR[2 * w] = r.d.r;
     dR[2 * w + 1] = r.d.i;
