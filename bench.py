@@ -81,13 +81,15 @@ def parse():
                     help="processes of the all-core CPU baseline (the box's share is 16 cores)")
     ap.add_argument("--warmup-seconds", type=float, default=2.0,
                     help="untimed forward launches before anything is timed (clock ramp)")
-    ap.add_argument("--section", choices=("all", "lm", "open", "served", "hessian"),
+    ap.add_argument("--section", choices=("all", "lm", "lmloop", "open", "served", "hessian"),
                     default="all",
                     help="'lm': only the 20-layer Newton-LM figures; 'hessian': only --steps "
                          "misfit + gradient + exact-Hessian evaluations of that LM after "
-                         "--warmup untimed ones (for profiling its kernels); 'open': only the "
-                         "open-loop rate sweep; 'served': only the headline served steps and "
-                         "their roofline")
+                         "--warmup untimed ones (for profiling its kernels); 'lmloop': that LM "
+                         "run for --steps iterations on the device loop, after a --warmup "
+                         "iteration run (for profiling a whole iteration's kernels); 'open': "
+                         "only the open-loop rate sweep; 'served': only the headline served "
+                         "steps and their roofline")
     ap.add_argument("--traffic-json", default=os.path.join(HERE, "profiles", "traffic_multi_served.json"),
                     help="HBM bytes per launch of the multi-model kernel from rocprofv3 --pmc")
     return ap.parse_args()
@@ -488,6 +490,25 @@ def lm_problem(torch, dev):
     return refs, p, w, obs, kt._pack_params(*refs[:4]).numpy() * 1.02
 
 
+def lm_loop_figures(torch, steps, warmup):
+    """The 20-layer Newton-LM (128 p x 512 omega, 5 % start perturbation) on
+    the device loop: a `warmup`-iteration run, then one `steps`-iteration
+    run timed whole."""
+    from seismicinversion_amd.taup_inversion import invert_taup
+    from seismicinversion_amd.workloads import seeded_model, uniform_slownesses
+
+    m = seeded_model(20, 42)
+    p = list(uniform_slownesses(128, 0.6))
+    invert_taup(m, p, nfreq=512, perturbation=0.05, max_iter=max(1, warmup), tol=0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = invert_taup(m, p, nfreq=512, perturbation=0.05, max_iter=steps, tol=0.0)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"iterations": res.n_iterations, "s": dt, "ms_per_iteration": dt * 1e3 / res.n_iterations,
+            "misfit_first_last": [res.misfit_history[0], res.misfit_history[-1]]}
+
+
 def hessian_figures(torch, dev, steps, warmup):
     """Misfit + gradient + exact Hessian of the 20-layer LM: `warmup`
     untimed evaluations, then `steps` timed one by one."""
@@ -631,6 +652,12 @@ def main():
         torch.cuda.set_device(dev)
         warm_clocks(torch, dev, args.warmup_seconds)
         print(json.dumps({"section": "lm", "lm_20_layers": lm_figures(torch, dev)}))
+        return
+    if args.section == "lmloop":
+        dev = torch.device("cuda", 0)
+        torch.cuda.set_device(dev)
+        print(json.dumps({"section": "lmloop", "lm_20_layers_loop": lm_loop_figures(
+            torch, args.steps, args.warmup)}))
         return
     if args.section == "hessian":
         dev = torch.device("cuda", 0)
