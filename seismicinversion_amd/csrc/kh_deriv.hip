@@ -245,25 +245,39 @@ __global__ void __launch_bounds__(256) jvp_kernel(const double* __restrict__ lay
   if (R && dir == 0) R[o] = r.v;
 }
 
-// Stages, for the adjoint sweep, each layer's (eta, nu, h) and each
-// interface's coefficients at slowness pv.  Thread l computes layer l and,
-// for the interface below it, layer l + 1 again, so the full LayerZ records
-// never need LDS of their own.
+// The staged tables of every (direction, slowness), computed once for all
+// the frequency tiles that read them: one thread per (direction, p, layer)
+// writes layer l's (eta, nu, h) and, below it, interface l's coefficients
+// (layer l + 1 computed again, so no table needs the whole LayerZ record).
+// Block (dir, p) holds nl LayerPh<Z> records and then nl - 1 IfaceZ<Z>
+// records, the layout the adjoint kernel copies into its LDS.
 template <class Z>
-__device__ __forceinline__ void stage_ph(const double* __restrict__ layers,
-                                         const double* __restrict__ drow_base, int nl, double pv,
-                                         LayerPh<Z>* s_ph, IfaceZ<Z>* s_if) {
-  auto lay = [&](int l) {
+__host__ __device__ __forceinline__ size_t stage_block_bytes(int nl) {
+  return (size_t)nl * sizeof(LayerPh<Z>) + (size_t)(nl - 1) * sizeof(IfaceZ<Z>);
+}
+
+template <class Z>
+__global__ void __launch_bounds__(128) stage_deriv_tables_kernel(
+    const double* __restrict__ layers, const double* __restrict__ dlayers, int nl,
+    const double* __restrict__ p, int np, int ndir, char* __restrict__ tab) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)ndir * np * nl) return;
+  const int l = (int)(t % nl);
+  const long dp = t / nl;
+  const int ip = (int)(dp % np), dir = (int)(dp / np);
+  const double* drow = dlayers ? dlayers + (size_t)dir * nl * NCOL : nullptr;
+  const double pv = p[ip];
+  auto lay = [&](int ll) {
     Z x[NCOL];
-    row_z(layers + (size_t)l * NCOL, drow_base ? drow_base + (size_t)l * NCOL : nullptr, x);
-    return layer_t<Z>(x, l, pv);
+    row_z(layers + (size_t)ll * NCOL, drow ? drow + (size_t)ll * NCOL : nullptr, x);
+    return layer_t<Z>(x, ll, pv);
   };
-  for (int l = threadIdx.x; l < nl; l += blockDim.x) {
-    const LayerZ<Z> a = lay(l);
-    s_ph[l] = layer_ph(a);
-    if (l < nl - 1) s_if[l] = interface_t<Z>(l, pv, a, lay(l + 1));
-  }
-  __syncthreads();
+  char* blk = tab + (size_t)dp * stage_block_bytes<Z>(nl);
+  LayerPh<Z>* ph = reinterpret_cast<LayerPh<Z>*>(blk);
+  IfaceZ<Z>* fz = reinterpret_cast<IfaceZ<Z>*>(blk + (size_t)nl * sizeof(LayerPh<Z>));
+  const LayerZ<Z> a = lay(l);
+  ph[l] = layer_ph(a);
+  if (l < nl - 1) fz[l] = interface_t<Z>(l, pv, a, lay(l + 1));
 }
 
 // grid.x = ndir * np * ngroups, blockDim = 64 * wpb: wave k of group g
@@ -278,7 +292,7 @@ __global__ void __launch_bounds__(AD_BLOCK_MAX) adjoint_kernel(
     long nlanes,
     IfaceZ<Z>* __restrict__ cot_if, LayerCot<Z>* __restrict__ cot_ly, Z* __restrict__ chi_part,
     cd* __restrict__ R, cd* __restrict__ dR, int ntiles, int ntp, int red_mode, int partial,
-    int kstride) {
+    int kstride, const char* __restrict__ tables) {
   extern __shared__ double smem_a[];
   LayerPh<Z>* s_layer = reinterpret_cast<LayerPh<Z>*>(smem_a);
   IfaceZ<Z>* s_if = reinterpret_cast<IfaceZ<Z>*>(s_layer + nl);
@@ -292,8 +306,13 @@ __global__ void __launch_bounds__(AD_BLOCK_MAX) adjoint_kernel(
   b /= ngroups;
   const int ip = b % np;
   const int dir = b / np;
-  stage_ph<Z>(layers, dlayers ? dlayers + (size_t)dir * nl * NCOL : nullptr, nl, p[ip], s_layer,
-              s_if);
+  {  // this (direction, slowness)'s staged tables: LayerPh records, then IfaceZ records
+    const size_t nb = stage_block_bytes<Z>(nl);
+    const double* src = reinterpret_cast<const double*>(tables + ((size_t)dir * np + ip) * nb);
+    double* dst = reinterpret_cast<double*>(s_layer);
+    for (int e = threadIdx.x; e < (int)(nb / sizeof(double)); e += blockDim.x) dst[e] = src[e];
+    __syncthreads();
+  }
   const int tile = grp * wpb + wave;
   const bool wave_ok = tile < ntiles;
   if (!wave_ok) return;  // no workgroup barrier follows the staging
@@ -597,7 +616,7 @@ __global__ void __launch_bounds__(CH_BLOCK) chain_near_kernel(
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct DerivLayout {
-  size_t ws, cif, cly, chi, jac, total;
+  size_t ws, cif, cly, chi, jac, tab, total;
   long nlanes;
   int ntiles;  // frequency tiles of AD_WAVE lanes
   int wpb;     // waves (tiles) per workgroup
@@ -622,7 +641,8 @@ static DerivLayout deriv_layout(int nl, int np, int nw, int ndir, int K) {
   d.jac = align256(d.chi + (size_t)nblocks * sizeof(Z));
   // the far-chain factors of the partial Hessian (dual calls only)
   const size_t jac = std::is_same<Z, dcd>::value ? (size_t)np * nl * NJCOL * sizeof(LocalJac) : 0;
-  d.total = align256(d.jac + jac);
+  d.tab = align256(d.jac + jac);
+  d.total = align256(d.tab + (size_t)ndir * np * stage_block_bytes<Z>(nl));
   return d;
 }
 
@@ -679,12 +699,16 @@ static int run_adjoint(const char* what, const double* layers, const double* dla
   auto* cif = reinterpret_cast<IfaceZ<Z>*>(wsb + d.cif);
   auto* cly = reinterpret_cast<LayerCot<Z>*>(wsb + d.cly);
   auto* chip = reinterpret_cast<Z*>(wsb + d.chi);
+  const long nstage = (long)ndir * np * nl;
+  hipLaunchKernelGGL((stage_deriv_tables_kernel<Z>), dim3((unsigned)((nstage + 127) / 128)),
+                     dim3(128), 0, st, layers, dlayers, nl, p, np, ndir, wsb + d.tab);
+  KH_TRY(check_launch(what));
   hipLaunchKernelGGL((adjoint_kernel<Z, MISFIT>), dim3((unsigned)nblocks), dim3(AD_WAVE * d.wpb), lds, st,
                      layers, dlayers, nl, p, np, reinterpret_cast<const cd*>(omega), nw, fs,
                      reinterpret_cast<const cd*>(Gin), reinterpret_cast<const cd*>(dGin), second_only,
                      reinterpret_cast<Z*>(wsb + d.ws), d.nlanes, cif, cly, chip,
                      reinterpret_cast<cd*>(R), reinterpret_cast<cd*>(dR), d.ntiles, d.ntp,
-                     mode, partial, d.kstride);
+                     mode, partial, d.kstride, wsb + d.tab);
   KH_TRY(check_launch(what));
   const size_t lds_c = (size_t)(nl - 1) * sizeof(IfaceZ<Z>) + (size_t)nl * sizeof(LayerCot<Z>);
   if constexpr (std::is_same<Z, dcd>::value) {
