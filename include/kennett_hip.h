@@ -414,7 +414,7 @@ int kh_lm_prepare(int nl, int np, const double* chi_rows, int chi_stride, const 
                   void* stream);
 
 /* Workspace bytes of kh_lm_trials for a launch of ntrials trials (0 when
- * P <= 85: the systems fit in LDS). */
+ * P <= 83: the systems fit in LDS). */
 size_t kh_lm_trials_workspace_bytes(int nl, int ntrials);
 
 /* Trials t = first..first+ntrials-1 of the damping schedule at once, one

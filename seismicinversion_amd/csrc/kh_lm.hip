@@ -37,7 +37,7 @@ enum { LM_LAM = 0, LM_CHI, LM_CHI_PREV, LM_ITER, LM_STATUS, LM_TRIAL };
 
 constexpr int LM_MAX_P = 4 * KH_MAX_LAYERS_DERIV - 5;  // packed parameters at the layer cap
 constexpr int LM_BLOCK = 512;
-constexpr int LM_LDS_MAX_P = 85;  // P x (P + 1) doubles and the pivot tables within 64 KB of LDS
+constexpr int LM_LDS_MAX_P = 83;  // P x (P + 1) doubles and the pivot tables within 64 KB of LDS
 
 static_assert(KH_LM_STATE_LEN >= LM_TRIAL + 1, "loop state layout");
 
@@ -185,23 +185,34 @@ __device__ __forceinline__ bool lm_trials_open(const double* state, int first) {
 // One workgroup per trial t = first + blockIdx.x: A = [H_log + lambda_t I |
 // -g_log] (row-major, leading dimension P + 1) in LDS or in the caller's
 // workspace, LU with partial pivoting (the first row of largest magnitude
-// in the column), the right-hand side eliminated along, then back
-// substitution.  Rows are exchanged through a permutation in LDS rather
-// than moved, so a column step is a pivot search in one wave and an update
-// by all, two barriers; the multipliers use one reciprocal of the pivot per
-// step, as LAPACK's dgetf2 scales the column.  Back substitution runs in
-// the first wave alone.  Writes trial_x[t] = log_x + dx and the trial's
-// layer table.  A zero pivot gives inf/NaN entries, whose trial misfit is
-// NaN and is never accepted.
+// in the column, as LAPACK's idamax picks it), the right-hand side
+// eliminated along, then back substitution.  Rows are exchanged through a
+// permutation in LDS rather than moved.  A column step is two phases: the
+// first wave searches the pivot, copies the pivot row to LDS and forms the
+// multipliers (one reciprocal of the pivot per column, as LAPACK's dgetf2
+// scales the column); then every work-item updates its elements of the
+// trailing matrix, one read and one write of A each.  Back substitution
+// runs in the first wave with the right-hand side in LDS.  Writes trial_x[t]
+// = log_x + dx and the trial's layer table.  A zero pivot gives inf/NaN
+// entries, whose trial misfit is NaN and is never accepted.
+constexpr int LM_LU_BLOCK = 1024;
+
+__device__ __forceinline__ void lm_wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
 template <bool SHARED>
-__global__ void __launch_bounds__(LM_BLOCK) lm_trials_kernel(
+__global__ void __launch_bounds__(LM_LU_BLOCK) lm_trials_kernel(
     int nl, int first, const double* __restrict__ H, const double* __restrict__ g,
     const double* __restrict__ log_x, const double* __restrict__ ref,
     const double* __restrict__ state, double* __restrict__ trial_x,
     double* __restrict__ tables, double* __restrict__ ws) {
   extern __shared__ double smem[];
   __shared__ int s_perm[LM_MAX_P];
-  __shared__ double s_x[LM_MAX_P];
+  __shared__ double s_l[LM_MAX_P];        // multipliers of the current column, then x
+  __shared__ double s_row[LM_MAX_P + 1];  // the pivot row, then the right-hand side
   if (!lm_trials_open(state, first)) return;  // this kernel never writes the state
   const int P = lm_params(nl);
   const int ld = P + 1;
@@ -226,7 +237,7 @@ __global__ void __launch_bounds__(LM_BLOCK) lm_trials_kernel(
   __syncthreads();
 
   for (int k = 0; k < P; ++k) {
-    if (tid < 64) {  // pivot search over the logical rows k.. in the first wave
+    if (tid < 64) {
       double best = -1.0;
       int bi = P;
       for (int i = k + tid; i < P; i += 64) {
@@ -249,35 +260,192 @@ __global__ void __launch_bounds__(LM_BLOCK) lm_trials_kernel(
         s_perm[k] = s_perm[bi];
         s_perm[bi] = r;
       }
+      lm_wave_lds_sync();
+      const double* rowk = A + (size_t)s_perm[k] * ld;
+      const double inv = 1.0 / rowk[k];
+      for (int j = k + 1 + tid; j < ld; j += 64) s_row[j] = rowk[j];
+      for (int i = k + 1 + tid; i < P; i += 64) s_l[i] = A[(size_t)s_perm[i] * ld + k] * inv;
     }
     __syncthreads();
-    const double* rowk = A + (size_t)s_perm[k] * ld;
-    const double inv = 1.0 / rowk[k];
     const int rows = P - 1 - k, cols = ld - 1 - k;
+#pragma unroll 4
     for (long idx = tid; idx < (long)rows * cols; idx += nthr) {
-      const int r = (int)(idx / cols);
-      double* rowi = A + (size_t)s_perm[k + 1 + r] * ld;
-      const int j = k + 1 + (int)(idx - (long)r * cols);
-      rowi[j] = fma(-(rowi[k] * inv), rowk[j], rowi[j]);
+      const int r = k + 1 + (int)(idx / cols);
+      const int j = k + 1 + (int)(idx - (long)(r - k - 1) * cols);
+      double* a = A + (size_t)s_perm[r] * ld + j;
+      *a = fma(-s_l[r], s_row[j], *a);
     }
     __syncthreads();
   }
   if (tid >= 64) return;  // back substitution and the outputs: the first wave
 
+  for (int i = tid; i < P; i += 64) s_row[i] = A[(size_t)s_perm[i] * ld + P];
+  lm_wave_lds_sync();
   for (int i = P - 1; i >= 0; --i) {
-    const double* rowi = A + (size_t)s_perm[i] * ld;
-    const double xi = rowi[P] / rowi[i];
-    if (tid == 0) s_x[i] = xi;
-    for (int j = tid; j < i; j += 64) {
-      double* rowj = A + (size_t)s_perm[j] * ld;
-      rowj[P] = fma(-rowj[i], xi, rowj[P]);
-    }
-    __syncthreads();  // only the first wave is left: orders its LDS traffic
+    const double xi = s_row[i] / A[(size_t)s_perm[i] * ld + i];
+    if (tid == 0) s_l[i] = xi;
+    for (int j = tid; j < i; j += 64) s_row[j] = fma(-A[(size_t)s_perm[j] * ld + i], xi, s_row[j]);
+    lm_wave_lds_sync();
   }
 
   double* xt = trial_x + (size_t)t * P;
-  for (int j = tid; j < P; j += 64) xt[j] = log_x[j] + s_x[j];
+  for (int j = tid; j < P; j += 64) xt[j] = log_x[j] + s_l[j];
+  __threadfence_block();
+  lm_wave_lds_sync();
+  double* tab = tables + (size_t)t * nl * NCOL;
+  for (int r = tid; r < nl * NCOL; r += 64) tab[r] = lm_table_entry(ref, nl, xt, r / NCOL, r % NCOL);
+}
+
+// The same solve for systems too large for LDS (A in the caller's
+// workspace), blocked: panels of LM_NB columns are factorised in LDS by the
+// first wave (pivot search, row exchange, multipliers, update of the
+// panel's later columns), then every work-item takes a column to the right
+// of the panel (the right-hand side included): it forms that column's
+// entries in the panel's pivot rows (forward substitution with the panel's
+// multipliers) and updates the rows below with them.  Each element of A
+// receives the same fused multiply-adds in the same order as in
+// lm_trials_kernel (the same arithmetic per element), while the trailing
+// matrix is read and written once per panel instead of once per column.
+constexpr int LM_NB = 8;
+
+__global__ void __launch_bounds__(LM_LU_BLOCK) lm_trials_blocked_kernel(
+    int nl, int first, const double* __restrict__ H, const double* __restrict__ g,
+    const double* __restrict__ log_x, const double* __restrict__ ref,
+    const double* __restrict__ state, double* __restrict__ trial_x,
+    double* __restrict__ tables, double* __restrict__ ws) {
+  __shared__ int s_perm[LM_MAX_P];
+  __shared__ double s_pan[LM_MAX_P * LM_NB];  // panel: logical rows kb.., LM_NB columns
+  __shared__ double s_l[LM_MAX_P];            // x
+  __shared__ double s_row[LM_MAX_P + 1];      // the right-hand side in back substitution
+  if (!lm_trials_open(state, first)) return;  // this kernel never writes the state
+  const int P = lm_params(nl);
+  const int ld = P + 1;
+  const int t = first + (int)blockIdx.x;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  double* A = ws + (size_t)blockIdx.x * P * ld;
+  double lam = state[LM_LAM];
+  for (int k = 0; k < t; ++k) lam *= 5.0;
+
+  for (long idx = tid; idx < (long)P * ld; idx += nthr) {
+    const int i = (int)(idx / ld), j = (int)(idx - (long)i * ld);
+    double v;
+    if (j < P) {
+      v = H[(size_t)i * P + j];
+      if (i == j) v += lam;
+    } else {
+      v = -g[i];
+    }
+    A[idx] = v;
+  }
+  for (int i = tid; i < P; i += nthr) s_perm[i] = i;
   __syncthreads();
+
+  for (int kb = 0; kb < P; kb += LM_NB) {
+    const int nbk = min(LM_NB, P - kb), m = P - kb;
+    for (int e = tid; e < m * nbk; e += nthr) {
+      const int r = e / nbk, c = e - (e / nbk) * nbk;
+      s_pan[r * LM_NB + c] = A[(size_t)s_perm[kb + r] * ld + kb + c];
+    }
+    __syncthreads();
+    if (tid < 64) {  // the panel, column by column
+      for (int c = 0; c < nbk; ++c) {
+        double best = -1.0;
+        int bi = m;
+        for (int r = c + tid; r < m; r += 64) {
+          const double v = fabs(s_pan[r * LM_NB + c]);
+          if (v > best) {
+            best = v;
+            bi = r;
+          }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+          const double ob = __shfl_xor(best, off, 64);
+          const int oi = __shfl_xor(bi, off, 64);
+          if (ob > best || (ob == best && oi < bi)) {
+            best = ob;
+            bi = oi;
+          }
+        }
+        if (bi < m && bi != c) {  // an all-NaN column keeps its row
+          if (tid < nbk) {
+            const double a = s_pan[c * LM_NB + tid];
+            s_pan[c * LM_NB + tid] = s_pan[bi * LM_NB + tid];
+            s_pan[bi * LM_NB + tid] = a;
+          }
+          if (tid == 0) {
+            const int r = s_perm[kb + c];
+            s_perm[kb + c] = s_perm[kb + bi];
+            s_perm[kb + bi] = r;
+          }
+        }
+        lm_wave_lds_sync();
+        const double inv = 1.0 / s_pan[c * LM_NB + c];
+        for (int r = c + 1 + tid; r < m; r += 64) {
+          const double l = s_pan[r * LM_NB + c] * inv;
+          s_pan[r * LM_NB + c] = l;
+          for (int cc = c + 1; cc < nbk; ++cc)
+            s_pan[r * LM_NB + cc] = fma(-l, s_pan[c * LM_NB + cc], s_pan[r * LM_NB + cc]);
+        }
+        lm_wave_lds_sync();
+      }
+    }
+    __syncthreads();
+    // the panel's pivot rows (upper part) back to A, for the back substitution
+    for (int e = tid; e < nbk * nbk; e += nthr) {
+      const int i = e / nbk, c = e - (e / nbk) * nbk;
+      if (c >= i) A[(size_t)s_perm[kb + i] * ld + kb + c] = s_pan[i * LM_NB + c];
+    }
+    // the columns right of the panel: pivot-row entries, then the rows below.
+    // G work-items per column (each takes every G-th row below the panel)
+    // all read the column's pivot-row entries, so those are written back
+    // only after every work-item has read them.  ncol * G <= nthr: one
+    // column per work-item at most.
+    const int ncol = ld - (kb + nbk);
+    const int G = ncol > 0 && nthr / ncol > 0 ? nthr / ncol : 1;
+    const bool mine = ncol > 0 && tid < ncol * G;
+    const int j = mine ? kb + nbk + tid % ncol : 0, grp = mine ? tid / ncol : 0;
+    double u[LM_NB];
+    if (mine) {
+#pragma unroll
+      for (int i = 0; i < LM_NB; ++i) u[i] = i < nbk ? A[(size_t)s_perm[kb + i] * ld + j] : 0.0;
+#pragma unroll
+      for (int i = 1; i < LM_NB; ++i)
+#pragma unroll
+        for (int c = 0; c < LM_NB; ++c)
+          if (i < nbk && c < i) u[i] = fma(-s_pan[i * LM_NB + c], u[c], u[i]);
+#pragma unroll 4
+      for (int r = nbk + grp; r < m; r += G) {
+        double* a = A + (size_t)s_perm[kb + r] * ld + j;
+        double v = *a;
+#pragma unroll
+        for (int c = 0; c < LM_NB; ++c)
+          if (c < nbk) v = fma(-s_pan[r * LM_NB + c], u[c], v);
+        *a = v;
+      }
+    }
+    __syncthreads();
+    if (mine && grp == 0) {
+#pragma unroll
+      for (int i = 0; i < LM_NB; ++i)
+        if (i < nbk) A[(size_t)s_perm[kb + i] * ld + j] = u[i];
+    }
+    __syncthreads();
+  }
+  if (tid >= 64) return;  // back substitution and the outputs: the first wave
+
+  for (int i = tid; i < P; i += 64) s_row[i] = A[(size_t)s_perm[i] * ld + P];
+  lm_wave_lds_sync();
+  for (int i = P - 1; i >= 0; --i) {
+    const double xi = s_row[i] / A[(size_t)s_perm[i] * ld + i];
+    if (tid == 0) s_l[i] = xi;
+    for (int j = tid; j < i; j += 64) s_row[j] = fma(-A[(size_t)s_perm[j] * ld + i], xi, s_row[j]);
+    lm_wave_lds_sync();
+  }
+
+  double* xt = trial_x + (size_t)t * P;
+  for (int j = tid; j < P; j += 64) xt[j] = log_x[j] + s_l[j];
+  __threadfence_block();
+  lm_wave_lds_sync();
   double* tab = tables + (size_t)t * nl * NCOL;
   for (int r = tid; r < nl * NCOL; r += 64) tab[r] = lm_table_entry(ref, nl, xt, r / NCOL, r % NCOL);
 }
@@ -441,14 +609,14 @@ extern "C" int kh_lm_trials(int nl, int first, int ntrials, const double* H_log,
   hipStream_t st = as_stream(stream);
   if (P <= LM_LDS_MAX_P) {
     const size_t lds = (size_t)P * (P + 1) * sizeof(double);
-    hipLaunchKernelGGL(lm_trials_kernel<true>, dim3((unsigned)ntrials), dim3(LM_BLOCK), lds, st,
+    hipLaunchKernelGGL(lm_trials_kernel<true>, dim3((unsigned)ntrials), dim3(LM_LU_BLOCK), lds, st,
                        nl, first, H_log, g_log, log_x, ref_layers, state, trial_x, trial_tables,
                        static_cast<double*>(nullptr));
   } else {
     const size_t need = kh_lm_trials_workspace_bytes(nl, ntrials);
     KH_REQUIRE(workspace && workspace_bytes >= need, KH_ERR_INVALID_ARGUMENT,
                "kh_lm_trials: workspace of %zu bytes needed, %zu given", need, workspace_bytes);
-    hipLaunchKernelGGL(lm_trials_kernel<false>, dim3((unsigned)ntrials), dim3(LM_BLOCK), 0, st,
+    hipLaunchKernelGGL(lm_trials_blocked_kernel, dim3((unsigned)ntrials), dim3(LM_LU_BLOCK), 0, st,
                        nl, first, H_log, g_log, log_x, ref_layers, state, trial_x, trial_tables,
                        static_cast<double*>(workspace));
   }

@@ -307,6 +307,12 @@ class _DeviceLoop:
         return self.hist[:n].cpu().numpy(), self.log_x.cpu().numpy(), int(state[4])
 
 
+# Largest packed-parameter count for which invert_taup takes the device loop
+# by default: the damped P x P systems then fit in LDS (kh_lm_trials_workspace_bytes
+# is 0 up to there).
+DEVICE_LOOP_MAX_P = 83
+
+
 def _log_transform(m, g, H):
     """g_log = m g, H_log = m m^T H + diag(g m) (taup_inversion.py:180-185)."""
     return m * g, np.outer(m, m) * H + np.diag(g * m)
@@ -348,14 +354,18 @@ def invert_taup(true_model: LayerModel, p_values: list[float], nfreq: int = 64,
                 perturbation: float = 0.15, max_iter: int = 100, seed: int = 42,
                 tol: float = 1e-8, hessian_mode: str = "gram",
                 max_trials: int = 10, use_graphs: bool = False,
-                device_loop: bool = True) -> InversionResult:
+                device_loop: bool | None = None) -> InversionResult:
     """Newton-LM inversion from a perturbed start model (taup_inversion.py:262-470).
 
-    ``device_loop`` (default) runs the whole iteration on the device: the
-    damped solves of all trials, their misfits and the accept / reject
-    decision included (_DeviceLoop); otherwise the host solves and decides
-    as the reference does, one trial at a time.  ``use_graphs`` replays the
-    iterations from a captured HIP graph (same numbers, fewer launches)."""
+    ``device_loop`` True runs the whole iteration on the device: the damped
+    solves of all trials, their misfits and the accept / reject decision
+    included (_DeviceLoop); False solves and decides on the host as the
+    reference does, one trial at a time.  The default (None) takes the
+    device loop while the damped systems fit in LDS (DEVICE_LOOP_MAX_P
+    packed parameters, 22 layers), where it is the faster of the two; for
+    deeper models the host's LAPACK solve is faster than one workgroup's.
+    ``use_graphs`` replays the iterations from a captured HIP graph (same
+    numbers, fewer launches)."""
     t = model_to_tensors(true_model)
     refs = (t["alpha"], t["beta"], t["rho"], t["thickness"], t["Q_alpha"], t["Q_beta"])
     dw = 2.0 * np.pi / 64.0
@@ -379,6 +389,8 @@ def invert_taup(true_model: LayerModel, p_values: list[float], nfreq: int = 64,
     init_params = true_params * perturb
     initial_model = _to_model(_unpack_params(torch.tensor(init_params), *refs[:4]), true_model)
 
+    if device_loop is None:
+        device_loop = len(true_params) <= DEVICE_LOOP_MAX_P
     if device_loop:
         loop = _DeviceLoop(prob, true_params, init_params, max_iter, tol, hessian_mode, max_trials)
         hist, log_x, status = loop.run(use_graphs)

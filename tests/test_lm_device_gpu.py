@@ -223,14 +223,15 @@ def test_decide_accepts_the_first_improving_trial():
     assert torch.equal(log_x, tx[0]) and after0[0] == 0.15
 
 
-@pytest.mark.parametrize("mode", ["gram", "hvp"])
-def test_device_loop_follows_host_loop(mode):
+# 6 layers: P = 19, the systems in LDS; 30 layers: P = 115, the blocked LU
+@pytest.mark.parametrize("nl,mode", [(6, "gram"), (6, "hvp"), (30, "gram")])
+def test_device_loop_follows_host_loop(nl, mode):
     """Same evaluations, accept / reject rule and damping; only the solve
     (LU here, LAPACK there) and the order of a few sums differ."""
-    m = seeded_model(6, 8)
+    m = seeded_model(nl, 8)
     p = list(np.linspace(0.05, 0.45, 10))
     kw = dict(nfreq=48, perturbation=0.05, max_iter=6, tol=0.0, hessian_mode=mode)
-    dev = invert_taup(m, p, **kw)
+    dev = invert_taup(m, p, device_loop=True, **kw)
     host = invert_taup(m, p, device_loop=False, **kw)
     print("device", dev.misfit_history, "\nhost  ", host.misfit_history, flush=True)
     assert dev.n_iterations == host.n_iterations == 6
