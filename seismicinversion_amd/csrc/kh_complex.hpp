@@ -28,6 +28,13 @@ __host__ __device__ __forceinline__ cd operator*(cd a, cd b) {
 __host__ __device__ __forceinline__ cd operator*(cd a, double s) { return cd{a.r * s, a.i * s}; }
 __host__ __device__ __forceinline__ cd operator*(double s, cd a) { return cd{a.r * s, a.i * s}; }
 __host__ __device__ __forceinline__ cd conj(cd a) { return cd{a.r, -a.i}; }
+// Stores a result that no kernel of the launch reads back, marked
+// non-temporal so that the output stream does not evict from L2 the tables
+// the sweep re-reads.
+__device__ __forceinline__ void store_stream(cd* p, const cd& z) {
+  __builtin_nontemporal_store(z.r, &p->r);
+  __builtin_nontemporal_store(z.i, &p->i);
+}
 // acc + |a - b|^2: the one association every misfit sum of the library uses,
 // so that sums over the same terms in the same order agree bit for bit.
 __host__ __device__ __forceinline__ double add_sq_diff(double acc, cd a, cd b) {

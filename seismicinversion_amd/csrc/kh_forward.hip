@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GRID ?
 #pragma unroll
   for (int k = 0; k < W; ++k) {
     const int w = base + k * blockDim.x;
-    if (w < nw) Rrow[w] = out[k];
+    if (w < nw) store_stream(Rrow + w, out[k]);
   }
 }
 

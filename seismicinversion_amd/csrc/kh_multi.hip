@@ -151,7 +151,7 @@ __device__ __forceinline__ int sweep_item(const kh_model_desc& md, int t0, int t
     for (int k = 0; k < WL; ++k) {
       const int w = base + k * blockDim.x;
       if (w < nw) {
-        Rrow[w] = out[k];
+        store_stream(Rrow + w, out[k]);
         bad += finite_cd(out[k]) ? 0 : 1;
       }
     }
