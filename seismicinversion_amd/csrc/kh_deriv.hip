@@ -8,8 +8,9 @@
 //   adjoint_kernel<Z, M>  one wave per (direction, p, 64-frequency tile):
 //                         forward keeping the whole tape (or sqrt(N)
 //                         checkpoints, when the workspace holds only those),
-//                         reverse sweep, per-level cotangents reduced over the
-//                         wave (fixed butterfly) and written per tile.
+//                         reverse sweep, per-level cotangents summed over the
+//                         wave in registers (a fixed exchange tree) and
+//                         written per tile.
 //                         Z = cd: vector-Jacobian product (M = 0, given
 //                         cotangent) or misfit gradient (M = 1);
 //                         Z = Dual<cd>, M = 1: tangent-over-adjoint, i.e. the
@@ -21,6 +22,7 @@
 
 #include "kh_deriv.hpp"
 #include "kh_runtime.hpp"
+#include "kh_wave.hpp"
 
 namespace kh {
 
@@ -56,102 +58,47 @@ struct Flat<dcd> {
   }
 };
 
-// Row stride (doubles) of the reduction buffer: one pad element spreads the
-// rows that different lanes sum over the LDS banks.  Reduction modes, by the
-// LDS a workgroup can have (with deep models the LDS per workgroup, not the
-// registers, decides how many workgroups share a CU):
-//   RED_LEVEL  an interface's 10 scalars and the layer below it (3) in one
-//              pass (one row per double);
-//   RED_RECORD the interface, then the layer: two passes;
-//   RED_HALF   the interface 5 scalars at a time, then the layer: three
-//              passes in half the buffer.
-constexpr int RED_STRIDE = AD_WAVE + 1;
-enum { RED_LEVEL = 0, RED_RECORD = 1, RED_HALF = 2 };
-template <class Z>
-constexpr int red_rows(int mode) {
-  return (mode == RED_LEVEL ? 13 : mode == RED_RECORD ? 10 : 5) * Flat<Z>::N;
-}
-
-// Orders one wave's LDS accesses around a reduction pass.  The buffer is the
-// wave's own and the LDS executes a wave's instructions in order, so its
-// lanes' writes are seen by the reads that follow and the next pass's
-// writes come after this pass's reads, without a workgroup barrier: the
-// wavefront-scope fences only keep the compiler from moving LDS accesses
-// across the point (no instruction, no wait on the wave's outstanding
-// global loads and stores, no waiting for the other wave of the workgroup).
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
-}
-
-// Per-wave sums of the per-lane cotangent records, written per tile.  Each
-// lane writes its record as a column of an LDS matrix (one row per double of
-// the record), then lane e adds row e in a fixed order and stores it: about a
-// tenth of the LDS operations of a shuffle butterfly per double, and the
-// same order on every run.
+// Per-wave sums of the per-lane cotangent records, written per tile: the
+// doubles of a record are summed over the 64 lanes in registers by
+// wave_reduce_scatter (a fixed exchange tree, the same order on every run),
+// after which each lane stores the sum it holds.  No LDS, and no wait on an
+// LDS round trip per summand.
 template <class Z>
 struct WaveSink {
   IfaceZ<Z>* cif;
   LayerCot<Z>* cly;
-  double* red;  // LDS, red_rows<Z>(mode) x RED_STRIDE, this wave's own
-  int mode;
   bool store;  // false for a wave with no tile
 
-  // one pass over scalars [F0, F1) of the record v; the sums of its first n0
-  // doubles go to d0, the rest to d1
-  template <int F0, int F1, int NF>
-  __device__ void reduce_pass(const Z (&v)[NF], double* __restrict__ d0, int n0,
-                              double* __restrict__ d1) {
+  // the sums of the doubles of v: the first n0 to d0, the rest to d1
+  template <int NF>
+  __device__ void reduce(const Z (&v)[NF], double* __restrict__ d0, int n0,
+                         double* __restrict__ d1) {
     constexpr int N = Flat<Z>::N;
-    const int lane = threadIdx.x % AD_WAVE;
+    double x[NF * N];
 #pragma unroll
-    for (int f = F0; f < F1; ++f) {
+    for (int f = 0; f < NF; ++f) {
 #pragma unroll
-      for (int k = 0; k < N; ++k) red[((f - F0) * N + k) * RED_STRIDE + lane] = Flat<Z>::get(v[f], k);
+      for (int k = 0; k < N; ++k) x[f * N + k] = Flat<Z>::get(v[f], k);
     }
-    wave_lds_sync();
-    if (store && lane < (F1 - F0) * N) {
-      const double* row = red + lane * RED_STRIDE;
-      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-#pragma unroll 4
-      for (int j = 0; j < AD_WAVE; j += 4) {
-        s0 += row[j];
-        s1 += row[j + 1];
-        s2 += row[j + 2];
-        s3 += row[j + 3];
-      }
-      const int e = F0 * N + lane;
-      *(e < n0 ? d0 + e : d1 + (e - n0)) = (s0 + s1) + (s2 + s3);
-    }
-    wave_lds_sync();  // the buffer is rewritten by the next pass
+    const double s = wave_reduce_scatter<NF * N>(x);
+    const int e = wave_rs_index(NF * N, (int)(threadIdx.x % AD_WAVE));
+    if (store && e >= 0) *(e < n0 ? d0 + e : d1 + (e - n0)) = s;
   }
   __device__ void iface(int i, const IfaceZ<Z>& f) {
     const Z v[10] = {f.rd00, f.rd01, f.rd11, f.ru00, f.ru01,
                      f.ru11, f.td00, f.td01, f.td10, f.td11};
-    double* dst = reinterpret_cast<double*>(cif + i);
-    if (mode == RED_HALF) {
-      reduce_pass<0, 5>(v, dst, 1 << 30, nullptr);
-      reduce_pass<5, 10>(v, dst, 1 << 30, nullptr);
-    } else {
-      reduce_pass<0, 10>(v, dst, 1 << 30, nullptr);
-    }
+    reduce(v, reinterpret_cast<double*>(cif + i), 1 << 30, nullptr);
   }
   __device__ void layer(int l, const LayerCot<Z>& c) {
     const Z v[3] = {c.eta, c.neta, c.h};
-    reduce_pass<0, 3>(v, reinterpret_cast<double*>(cly + l), 1 << 30, nullptr);
+    reduce(v, reinterpret_cast<double*>(cly + l), 1 << 30, nullptr);
   }
-  // interface i and layer i + 1
+  // interface i and layer i + 1 in one reduction
   __device__ void level(int i, const IfaceZ<Z>& f, const LayerCot<Z>& c) {
-    if (mode != RED_LEVEL) {
-      iface(i, f);
-      layer(i + 1, c);
-      return;
-    }
     const Z v[13] = {f.rd00, f.rd01, f.rd11, f.ru00, f.ru01, f.ru11, f.td00,
                      f.td01, f.td10, f.td11, c.eta,  c.neta, c.h};
-    reduce_pass<0, 13>(v, reinterpret_cast<double*>(cif + i), 10 * Flat<Z>::N,
-                       reinterpret_cast<double*>(cly + i + 1));
+    reduce(v, reinterpret_cast<double*>(cif + i), 10 * Flat<Z>::N,
+           reinterpret_cast<double*>(cly + i + 1));
   }
 };
 
@@ -291,15 +238,13 @@ __global__ void __launch_bounds__(AD_BLOCK_MAX) adjoint_kernel(
     const cd* __restrict__ Gin, const cd* __restrict__ dGin, int second_only, Z* __restrict__ ws,
     long nlanes,
     IfaceZ<Z>* __restrict__ cot_if, LayerCot<Z>* __restrict__ cot_ly, Z* __restrict__ chi_part,
-    cd* __restrict__ R, cd* __restrict__ dR, int ntiles, int ntp, int red_mode, int partial,
+    cd* __restrict__ R, cd* __restrict__ dR, int ntiles, int ntp, int partial,
     int kstride, const char* __restrict__ tables) {
   extern __shared__ double smem_a[];
   LayerPh<Z>* s_layer = reinterpret_cast<LayerPh<Z>*>(smem_a);
   IfaceZ<Z>* s_if = reinterpret_cast<IfaceZ<Z>*>(s_layer + nl);
   const int wpb = (int)(blockDim.x / AD_WAVE);
   const int wave = (int)(threadIdx.x / AD_WAVE), lane = (int)(threadIdx.x % AD_WAVE);
-  double* s_red = reinterpret_cast<double*>(s_if + (nl - 1)) +
-                  (size_t)wave * red_rows<Z>(red_mode) * RED_STRIDE;
   const int ngroups = ntp / wpb;
   int b = blockIdx.x;
   const int grp = b % ngroups;
@@ -322,7 +267,7 @@ __global__ void __launch_bounds__(AD_BLOCK_MAX) adjoint_kernel(
   const cd om = omega[wc];
   const size_t slot = ((size_t)dir * np + ip) * ntp + tile;  // (dir, p, tile)
   GlobalStore<Z> st{ws + (long)slot * AD_WAVE + lane, nlanes};
-  WaveSink<Z> sink{cot_if + slot * (nl - 1), cot_ly + slot * nl, s_red, red_mode, wave_ok};
+  WaveSink<Z> sink{cot_if + slot * (nl - 1), cot_ly + slot * nl, wave_ok};
   const cd gin = Gin[(size_t)ip * nw + wc];
   const cd dgin = dGin ? dGin[((size_t)dir * np + ip) * nw + wc] : cd{0.0, 0.0};
   Z chi = lift<Z>(0.0);
@@ -685,17 +630,8 @@ static int run_adjoint(const char* what, const double* layers, const double* dla
   char* wsb = reinterpret_cast<char*>(workspace);
   const long nblocks = (long)ndir * np * (d.ntp / d.wpb);
   KH_REQUIRE(nblocks <= 0x7fffffffL, KH_ERR_UNSUPPORTED, "%s: grid too large", what);
-  // the fewest reduction passes whose buffers still let four waves (the
-  // register limit of the dual kernel: one per SIMD) share a CU's 160 KB of
-  // LDS
-  const size_t tables = (size_t)nl * sizeof(LayerPh<Z>) + (size_t)(nl - 1) * sizeof(IfaceZ<Z>);
-  const int blocks_per_cu = 4 / d.wpb;
-  auto lds_of = [&](int mode) {
-    return tables + (size_t)d.wpb * red_rows<Z>(mode) * RED_STRIDE * sizeof(double);
-  };
-  int mode = RED_LEVEL;
-  while (mode < RED_HALF && (size_t)blocks_per_cu * lds_of(mode) > (size_t)160 * 1024) ++mode;
-  const size_t lds = lds_of(mode);
+  // LDS: the (direction, slowness)'s staged tables
+  const size_t lds = (size_t)nl * sizeof(LayerPh<Z>) + (size_t)(nl - 1) * sizeof(IfaceZ<Z>);
   auto* cif = reinterpret_cast<IfaceZ<Z>*>(wsb + d.cif);
   auto* cly = reinterpret_cast<LayerCot<Z>*>(wsb + d.cly);
   auto* chip = reinterpret_cast<Z*>(wsb + d.chi);
@@ -708,7 +644,7 @@ static int run_adjoint(const char* what, const double* layers, const double* dla
                      reinterpret_cast<const cd*>(Gin), reinterpret_cast<const cd*>(dGin), second_only,
                      reinterpret_cast<Z*>(wsb + d.ws), d.nlanes, cif, cly, chip,
                      reinterpret_cast<cd*>(R), reinterpret_cast<cd*>(dR), d.ntiles, d.ntp,
-                     mode, partial, d.kstride, wsb + d.tab);
+                     partial, d.kstride, wsb + d.tab);
   KH_TRY(check_launch(what));
   const size_t lds_c = (size_t)(nl - 1) * sizeof(IfaceZ<Z>) + (size_t)nl * sizeof(LayerCot<Z>);
   if constexpr (std::is_same<Z, dcd>::value) {
