@@ -227,6 +227,29 @@ __global__ void __launch_bounds__(128) stage_deriv_tables_kernel(
   if (l < nl - 1) fz[l] = interface_t<Z>(l, pv, a, lay(l + 1));
 }
 
+// The primal phase factors of every (slowness, layer, frequency), entry
+// (ip * nl + l) * nw + w, one thread each, from direction 0's staged layer
+// records (the primal parts are the same for every direction): the dual
+// sweep reads them instead of evaluating them again for every direction.
+template <class Z>
+__global__ void __launch_bounds__(256) phase_table_kernel(const char* __restrict__ tab, int nl,
+                                                          int np, const cd* __restrict__ omega,
+                                                          int nw, PhPair* __restrict__ out) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)np * nl * nw) return;
+  const int w = (int)(t % nw);
+  const long pl = t / nw;
+  const int l = (int)(pl % nl), ip = (int)(pl / nl);
+  const LayerPh<Z>& L =
+      reinterpret_cast<const LayerPh<Z>*>(tab + (size_t)ip * stage_block_bytes<Z>(nl))[l];
+  PhPair q{cd{1.0, 0.0}, cd{1.0, 0.0}};
+  if (!L.halfspace) {
+    q.a = phase_t(prim(L.eta), omega[w], prim(L.h));
+    q.b = phase_t(prim(L.neta), omega[w], prim(L.h));
+  }
+  out[t] = q;
+}
+
 // grid.x = ndir * np * ngroups, blockDim = 64 * wpb: wave k of group g
 // sweeps frequency tile g * wpb + k of (dir, p), all waves sharing one staged
 // table.  ntp = ngroups * wpb tiles have workspace and cotangent slots; a
@@ -239,7 +262,7 @@ __global__ void __launch_bounds__(AD_BLOCK_MAX) adjoint_kernel(
     long nlanes,
     IfaceZ<Z>* __restrict__ cot_if, LayerCot<Z>* __restrict__ cot_ly, Z* __restrict__ chi_part,
     cd* __restrict__ R, cd* __restrict__ dR, int ntiles, int ntp, int partial,
-    int kstride, const char* __restrict__ tables) {
+    int kstride, const char* __restrict__ tables, const PhPair* __restrict__ ph) {
   extern __shared__ double smem_a[];
   LayerPh<Z>* s_layer = reinterpret_cast<LayerPh<Z>*>(smem_a);
   IfaceZ<Z>* s_if = reinterpret_cast<IfaceZ<Z>*>(s_layer + nl);
@@ -283,7 +306,8 @@ __global__ void __launch_bounds__(AD_BLOCK_MAX) adjoint_kernel(
   const int lsh = std::is_same<Z, cd>::value
                       ? 0 : shallowest_moved_layer(dlayers ? dlayers + (size_t)dir * nl * NCOL
                                                            : nullptr, nl);
-  const Z r = lane_adjoint<Z>(s_layer, s_if, nl, om, free_surface, st, sink, [&](const Z& Rv) {
+  // dual Z reads the primal phase factors from the table ph (always given)
+  const Z r = lane_adjoint<Z, !std::is_same<Z, cd>::value>(s_layer, s_if, nl, om, free_surface, st, sink, [&](const Z& Rv) {
     if (!valid) return lift<Z>(0.0);
     if (MISFIT) {
       const Z res = Rv - lift_c<Z>(gin);
@@ -295,7 +319,7 @@ __global__ void __launch_bounds__(AD_BLOCK_MAX) adjoint_kernel(
     Z g = lift_c<Z>(gin);
     if constexpr (!std::is_same<Z, cd>::value) g.d = dgin;
     return g;
-  }, ltan, lrev, kstride, lsh);
+  }, ltan, lrev, kstride, lsh, PhaseTab{ph ? ph + (size_t)ip * nl * nw + wc : nullptr, nw});
   if (wave_ok && lane == 0) {
     const Z zero = lift<Z>(0.0);
     sink.cly[nl - 1] = LayerCot<Z>{zero, zero, zero};
@@ -561,7 +585,7 @@ __global__ void __launch_bounds__(CH_BLOCK) chain_near_kernel(
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct DerivLayout {
-  size_t ws, cif, cly, chi, jac, tab, total;
+  size_t ws, cif, cly, chi, jac, tab, ph, total;
   long nlanes;
   int ntiles;  // frequency tiles of AD_WAVE lanes
   int wpb;     // waves (tiles) per workgroup
@@ -587,7 +611,10 @@ static DerivLayout deriv_layout(int nl, int np, int nw, int ndir, int K) {
   // the far-chain factors of the partial Hessian (dual calls only)
   const size_t jac = std::is_same<Z, dcd>::value ? (size_t)np * nl * NJCOL * sizeof(LocalJac) : 0;
   d.tab = align256(d.jac + jac);
-  d.total = align256(d.tab + (size_t)ndir * np * stage_block_bytes<Z>(nl));
+  d.ph = align256(d.tab + (size_t)ndir * np * stage_block_bytes<Z>(nl));
+  // the primal phase table of the dual sweep
+  const size_t phb = std::is_same<Z, dcd>::value ? (size_t)np * nl * nw * sizeof(PhPair) : 0;
+  d.total = align256(d.ph + phb);
   return d;
 }
 
@@ -639,12 +666,20 @@ static int run_adjoint(const char* what, const double* layers, const double* dla
   hipLaunchKernelGGL((stage_deriv_tables_kernel<Z>), dim3((unsigned)((nstage + 127) / 128)),
                      dim3(128), 0, st, layers, dlayers, nl, p, np, ndir, wsb + d.tab);
   KH_TRY(check_launch(what));
+  PhPair* ph = nullptr;
+  if constexpr (std::is_same<Z, dcd>::value) {
+    ph = reinterpret_cast<PhPair*>(wsb + d.ph);
+    const long nph = (long)np * nl * nw;
+    hipLaunchKernelGGL((phase_table_kernel<Z>), dim3((unsigned)((nph + 255) / 256)), dim3(256), 0,
+                       st, wsb + d.tab, nl, np, reinterpret_cast<const cd*>(omega), nw, ph);
+    KH_TRY(check_launch(what));
+  }
   hipLaunchKernelGGL((adjoint_kernel<Z, MISFIT>), dim3((unsigned)nblocks), dim3(AD_WAVE * d.wpb), lds, st,
                      layers, dlayers, nl, p, np, reinterpret_cast<const cd*>(omega), nw, fs,
                      reinterpret_cast<const cd*>(Gin), reinterpret_cast<const cd*>(dGin), second_only,
                      reinterpret_cast<Z*>(wsb + d.ws), d.nlanes, cif, cly, chip,
                      reinterpret_cast<cd*>(R), reinterpret_cast<cd*>(dR), d.ntiles, d.ntp,
-                     partial, d.kstride, wsb + d.tab);
+                     partial, d.kstride, wsb + d.tab, ph);
   KH_TRY(check_launch(what));
   const size_t lds_c = (size_t)(nl - 1) * sizeof(IfaceZ<Z>) + (size_t)nl * sizeof(LayerCot<Z>);
   if constexpr (std::is_same<Z, dcd>::value) {

@@ -108,41 +108,35 @@ __host__ __device__ __forceinline__ IfaceZ<cd> prim_iface(const IfaceZ<Dual<T>>&
                     prim(f.ru11), prim(f.td00), prim(f.td01), prim(f.td10), prim(f.td11)};
 }
 
-// The phases of a layer whose parameters carry no tangent: those of the
-// primal parts, lifted with zero tangents.
-template <class Z, class LZ>
-__host__ __device__ __forceinline__ void phases_flat(const LZ& lb, cd om, Z& ea, Z& eb) {
-  if constexpr (std::is_same<Z, cd>::value) {
-    phases_of(lb, om, ea, eb);
-  } else {
-    cd a{1.0, 0.0}, b{1.0, 0.0};
-    if (!lb.halfspace) {
-      a = phase_t(prim(lb.eta), om, prim(lb.h));
-      b = phase_t(prim(lb.neta), om, prim(lb.h));
-    }
-    ea = lift_c<Z>(a);
-    eb = lift_c<Z>(b);
-  }
-}
-
 // A forward step whose inputs carry no tangent: every layer it reads lies
 // below the deepest layer the direction moves, so the state below, the
-// interface and the phases all have zero tangents.  The plain step on the
+// interface and the phases (a, b: the primal phase factors of the layer
+// below the interface) all have zero tangents.  The plain step on the
 // primal parts, lifted back with zero tangents: bitwise the primal of the
-// full step, and the tangents it leaves out are exactly zero.  The phases
-// are returned too (lifted), for the reverse sweep's buffer.
-template <class Z, class LZ>
-__host__ __device__ __forceinline__ M2<Z> step_fwd_flat(const M2<Z>& S, const LZ& lb, cd om,
-                                                        const IfaceZ<Z>& f, Z& ea, Z& eb) {
-  phases_flat(lb, om, ea, eb);
+// full step, and the tangents it leaves out are exactly zero.
+template <class Z>
+__host__ __device__ __forceinline__ M2<Z> step_fwd_prim(const M2<Z>& S, const cd& a, const cd& b,
+                                                        const IfaceZ<Z>& f) {
   if constexpr (std::is_same<Z, cd>::value) {
-    return step_fwd(S, ea, eb, f);
+    return step_fwd(S, a, b, f);
   } else {
     const M2<cd> s{prim(S.a00), prim(S.a01), prim(S.a10), prim(S.a11)};
-    const M2<cd> r = step_fwd(s, prim(ea), prim(eb), prim_iface(f));
+    const M2<cd> r = step_fwd(s, a, b, prim_iface(f));
     return M2<Z>{lift_c<Z>(r.a00), lift_c<Z>(r.a01), lift_c<Z>(r.a10), lift_c<Z>(r.a11)};
   }
 }
+
+// Primal phase factors (ea, eb) of one layer at one frequency, and a lane's
+// view of a table of them: entry l (layer l) at p[l * stride].  The table
+// holds what the sweep would compute from the same primal eta, neta and h
+// ((1, 0) for the half-space), so reading it changes no result.
+struct PhPair {
+  cd a, b;
+};
+struct PhaseTab {
+  const PhPair* p;  // nullptr: the sweep computes the phases itself
+  long stride;
+};
 
 // Storage: st(e) returns a reference to Z-slot e of this lane.
 // Sink: sink.iface(i, IfaceZ<Z>), sink.layer(l, LayerCot<Z>) and
@@ -152,10 +146,12 @@ __host__ __device__ __forceinline__ M2<Z> step_fwd_flat(const M2<Z>& S, const LZ
 // LayerPh<Z>).
 // ltan: the deepest layer whose parameters carry a tangent (dual Z): forward
 // steps at interfaces i > ltan read only tangent-free layers and run on the
-// primal parts alone (step_fwd_flat; results unchanged).  lsh: the
+// primal parts alone (step_fwd_prim; results unchanged).  lsh: the
 // shallowest such layer (0 when unknown); steps at interfaces whose two
-// layers lie outside [lsh, ltan] take the coefficients and phases as plain
-// cd (results unchanged).  lrev: the reverse
+// layers lie outside [lsh, ltan] take the coefficients as plain cd, and
+// every layer outside [lsh, ltan] gives its phases as plain cd (results
+// unchanged).  pt: the primal phase factors of this lane's frequency per
+// layer, read when TAB (else computed in place).  lrev: the reverse
 // sweep stops after interface lrev, so the sink receives interfaces
 // 0..min(lrev, nl-2) and layers 0..min(lrev + 1, nl-1) only.
 // Checkpoints and buffer in separate storage: st holds the states S_j at
@@ -163,12 +159,12 @@ __host__ __device__ __forceinline__ M2<Z> step_fwd_flat(const M2<Z>& S, const LZ
 // buffer of a segment being walked back (K * 6 slots).  K = ckpt_stride(nl)
 // balances the two; a smaller K keeps the buffer small enough for on-chip
 // storage (the reverse reads it right after the recompute writes it).
-template <class Z, class LZ, class Store, class BufStore, class Sink, class GFn>
+template <class Z, bool TAB = false, class LZ, class Store, class BufStore, class Sink, class GFn>
 __host__ __device__ inline Z lane_adjoint_k(const LZ* __restrict__ L,
                                             const IfaceZ<Z>* __restrict__ F, int nl, cd om,
                                             int free_surface, Store& st, BufStore& bst, int K,
                                             Sink& sink, GFn&& G_of, int ltan, int lrev,
-                                            int lsh = 0) {
+                                            int lsh = 0, PhaseTab pt = PhaseTab{nullptr, 0}) {
   auto save = [&](int slot, const M2<Z>& S) {
     st(slot * 4 + 0) = S.a00;
     st(slot * 4 + 1) = S.a01;
@@ -189,7 +185,16 @@ __host__ __device__ inline Z lane_adjoint_k(const LZ* __restrict__ L,
   // numbers, a third fewer products with them)
   constexpr bool DUAL = !std::is_same<Z, cd>::value;
   auto flat = [&](int i) { return DUAL && (i + 1 < lsh || i > ltan); };
+  // layer l's phases carry a tangent (dual Z only)
+  auto moved = [&](int l) { return DUAL && l >= lsh && l <= ltan; };
+  constexpr bool tab = TAB;
   auto phases_prim = [&](int l, cd& ea, cd& eb) {
+    if (tab) {
+      const PhPair q = pt.p[(long)l * pt.stride];
+      ea = q.a;
+      eb = q.b;
+      return;
+    }
     ea = cd{1.0, 0.0};
     eb = cd{1.0, 0.0};
     if (!L[l].halfspace) {
@@ -197,36 +202,55 @@ __host__ __device__ inline Z lane_adjoint_k(const LZ* __restrict__ L,
       eb = phase_t(prim(L[l].neta), om, prim(L[l].h));
     }
   };
-  // a forward step at level i <= ltan (the state carries a tangent)
-  auto step_dual = [&](const M2<Z>& T, int i, Z& ea, Z& eb) {
+  // the forward step at interface i, given the phases of layer i + 1: the
+  // primal ones (a, b) when it does not move, else the dual ones (ea, eb)
+  auto fwd_step = [&](const M2<Z>& T, int i, const cd& a, const cd& b, const Z& ea,
+                      const Z& eb) -> M2<Z> {
     if constexpr (DUAL) {
-      if (flat(i)) {
-        cd a, b;
-        phases_prim(i + 1, a, b);
-        ea = lift_c<Z>(a);
-        eb = lift_c<Z>(b);
-        return step_fwd(T, a, b, primal_uniform(prim_iface(F[i])));
-      }
+      if (i > ltan) return step_fwd_prim(T, a, b, F[i]);
+      if (flat(i)) return step_fwd(T, a, b, primal_uniform(prim_iface(F[i])));
+      if (!moved(i + 1)) return step_fwd(T, a, b, F[i]);
+      return step_fwd(T, ea, eb, F[i]);
+    } else {
+      return step_fwd(T, a, b, F[i]);
     }
-    phases_of(L[i + 1], om, ea, eb);
-    return step_fwd(T, ea, eb, F[i]);
   };
 
-  // 1. forward with checkpoints
+  // 1. forward with checkpoints (with a table, each layer's phases are
+  // loaded one step ahead)
   const IfaceZ<Z>& fb = F[nl - 2];
   M2<Z> S{fb.rd00, fb.rd01, fb.rd01, fb.rd11};
   if ((nl - 2) % K == 0 && (nl - 2) / K <= ck_max) save((nl - 2) / K, S);
+  PhPair ahead{cd{1.0, 0.0}, cd{1.0, 0.0}};
+  if (tab && nl >= 3) ahead = pt.p[(long)(nl - 2) * pt.stride];
   for (int i = nl - 3; i >= 0; --i) {
-    Z ea, eb;
-    if (i > ltan) S = step_fwd_flat(S, L[i + 1], om, F[i], ea, eb);
-    else S = step_dual(S, i, ea, eb);
+    cd a{1.0, 0.0}, b{1.0, 0.0};
+    Z ea = lift<Z>(1.0), eb = lift<Z>(1.0);
+    const bool mv = moved(i + 1);
+    if constexpr (tab) {
+      a = ahead.a;
+      b = ahead.b;
+      ahead = pt.p[(long)(i >= 1 ? i : 0) * pt.stride];  // unconditional: no merge, no early wait
+    } else if (!mv) {
+      phases_prim(i + 1, a, b);
+    }
+    if (mv) phases_of(L[i + 1], om, ea, eb);
+    S = fwd_step(S, i, a, b, ea, eb);
     if (i % K == 0 && i / K <= ck_max) save(i / K, S);
   }
 
   // 2. surface value and its cotangent
   const LZ& water = L[0];
   Z ea0 = lift<Z>(1.0);
-  if (!water.halfspace) ea0 = phase_t(water.eta, om, water.h);
+  if (!water.halfspace) {
+    if (moved(0)) {
+      ea0 = phase_t(water.eta, om, water.h);
+    } else {
+      cd a, b;
+      phases_prim(0, a, b);
+      ea0 = lift_c<Z>(a);
+    }
+  }
   const Z e2 = ea0 * ea0;
   const Z top = e2 * S.a00;
   const Z one = lift<Z>(1.0);
@@ -259,8 +283,10 @@ __host__ __device__ inline Z lane_adjoint_k(const LZ* __restrict__ L,
   // then() runs once the step's arithmetic is done (the whole-tape sweep
   // issues the next level's state load there, so that its latency overlaps
   // this level's cotangent reduction)
-  // (phases of type Z, or plain cd at a flat level)
-  auto level = [&](int i, const M2<Z>& Sin, const auto& ea, const auto& eb, auto&& then) {
+  // (phases of type Z, or plain cd where layer i + 1 does not move;
+  // flat_iface: std::true_type where interface i carries no tangent)
+  auto level = [&](int i, const M2<Z>& Sin, const auto& ea, const auto& eb, auto flat_iface,
+                   auto&& then) {
     using EZ = std::decay_t<decltype(ea)>;
     IfaceZ<Z> fbar;
     fbar.rd00 = fbar.rd01 = fbar.rd11 = zero_z<Z>();
@@ -268,10 +294,10 @@ __host__ __device__ inline Z lane_adjoint_k(const LZ* __restrict__ L,
     fbar.td00 = fbar.td01 = fbar.td10 = fbar.td11 = zero_z<Z>();
     M2<Z> rb;
     Z eab, ebb;
-    if constexpr (std::is_same<EZ, Z>::value)
-      step_rev(Sin, ea, eb, primal_uniform(F[i]), sb, rb, fbar, eab, ebb);
-    else
+    if constexpr (decltype(flat_iface)::value)
       step_rev(Sin, ea, eb, primal_uniform(prim_iface(F[i])), sb, rb, fbar, eab, ebb);
+    else
+      step_rev(Sin, ea, eb, primal_uniform(F[i]), sb, rb, fbar, eab, ebb);
     then();
     LayerCot<Z> c{zero_z<Z>(), zero_z<Z>(), zero_z<Z>()};
     const LZ& lb = L[i + 1];
@@ -291,28 +317,39 @@ __host__ __device__ inline Z lane_adjoint_k(const LZ* __restrict__ L,
   };
   if (K == 1) {  // the whole tape: S_{i+1} is checkpoint i + 1
     M2<Z> next;
-    if (ilast >= 0 && nl - 2 > 0) next = load(1);
+    PhPair nph{cd{1.0, 0.0}, cd{1.0, 0.0}};  // with a table: layer i + 1's phases
+    if (ilast >= 0 && nl - 2 > 0) {
+      next = load(1);
+      if (tab) nph = pt.p[pt.stride];
+    }
     for (int i = 0; i <= ilast; ++i) {
       if (i == nl - 2) {
         bottom(i);
         continue;
       }
       const M2<Z> cur = next;
+      const PhPair q = nph;
       auto prefetch = [&] {
-        if (i + 1 <= ilast && i + 1 != nl - 2) next = load(i + 2);
+        if (i + 1 <= ilast && i + 1 != nl - 2) {
+          next = load(i + 2);
+          if (tab) nph = pt.p[(long)(i + 2) * pt.stride];
+        }
       };
+      cd a = q.a, b = q.b;
+      if (!tab && !moved(i + 1)) phases_prim(i + 1, a, b);
       if constexpr (DUAL) {
         if (flat(i)) {
-          cd a, b;
-          phases_prim(i + 1, a, b);
-          level(i, cur, a, b, prefetch);
-          continue;
+          level(i, cur, a, b, std::true_type{}, prefetch);
+        } else if (!moved(i + 1)) {
+          level(i, cur, a, b, std::false_type{}, prefetch);
+        } else {
+          Z ea, eb;
+          phases_of(L[i + 1], om, ea, eb);
+          level(i, cur, ea, eb, std::false_type{}, prefetch);
         }
+      } else {
+        level(i, cur, a, b, std::false_type{}, prefetch);
       }
-      Z ea, eb;
-      if (i > ltan) phases_flat(L[i + 1], om, ea, eb);
-      else phases_of(L[i + 1], om, ea, eb);
-      level(i, cur, ea, eb, prefetch);
     }
     return R;
   }
@@ -328,20 +365,22 @@ __host__ __device__ inline Z lane_adjoint_k(const LZ* __restrict__ L,
     }
     for (int i = i1; i >= i0; --i) {
       if (i == nl - 2) continue;  // bottom interface: no step (S_{nl-2} = Rd)
-      Z ea, eb;
       const int k = i - i0;
       buf(k, 0) = T.a00;
       buf(k, 1) = T.a01;
       buf(k, 2) = T.a10;
       buf(k, 3) = T.a11;
-      if (i == i0) {  // the state this step would give lies above the segment
-        if (i > ltan) phases_flat(L[i + 1], om, ea, eb);
-        else phases_of(L[i + 1], om, ea, eb);
-      } else if (i > ltan) {
-        T = step_fwd_flat(T, L[i + 1], om, F[i], ea, eb);
+      cd a{1.0, 0.0}, b{1.0, 0.0};
+      Z ea, eb;
+      if (moved(i + 1)) {
+        phases_of(L[i + 1], om, ea, eb);
       } else {
-        T = step_dual(T, i, ea, eb);
+        phases_prim(i + 1, a, b);
+        ea = lift_c<Z>(a);
+        eb = lift_c<Z>(b);
       }
+      // the state the step at i0 would give lies above the segment
+      if (i != i0) T = fwd_step(T, i, a, b, ea, eb);
       buf(k, 4) = ea;
       buf(k, 5) = eb;
     }
@@ -355,11 +394,15 @@ __host__ __device__ inline Z lane_adjoint_k(const LZ* __restrict__ L,
       const M2<Z> Sin{buf(k, 0), buf(k, 1), buf(k, 2), buf(k, 3)};
       if constexpr (DUAL) {
         if (flat(i)) {  // the buffered phases have no tangent here
-          level(i, Sin, prim(buf(k, 4)), prim(buf(k, 5)), [] {});
+          level(i, Sin, prim(buf(k, 4)), prim(buf(k, 5)), std::true_type{}, [] {});
+          continue;
+        }
+        if (!moved(i + 1)) {  // nor here (the interface has one)
+          level(i, Sin, prim(buf(k, 4)), prim(buf(k, 5)), std::false_type{}, [] {});
           continue;
         }
       }
-      level(i, Sin, buf(k, 4), buf(k, 5), [] {});
+      level(i, Sin, buf(k, 4), buf(k, 5), std::false_type{}, [] {});
     }
   }
   return R;
@@ -374,14 +417,15 @@ struct BufAfter {
   __host__ __device__ Z& operator()(int e) { return st(off + e); }
 };
 
-template <class Z, class LZ, class Store, class Sink, class GFn>
+template <class Z, bool TAB = false, class LZ, class Store, class Sink, class GFn>
 __host__ __device__ inline Z lane_adjoint(const LZ* __restrict__ L,
                                           const IfaceZ<Z>* __restrict__ F, int nl, cd om,
                                           int free_surface, Store& st, Sink& sink, GFn&& G_of,
                                           int ltan = 1 << 30, int lrev = 1 << 30, int K = 1,
-                                          int lsh = 0) {
+                                          int lsh = 0, PhaseTab pt = PhaseTab{nullptr, 0}) {
   BufAfter<Z, Store> bst{st, ckpt_count(nl, K) * 4};
-  return lane_adjoint_k<Z>(L, F, nl, om, free_surface, st, bst, K, sink, G_of, ltan, lrev, lsh);
+  return lane_adjoint_k<Z, TAB>(L, F, nl, om, free_surface, st, bst, K, sink, G_of, ltan, lrev,
+                                lsh, pt);
 }
 
 // One dual level above a scalar type, with the seeding of its inputs.

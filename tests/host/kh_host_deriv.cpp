@@ -126,7 +126,7 @@ extern "C" int host_misfit_hvp(const double* layers, const double* dlayers, int 
 }
 
 // host_misfit_hvp with the kernels' shortcuts: mode 1 runs the forward steps
-// below the deepest moved layer on primal parts only (step_fwd_flat), mode 2
+// below the deepest moved layer on primal parts only (step_fwd_prim), mode 2
 // also stops the reverse sweep one interface below it and leaves the deeper
 // layers' entries of hvp at 0 (kh_misfit_hvp_partial).  Mode 0 is
 // host_misfit_hvp.  Modes 5, 6, 7 are modes 2, 3, 4 with the steps above the
