@@ -141,6 +141,25 @@ struct GlobalStore {
   __device__ Z& operator()(int e) { return base[(long)e * stride]; }
 };
 
+// A lane's tape: elements e >= from_e (the states below the deepest layer a
+// direction moves) in the tape shared by every direction, which is only
+// read; the rest in the (direction, p, tile)'s own storage.  The element's
+// address is selected, not its value, so a load issued a level ahead is not
+// waited for at the selection.
+template <class Z>
+struct TapeStore {
+  Z* own;
+  long own_stride;
+  const Z* sh;
+  long sh_stride;
+  int from_e;
+  __device__ Z& operator()(int e) {
+    Z* const a = own + (long)e * own_stride;
+    Z* const b = const_cast<Z*>(sh) + (long)e * sh_stride;
+    return *(e >= from_e ? b : a);
+  }
+};
+
 template <class Z>
 __device__ __forceinline__ void stage_z(const double* __restrict__ layers,
                                         const double* __restrict__ drow_base, int nl, double pv,
@@ -250,6 +269,40 @@ __global__ void __launch_bounds__(256) phase_table_kernel(const char* __restrict
   out[t] = q;
 }
 
+// The forward states S_j (j = nl-2 .. 0) of the primal sweep of every
+// (slowness, frequency), one thread each, as Z with zero tangents: element
+// (j * 4 + q) of lane (ip, w) at ((ip * (nl - 1)) * 4 + j * 4 + q) * nw + w.
+// The steps are step_fwd_prim's (direction 0's staged interfaces, the
+// phase table): the states every direction's sweep would compute below
+// the deepest layer it moves.
+template <class Z>
+__global__ void __launch_bounds__(256) primal_tape_kernel(const char* __restrict__ tab, int nl,
+                                                          int np, int nw,
+                                                          const PhPair* __restrict__ ph,
+                                                          Z* __restrict__ out) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)np * nw) return;
+  const int w = (int)(t % nw), ip = (int)(t / nw);
+  const IfaceZ<Z>* F = reinterpret_cast<const IfaceZ<Z>*>(
+      tab + (size_t)ip * stage_block_bytes<Z>(nl) + (size_t)nl * sizeof(LayerPh<Z>));
+  const PhPair* q = ph + (size_t)ip * nl * nw + w;
+  Z* o = out + (size_t)ip * (nl - 1) * 4 * nw + w;
+  auto put = [&](int j, const M2<Z>& S) {
+    o[(size_t)(j * 4 + 0) * nw] = S.a00;
+    o[(size_t)(j * 4 + 1) * nw] = S.a01;
+    o[(size_t)(j * 4 + 2) * nw] = S.a10;
+    o[(size_t)(j * 4 + 3) * nw] = S.a11;
+  };
+  const IfaceZ<cd> fb = prim_iface(F[nl - 2]);
+  M2<Z> S{lift_c<Z>(fb.rd00), lift_c<Z>(fb.rd01), lift_c<Z>(fb.rd01), lift_c<Z>(fb.rd11)};
+  put(nl - 2, S);
+  for (int i = nl - 3; i >= 0; --i) {
+    const PhPair e = q[(size_t)(i + 1) * nw];
+    S = step_fwd_prim(S, e.a, e.b, F[i]);
+    put(i, S);
+  }
+}
+
 // grid.x = ndir * np * ngroups, blockDim = 64 * wpb: wave k of group g
 // sweeps frequency tile g * wpb + k of (dir, p), all waves sharing one staged
 // table.  ntp = ngroups * wpb tiles have workspace and cotangent slots; a
@@ -262,7 +315,8 @@ __global__ void __launch_bounds__(AD_BLOCK_MAX) adjoint_kernel(
     long nlanes,
     IfaceZ<Z>* __restrict__ cot_if, LayerCot<Z>* __restrict__ cot_ly, Z* __restrict__ chi_part,
     cd* __restrict__ R, cd* __restrict__ dR, int ntiles, int ntp, int partial,
-    int kstride, const char* __restrict__ tables, const PhPair* __restrict__ ph) {
+    int kstride, const char* __restrict__ tables, const PhPair* __restrict__ ph,
+    const Z* __restrict__ pts) {
   extern __shared__ double smem_a[];
   LayerPh<Z>* s_layer = reinterpret_cast<LayerPh<Z>*>(smem_a);
   IfaceZ<Z>* s_if = reinterpret_cast<IfaceZ<Z>*>(s_layer + nl);
@@ -289,7 +343,6 @@ __global__ void __launch_bounds__(AD_BLOCK_MAX) adjoint_kernel(
   const int wc = valid ? w : nw - 1;
   const cd om = omega[wc];
   const size_t slot = ((size_t)dir * np + ip) * ntp + tile;  // (dir, p, tile)
-  GlobalStore<Z> st{ws + (long)slot * AD_WAVE + lane, nlanes};
   WaveSink<Z> sink{cot_if + slot * (nl - 1), cot_ly + slot * nl, wave_ok};
   const cd gin = Gin[(size_t)ip * nw + wc];
   const cd dgin = dGin ? dGin[((size_t)dir * np + ip) * nw + wc] : cd{0.0, 0.0};
@@ -306,6 +359,14 @@ __global__ void __launch_bounds__(AD_BLOCK_MAX) adjoint_kernel(
   const int lsh = std::is_same<Z, cd>::value
                       ? 0 : shallowest_moved_layer(dlayers ? dlayers + (size_t)dir * nl * NCOL
                                                            : nullptr, nl);
+  // with the whole tape, the states below the deepest moved layer come from
+  // the tape every direction shares (pts, when given)
+  PhaseTab pt{ph ? ph + (size_t)ip * nl * nw + wc : nullptr, nw};
+  const bool shared = pts != nullptr && kstride == 1 && ltan <= nl - 3;
+  if (shared) pt.tape_from = ltan + 1;
+  Z* const own = ws + (long)slot * AD_WAVE + lane;
+  TapeStore<Z> st{own, nlanes, shared ? pts + (size_t)ip * (nl - 1) * 4 * nw + wc : own,
+                  shared ? (long)nw : nlanes, shared ? 4 * (ltan + 1) : 1 << 30};
   // dual Z reads the primal phase factors from the table ph (always given)
   const Z r = lane_adjoint<Z, !std::is_same<Z, cd>::value>(s_layer, s_if, nl, om, free_surface, st, sink, [&](const Z& Rv) {
     if (!valid) return lift<Z>(0.0);
@@ -319,7 +380,7 @@ __global__ void __launch_bounds__(AD_BLOCK_MAX) adjoint_kernel(
     Z g = lift_c<Z>(gin);
     if constexpr (!std::is_same<Z, cd>::value) g.d = dgin;
     return g;
-  }, ltan, lrev, kstride, lsh, PhaseTab{ph ? ph + (size_t)ip * nl * nw + wc : nullptr, nw});
+  }, ltan, lrev, kstride, lsh, pt);
   if (wave_ok && lane == 0) {
     const Z zero = lift<Z>(0.0);
     sink.cly[nl - 1] = LayerCot<Z>{zero, zero, zero};
@@ -585,7 +646,7 @@ __global__ void __launch_bounds__(CH_BLOCK) chain_near_kernel(
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct DerivLayout {
-  size_t ws, cif, cly, chi, jac, tab, ph, total;
+  size_t ws, cif, cly, chi, jac, tab, ph, pts, total;
   long nlanes;
   int ntiles;  // frequency tiles of AD_WAVE lanes
   int wpb;     // waves (tiles) per workgroup
@@ -614,7 +675,11 @@ static DerivLayout deriv_layout(int nl, int np, int nw, int ndir, int K) {
   d.ph = align256(d.tab + (size_t)ndir * np * stage_block_bytes<Z>(nl));
   // the primal phase table of the dual sweep
   const size_t phb = std::is_same<Z, dcd>::value ? (size_t)np * nl * nw * sizeof(PhPair) : 0;
-  d.total = align256(d.ph + phb);
+  // with the whole tape, the primal states every direction shares
+  d.pts = align256(d.ph + phb);
+  const size_t ptsb =
+      (std::is_same<Z, dcd>::value && K == 1) ? (size_t)np * (nl - 1) * 4 * nw * sizeof(Z) : 0;
+  d.total = align256(d.pts + ptsb);
   return d;
 }
 
@@ -667,19 +732,27 @@ static int run_adjoint(const char* what, const double* layers, const double* dla
                      dim3(128), 0, st, layers, dlayers, nl, p, np, ndir, wsb + d.tab);
   KH_TRY(check_launch(what));
   PhPair* ph = nullptr;
+  Z* pts = nullptr;
   if constexpr (std::is_same<Z, dcd>::value) {
     ph = reinterpret_cast<PhPair*>(wsb + d.ph);
     const long nph = (long)np * nl * nw;
     hipLaunchKernelGGL((phase_table_kernel<Z>), dim3((unsigned)((nph + 255) / 256)), dim3(256), 0,
                        st, wsb + d.tab, nl, np, reinterpret_cast<const cd*>(omega), nw, ph);
     KH_TRY(check_launch(what));
+    if (d.kstride == 1) {
+      pts = reinterpret_cast<Z*>(wsb + d.pts);
+      const long npw = (long)np * nw;
+      hipLaunchKernelGGL((primal_tape_kernel<Z>), dim3((unsigned)((npw + 255) / 256)), dim3(256), 0,
+                         st, wsb + d.tab, nl, np, nw, ph, pts);
+      KH_TRY(check_launch(what));
+    }
   }
   hipLaunchKernelGGL((adjoint_kernel<Z, MISFIT>), dim3((unsigned)nblocks), dim3(AD_WAVE * d.wpb), lds, st,
                      layers, dlayers, nl, p, np, reinterpret_cast<const cd*>(omega), nw, fs,
                      reinterpret_cast<const cd*>(Gin), reinterpret_cast<const cd*>(dGin), second_only,
                      reinterpret_cast<Z*>(wsb + d.ws), d.nlanes, cif, cly, chip,
                      reinterpret_cast<cd*>(R), reinterpret_cast<cd*>(dR), d.ntiles, d.ntp,
-                     partial, d.kstride, wsb + d.tab, ph);
+                     partial, d.kstride, wsb + d.tab, ph, pts);
   KH_TRY(check_launch(what));
   const size_t lds_c = (size_t)(nl - 1) * sizeof(IfaceZ<Z>) + (size_t)nl * sizeof(LayerCot<Z>);
   if constexpr (std::is_same<Z, dcd>::value) {

@@ -136,6 +136,10 @@ struct PhPair {
 struct PhaseTab {
   const PhPair* p;  // nullptr: the sweep computes the phases itself
   long stride;
+  // With the whole tape (K = 1): the first level whose state the tape
+  // storage already holds, shared by every direction (a primal state with
+  // zero tangent); the forward sweep starts from it.  Default: none.
+  int tape_from = 1 << 30;
 };
 
 // Storage: st(e) returns a reference to Z-slot e of this lane.
@@ -220,10 +224,8 @@ __host__ __device__ inline Z lane_adjoint_k(const LZ* __restrict__ L,
   // loaded one step ahead)
   const IfaceZ<Z>& fb = F[nl - 2];
   M2<Z> S{fb.rd00, fb.rd01, fb.rd01, fb.rd11};
-  if ((nl - 2) % K == 0 && (nl - 2) / K <= ck_max) save((nl - 2) / K, S);
   PhPair ahead{cd{1.0, 0.0}, cd{1.0, 0.0}};
-  if (tab && nl >= 3) ahead = pt.p[(long)(nl - 2) * pt.stride];
-  for (int i = nl - 3; i >= 0; --i) {
+  auto fwd_level = [&](int i) {
     cd a{1.0, 0.0}, b{1.0, 0.0};
     Z ea = lift<Z>(1.0), eb = lift<Z>(1.0);
     const bool mv = moved(i + 1);
@@ -236,7 +238,24 @@ __host__ __device__ inline Z lane_adjoint_k(const LZ* __restrict__ L,
     }
     if (mv) phases_of(L[i + 1], om, ea, eb);
     S = fwd_step(S, i, a, b, ea, eb);
-    if (i % K == 0 && i / K <= ck_max) save(i / K, S);
+  };
+  if (K == 1 && pt.tape_from <= nl - 2) {
+    // the storage holds the states from tape_from down: start there and
+    // save every state computed (unconditional stores)
+    S = load(pt.tape_from);
+    const int i0 = pt.tape_from - 1;
+    if (tab && i0 >= 0) ahead = pt.p[(long)(i0 + 1) * pt.stride];
+    for (int i = i0; i >= 0; --i) {
+      fwd_level(i);
+      save(i, S);
+    }
+  } else {
+    if ((nl - 2) % K == 0 && (nl - 2) / K <= ck_max) save((nl - 2) / K, S);
+    if (tab && nl >= 3) ahead = pt.p[(long)(nl - 2) * pt.stride];
+    for (int i = nl - 3; i >= 0; --i) {
+      fwd_level(i);
+      if (i % K == 0 && i / K <= ck_max) save(i / K, S);
+    }
   }
 
   // 2. surface value and its cotangent
