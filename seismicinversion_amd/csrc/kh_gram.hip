@@ -9,8 +9,10 @@
 // slabs of both 64-row panels in LDS with coalesced loads (every element of
 // X is read from memory once per super-tile pair), and wave w accumulates
 // the 16 x 64 strip of row tile w against the four column tiles of B.  The
-// chunk count is chosen from (P, K) alone, about 512 workgroups in all, so
-// the launch fills the device for thin and for tall X alike.  Per-chunk
+// next slab is loaded into registers while the current one is multiplied,
+// so the memory latency overlaps the MFMAs.  The chunk count is chosen from
+// (P, K) alone, about 1024 workgroups in all, so the launch fills the
+// device for thin and for tall X alike.  Per-chunk
 // partial 64 x 64 blocks go to the workspace; the reduction adds the chunks
 // of each entry in a fixed order (bitwise reproducible) and reads the upper
 // triangle for both halves, so G is exactly symmetric.
@@ -44,7 +46,7 @@ static GramPlan gram_plan(int P, long K) {
   g.nsup = (P + GR_SUP - 1) / GR_SUP;
   g.nsp = g.nsup * (g.nsup + 1) / 2;
   const long slabs = std::max(1L, (K + GR_KS - 1) / GR_KS);
-  const long want = std::max(1L, std::min(128L, 512L / g.nsp));
+  const long want = std::max(1L, std::min(256L, 1024L / g.nsp));
   const long nch = std::min(slabs, want);
   g.kc = ((K + nch - 1) / nch + GR_KS - 1) / GR_KS * GR_KS;
   g.nchunks = K > 0 ? (K + g.kc - 1) / g.kc : 0;
@@ -76,15 +78,32 @@ __global__ void __launch_bounds__(GR_BLOCK) gram_partial_kernel(const double* __
   d4 acc[4];
 #pragma unroll
   for (int bt = 0; bt < 4; ++bt) acc[bt] = d4{0.0, 0.0, 0.0, 0.0};
-  for (long ks = k0; ks < k1; ks += GR_KS) {
-    for (int e = threadIdx.x; e < GR_SUP * GR_KS; e += GR_BLOCK) {
+  // a slab is GR_SUP x GR_KS elements of each panel, GR_PER per thread
+  constexpr int GR_PER = GR_SUP * GR_KS / GR_BLOCK;
+  static_assert(GR_PER * GR_BLOCK == GR_SUP * GR_KS, "whole slabs per thread");
+  double va[GR_PER], vb[GR_PER];
+  auto fetch = [&](long ks) {
+#pragma unroll
+    for (int q = 0; q < GR_PER; ++q) {
+      const int e = (int)threadIdx.x + q * GR_BLOCK;
       const int r = e / GR_KS, c = e - (e / GR_KS) * GR_KS;  // a row's slab is contiguous
       const long k = ks + c;
       const int ra = A * GR_SUP + r, rb = B * GR_SUP + r;
-      sA[r * GR_LD + c] = (ra < P && k < k1) ? X[(size_t)ra * K + k] : 0.0;
-      sB[r * GR_LD + c] = (rb < P && k < k1) ? X[(size_t)rb * K + k] : 0.0;
+      va[q] = (ra < P && k < k1) ? X[(size_t)ra * K + k] : 0.0;
+      vb[q] = (rb < P && k < k1) ? X[(size_t)rb * K + k] : 0.0;
+    }
+  };
+  if (k0 < k1) fetch(k0);
+  for (long ks = k0; ks < k1; ks += GR_KS) {
+#pragma unroll
+    for (int q = 0; q < GR_PER; ++q) {
+      const int e = (int)threadIdx.x + q * GR_BLOCK;
+      const int r = e / GR_KS, c = e - (e / GR_KS) * GR_KS;
+      sA[r * GR_LD + c] = va[q];
+      sB[r * GR_LD + c] = vb[q];
     }
     __syncthreads();
+    if (ks + GR_KS < k1) fetch(ks + GR_KS);  // in flight during the MFMAs below
 #pragma unroll
     for (int s = 0; s < GR_KS; s += 4) {
       const double a = sA[(w * GR_TILE + row) * GR_LD + s + kk];
