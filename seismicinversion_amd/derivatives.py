@@ -22,7 +22,7 @@ __all__ = ["jvp", "vjp", "misfit_grad", "misfit_hvp", "vjp_hvp", "paced", "WORKS
 
 # Directions per launch are capped so that one launch's workspace stays below
 # this (the 20-layer LM's 75 directions at 128 x 512 keep their whole forward
-# tape in 12.5 GB: one launch).
+# tape in 12.0 GB: one launch).
 WORKSPACE_LIMIT = 16 << 30
 
 
@@ -88,8 +88,12 @@ def _dir_chunks(nl, n_p, nw, ndir):
     paced, whose launches stay under the pacing cap), and whether they keep
     the whole tape."""
     tape = _tape_fits(nl, n_p, nw, True)
-    one = _ws_bytes(nl, n_p, nw, 1, True, tape) or 1
-    step = max(1, min(ndir, WORKSPACE_LIMIT // one))
+    # the workspace is a part per call (the phase and shared-tape tables)
+    # plus a part per direction
+    one = _ws_bytes(nl, n_p, nw, 1, True, tape)
+    per = max(1, _ws_bytes(nl, n_p, nw, 2, True, tape) - one)
+    fixed = max(0, one - per)
+    step = max(1, min(ndir, (WORKSPACE_LIMIT - fixed) // per))
     cfg = getattr(_pacing, "cfg", None)
     if cfg is not None:
         step = max(1, min(step, int(cfg[0] // max(1, n_p * nw * (nl - 1)))))
