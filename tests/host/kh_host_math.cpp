@@ -4,6 +4,7 @@
 #include "kh_gmm_lanes.hpp"
 #include "kh_j0.hpp"
 #include "kh_sweep.hpp"
+#include "kh_wave.hpp"
 
 #include <vector>
 
@@ -107,3 +108,6 @@ extern "C" int host_gmm_forward(const double* layers, int nl, const double* p, i
   }
   return 0;
 }
+
+// The lane -> value map of the device's wave reduce-scatter (kh_wave.hpp).
+extern "C" int host_wave_rs_index(int n, int lane) { return wave_rs_index(n, lane); }

@@ -225,3 +225,37 @@ def test_j0_split_form_as_the_gather_kernel_forms_it(host_lib):
     err = (np.abs(out - scipy_j0(x)) - ampx * 0.5 * np.spacing(x)) / env
     assert err.max() < 3.5e-15, (err.max(), x[err.argmax()])
     assert np.sum(x >= 12.0) > 40000
+
+
+def _reduce_scatter_model(n):
+    """The exchange tree of kh_wave.hpp run on value indices: at each step a
+    lane and its partner hold the same list; the lane in the lower half of
+    the group keeps the first half of the (zero-padded) pairs, the upper
+    lane the second.  Partners: lane ^ 32 (v_permlane32_swap), lane ^ 16
+    (v_permlane16_swap), the mirror lane of a 16-lane row (DPP row_mirror),
+    of an 8-lane half row (row_half_mirror), lane ^ 2 and lane ^ 1 (quad
+    permutations)."""
+    lists = [list(range(n)) for _ in range(64)]
+    partner = [lambda l: l ^ 32, lambda l: l ^ 16, lambda l: (l & ~15) | (15 - (l & 15)),
+               lambda l: (l & ~7) | (7 - (l & 7)), lambda l: l ^ 2, lambda l: l ^ 1]
+    for step, s in enumerate(range(5, -1, -1)):
+        new = []
+        for lane in range(64):
+            q = partner[step](lane)
+            assert lists[q] == lists[lane] and ((q >> s) & 1) != ((lane >> s) & 1)
+            cur = lists[lane] + ([-1] if len(lists[lane]) % 2 else [])
+            h = len(cur) // 2
+            new.append(cur[h:] if (lane >> s) & 1 else cur[:h])
+        lists = new
+    return [lst[0] for lst in lists]
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 12, 13, 26, 40, 52, 63, 64])
+def test_wave_reduce_scatter_index_map(host_lib, n):
+    """wave_rs_index, which tells each lane whose sum it holds, against a
+    model of the exchange tree; every value is held by exactly one lane."""
+    f = host_lib.host_wave_rs_index
+    f.argtypes = [ctypes.c_int, ctypes.c_int]
+    got = [f(n, lane) for lane in range(64)]
+    assert got == _reduce_scatter_model(n)
+    assert sorted(i for i in got if i >= 0) == list(range(n))
