@@ -15,6 +15,12 @@
 // below x = 12 an interval polynomial (its table staged in LDS), above it
 // Hankel's form with 1/x and sqrt(2/(pi x)) formed as products of per-lane
 // factors 1/(w r), sqrt(2/(pi w r)) and per-slowness factors 1/p, sqrt(1/p).
+// Where a chunk's slownesses are uniform (p_j = p_0 + j dp to a few ulps,
+// checked on the device), x = w r p advances by the fixed angle w r dp from
+// one slowness to the next, so Hankel's cos and sin of x - pi/4 are rotated
+// by that angle (four fused multiply-adds) instead of evaluated: directly
+// only where a lane first enters the large regime in the chunk, whose 128
+// rotations change the factor by a few units of 2^-53.
 #include "kh_complex.hpp"
 #include "kh_j0.hpp"
 #include "kh_runtime.hpp"
@@ -44,6 +50,8 @@ __global__ void __launch_bounds__(BS_WAVE) bessel_partial_kernel(
   const int chunk = b / n_rgroups;
   const int j0_ = chunk * BS_PCHUNK;
   const int nj = min(BS_PCHUNK, np - j0_);
+  const double p0 = p[j0_];
+  int lin = dp > 0.0 && p0 >= 0.0;  // the chunk's slownesses are p0 + t dp
   for (int t = threadIdx.x; t < nj; t += blockDim.x) {
     const double pj = p[j0_ + t];
     const double taper = 0.5 * (1.0 + cos(M_PI * pj / p_max));
@@ -52,7 +60,9 @@ __global__ void __launch_bounds__(BS_WAVE) bessel_partial_kernel(
     s_tp[t] = taper * pj;
     s_ip[t] = ip;
     s_sip[t] = sqrt(ip);
+    lin &= fabs(pj - fma((double)t, dp, p0)) <= 8.0 * 2.220446049250313e-16 * fabs(pj);
   }
+  const bool uniform = __all(lin) != 0;  // one wave: the same in every lane
   for (int t = threadIdx.x; t < J0_XA * (J0_DEG_SMALL + 1); t += blockDim.x)
     s_j0[t] = (&J0_SMALL[0][0])[t];
   __syncthreads();
@@ -76,19 +86,61 @@ __global__ void __launch_bounds__(BS_WAVE) bessel_partial_kernel(
   for (int q = 0; q < BS_RG; ++q) acc[q] = cd{0.0, 0.0};
 
   const cd* Rcol = R + (size_t)j0_ * nwm + kc;
-  for (int t = 0; t < nj; ++t) {
-    const double pj = s_p[t];
-    const double wgt = (s_tp[t] * w) * dp;
-    const cd Rv = Rcol[(size_t)t * nwm];
-    const cd Wv = Rv * wgt;
-    const double x0 = w * pj;
-    const double ip = s_ip[t], sip = s_sip[t];
+  if (uniform) {
+    // (hc, hs) = cos, sin of x - pi/4 at the current slowness once the lane
+    // is in the large regime (started); (tc, ts) = cos, sin of w r dp
+    double hc[BS_RG], hs[BS_RG], tc[BS_RG], ts[BS_RG];
+    bool started[BS_RG];
 #pragma unroll
     for (int q = 0; q < BS_RG; ++q) {
-      const double x = fabs(x0 * rr[q]);
-      const double J = x < (double)J0_XA ? j0_small(x, s_j0) : j0_large(x, cu[q] * ip, ca[q] * sip);
-      acc[q].r += J * Wv.r;
-      acc[q].i += J * Wv.i;
+      sincos_phase(fabs(w * rr[q]) * dp, &ts[q], &tc[q]);
+      hc[q] = 1.0;
+      hs[q] = 0.0;
+      started[q] = false;
+    }
+    for (int t = 0; t < nj; ++t) {
+      const double pj = s_p[t];
+      const double wgt = (s_tp[t] * w) * dp;
+      const cd Rv = Rcol[(size_t)t * nwm];
+      const cd Wv = Rv * wgt;
+      const double x0 = w * pj;
+      const double ip = s_ip[t], sip = s_sip[t];
+#pragma unroll
+      for (int q = 0; q < BS_RG; ++q) {
+        const double x = fabs(x0 * rr[q]);
+        double J;
+        if (x < (double)J0_XA) {
+          J = j0_small(x, s_j0);
+        } else {
+          if (!started[q]) {
+            j0_phase(x, &hs[q], &hc[q]);
+            started[q] = true;
+          }
+          J = j0_large_cs(cu[q] * ip, ca[q] * sip, hc[q], hs[q]);
+          const double c2 = fma(hc[q], tc[q], -(hs[q] * ts[q]));  // advance by w r dp
+          hs[q] = fma(hs[q], tc[q], hc[q] * ts[q]);
+          hc[q] = c2;
+        }
+        acc[q].r += J * Wv.r;
+        acc[q].i += J * Wv.i;
+      }
+    }
+  } else {
+    for (int t = 0; t < nj; ++t) {
+      const double pj = s_p[t];
+      const double wgt = (s_tp[t] * w) * dp;
+      const cd Rv = Rcol[(size_t)t * nwm];
+      const cd Wv = Rv * wgt;
+      const double x0 = w * pj;
+      const double ip = s_ip[t], sip = s_sip[t];
+#pragma unroll
+      for (int q = 0; q < BS_RG; ++q) {
+        const double x = fabs(x0 * rr[q]);
+        const double J =
+            x < (double)J0_XA ? j0_small(x, s_j0) : j0_large(x, cu[q] * ip, ca[q] * sip);
+        acc[q].r += J * Wv.r;
+        acc[q].i += J * Wv.i;
+      }
     }
   }
   if (k < nwm) {

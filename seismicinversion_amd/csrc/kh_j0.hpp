@@ -33,17 +33,8 @@ __host__ __device__ __forceinline__ double j0_small(double x, const double* __re
   return v;
 }
 
-// x >= J0_XA (finite), u ~ 1/x, amp ~ sqrt(2/(pi x)).
-__host__ __device__ __forceinline__ double j0_large(double x, double u, double amp) {
-  const double z = u * u;
-  double P = J0_P[J0_DEG_PQ];
-  double Q = J0_Q[J0_DEG_PQ];
-#pragma unroll
-  for (int k = J0_DEG_PQ - 1; k >= 0; --k) {
-    P = fma_k(P, z, J0_P[k]);
-    Q = fma_k(Q, z, J0_Q[k]);
-  }
-  Q *= u;
+// cos and sin of x - pi/4 for the large regime (x >= J0_XA, finite).
+__host__ __device__ __forceinline__ void j0_phase(double x, double* sp, double* cp) {
   double s, c;
   if (x < 1.0e8) {
     // x - pi/4 = m pi/2 + r with r = x - (m + 1/2) pi/2
@@ -60,7 +51,30 @@ __host__ __device__ __forceinline__ double j0_large(double x, double u, double a
     c = (cx + sx) * 0.7071067811865476;
     s = (sx - cx) * 0.7071067811865476;
   }
+  *sp = s;
+  *cp = c;
+}
+
+// Hankel's form given u ~ 1/x, amp ~ sqrt(2/(pi x)) and (c, s) = (cos, sin)
+// of x - pi/4.
+__host__ __device__ __forceinline__ double j0_large_cs(double u, double amp, double c, double s) {
+  const double z = u * u;
+  double P = J0_P[J0_DEG_PQ];
+  double Q = J0_Q[J0_DEG_PQ];
+#pragma unroll
+  for (int k = J0_DEG_PQ - 1; k >= 0; --k) {
+    P = fma_k(P, z, J0_P[k]);
+    Q = fma_k(Q, z, J0_Q[k]);
+  }
+  Q *= u;
   return amp * fma(P, c, -(Q * s));
+}
+
+// x >= J0_XA (finite), u ~ 1/x, amp ~ sqrt(2/(pi x)).
+__host__ __device__ __forceinline__ double j0_large(double x, double u, double amp) {
+  double s, c;
+  j0_phase(x, &s, &c);
+  return j0_large_cs(u, amp, c, s);
 }
 
 // J0(x) for any double x (even function; J0(+-inf) = 0, NaN gives NaN).
