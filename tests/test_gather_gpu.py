@@ -139,6 +139,28 @@ def test_bessel_sum_linearity_and_zero_offset():
     assert rel(U1, U_o) < 1e-12
 
 
+@pytest.mark.parametrize("kind", ["offset_grid", "irregular"])
+def test_bessel_sum_uniform_and_irregular_slownesses(kind):
+    """The sum against the oracle for slownesses p0 + j dp with p0 != dp
+    (chunks of uniform slownesses rotate Hankel's phase by w r dp from one
+    slowness to the next) and for sorted random slownesses (no fixed step:
+    every J0 is evaluated directly), over the small and large regimes."""
+    rng = np.random.default_rng(21)
+    np_, nw, dp = 300, 97, 0.9 / 300
+    if kind == "offset_grid":
+        pn = 0.05 + np.arange(np_) * dp
+    else:
+        pn = np.sort(rng.uniform(0.01, 0.95, np_))
+    offsets = np.array([0.0, 0.7, 3.0, 12.5, 40.0])
+    dw = 2 * np.pi / 16.0
+    Rn = rng.standard_normal((np_, nw - 1)) + 1j * rng.standard_normal((np_, nw - 1))
+    U = engine.bessel_sum(torch.from_numpy(Rn).cuda(), torch.from_numpy(pn), dp, pn[-1], 1.0, dw,
+                          torch.from_numpy(offsets)).cpu().numpy()
+    w = np.arange(1, nw) * dw
+    U_o = K.bessel_sum(Rn, pn, w, offsets, dp, p_max=pn[-1])
+    assert rel(U, U_o) < 1e-12
+
+
 def test_full_c2_gather_properties():
     """Full gather workload: 20 layers, 8192 p, nt = 2048, 100 offsets."""
     cfg = gather_config_c2()
