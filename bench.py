@@ -81,7 +81,8 @@ def parse():
                     help="processes of the all-core CPU baseline (the box's share is 16 cores)")
     ap.add_argument("--warmup-seconds", type=float, default=2.0,
                     help="untimed forward launches before anything is timed (clock ramp)")
-    ap.add_argument("--section", choices=("all", "lm", "lmloop", "open", "served", "hessian"),
+    ap.add_argument("--section", choices=("all", "lm", "lmloop", "open", "served", "hessian",
+                                          "gather"),
                     default="all",
                     help="'lm': only the 20-layer Newton-LM figures; 'hessian': only --steps "
                          "misfit + gradient + exact-Hessian evaluations of that LM after "
@@ -89,7 +90,8 @@ def parse():
                          "run for --steps iterations on the device loop, after a --warmup "
                          "iteration run (for profiling a whole iteration's kernels); 'open': "
                          "only the open-loop rate sweep; 'served': only the headline served "
-                         "steps and their roofline")
+                         "steps and their roofline; 'gather': only the 20-layer gather "
+                         "(8192 p x 1023 omega, 100 offsets), --steps runs after --warmup")
     ap.add_argument("--traffic-json", default=os.path.join(HERE, "profiles", "traffic_multi_served.json"),
                     help="HBM bytes per launch of the multi-model kernel from rocprofv3 --pmc")
     return ap.parse_args()
@@ -302,6 +304,28 @@ def c1_figure(torch, dev, c1_cpu):
             "dropin_host_arrays_ms": t_drop * 1e3, "device_resident_ms": ms,
             "dropin_over_cpu": c1_cpu["ms"] / (t_drop * 1e3),
             "max_row_rel_diff_vs_oracle": diff}
+
+
+def gather_figures(torch, dev, steps, warmup):
+    """The 20-layer gather (8192 p x 1023 omega, 100 offsets, nt 2048), inputs
+    resident, timed over `steps` runs after `warmup` untimed ones (for
+    profiling its kernels)."""
+    from seismicinversion_amd import engine
+    from seismicinversion_amd.kennett_gather_gpu import gather_device
+    from seismicinversion_amd.source import ricker_spectrum
+    from seismicinversion_amd.workloads import gather_config_c2
+
+    cfg = gather_config_c2()
+    L = engine.layers_tensor(cfg.model, dev)
+    dw, gamma = 2 * np.pi / cfg.T, np.pi / cfg.T
+    off = torch.from_numpy(cfg.offsets).to(dev)
+    S = torch.from_numpy(ricker_spectrum(np.arange(1, cfg.nw) * dw, cfg.ricker_omega_max)).to(dev)
+    gat = lambda: gather_device(L, off, cfg.T, cfg.nw, cfg.np_slow, cfg.p_max, gamma, S, True)  # noqa: E731
+    for _ in range(max(1, warmup)):
+        gat()
+    return {"workload": "20 layers, 8192 p x 1023 omega, 100 offsets, nt 2048",
+            "steps": steps, "ms": event_ms(gat, max(1, steps), torch),
+            "bessel_terms": len(cfg.offsets) * cfg.np_slow * (cfg.nw - 1)}
 
 
 def kernel_figures(torch, dev):
@@ -663,6 +687,13 @@ def main():
         dev = torch.device("cuda", 0)
         torch.cuda.set_device(dev)
         print(json.dumps({"section": "hessian", "lm_20_layers_evaluation": hessian_figures(
+            torch, dev, args.steps, args.warmup)}))
+        return
+    if args.section == "gather":
+        dev = torch.device("cuda", 0)
+        torch.cuda.set_device(dev)
+        warm_clocks(torch, dev, args.warmup_seconds)
+        print(json.dumps({"section": "gather", "gather": gather_figures(
             torch, dev, args.steps, args.warmup)}))
         return
     if args.section == "open":
