@@ -28,10 +28,11 @@
 namespace kh {
 
 constexpr int BS_WAVE = 64;   // frequencies per stage-1 workgroup (one wave)
-// offsets per stage-1 workgroup: 4 keeps the kernel at 119 VGPRs, four waves
-// per SIMD; 10 needed 179 (two waves) and ran 1.3x slower on the 100-offset
-// gather (2.86 against 2.21 ms; 2 and 3 offsets time as 4)
-constexpr int BS_RG = 4;
+// offsets per stage-1 workgroup: with the rotated Hankel phase, 3 times best
+// on the 100-offset gather (1.66 ms; 2: 1.71, 4: 1.79 at 142 VGPRs, three
+// waves per SIMD; 5: 2.31); before it, 4 offsets at 119 VGPRs and 10 at 179
+// (two waves, 1.3x slower)
+constexpr int BS_RG = 3;
 constexpr int BS_PCHUNK = 128;  // slownesses per stage-1 workgroup
 
 __global__ void __launch_bounds__(BS_WAVE) bessel_partial_kernel(
