@@ -29,6 +29,7 @@
 // Loop state: KH_LM_STATE_LEN float64 values (see kennett_hip.h).
 #include "kh_physics.hpp"
 #include "kh_runtime.hpp"
+#include "kh_wave.hpp"
 #include "kh_sweep.hpp"
 
 namespace kh {
@@ -197,6 +198,30 @@ __device__ __forceinline__ bool lm_trials_open(const double* state, int first) {
 // entries, whose trial misfit is NaN and is never accepted.
 constexpr int LM_LU_BLOCK = 1024;
 
+// The largest |pivot| candidate over the wave and its row: the first row of
+// largest magnitude (ties to the lower row, NaN never chosen), the order
+// LAPACK's idamax uses, reduced over DPP and permlane exchanges (no LDS).
+template <int S>
+__device__ __forceinline__ void lm_argmax_step(double& best, int& bi) {
+  const long long b = __double_as_longlong(best);
+  const unsigned lo = wave_partner_b32<S>((unsigned)b);
+  const unsigned hi = wave_partner_b32<S>((unsigned)((unsigned long long)b >> 32));
+  const double ob = __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+  const int oi = (int)wave_partner_b32<S>((unsigned)bi);
+  if (ob > best || (ob == best && oi < bi)) {
+    best = ob;
+    bi = oi;
+  }
+}
+__device__ __forceinline__ void lm_wave_argmax(double& best, int& bi) {
+  lm_argmax_step<0>(best, bi);
+  lm_argmax_step<1>(best, bi);
+  lm_argmax_step<2>(best, bi);
+  lm_argmax_step<3>(best, bi);
+  lm_argmax_step<4>(best, bi);
+  lm_argmax_step<5>(best, bi);
+}
+
 __device__ __forceinline__ void lm_wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
   __builtin_amdgcn_wave_barrier();
@@ -247,14 +272,7 @@ __global__ void __launch_bounds__(LM_LU_BLOCK) lm_trials_kernel(
           bi = i;
         }
       }
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ob = __shfl_xor(best, off, 64);
-        const int oi = __shfl_xor(bi, off, 64);
-        if (ob > best || (ob == best && oi < bi)) {
-          best = ob;
-          bi = oi;
-        }
-      }
+      lm_wave_argmax(best, bi);
       if (tid == 0 && bi < P && bi != k) {  // an all-NaN column keeps its row
         const int r = s_perm[k];
         s_perm[k] = s_perm[bi];
@@ -267,13 +285,19 @@ __global__ void __launch_bounds__(LM_LU_BLOCK) lm_trials_kernel(
       for (int i = k + 1 + tid; i < P; i += 64) s_l[i] = A[(size_t)s_perm[i] * ld + k] * inv;
     }
     __syncthreads();
-    const int rows = P - 1 - k, cols = ld - 1 - k;
+    // each work-item keeps one column of the trailing matrix and strides
+    // over its rows (every element updated once, as before)
+    static_assert(LM_LU_BLOCK >= LM_LDS_MAX_P, "one work-item per trailing column at least");
+    const int cols = ld - 1 - k, per = nthr / cols;
+    const int c = tid % cols, r0 = tid / cols;
+    if (r0 < per) {
+      const int j = k + 1 + c;
+      const double pj = s_row[j];
 #pragma unroll 4
-    for (long idx = tid; idx < (long)rows * cols; idx += nthr) {
-      const int r = k + 1 + (int)(idx / cols);
-      const int j = k + 1 + (int)(idx - (long)(r - k - 1) * cols);
-      double* a = A + (size_t)s_perm[r] * ld + j;
-      *a = fma(-s_l[r], s_row[j], *a);
+      for (int r = k + 1 + r0; r < P; r += per) {
+        double* a = A + (size_t)s_perm[r] * ld + j;
+        *a = fma(-s_l[r], pj, *a);
+      }
     }
     __syncthreads();
   }
@@ -358,14 +382,7 @@ __global__ void __launch_bounds__(LM_LU_BLOCK) lm_trials_blocked_kernel(
             bi = r;
           }
         }
-        for (int off = 32; off > 0; off >>= 1) {
-          const double ob = __shfl_xor(best, off, 64);
-          const int oi = __shfl_xor(bi, off, 64);
-          if (ob > best || (ob == best && oi < bi)) {
-            best = ob;
-            bi = oi;
-          }
-        }
+        lm_wave_argmax(best, bi);
         if (bi < m && bi != c) {  // an all-NaN column keeps its row
           if (tid < nbk) {
             const double a = s_pan[c * LM_NB + tid];

@@ -90,6 +90,26 @@ struct WaveRs {
   }
 };
 
+// The 32-bit value x of this lane's partner at step S of an all-reduce
+// over the wave: lane ^ 1, lane ^ 2 (quad permutations), the mirror lane of
+// an 8-lane half row, of a 16-lane row, then lane ^ 16 and lane ^ 32 (the
+// permlane swaps).  After the six steps, combining each lane's value with
+// its partner's at every step, every lane holds the combination of all 64.
+template <int S>
+__device__ __forceinline__ unsigned wave_partner_b32(unsigned x) {
+  if constexpr (S == 0) return (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0xb1, 0xf, 0xf, false);
+  else if constexpr (S == 1) return (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x4e, 0xf, 0xf, false);
+  else if constexpr (S == 2) return (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xf, 0xf, false);
+  else if constexpr (S == 3) return (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x140, 0xf, 0xf, false);
+  else {
+    // permlaneN_swap(x, x): the lower group's copy of the upper group's x
+    // in the second result, the upper group's copy of the lower's in the first
+    auto r = S == 4 ? __builtin_amdgcn_permlane16_swap(x, x, false, false)
+                    : __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    return ((threadIdx.x >> S) & 1) ? r[0] : r[1];
+  }
+}
+
 template <int N>
 __device__ __forceinline__ double wave_reduce_scatter(const double (&x)[N]) {
   static_assert(N >= 1 && N <= 64, "1 to 64 values per lane");
