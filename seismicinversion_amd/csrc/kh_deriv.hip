@@ -446,7 +446,6 @@ __global__ void __launch_bounds__(CH_BLOCK) chain_kernel(const double* __restric
   }
   __syncthreads();
   const Z zero = lift<Z>(0.0);
-  const IfaceZ<Z> none{zero, zero, zero, zero, zero, zero, zero, zero, zero, zero};
   const double* dl = dlayers ? dlayers + (size_t)dir * nl * NCOL : nullptr;
   // with `partial`, the reverse sweep stopped one interface below the
   // deepest moved layer (direction 0 excepted): only the layers above it
@@ -459,8 +458,10 @@ __global__ void __launch_bounds__(CH_BLOCK) chain_kernel(const double* __restric
       if (hvp) hvp[(size_t)dir * np * nl * NCOL + g] = 0.0;
       continue;
     }
-    const Z v = chain_column<Z>(layers, dl, nl, l, p[ip], l >= 1 ? s_if[l - 1] : none,
-                                l <= nl - 2 ? s_if[l] : none, s_ly[l], col);
+    // chain_column ignores the interface above layer 0 and below layer nl-1:
+    // any table entry stands in (a placeholder value would live in scratch)
+    const Z v = chain_column<Z>(layers, dl, nl, l, p[ip], s_if[l >= 1 ? l - 1 : 0],
+                                s_if[l <= nl - 2 ? l : nl - 2], s_ly[l], col);
     if (grad && dir == 0) grad[g] = prim(v).r;
     if (hvp) hvp[(size_t)dir * np * nl * NCOL + g] = tangent_re(v);
   }
@@ -631,13 +632,11 @@ __global__ void __launch_bounds__(CH_BLOCK) chain_near_kernel(
   const LayerCot<dcd>* cly = cot_ly + base * nl;
   const double* dl = dlayers + (size_t)dir * nl * NCOL;
   const Reach rc = reach_of(dl, nl, dir, partial);
-  const dcd zero = lift<dcd>(0.0);
-  const IfaceZ<dcd> none{zero, zero, zero, zero, zero, zero, zero, zero, zero, zero};
   for (int j = j0; rc.lo + j / NCOL <= rc.hi; j += NEAR_THREADS) {
     const int l = rc.lo + j / NCOL, col = j - (j / NCOL) * NCOL;
     const size_t g = ((size_t)ip * nl + l) * NCOL + col;
-    const dcd v = chain_column<dcd>(layers, dl, nl, l, p[ip], l >= 1 ? cif[l - 1] : none,
-                                    l <= nl - 2 ? cif[l] : none, cly[l], col);
+    const dcd v = chain_column<dcd>(layers, dl, nl, l, p[ip], cif[l >= 1 ? l - 1 : 0],
+                                    cif[l <= nl - 2 ? l : nl - 2], cly[l], col);
     if (grad && dir == 0) grad[g] = prim(v).r;
     hvp[(size_t)dir * np * nl * NCOL + g] = tangent_re(v);
   }

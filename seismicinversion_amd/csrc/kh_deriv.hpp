@@ -502,7 +502,7 @@ __host__ __device__ __forceinline__ IfaceZ<Z> outer_iface(const IfaceZ<W>& f) {
 // dual Z).  The thickness enters only through the phase, whose cotangent
 // is lc.h.
 template <class Z>
-__host__ __device__ inline Z chain_column(const double* layers, const double* dlayers, int nl,
+__host__ __device__ __forceinline__ Z chain_column(const double* layers, const double* dlayers, int nl,
                                           int l, double p, const IfaceZ<Z>& cb_up,
                                           const IfaceZ<Z>& cb_dn, const LayerCot<Z>& lc,
                                           int col) {
@@ -518,14 +518,24 @@ __host__ __device__ inline Z chain_column(const double* layers, const double* dl
     }
     return layer_t<W>(x, ll, p);
   };
-  const LayerZ<W> mid = layer(l);
-  Z acc = re_dot(Up<Z>::outer(mid.eta), lc.eta) + re_dot(Up<Z>::outer(mid.neta), lc.neta);
-  if (l >= 1) {
-    const IfaceZ<W> f = interface_t<W>(l - 1, p, layer(l - 1), mid);
-    acc = acc + iface_dot(outer_iface<W, Z>(f), cb_up);
+  Z acc;
+  {
+    const LayerZ<W> mid = layer(l);
+    acc = re_dot(Up<Z>::outer(mid.eta), lc.eta) + re_dot(Up<Z>::outer(mid.neta), lc.neta);
+    if (l >= 1) {
+      const IfaceZ<W> f = interface_t<W>(l - 1, p, layer(l - 1), mid);
+      acc = acc + iface_dot(outer_iface<W, Z>(f), cb_up);
+    }
   }
   if (l <= nl - 2) {
-    const IfaceZ<W> f = interface_t<W>(l, p, mid, layer(l + 1));
+    // layer l again rather than kept live across the interface above (the
+    // same numbers; the opaque copy of l stops the compiler from merging
+    // the two evaluations, which would keep 40 doubles live)
+    int lo = l;
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(lo));
+#endif
+    const IfaceZ<W> f = interface_t<W>(l, p, layer(lo), layer(l + 1));
     acc = acc + iface_dot(outer_iface<W, Z>(f), cb_dn);
   }
   return acc;
