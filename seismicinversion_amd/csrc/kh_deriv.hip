@@ -411,6 +411,21 @@ __device__ __forceinline__ double tangent_re(const dcd& z) { return z.d.r; }
 // fills the device.
 constexpr int CH_BLOCK = 128;
 
+// g[0] + g[stride] + ... + g[(ntiles - 1) stride], added in that order from
+// 0.0, with eight loads in flight at a time.
+__device__ __forceinline__ double tile_sum(const double* g, size_t stride, int ntiles) {
+  double acc = 0.0;
+  for (int t0 = 0; t0 < ntiles; t0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = t0 + u < ntiles ? g[(size_t)(t0 + u) * stride] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (t0 + u < ntiles) acc += v[u];
+  }
+  return acc;
+}
+
 template <class Z>
 __global__ void __launch_bounds__(CH_BLOCK) chain_kernel(const double* __restrict__ layers, const double* __restrict__ dlayers,
                              int nl, const double* __restrict__ p, int np, int ndir,
@@ -434,15 +449,8 @@ __global__ void __launch_bounds__(CH_BLOCK) chain_kernel(const double* __restric
   double* sif = reinterpret_cast<double*>(s_if);
   double* sly = reinterpret_cast<double*>(s_ly);
   for (int e = threadIdx.x; e < nall; e += blockDim.x) {
-    double acc = 0.0;
-    if (e < nif) {
-      for (int t = 0; t < ntiles; ++t) acc += gif[(size_t)t * nif + e];
-      sif[e] = acc;
-    } else {
-      const int e2 = e - nif;
-      for (int t = 0; t < ntiles; ++t) acc += gly[(size_t)t * (nall - nif) + e2];
-      sly[e2] = acc;
-    }
+    if (e < nif) sif[e] = tile_sum(gif + e, (size_t)nif, ntiles);
+    else sly[e - nif] = tile_sum(gly + (e - nif), (size_t)(nall - nif), ntiles);
   }
   __syncthreads();
   const Z zero = lift<Z>(0.0);
@@ -526,15 +534,9 @@ __global__ void __launch_bounds__(CH_BLOCK) cot_sum_kernel(int nl, int np,
   double* gif = reinterpret_cast<double*>(cot_if + base * (nl - 1));
   double* gly = reinterpret_cast<double*>(cot_ly + base * nl);
   for (int e = threadIdx.x; e < nall; e += blockDim.x) {
-    double acc = 0.0;
-    if (e < nif) {
-      for (int t = 0; t < ntiles; ++t) acc += gif[(size_t)t * nif + e];
-      gif[e] = acc;
-    } else {
-      const int e2 = e - nif;
-      for (int t = 0; t < ntiles; ++t) acc += gly[(size_t)t * (nall - nif) + e2];
-      gly[e2] = acc;
-    }
+    double* const g = e < nif ? gif + e : gly + (e - nif);
+    const size_t stride = e < nif ? (size_t)nif : (size_t)(nall - nif);
+    *g = tile_sum(g, stride, ntiles);
   }
   if (chi && chi_part && threadIdx.x == 0) {
     Z sz = lift<Z>(0.0);
