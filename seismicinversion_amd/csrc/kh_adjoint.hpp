@@ -95,8 +95,8 @@ __host__ __device__ __forceinline__ M2<Z> step_fwd(const M2<Z>& r, const EZ& ea,
 }
 
 // Adjoint of step_fwd.  Given r = S_{i+1}, the phases, f and sb = S_i_bar:
-// returns rb = S_{i+1}_bar, adds the coefficient cotangents into fb and
-// returns the phase cotangents eab, ebb.
+// returns rb = S_{i+1}_bar, the coefficient cotangents in fb (the Rd and Td
+// ones stored, the Ru ones added to fb's) and the phase cotangents eab, ebb.
 template <class Z, class EZ, class FZ>
 __host__ __device__ __forceinline__ void step_rev(const M2<Z>& r, const EZ& ea, const EZ& eb,
                                                   const IfaceZ<FZ>& f, const M2<Z>& sb,
@@ -104,9 +104,9 @@ __host__ __device__ __forceinline__ void step_rev(const M2<Z>& r, const EZ& ea, 
   const StepVals<Z, EZ> v = step_vals(r, ea, eb, f);  // recompute the forward intermediates
 
   // S = Rd + Y Td
-  fb.rd00 = fb.rd00 + sb.a00;
-  fb.rd01 = fb.rd01 + (sb.a01 + sb.a10);
-  fb.rd11 = fb.rd11 + sb.a11;
+  fb.rd00 = sb.a00;
+  fb.rd01 = sb.a01 + sb.a10;
+  fb.rd11 = sb.a11;
   const Z yb00 = cfma(sb.a01, conj(f.td01), sb.a00 * conj(f.td00));
   const Z yb01 = cfma(sb.a01, conj(f.td11), sb.a00 * conj(f.td10));
   const Z yb10 = cfma(sb.a11, conj(f.td01), sb.a10 * conj(f.td00));
@@ -124,10 +124,10 @@ __host__ __device__ __forceinline__ void step_rev(const M2<Z>& r, const EZ& ea, 
   const Z xb01 = cfma(conj(f.td01), yb11, conj(f.td00) * yb01);
   const Z xb10 = cfma(conj(f.td11), yb10, conj(f.td10) * yb00);
   const Z xb11 = cfma(conj(f.td11), yb11, conj(f.td10) * yb01);
-  fb.td00 = fb.td00 + tdb00;
-  fb.td01 = fb.td01 + tdb01;
-  fb.td10 = fb.td10 + tdb10;
-  fb.td11 = fb.td11 + tdb11;
+  fb.td00 = tdb00;
+  fb.td01 = tdb01;
+  fb.td10 = tdb10;
+  fb.td11 = tdb11;
   // X = M U
   Z mb00 = cfma(xb01, conj(v.u01), xb00 * conj(v.u00));
   Z mb01 = cfma(xb01, conj(v.u11), xb00 * conj(v.u10));

@@ -307,10 +307,8 @@ __host__ __device__ inline Z lane_adjoint_k(const LZ* __restrict__ L,
   auto level = [&](int i, const M2<Z>& Sin, const auto& ea, const auto& eb, auto flat_iface,
                    auto&& then) {
     using EZ = std::decay_t<decltype(ea)>;
-    IfaceZ<Z> fbar;
-    fbar.rd00 = fbar.rd01 = fbar.rd11 = zero_z<Z>();
+    IfaceZ<Z> fbar;  // step_rev stores the Rd and Td cotangents, adds the Ru ones
     fbar.ru00 = fbar.ru01 = fbar.ru11 = zero_z<Z>();
-    fbar.td00 = fbar.td01 = fbar.td10 = fbar.td11 = zero_z<Z>();
     M2<Z> rb;
     Z eab, ebb;
     if constexpr (decltype(flat_iface)::value)
