@@ -78,6 +78,22 @@ __device__ __forceinline__ void packed_lc(int k, int n, int& l, int& c) {
   l = k - c * n + 1;
 }
 
+// src[0] + src[stride] + ... + src[(n - 1) stride], added in that order
+// from 0.0, with eight loads in flight at a time.
+__device__ __forceinline__ double lm_strided_sum(const double* __restrict__ src, size_t stride,
+                                                 int n) {
+  double s = 0.0;
+  for (int q0 = 0; q0 < n; q0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = q0 + u < n ? src[(size_t)(q0 + u) * stride] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (q0 + u < n) s += v[u];
+  }
+  return s;
+}
+
 // The gradient and Hessian in the packed parameters, summed over the
 // slownesses in ascending order, from the per-slowness outputs of
 // kh_misfit_hvp_partial over a basis (rows (P, np, nl, 6); direction
@@ -106,16 +122,13 @@ __global__ void lm_assemble_kernel(int nl, int np, const double* __restrict__ ro
     const int k = mirror ? b : a;
     const int lj = mirror ? la : lb, cj = mirror ? ca : cb;
     const double* src = rows + (size_t)order[k] * np * per + (size_t)lj * NCOL + cj;
-    double s = 0.0;
-    for (int q = 0; q < np; ++q) s += src[(size_t)q * per];
+    const double s = lm_strided_sum(src, per, np);
     H[t] = G != nullptr ? G[(size_t)order[a] * P + order[b]] + s : s;
   } else if (t < PP + P) {
     const int k = (int)(t - PP);
     int l, c;
     packed_lc(k, n, l, c);
-    double s = 0.0;
-    for (int q = 0; q < np; ++q) s += grad[(size_t)q * per + (size_t)l * NCOL + c];
-    g[k] = s;
+    g[k] = lm_strided_sum(grad + (size_t)l * NCOL + c, per, np);
   }
 }
 
@@ -144,8 +157,7 @@ __global__ void __launch_bounds__(LM_BLOCK) lm_prepare_kernel(
     H_log[idx] = v;
   }
   if (threadIdx.x != 0) return;
-  double chi = 0.0;
-  for (int p = 0; p < np; ++p) chi += chi_rows[(size_t)p * chi_stride];
+  const double chi = lm_strided_sum(chi_rows, (size_t)chi_stride, np);
   double gg = 0.0, ee = 0.0, tt = 0.0, dmax = 0.0;
   for (int j = 0; j < P; ++j) {
     const double m = s_m[j];
@@ -525,8 +537,7 @@ __global__ void __launch_bounds__(256) lm_decide_kernel(int P, int first, int nt
     for (int k = 0; k < first; ++k) lam *= 5.0;
     int acc = -1;
     for (int t = first; t < first + ntrials; ++t) {
-      double s = 0.0;
-      for (int q = 0; q < np; ++q) s += chi_trials[(size_t)t * np + q];
+      const double s = lm_strided_sum(chi_trials + (size_t)t * np, 1, np);
       if (s < chi) {
         acc = t;
         break;
