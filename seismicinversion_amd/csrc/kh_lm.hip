@@ -499,7 +499,18 @@ __global__ void __launch_bounds__(LM_MISFIT_BLOCK) lm_trial_misfit_kernel(
   stage_slowness(tables + (size_t)t * nl * NCOL, nl, p[ip], s_layer, s_if);
   const cd* obs = R_obs + (size_t)ip * nw;
   double acc = 0.0;
-  for (int w = threadIdx.x; w < nw; w += LM_MISFIT_BLOCK) {
+  // frequencies w and w + LM_MISFIT_BLOCK swept together (two independent
+  // lane sweeps, each the single-frequency arithmetic), added in order
+  int w = threadIdx.x;
+  for (; w + LM_MISFIT_BLOCK < nw; w += 2 * LM_MISFIT_BLOCK) {
+    const cd om[2] = {omega[w], omega[w + LM_MISFIT_BLOCK]};
+    cd out[2];
+    sweep_lanes<2, false>(s_layer, s_if, static_cast<const LayerStep*>(nullptr), nl, om,
+                          free_surface, out);
+    acc = add_sq_diff(acc, out[0], obs[w]);
+    acc = add_sq_diff(acc, out[1], obs[w + LM_MISFIT_BLOCK]);
+  }
+  if (w < nw) {
     const cd om[1] = {omega[w]};
     cd out[1];
     sweep_lanes<1, false>(s_layer, s_if, static_cast<const LayerStep*>(nullptr), nl, om,
