@@ -343,6 +343,9 @@ __global__ void __launch_bounds__(LM_LU_BLOCK) lm_trials_kernel(
 // lm_trials_kernel (the same arithmetic per element), while the trailing
 // matrix is read and written once per panel instead of once per column.
 constexpr int LM_NB = 8;
+// panel row stride in LDS: one double of padding, so that the lanes of a
+// wave reading one column of consecutive rows hit different banks
+constexpr int LM_PS = LM_NB + 1;
 
 __global__ void __launch_bounds__(LM_LU_BLOCK) lm_trials_blocked_kernel(
     int nl, int first, const double* __restrict__ H, const double* __restrict__ g,
@@ -350,7 +353,7 @@ __global__ void __launch_bounds__(LM_LU_BLOCK) lm_trials_blocked_kernel(
     const double* __restrict__ state, double* __restrict__ trial_x,
     double* __restrict__ tables, double* __restrict__ ws) {
   __shared__ int s_perm[LM_MAX_P];
-  __shared__ double s_pan[LM_MAX_P * LM_NB];  // panel: logical rows kb.., LM_NB columns
+  __shared__ double s_pan[LM_MAX_P * LM_PS];  // panel: logical rows kb.., LM_NB columns
   __shared__ double s_l[LM_MAX_P];            // x
   __shared__ double s_row[LM_MAX_P + 1];      // the right-hand side in back substitution
   if (!lm_trials_open(state, first)) return;  // this kernel never writes the state
@@ -380,7 +383,7 @@ __global__ void __launch_bounds__(LM_LU_BLOCK) lm_trials_blocked_kernel(
     const int nbk = min(LM_NB, P - kb), m = P - kb;
     for (int e = tid; e < m * nbk; e += nthr) {
       const int r = e / nbk, c = e - (e / nbk) * nbk;
-      s_pan[r * LM_NB + c] = A[(size_t)s_perm[kb + r] * ld + kb + c];
+      s_pan[r * LM_PS + c] = A[(size_t)s_perm[kb + r] * ld + kb + c];
     }
     __syncthreads();
     if (tid < 64) {  // the panel, column by column
@@ -388,7 +391,7 @@ __global__ void __launch_bounds__(LM_LU_BLOCK) lm_trials_blocked_kernel(
         double best = -1.0;
         int bi = m;
         for (int r = c + tid; r < m; r += 64) {
-          const double v = fabs(s_pan[r * LM_NB + c]);
+          const double v = fabs(s_pan[r * LM_PS + c]);
           if (v > best) {
             best = v;
             bi = r;
@@ -397,9 +400,9 @@ __global__ void __launch_bounds__(LM_LU_BLOCK) lm_trials_blocked_kernel(
         lm_wave_argmax(best, bi);
         if (bi < m && bi != c) {  // an all-NaN column keeps its row
           if (tid < nbk) {
-            const double a = s_pan[c * LM_NB + tid];
-            s_pan[c * LM_NB + tid] = s_pan[bi * LM_NB + tid];
-            s_pan[bi * LM_NB + tid] = a;
+            const double a = s_pan[c * LM_PS + tid];
+            s_pan[c * LM_PS + tid] = s_pan[bi * LM_PS + tid];
+            s_pan[bi * LM_PS + tid] = a;
           }
           if (tid == 0) {
             const int r = s_perm[kb + c];
@@ -408,12 +411,20 @@ __global__ void __launch_bounds__(LM_LU_BLOCK) lm_trials_blocked_kernel(
           }
         }
         lm_wave_lds_sync();
-        const double inv = 1.0 / s_pan[c * LM_NB + c];
+        const double inv = 1.0 / s_pan[c * LM_PS + c];
+        // the pivot row in registers; each row read whole, updated, written
+        double prow[LM_NB];
+#pragma unroll
+        for (int cc = 0; cc < LM_NB; ++cc) prow[cc] = s_pan[c * LM_PS + cc];
         for (int r = c + 1 + tid; r < m; r += 64) {
-          const double l = s_pan[r * LM_NB + c] * inv;
-          s_pan[r * LM_NB + c] = l;
-          for (int cc = c + 1; cc < nbk; ++cc)
-            s_pan[r * LM_NB + cc] = fma(-l, s_pan[c * LM_NB + cc], s_pan[r * LM_NB + cc]);
+          double v[LM_NB];
+#pragma unroll
+          for (int cc = 0; cc < LM_NB; ++cc) v[cc] = s_pan[r * LM_PS + cc];
+          const double l = s_pan[r * LM_PS + c] * inv;
+#pragma unroll
+          for (int cc = 0; cc < LM_NB; ++cc)
+            if (cc > c && cc < nbk) s_pan[r * LM_PS + cc] = fma(-l, prow[cc], v[cc]);
+          s_pan[r * LM_PS + c] = l;
         }
         lm_wave_lds_sync();
       }
@@ -422,7 +433,7 @@ __global__ void __launch_bounds__(LM_LU_BLOCK) lm_trials_blocked_kernel(
     // the panel's pivot rows (upper part) back to A, for the back substitution
     for (int e = tid; e < nbk * nbk; e += nthr) {
       const int i = e / nbk, c = e - (e / nbk) * nbk;
-      if (c >= i) A[(size_t)s_perm[kb + i] * ld + kb + c] = s_pan[i * LM_NB + c];
+      if (c >= i) A[(size_t)s_perm[kb + i] * ld + kb + c] = s_pan[i * LM_PS + c];
     }
     // the columns right of the panel: pivot-row entries, then the rows below.
     // G work-items per column (each takes every G-th row below the panel)
@@ -441,14 +452,14 @@ __global__ void __launch_bounds__(LM_LU_BLOCK) lm_trials_blocked_kernel(
       for (int i = 1; i < LM_NB; ++i)
 #pragma unroll
         for (int c = 0; c < LM_NB; ++c)
-          if (i < nbk && c < i) u[i] = fma(-s_pan[i * LM_NB + c], u[c], u[i]);
-#pragma unroll 4
+          if (i < nbk && c < i) u[i] = fma(-s_pan[i * LM_PS + c], u[c], u[i]);
+#pragma unroll 8
       for (int r = nbk + grp; r < m; r += G) {
         double* a = A + (size_t)s_perm[kb + r] * ld + j;
         double v = *a;
 #pragma unroll
         for (int c = 0; c < LM_NB; ++c)
-          if (c < nbk) v = fma(-s_pan[r * LM_NB + c], u[c], v);
+          if (c < nbk) v = fma(-s_pan[r * LM_PS + c], u[c], v);
         *a = v;
       }
     }
@@ -462,13 +473,47 @@ __global__ void __launch_bounds__(LM_LU_BLOCK) lm_trials_blocked_kernel(
   }
   if (tid >= 64) return;  // back substitution and the outputs: the first wave
 
-  for (int i = tid; i < P; i += 64) s_row[i] = A[(size_t)s_perm[i] * ld + P];
+  // the right-hand side and U's diagonal in LDS (the panel buffer is free);
+  // lane l keeps the rows j = l + 64 k of the column being eliminated in
+  // registers and loads the next column while this one is applied (the
+  // same operations in the same order as a column-by-column sweep)
+  constexpr int KB = (LM_MAX_P + 63) / 64;
+  double* s_diag = s_pan;
+  for (int i = tid; i < P; i += 64) {
+    const double* row = A + (size_t)s_perm[i] * ld;
+    s_row[i] = row[P];
+    s_diag[i] = row[i];
+  }
+  const double* rowp[KB];
+#pragma unroll
+  for (int k = 0; k < KB; ++k) {
+    const int j = tid + 64 * k;
+    rowp[k] = A + (size_t)s_perm[j < P ? j : 0] * ld;
+  }
+  double col[KB];
+#pragma unroll
+  for (int k = 0; k < KB; ++k) {
+    const int j = tid + 64 * k;
+    col[k] = j < P - 1 ? rowp[k][P - 1] : 0.0;
+  }
   lm_wave_lds_sync();
   for (int i = P - 1; i >= 0; --i) {
-    const double xi = s_row[i] / A[(size_t)s_perm[i] * ld + i];
+    double nxt[KB];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      const int j = tid + 64 * k;
+      nxt[k] = j < i - 1 ? rowp[k][i - 1] : 0.0;
+    }
+    const double xi = s_row[i] / s_diag[i];
     if (tid == 0) s_l[i] = xi;
-    for (int j = tid; j < i; j += 64) s_row[j] = fma(-A[(size_t)s_perm[j] * ld + i], xi, s_row[j]);
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      const int j = tid + 64 * k;
+      if (j < i) s_row[j] = fma(-col[k], xi, s_row[j]);
+    }
     lm_wave_lds_sync();
+#pragma unroll
+    for (int k = 0; k < KB; ++k) col[k] = nxt[k];
   }
 
   double* xt = trial_x + (size_t)t * P;
