@@ -68,33 +68,12 @@ struct WaveSink {
   IfaceZ<Z>* cif;
   LayerCot<Z>* cly;
   bool store;  // false for a wave with no tile
-  // dual Z: sum and store only the tangent parts.  The primal parts of a
-  // direction's cotangents do not depend on the direction (every primal
-  // operation of the dual sweep is the plain one), so where another
-  // direction of the call stores them (the one whose sweep runs to the
-  // bottom), the rest need not sum them again.
-  bool tan_only = false;
 
   // the sums of the doubles of v: the first n0 to d0, the rest to d1
   template <int NF>
   __device__ void reduce(const Z (&v)[NF], double* __restrict__ d0, int n0,
                          double* __restrict__ d1) {
     constexpr int N = Flat<Z>::N;
-    if constexpr (N == 4) {
-      if (tan_only) {
-        double x[NF * 2];
-#pragma unroll
-        for (int f = 0; f < NF; ++f) {
-          x[2 * f] = Flat<Z>::get(v[f], 2);
-          x[2 * f + 1] = Flat<Z>::get(v[f], 3);
-        }
-        const double s = wave_reduce_scatter<NF * 2>(x);
-        const int e = wave_rs_index(NF * 2, (int)(threadIdx.x % AD_WAVE));
-        const int q = (e >> 1) * N + 2 + (e & 1);  // its place in the record
-        if (store && e >= 0) *(q < n0 ? d0 + q : d1 + (q - n0)) = s;
-        return;
-      }
-    }
     double x[NF * N];
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
@@ -335,7 +314,7 @@ __global__ void __launch_bounds__(AD_BLOCK_MAX) adjoint_kernel(
     const cd* __restrict__ Gin, const cd* __restrict__ dGin, int second_only, Z* __restrict__ ws,
     long nlanes,
     IfaceZ<Z>* __restrict__ cot_if, LayerCot<Z>* __restrict__ cot_ly, Z* __restrict__ chi_part,
-    cd* __restrict__ R, cd* __restrict__ dR, int ntiles, int ntp, int partial, int tan_split,
+    cd* __restrict__ R, cd* __restrict__ dR, int ntiles, int ntp, int partial,
     int kstride, const char* __restrict__ tables, const PhPair* __restrict__ ph,
     const Z* __restrict__ pts) {
   extern __shared__ double smem_a[];
@@ -365,9 +344,6 @@ __global__ void __launch_bounds__(AD_BLOCK_MAX) adjoint_kernel(
   const cd om = omega[wc];
   const size_t slot = ((size_t)dir * np + ip) * ntp + tile;  // (dir, p, tile)
   WaveSink<Z> sink{cot_if + slot * (nl - 1), cot_ly + slot * nl, wave_ok};
-  // tan_split (a PARTIAL_REST call): direction 0 sweeps to the bottom and
-  // stores the primal parts that every other direction's chain reads
-  sink.tan_only = tan_split && dir != 0;
   const cd gin = Gin[(size_t)ip * nw + wc];
   const cd dgin = dGin ? dGin[((size_t)dir * np + ip) * nw + wc] : cd{0.0, 0.0};
   Z chi = lift<Z>(0.0);
@@ -547,8 +523,7 @@ __global__ void __launch_bounds__(CH_BLOCK) cot_sum_kernel(int nl, int np,
                                                            LayerCot<Z>* __restrict__ cot_ly,
                                                            const Z* __restrict__ chi_part,
                                                            int ntiles, int ntp,
-                                                           double* __restrict__ chi,
-                                                           int tan_split) {
+                                                           double* __restrict__ chi) {
   const int ip = (int)(blockIdx.x % (unsigned)np);
   const int dir = (int)(blockIdx.x / (unsigned)np);
   const size_t base = ((size_t)dir * np + ip) * ntp;
@@ -558,12 +533,7 @@ __global__ void __launch_bounds__(CH_BLOCK) cot_sum_kernel(int nl, int np,
   const int nall = nif + nl * NLY;
   double* gif = reinterpret_cast<double*>(cot_if + base * (nl - 1));
   double* gly = reinterpret_cast<double*>(cot_ly + base * nl);
-  // tan_split: only direction 0's records hold primal parts (the doubles
-  // 0 and 1 of every 4-double dual); the others hold tangent parts only
-  const bool tan_only = tan_split && dir != 0 && Flat<Z>::N == 4;
-  const int ne = tan_only ? nall / 2 : nall;
-  for (int j = threadIdx.x; j < ne; j += blockDim.x) {
-    const int e = tan_only ? (j >> 1) * 4 + 2 + (j & 1) : j;
+  for (int e = threadIdx.x; e < nall; e += blockDim.x) {
     double* const g = e < nif ? gif + e : gly + (e - nif);
     const size_t stride = e < nif ? (size_t)nif : (size_t)(nall - nif);
     *g = tile_sum(g, stride, ntiles);
@@ -644,23 +614,6 @@ __global__ void __launch_bounds__(CH_BLOCK) chain_far_kernel(
   }
 }
 
-// own's tangent parts with pr's primal parts
-__device__ __forceinline__ dcd with_primal(const dcd& own, const dcd& pr) {
-  return dcd{pr.v, own.d};
-}
-__device__ __forceinline__ IfaceZ<dcd> with_primal(const IfaceZ<dcd>& o, const IfaceZ<dcd>& q) {
-  return IfaceZ<dcd>{with_primal(o.rd00, q.rd00), with_primal(o.rd01, q.rd01),
-                     with_primal(o.rd11, q.rd11), with_primal(o.ru00, q.ru00),
-                     with_primal(o.ru01, q.ru01), with_primal(o.ru11, q.ru11),
-                     with_primal(o.td00, q.td00), with_primal(o.td01, q.td01),
-                     with_primal(o.td10, q.td10), with_primal(o.td11, q.td11)};
-}
-__device__ __forceinline__ LayerCot<dcd> with_primal(const LayerCot<dcd>& o,
-                                                     const LayerCot<dcd>& q) {
-  return LayerCot<dcd>{with_primal(o.eta, q.eta), with_primal(o.neta, q.neta),
-                       with_primal(o.h, q.h)};
-}
-
 // The pairs each direction reaches, NEAR_THREADS threads per (dir, p)
 // (three layers' columns: a basis direction's reach), packed into full
 // waves; a wider reach is covered by the same threads in turn.
@@ -670,7 +623,7 @@ __global__ void __launch_bounds__(CH_BLOCK) chain_near_kernel(
     const double* __restrict__ layers, const double* __restrict__ dlayers, int nl,
     const double* __restrict__ p, int np, int ndir, const IfaceZ<dcd>* __restrict__ cot_if,
     const LayerCot<dcd>* __restrict__ cot_ly, int ntp, double* __restrict__ grad,
-    double* __restrict__ hvp, int partial, int tan_split) {
+    double* __restrict__ hvp, int partial) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long pr = t / NEAR_THREADS;
   if (pr >= (long)ndir * np) return;
@@ -679,22 +632,13 @@ __global__ void __launch_bounds__(CH_BLOCK) chain_near_kernel(
   const size_t base = ((size_t)dir * np + ip) * ntp;
   const IfaceZ<dcd>* cif = cot_if + base * (nl - 1);
   const LayerCot<dcd>* cly = cot_ly + base * nl;
-  // the primal parts: direction 0's records at this slowness (tan_split),
-  // else this direction's own
-  const IfaceZ<dcd>* cif0 = tan_split ? cot_if + (size_t)ip * ntp * (nl - 1) : cif;
-  const LayerCot<dcd>* cly0 = tan_split ? cot_ly + (size_t)ip * ntp * nl : cly;
   const double* dl = dlayers + (size_t)dir * nl * NCOL;
   const Reach rc = reach_of(dl, nl, dir, partial);
   for (int j = j0; rc.lo + j / NCOL <= rc.hi; j += NEAR_THREADS) {
     const int l = rc.lo + j / NCOL, col = j - (j / NCOL) * NCOL;
     const size_t g = ((size_t)ip * nl + l) * NCOL + col;
-    // chain_column ignores the interface above layer 0 and below layer nl-1:
-    // any table entry stands in (a placeholder value would live in scratch)
-    const int iu = l >= 1 ? l - 1 : 0, id = l <= nl - 2 ? l : nl - 2;
-    const dcd v = chain_column_of<dcd>(
-        layers, dl, nl, l, p[ip], [&] { return with_primal(cif[iu], cif0[iu]); },
-        [&] { return with_primal(cif[id], cif0[id]); },
-        [&] { return with_primal(cly[l], cly0[l]); }, col);
+    const dcd v = chain_column<dcd>(layers, dl, nl, l, p[ip], cif[l >= 1 ? l - 1 : 0],
+                                    cif[l <= nl - 2 ? l : nl - 2], cly[l], col);
     if (grad && dir == 0) grad[g] = prim(v).r;
     hvp[(size_t)dir * np * nl * NCOL + g] = tangent_re(v);
   }
@@ -777,9 +721,6 @@ static int run_adjoint(const char* what, const double* layers, const double* dla
              "%s: workspace of %zu bytes (sqrt(N) checkpoints) or %zu (whole tape) needed, "
              "%zu given", what, d.total, deriv_layout<Z>(nl, np, nw, ndir, 1).total, ws_bytes);
   char* wsb = reinterpret_cast<char*>(workspace);
-  // the split chain below (cot_sum, chain_far, chain_near) takes the primal
-  // parts of every direction's cotangents from direction 0's records
-  const bool tan_split = std::is_same<Z, dcd>::value && partial == PARTIAL_REST && dlayers && hvp;
   const long nblocks = (long)ndir * np * (d.ntp / d.wpb);
   KH_REQUIRE(nblocks <= 0x7fffffffL, KH_ERR_UNSUPPORTED, "%s: grid too large", what);
   // LDS: the (direction, slowness)'s staged tables
@@ -812,14 +753,13 @@ static int run_adjoint(const char* what, const double* layers, const double* dla
                      reinterpret_cast<const cd*>(Gin), reinterpret_cast<const cd*>(dGin), second_only,
                      reinterpret_cast<Z*>(wsb + d.ws), d.nlanes, cif, cly, chip,
                      reinterpret_cast<cd*>(R), reinterpret_cast<cd*>(dR), d.ntiles, d.ntp,
-                     partial, tan_split ? 1 : 0, d.kstride, wsb + d.tab, ph, pts);
+                     partial, d.kstride, wsb + d.tab, ph, pts);
   KH_TRY(check_launch(what));
   const size_t lds_c = (size_t)(nl - 1) * sizeof(IfaceZ<Z>) + (size_t)nl * sizeof(LayerCot<Z>);
   if constexpr (std::is_same<Z, dcd>::value) {
     if (partial && dlayers && hvp) {
       hipLaunchKernelGGL((cot_sum_kernel<Z>), dim3((unsigned)((long)ndir * np)), dim3(CH_BLOCK),
-                         0, st, nl, np, cif, cly, MISFIT ? chip : nullptr, d.ntiles, d.ntp, chi,
-                         tan_split ? 1 : 0);
+                         0, st, nl, np, cif, cly, MISFIT ? chip : nullptr, d.ntiles, d.ntp, chi);
       KH_TRY(check_launch(what));
       auto* jac = reinterpret_cast<LocalJac*>(wsb + d.jac);
       const long njac = (long)np * nl * NJCOL;
@@ -832,7 +772,7 @@ static int run_adjoint(const char* what, const double* layers, const double* dla
       const long near_threads = (long)ndir * np * NEAR_THREADS;
       hipLaunchKernelGGL(chain_near_kernel, dim3((unsigned)((near_threads + CH_BLOCK - 1) / CH_BLOCK)),
                          dim3(CH_BLOCK), 0, st, layers, dlayers, nl, p, np, ndir, cif, cly, d.ntp,
-                         grad, hvp, partial, tan_split ? 1 : 0);
+                         grad, hvp, partial);
       return check_launch(what);
     }
   }

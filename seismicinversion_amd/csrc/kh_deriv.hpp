@@ -499,14 +499,13 @@ __host__ __device__ __forceinline__ IfaceZ<Z> outer_iface(const IfaceZ<W>& f) {
 // the row-major (nl, 6) parameter and direction tables (dlayers only for
 // dual Z).  The thickness enters only through the phase, whose cotangent
 // is lc.h.
-// (chain_column_of: the cotangents come from callables cb_up(), cb_dn() and
-// lc(), each evaluated where its value is used.)
-template <class Z, class CU, class CD, class CL>
-__host__ __device__ __forceinline__ Z chain_column_of(const double* layers, const double* dlayers,
-                                                      int nl, int l, double p, CU&& cb_up,
-                                                      CD&& cb_dn, CL&& lc, int col) {
+template <class Z>
+__host__ __device__ __forceinline__ Z chain_column(const double* layers, const double* dlayers, int nl,
+                                          int l, double p, const IfaceZ<Z>& cb_up,
+                                          const IfaceZ<Z>& cb_dn, const LayerCot<Z>& lc,
+                                          int col) {
   using W = typename Up<Z>::W;
-  if (col == COL_H) return lc().h;
+  if (col == COL_H) return lc.h;
   // Layer ll with the outer seed on (l, col); one neighbour at a time, so
   // that two dual-of-dual layers and one interface are live at once.
   auto layer = [&](int ll) {
@@ -520,11 +519,10 @@ __host__ __device__ __forceinline__ Z chain_column_of(const double* layers, cons
   Z acc;
   {
     const LayerZ<W> mid = layer(l);
-    const LayerCot<Z> c = lc();
-    acc = re_dot(Up<Z>::outer(mid.eta), c.eta) + re_dot(Up<Z>::outer(mid.neta), c.neta);
+    acc = re_dot(Up<Z>::outer(mid.eta), lc.eta) + re_dot(Up<Z>::outer(mid.neta), lc.neta);
     if (l >= 1) {
       const IfaceZ<W> f = interface_t<W>(l - 1, p, layer(l - 1), mid);
-      acc = acc + iface_dot(outer_iface<W, Z>(f), cb_up());
+      acc = acc + iface_dot(outer_iface<W, Z>(f), cb_up);
     }
   }
   if (l <= nl - 2) {
@@ -536,19 +534,9 @@ __host__ __device__ __forceinline__ Z chain_column_of(const double* layers, cons
     asm volatile("" : "+v"(lo));
 #endif
     const IfaceZ<W> f = interface_t<W>(l, p, layer(lo), layer(l + 1));
-    acc = acc + iface_dot(outer_iface<W, Z>(f), cb_dn());
+    acc = acc + iface_dot(outer_iface<W, Z>(f), cb_dn);
   }
   return acc;
-}
-
-template <class Z>
-__host__ __device__ __forceinline__ Z chain_column(const double* layers, const double* dlayers, int nl,
-                                                   int l, double p, const IfaceZ<Z>& cb_up,
-                                                   const IfaceZ<Z>& cb_dn, const LayerCot<Z>& lc,
-                                                   int col) {
-  return chain_column_of<Z>(
-      layers, dlayers, nl, l, p, [&]() -> const IfaceZ<Z>& { return cb_up; },
-      [&]() -> const IfaceZ<Z>& { return cb_dn; }, [&]() -> const LayerCot<Z>& { return lc; }, col);
 }
 
 // The first-order factors of chain_column<cd> for parameter column col
