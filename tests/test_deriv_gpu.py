@@ -316,3 +316,29 @@ def test_whole_tape_and_sqrt_checkpoints_give_the_same_numbers(monkeypatch, nl):
         assert np.array_equal(a == 0, b == 0)
         if np.any(b != 0):
             assert rel(a, b) <= 1e-11
+
+
+def test_partial_rows_do_not_depend_on_the_launch_split():
+    """kh_misfit_hvp_partial over a parameter basis in one launch and split
+    into several paced launches (as a server caps them) gives bit-identical
+    misfit, gradient and Hessian rows: every direction's chain uses its own
+    cotangent sums, so no row depends on which launch its direction ran in."""
+    from seismicinversion_amd import kennett_torch as kt
+    from seismicinversion_amd.taup_inversion import _Problem
+    from seismicinversion_amd.workloads import uniform_slownesses
+
+    nl, n_p, nw = 30, 16, 128
+    m = seeded_model(nl, 42)
+    t = kt.model_to_tensors(m)
+    refs = [t[k] for k in ("alpha", "beta", "rho", "thickness", "Q_alpha", "Q_beta")]
+    p = list(uniform_slownesses(n_p, 0.6))
+    w = torch.arange(1, nw + 1, dtype=torch.float64) * (2 * np.pi / 64)
+    obs = engine.forward(engine.layers_tensor(m), torch.tensor(p), w.to(torch.complex128))
+    prob = _Problem(*refs, p, w, obs)
+    layers = prob.table(kt._pack_params(*refs[:4]).numpy() * 1.02)
+    whole = D.misfit_hvp(layers, prob.basis, prob.p, prob.omega, prob.R_obs, partial=True)
+    cap = 0.7 * prob.basis.shape[0] * n_p * nw * (nl - 1)  # two launches
+    with D.paced(cap):
+        split = D.misfit_hvp(layers, prob.basis, prob.p, prob.omega, prob.R_obs, partial=True)
+    for name, a, b in zip(("chi", "grad", "hvp"), whole[:3], split[:3]):
+        assert torch.equal(a, b), name
